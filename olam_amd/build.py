@@ -1,0 +1,78 @@
+"""Build helpers: the HIP library (gfx950), the oracle (test infrastructure) and the
+Fortran ISO_C_BINDING interface module.
+
+Everything is built in-tree so the shared objects travel with the repository:
+  olam_amd/lib/libolamgpu.so      product: HIP kernels + C-ABI (include/olamgpu.h)
+  olam_amd/lib/olamgpu_iface.mod  Fortran interface module (+ .o) when amdflang exists
+  oracle/build/liboracle.so       CPU restatement used only by tests and the CPU baseline
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libolamgpu.so")
+FORTRAN = os.path.join(PKG, "fortran")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 (v_pk_*_f32) is kept out of all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def build_hip(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    src = os.path.join(CSRC, "olamgpu.hip")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    return os.path.join(ROOT, "oracle", "build", "liboracle.so")
+
+
+def flang():
+    for cand in ("/opt/rocm/bin/amdflang", "/opt/rocm/lib/llvm/bin/flang", shutil.which("amdflang") or ""):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran():
+    """Compile the ISO_C_BINDING interface module and the Fortran test driver."""
+    fc = flang()
+    if fc is None:
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    iface = os.path.join(FORTRAN, "olamgpu_iface.f90")
+    driver = os.path.join(FORTRAN, "olamgpu_driver.f90")
+    subprocess.run([fc, "-c", "-O2", "-J", LIBDIR, "-o", os.path.join(LIBDIR, "olamgpu_iface.o"), iface],
+                   check=True)
+    exe = os.path.join(LIBDIR, "olamgpu_fdriver")
+    if os.path.exists(driver):
+        subprocess.run([fc, "-O2", "-I", LIBDIR, "-o", exe, driver, os.path.join(LIBDIR, "olamgpu_iface.o"),
+                        "-L", LIBDIR, "-lolamgpu", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN"],
+                       check=True)
+    return exe
+
+
+def build_all(verbose=False):
+    build_hip(verbose)
+    build_oracle()
+    if os.path.exists(os.path.join(FORTRAN, "olamgpu_iface.f90")):
+        build_fortran()

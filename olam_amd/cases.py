@@ -1,0 +1,227 @@
+"""Synthetic, seeded inputs for the RK3 dynamical core (no datasets are used).
+
+* ``transport_winds``: prescribed-flow tracer cases (the reference's own isolation
+  harness for scalar_transport_rk, timestep.f90:200-228): solid-body rotation at
+  ``u0`` from a discrete M-point streamfunction (exactly non-divergent on the mesh),
+  plus seeded Gaussian noise on the face mass fluxes, and a new density that follows
+  the discrete continuity equation with the same fluxes, so that tracer mass is
+  conserved and a constant tracer stays constant.
+* ``cosine_bell``: h = (1 + cos(pi r / R)) / 2 inside r < R = a/3, centre (3pi/2, 0).
+* ``lognormal_tracers``: ln q ~ N(mu, sigma^2) i.i.d. per cell-level.
+* ``jet_state``: dry atmosphere, T = 250 K above 11 km and a 6.5 K/km lapse rate
+  below, surface pressure 1000 hPa, rebalanced with the model's own discrete
+  hydrostatic weights (gdz_wgt*8, as ohhi.f90:725-770 does), a zonal jet
+  u = 35 sin^2(2 phi) F(p) with F = exp(-(ln(p/250 hPa)/0.5)^2 / 2), and a 1 K
+  Gaussian theta bump at a seeded position.  The jet is not in thermal-wind
+  balance (only the vertical is balanced); it is a benchmark state, not a climate.
+
+Random draws use a counter-based generator keyed by (seed, stream, level, canonical
+point id), so values do not depend on the order in which points are numbered.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import mesh as M
+from .abi import FortranGrid, FortranState
+
+_MASK64 = (1 << 64) - 1
+
+
+def _splitmix(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15))
+    z = x
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def counter_normal(seed, stream, ids, nlev):
+    """N(0,1) draws of shape (len(ids), nlev) keyed by (seed, stream, id, level)."""
+    with np.errstate(over="ignore"):
+        ids = np.asarray(ids, dtype=np.uint64)[:, None]
+        lev = np.arange(nlev, dtype=np.uint64)[None, :]
+        base = np.uint64((seed * 1000003 + stream * 7919) & _MASK64)
+        key = _splitmix(base ^ (ids * np.uint64(0x100000001B3)) ^ (lev * np.uint64(0xC2B2AE3D27D4EB4F)))
+        k2 = _splitmix(key ^ np.uint64(0xD6E8FEB86659FD93))
+    u1 = ((key >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
+    u2 = ((k2 >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def _latlon_w(mesh):
+    g = mesh.geo
+    lat = np.radians(g["glatw"])
+    lon = np.radians(g["glonw"])
+    return lat, lon
+
+
+def transport_winds(grid: FortranGrid, dt: float, u0: float = 40.0, noise: float = 1.0,
+                    wnoise: float = 0.01, seed: int = 1, scale_h: float = 8000.0):
+    """vmasc, wmasc, rho_old, rho (Fortran layout) for a prescribed-flow tracer step."""
+    mesh, vg, cm = grid.mesh, grid.vg, grid.cm
+    g = mesh.geo
+    mza = vg["mza"]
+    zt = vg["zt"]
+    rho_prof = 1.2 * np.exp(-zt / scale_h)                       # [k]
+    # streamfunction at M points for u = u0 cos(lat) eastward: psi = -a u0 sin(lat)
+    zm_hat = np.stack([g["xem"], g["yem"], g["zem"]], 1) / M.ERAD
+    psi = -M.ERAD * u0 * zm_hat[:, 2]
+    im1, im2 = mesh.v_im[:, 0], mesh.v_im[:, 1]
+    vn = (psi[im1] - psi[im2]) / g["dnu"]                         # face-normal velocity
+    nv, nw = mesh.nv, mesh.nw
+    vmasc = np.zeros((mza + 1, nv))
+    for k in range(2, mza + 1):
+        vmasc[k] = rho_prof[k] * vn * cm["arv"][k]
+    nz = counter_normal(seed, 1, mesh.v_canon, mza + 1).T         # [k, iv]
+    vmasc[2:] += (rho_prof[2:, None] * noise * nz[2:] * mesh.v_orient[None, :]) * cm["arv"][2:]
+    wmasc = np.zeros((mza + 1, nw))
+    if wnoise:
+        nzw = counter_normal(seed, 2, mesh.w_canon, mza + 1).T
+        zw = vg["zm"]
+        rho_w = 1.2 * np.exp(-zw / scale_h)
+        for k in range(2, mza):
+            wmasc[k] = rho_w[k] * wnoise * nzw[k] * cm["arw"][k]
+    # the winds are FP32 fields (mem_basic.f90:8-20); continuity uses their FP32 values
+    vmasc = vmasc.astype(np.float32).astype(np.float64)
+    wmasc = wmasc.astype(np.float32).astype(np.float64)
+    # discrete continuity (scalar_transport_rk.F90 conservation form)
+    rho_old = np.zeros((mza + 1, nw))
+    # rho_old is FP32 in the reference (timestep.f90:38); build rho from its FP32 value
+    rho_old[2:] = rho_prof[2:, None].astype(np.float32).astype(np.float64)
+    div = np.zeros((mza + 1, nw))
+    for j in range(7):
+        act = j < mesh.npoly
+        iv = mesh.w_iv[act, j]
+        div[:, act] += mesh.dirv[act, j][None, :] * vmasc[:, iv]
+    div[2:] += wmasc[1:mza] - wmasc[2:mza + 1]
+    rho = np.zeros_like(rho_old)
+    rho[2:] = rho_old[2:] + dt / cm["volt"][2:] * div[2:]
+    out = {}
+    out["vmasc"] = grid.to_f(vmasc[1:].T.astype(np.float32), "v")
+    out["wmasc"] = grid.to_f(wmasc[1:].T.astype(np.float32), "w")
+    out["rho_old"] = grid.to_f(rho_old[1:].T.astype(np.float32), "w")
+    out["rho"] = grid.to_f(rho[1:].T, "w", np.float64)
+    return out
+
+
+def cosine_bell(grid: FortranGrid, lon0=1.5 * math.pi, lat0=0.0, radius=M.ERAD / 3.0):
+    mesh = grid.mesh
+    lat, lon = _latlon_w(mesh)
+    r = M.ERAD * np.arccos(np.clip(np.sin(lat0) * np.sin(lat) + np.cos(lat0) * np.cos(lat) * np.cos(lon - lon0), -1, 1))
+    h = np.where(r < radius, 0.5 * (1.0 + np.cos(math.pi * r / radius)), 0.0)
+    q = np.repeat(h[:, None], grid.mza, axis=1).astype(np.float32)
+    return grid.to_f(q, "w")
+
+
+def lognormal_tracers(grid: FortranGrid, ntr: int, mu=-20.0, sigma=2.0, seed=7):
+    mesh = grid.mesh
+    out = []
+    for n in range(ntr):
+        z = counter_normal(seed, 100 + n, mesh.w_canon, grid.mza)
+        q = np.exp(mu + sigma * z).astype(np.float32)
+        q[:, 0] = q[:, 1]
+        out.append(grid.to_f(q, "w"))
+    return out
+
+
+def seeded_field(grid: FortranGrid, seed, stream, kind="w", scale=1.0):
+    mesh = grid.mesh
+    ids = mesh.w_canon if kind == "w" else mesh.v_canon
+    z = counter_normal(seed, stream, ids, grid.mza).astype(np.float32) * np.float32(scale)
+    return grid.to_f(z, kind)
+
+
+def _hydrostatic_profile(z):
+    """Continuous T, p at height z for the configured sounding."""
+    t_strat, zt_trop, lapse, ps = 250.0, 11000.0, 0.0065, 1.0e5
+    t0 = t_strat + lapse * zt_trop
+    T = np.where(z < zt_trop, t0 - lapse * z, t_strat)
+    p_trop = ps * (t_strat / t0) ** (M.GRAV / (M.RDRY * lapse))
+    p = np.where(z < zt_trop, ps * (np.maximum(T, 1.0) / t0) ** (M.GRAV / (M.RDRY * lapse)),
+                 p_trop * np.exp(-M.GRAV * (z - zt_trop) / (M.RDRY * t_strat)))
+    return T, p
+
+
+def jet_state(grid: FortranGrid, seed: int = 3, u_jet: float = 35.0, bump: float = 1.0):
+    """Full dynamics state (olamgpu_state fields) in Fortran layout."""
+    mesh, vg, cm = grid.mesh, grid.vg, grid.cm
+    g = mesh.geo
+    mza = vg["mza"]
+    nw, nv = mesh.nw, mesh.nv
+    zt = vg["zt"]
+    T, p = _hydrostatic_profile(zt)
+    theta0 = T * (M.P00 / p) ** M.ROCP                       # [k]
+    lat, lon = _latlon_w(mesh)
+    # seeded theta bump
+    rng = np.random.default_rng(seed)
+    blat, blon = math.radians(rng.uniform(-40, 40)), math.radians(rng.uniform(0, 360))
+    rdist = M.ERAD * np.arccos(np.clip(np.sin(blat) * np.sin(lat) + np.cos(blat) * np.cos(lat) * np.cos(lon - blon), -1, 1))
+    horiz = np.exp(-(rdist / 1.0e6) ** 2)
+    theta = np.zeros((mza + 1, nw))
+    for k in range(2, mza + 1):
+        theta[k] = theta0[k] + bump * horiz * math.exp(-((zt[k] - 5000.0) / 2000.0) ** 2)
+    # discrete hydrostatic rebalance, per column, upward from level 2
+    alpha = M.ALPHA_DRY
+
+    def rho_of(pp, th):
+        return (pp / alpha) ** (1.0 / M.CPOCV) / th
+
+    pr = np.zeros((mza + 1, nw))
+    rh = np.zeros((mza + 1, nw))
+    pr[2] = p[2]
+    rh[2] = rho_of(pr[2], theta[2])
+    dzm = vg["dzm"]
+    zb, ztp = vg["zwgt_bot8"], vg["zwgt_top8"]
+    for k in range(2, mza):
+        pk, rk = pr[k], rh[k]
+        pn = pk * np.exp(-dzm[k] * M.GRAV / (M.RDRY * 250.0))
+        for _ in range(30):
+            rn = rho_of(pn, theta[k + 1])
+            f = pk - pn - dzm[k] * M.GRAV * (zb[k] * rk + ztp[k] * rn)
+            dfdp = -1.0 - dzm[k] * M.GRAV * ztp[k] * rn / (M.CPOCV * pn)
+            pn = pn - f / dfdp
+        pr[k + 1] = pn
+        rh[k + 1] = rho_of(pn, theta[k + 1])
+    # press must be the model's diagnostic alpha (rho theta)^(cp/cv)
+    pr[2:] = alpha * (rh[2:] * theta[2:]) ** M.CPOCV
+    # zonal jet on V faces
+    xv = np.stack([g["xev"], g["yev"], g["zev"]], 1) / M.ERAD
+    latv = np.arcsin(np.clip(xv[:, 2], -1, 1))
+    lonv = np.arctan2(xv[:, 1], xv[:, 0])
+    east = np.stack([-np.sin(lonv), np.cos(lonv), np.zeros_like(lonv)], 1)
+    vn = np.stack([g["vnx"], g["vny"], g["vnz"]], 1)
+    en = np.einsum("ij,ij->i", east, vn)
+    iw1, iw2 = mesh.v_iw[:, 0], mesh.v_iw[:, 1]
+    vc = np.zeros((mza + 1, nv))
+    vmc = np.zeros((mza + 1, nv))
+    for k in range(2, mza + 1):
+        pf = 0.5 * (pr[k, iw1] + pr[k, iw2])
+        F = np.exp(-0.5 * (np.log(pf / 25000.0) / 0.5) ** 2)
+        u = u_jet * np.sin(2.0 * latv) ** 2 * F
+        vc[k] = (u * en).astype(np.float32)
+        vmc[k] = vc[k] * 0.5 * (rh[k, iw1] + rh[k, iw2])
+    # cell velocities from vc (vel_t3d_hex with wc = 0)
+    vxe = np.zeros((mza + 1, nw)); vye = np.zeros_like(vxe); vze = np.zeros_like(vxe)
+    for j in range(7):
+        act = j < mesh.npoly
+        iv = mesh.w_iv[act, j]
+        vxe[:, act] += g["ecvec_vx"][act, j][None, :] * vc[:, iv]
+        vye[:, act] += g["ecvec_vy"][act, j][None, :] * vc[:, iv]
+        vze[:, act] += g["ecvec_vz"][act, j][None, :] * vc[:, iv]
+    for a in (vxe, vye, vze, theta):
+        a[1] = a[2]
+    st = FortranState(grid)
+    A = st.arrays
+    A["thil"][...] = grid.to_f(theta[1:].T.astype(np.float32))
+    A["rho"][...] = grid.to_f(rh[1:].T, "w", np.float64)
+    A["press"][...] = grid.to_f(pr[1:].T, "w", np.float64)
+    A["vmc"][...] = grid.to_f(vmc[1:].T.astype(np.float32), "v")
+    A["vc"][...] = grid.to_f(vc[1:].T.astype(np.float32), "v")
+    A["vxe"][...] = grid.to_f(vxe[1:].T.astype(np.float32))
+    A["vye"][...] = grid.to_f(vye[1:].T.astype(np.float32))
+    A["vze"][...] = grid.to_f(vze[1:].T.astype(np.float32))
+    return st

@@ -1,0 +1,85 @@
+// common.h -- device-side layout and wave helpers shared by the HIP kernels.
+//
+// Layout in HBM.  Every 3-D field is stored column by column exactly as the
+// reference's Fortran arrays (vertical index contiguous), with the dummy point
+// removed, points renumbered for locality, and the column stride padded to LDK
+// (a multiple of 16 elements, >= mza).  Fortran level k sits at offset k-1 of its
+// column.  One 64-lane wavefront owns one column: lane l holds level k = l + 1, so
+// every column load/store is a single coalesced 4*mza-byte access and neighbour
+// columns are gathered whole.  Per-point neighbour records are padded to 8 slots so
+// a wave reads them with scalar loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WAVE 64
+#define WPB 4          // waves per 256-thread block
+#define NSLOT 8        // padded neighbour slots per W point
+
+template <class R>
+struct DMesh {
+  int nw, nv, nm, mza, ldk;
+  // W records (nw x NSLOT)
+  const int *w_npoly, *w_nbr, *w_face, *w_back;
+  const R *w_dirv, *w_gxc, *w_gyc, *w_dxs, *w_dys, *w_dxn, *w_dyn;
+  const R *w_ecx, *w_ecy, *w_ecz, *w_xxyy;
+  const R *wnx, *wny, *wnz;
+  // V records (nv x 2)
+  const int *v_iw, *v_im;
+  const R *vnx, *vny, *vnz;
+  // M records (nm x 4)
+  const int *m_iv, *m_im;
+  const R *m_vfact, *m_xxyym;
+  // per level (ldk)
+  const R *dzim, *gdz_wgtm, *gdz_wgtp, *aw11, *aw21, *aw12, *aw22;
+  const double *zwgt_top8, *zwgt_bot8, *gdz_wgtm8, *gdz_wgtp8;
+  // control volumes (n x ldk)
+  const R *arw, *volti, *volwi, *arv, *volvi;
+  const double *volt;
+};
+
+// Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous range of
+// logical blocks so neighbouring columns (which share gathers) hit the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  int per = nb >> 3, rem = nb & 7;
+  int x = b & 7, idx = b >> 3;
+  return x * per + (x < rem ? x : rem) + idx;
+}
+
+__device__ __forceinline__ int wave_item() {
+  int b = xcd_block(blockIdx.x, gridDim.x);
+  return __builtin_amdgcn_readfirstlane(b * WPB + (threadIdx.x >> 6));
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+template <class T>
+__device__ __forceinline__ T shdn(T v, int d) { return __shfl_down(v, d, WAVE); }
+template <class T>
+__device__ __forceinline__ T shup(T v, int d) { return __shfl_up(v, d, WAVE); }
+
+// Parallel cyclic reduction of a tridiagonal system held one row per lane:
+// a x(l-1) + b x(l) + c x(l+1) = d.  Rows outside the system must be identity rows
+// (a = c = d = 0, b = 1).  Returns x for this lane.  FP64 throughout.
+__device__ __forceinline__ double pcr_solve(double a, double b, double c, double d) {
+  const int l = lane_id();
+#pragma unroll
+  for (int s = 1; s < WAVE; s <<= 1) {
+    double rb = 1.0 / b;
+    double am = shup(a, s), rbm = shup(rb, s), cm = shup(c, s), dm = shup(d, s);
+    double ap = shdn(a, s), rbp = shdn(rb, s), cp = shdn(c, s), dp = shdn(d, s);
+    if (l < s) { am = 0.0; rbm = 1.0; cm = 0.0; dm = 0.0; }
+    if (l + s >= WAVE) { ap = 0.0; rbp = 1.0; cp = 0.0; dp = 0.0; }
+    double alpha = -a * rbm;
+    double gamma = -c * rbp;
+    double na = alpha * am;
+    double nc = gamma * cp;
+    double nb = b + alpha * cm + gamma * ap;
+    double nd = d + alpha * dm + gamma * dp;
+    a = na; b = nb; c = nc; d = nd;
+  }
+  return d / b;
+}
+
+template <class T>
+__device__ __forceinline__ T ld(const T *p, long i, bool ok) { return ok ? p[i] : T(0); }

@@ -1,0 +1,404 @@
+// kernels_dyn.h -- prog_wrtv_rk: the acoustic RK step of W/V momentum, rho, rho*thil
+// and press, with divergence/vorticity damping and vel_t3d.
+//
+// Reference: omodel/prog_wrtv_rk.f90, omodel/vort_div_damp.f90, omodel/vel_t3d.f90,
+// omodel/grad_lib.f90.  Per RK stage four launches replace the reference's ten
+// OpenMP loops: k_dyn_grad (grad_t2d of thil, vxe, vye, vze), k_dyn_w (fluxes_wrtv's
+// face and vertical upwinding fused with prog_wrt_begs, including the FP64
+// tridiagonal W solve by parallel cyclic reduction inside the wavefront),
+// k_dyn_v (prog_v_begs) and k_dyn_vel (vel_t3d_hex).
+#pragma once
+#include "common.h"
+
+template <class R, class RP>
+struct DynBuf {
+  // prognostic state
+  const R *thil_in;
+  R *thil_out;
+  R *wmc, *wc, *vxe, *vye, *vze, *vmc, *vc, *vmasc, *wmasc;
+  RP *rho, *press;
+  const R *alpha_press, *pwfac, *pvfac;
+  // RK start values and volume-weighted short-step tendencies
+  const RP *rho0, *rth0;
+  const R *wmc0, *vmc0, *thilt_s, *vmxet_s, *vmyet_s, *vmzet_s, *vmt_s;
+  R *vmxet_rk, *vmyet_rk, *vmzet_rk;
+  // horizontal gradients: [thil, vxe, vye, vze] x [gx, gy]
+  const R *g[8];
+  int thil_order, wind_order, istage, nrk;
+  double dt8;
+  R fcor;
+};
+
+// ---------------------------------------------------------------- setup loop
+// prog_wrtv_rk.f90:165-200: RK start values and volume-weighted tendencies.
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, const RP *rho, const R *wmc,
+                                                     const R *thilt, const R *vmxet, const R *vmyet,
+                                                     const R *vmzet, RP *rth0, RP *rho0, R *wmc0, R *thilt_s,
+                                                     R *vmxet_s, R *vmyet_s, R *vmzet_s) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  if (k < 2 || k > m.mza) return;
+  const long ic = (long)c * m.ldk + l;
+  const R v4 = (R)m.volt[ic];
+  rth0[ic] = (RP)thil[ic] * rho[ic];
+  rho0[ic] = rho[ic];
+  wmc0[ic] = wmc[ic];
+  thilt_s[ic] = thilt[ic] * v4;
+  vmxet_s[ic] = vmxet[ic] * v4;
+  vmyet_s[ic] = vmyet[ic] * v4;
+  vmzet_s[ic] = vmzet[ic] * v4;
+}
+
+// ---------------------------------------------------------------- damping
+// divh_damp divergence, vort_div_damp.f90:140-156.  div2d(:,:,1) is SAVEd by the
+// reference and never reset, so the divergence accumulates onto what the previous
+// call left there (the last Laplacian for level 3); reproduced as is.
+template <class R>
+__global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, R *div) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  if (k < 2 || k > m.mza) return;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const int np = m.w_npoly[c];
+  R d = div[ic];
+  for (int j = 0; j < np; ++j) {
+    const long iv = (long)m.w_face[c * NSLOT + j] * ldk + l;
+    d = d - m.arv[iv] * m.w_dirv[c * NSLOT + j] * vmc[iv];
+  }
+  div[ic] = d * m.volti[ic];
+}
+
+// laplacian2d, grad_lib.f90:494-524
+template <class R>
+__global__ __launch_bounds__(256) void k_lap_w(DMesh<R> m, const R *src, R *dst) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  if (k > m.mza) return;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const int np = m.w_npoly[c];
+  R d = R(0);
+  if (k >= 2) {
+    const R s0 = src[ic];
+    for (int j = 0; j < np; ++j)
+      d = d + (src[(long)m.w_nbr[c * NSLOT + j] * ldk + l] - s0) * m.w_xxyy[c * NSLOT + j];
+  }
+  dst[ic] = d;
+}
+
+// vort_damp vorticity at M points, vort_div_damp.f90:304-336 (flat terrain)
+template <class R>
+__global__ __launch_bounds__(256) void k_vort(DMesh<R> m, const R *vc, R *vort) {
+  const int im = wave_item();
+  if (im >= m.nm) return;
+  const int l = lane_id(), k = l + 1;
+  if (k > m.mza) return;
+  const long ldk = m.ldk;
+  R v = R(0);
+  if (k >= 2)
+    for (int j = 0; j < 3; ++j) v = v + vc[(long)m.m_iv[im * 4 + j] * ldk + l] * m.m_vfact[im * 4 + j];
+  vort[(long)im * ldk + l] = v;
+}
+
+// laplacian_m, grad_lib.f90:531-559
+template <class R>
+__global__ __launch_bounds__(256) void k_lap_m(DMesh<R> m, const R *src, R *dst) {
+  const int im = wave_item();
+  if (im >= m.nm) return;
+  const int l = lane_id(), k = l + 1;
+  if (k > m.mza) return;
+  const long ldk = m.ldk, ic = (long)im * ldk + l;
+  R d = R(0);
+  if (k >= 2) {
+    const R s0 = src[ic];
+    for (int j = 0; j < 3; ++j)
+      d = d + (src[(long)m.m_im[im * 4 + j] * ldk + l] - s0) * m.m_xxyym[im * 4 + j];
+  }
+  dst[ic] = d;
+}
+
+// V setup (prog_wrtv_rk.f90:256-262) with the damping tendencies added to vmt_short
+// (vort_div_damp.f90:216-228 and :424-441).
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, const R *vmt, R *vmc0, R *vmt_s,
+                                                     const R *div, const R *fdivh, const R *vort,
+                                                     const R *fvorth, const RP *rho) {
+  const int iv = wave_item();
+  if (iv >= m.nv) return;
+  const int l = lane_id(), k = l + 1;
+  if (k > m.mza) return;
+  const long ldk = m.ldk, ie = (long)iv * ldk + l;
+  vmc0[ie] = vmc[ie];
+  R t = vmt[ie];
+  if (k >= 2) {
+    const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
+    if (div) t = t + fdivh[ie] * (div[(long)iw2 * ldk + l] - div[(long)iw1 * ldk + l]);
+    if (vort) {
+      const int im1 = m.v_im[iv * 2], im2 = m.v_im[iv * 2 + 1];
+      t = t + fvorth[iv] * (vort[(long)im2 * ldk + l] - vort[(long)im1 * ldk + l]) *
+                  (R)(rho[(long)iw1 * ldk + l] + rho[(long)iw2 * ldk + l]);
+    }
+  }
+  vmt_s[ie] = t;
+}
+
+// ---------------------------------------------------------------- gradients
+// grad_t2d of thil, vxe, vye, vze (fluxes_wrtv, prog_wrtv_rk.f90:976-987)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, const R *vxe, const R *vye,
+                                                  const R *vze, R *g0, R *g1, R *g2, R *g3, R *g4, R *g5, R *g6,
+                                                  R *g7, int do_thil, int do_wind) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza, body = act && k >= 2;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const int np = m.w_npoly[c];
+  const R *f[4] = {thil, vxe, vye, vze};
+  R *gx[4] = {g0, g2, g4, g6};
+  R *gy[4] = {g1, g3, g5, g7};
+  for (int q = 0; q < 4; ++q) {
+    if ((q == 0 && !do_thil) || (q > 0 && !do_wind)) continue;
+    const R s0 = ld(f[q], ic, act);
+    R ax = R(0), ay = R(0);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      if (j < np) {
+        R d = ld(f[q], (long)m.w_nbr[c * NSLOT + j] * ldk + l, act) - s0;
+        ax = ax + m.w_gxc[c * NSLOT + j] * d;
+        ay = ay + m.w_gyc[c * NSLOT + j] * d;
+      }
+    }
+    if (!body) { ax = R(0); ay = R(0); }
+    if (act) { gx[q][ic] = ax; gy[q][ic] = ay; }
+  }
+}
+
+// vertical upwinded face value at W level k of a column held one level per lane
+// (fluxes_wrtv scalar_w_column :1085-1126 with grad_z_quadratic, bounded)
+template <class R>
+__device__ __forceinline__ R vert_upw(R q, R wsign, int k, int mza, R a11, R a21, R a12, R a22) {
+  const bool inner = k >= 2 && k <= mza - 1;
+  const R qp1 = shdn(q, 1);
+  const R ds = inner ? qp1 - q : R(0);
+  const R dsm1 = shup(ds, 1);
+  const R scpb = q + ds * a11 + dsm1 * a21;
+  const R scpt = q + ds * a12 + dsm1 * a22;
+  const R scpb_p1 = shdn(scpb, 1);
+  R u = (wsign < R(0)) ? scpb_p1 : scpt;
+  if (k == 1) u = qp1;
+  if (k == mza) u = q;
+  return u;
+}
+
+// ---------------------------------------------------------------- prog_wrt_begs
+// prog_wrtv_rk.f90:494-884 (global domain, no nudging, b20 = 1) fused with the face
+// values of fluxes_wrtv's scalar_v_column (:1022-1081) and scalar_w_column.
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const int np = m.w_npoly[c];
+  const R cpocv = (R)(1004.0 / 717.0), rocv = (R)(287.0 / 717.0);
+  const RP fr = RP(0.70), fp2 = RP(0.70), fp3 = RP(0.70);
+  const RP pc2 = fp2 * (RP)cpocv, pc3 = fp3 * (RP)cpocv;
+  const double dt8 = b.dt8;
+  const R dts = (R)dt8;
+
+  // own column
+  const R thc = ld(b.thil_in, ic, act), vxc = ld(b.vxe, ic, act), vyc = ld(b.vye, ic, act),
+          vzc = ld(b.vze, ic, act);
+  R gxs[4] = {R(0), R(0), R(0), R(0)}, gys[4] = {R(0), R(0), R(0), R(0)};
+  for (int q = 0; q < 4; ++q) {
+    const int ord = q == 0 ? b.thil_order : b.wind_order;
+    if (ord > 1) { gxs[q] = ld(b.g[2 * q], ic, act); gys[q] = ld(b.g[2 * q + 1], ic, act); }
+  }
+  const R qc[4] = {thc, vxc, vyc, vzc};
+  // horizontal advective fluxes (:632-661)
+  RP hflx = RP(0);
+  R tt = ld(b.thilt_s, ic, act), tx = ld(b.vmxet_s, ic, act), ty = ld(b.vmyet_s, ic, act),
+    tz = ld(b.vmzet_s, ic, act);
+  for (int j = 0; j < np; ++j) {
+    const int iv = m.w_face[c * NSLOT + j];
+    const int nbr = m.w_nbr[c * NSLOT + j];
+    const R dv = m.w_dirv[c * NSLOT + j];
+    const R dxs = m.w_dxs[c * NSLOT + j], dys = m.w_dys[c * NSLOT + j];
+    const R dxn = m.w_dxn[c * NSLOT + j], dyn = m.w_dyn[c * NSLOT + j];
+    const long ie = (long)iv * ldk + l, in = (long)nbr * ldk + l;
+    const R vmcv = ld(b.vmc, ie, act);
+    const R vmca = vmcv * ld(m.arv, ie, act);
+    const bool sel = (dv < R(0)) ? (vmcv < R(0)) : !(vmcv < R(0));
+    R f[4];
+    for (int q = 0; q < 4; ++q) {
+      const int ord = q == 0 ? b.thil_order : b.wind_order;
+      const R *src = q == 0 ? b.thil_in : (q == 1 ? b.vxe : (q == 2 ? b.vye : b.vze));
+      const R qn = ld(src, in, act);
+      R fs = qc[q], fn = qn;
+      if (ord > 1) {
+        fs = qc[q] + dxs * gxs[q] + dys * gys[q];
+        fn = qn + dxn * ld(b.g[2 * q], in, act) + dyn * ld(b.g[2 * q + 1], in, act);
+      }
+      f[q] = sel ? fn : fs;
+    }
+    if (dv > R(0)) {
+      hflx = hflx + (RP)vmca;
+      tt = tt + vmca * f[0]; tx = tx + vmca * f[1]; ty = ty + vmca * f[2]; tz = tz + vmca * f[3];
+    } else {
+      hflx = hflx - (RP)vmca;
+      tt = tt - vmca * f[0]; tx = tx - vmca * f[1]; ty = ty - vmca * f[2]; tz = tz - vmca * f[3];
+    }
+  }
+  // Coriolis, global (:675-681)
+  const RP rho_c = act ? b.rho[ic] : RP(0);
+  const double vol = act ? m.volt[ic] : 0.0;
+  {
+    const R mass = (R)(rho_c * (RP)vol);
+    tx = tx + mass * b.fcor * vyc;
+    ty = ty - mass * b.fcor * vxc;
+  }
+  // vertical upwinded values (own column, wmc sign)
+  const R wmcc = ld(b.wmc, ic, act);
+  const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
+  const R tupw = vert_upw(thc, wmcc, k, mza, a11, a21, a12, a22);
+  const R xupw = vert_upw(vxc, wmcc, k, mza, a11, a21, a12, a22);
+  const R yupw = vert_upw(vyc, wmcc, k, mza, a11, a21, a12, a22);
+  const R zupw = vert_upw(vzc, wmcc, k, mza, a11, a21, a12, a22);
+  // explicit density and theta tendencies (:687-723)
+  const R vti = ld(m.volti, ic, act);
+  const R alp = ld(b.alpha_press, ic, act);
+  const RP rhothil = rho_c * (RP)thc;
+  const RP b10 = (RP)dt8 * (RP)vti;
+  const RP b5 = body ? (RP)(alp * (R)pow((R)rhothil, rocv)) : RP(0);
+  const R b15 = (R)(b10 * b5);
+  RP delex_rho = b10 * hflx;
+  RP delex_rt = b10 * (RP)tt;
+  if (b.istage > 1) {
+    delex_rho = delex_rho + ld(b.rho0, ic, act) - rho_c;
+    delex_rt = delex_rt + ld(b.rth0, ic, act) - rhothil;
+  }
+  const RP press_ex = b5 * (rhothil + pc2 * delex_rt);
+  const RP rho_ex = rho_c + fr * delex_rho;
+  // explicit W momentum (:727-745), W levels ka..mza-1
+  const RP pex_p1 = shdn(press_ex, 1), rex_p1 = shdn(rho_ex, 1);
+  const R tx_p1 = shdn(tx, 1), ty_p1 = shdn(ty, 1), tz_p1 = shdn(tz, 1);
+  const R pwf = ld(b.pwfac, ic, act), vwi = ld(m.volwi, ic, act);
+  R delex_wm = R(0);
+  if (inner)
+    delex_wm = ld(b.wmc0, ic, act) +
+               dts * ((R)(press_ex - pex_p1) * pwf -
+                      (R)((RP)m.gdz_wgtm8[l] * rho_ex + (RP)m.gdz_wgtp8[l] * rex_p1) +
+                      (m.wnx[c] * (tx + tx_p1) + m.wny[c] * (ty + ty_p1) + m.wnz[c] * (tz + tz_p1)) * vwi);
+  // implicit W matrix (:749-769); b20 = 1 without Rayleigh friction on W
+  const R c8 = dts * (R)pc2;
+  const R c9 = -dts * dts * (R)fr;
+  const R wcc = ld(b.wc, ic, act);
+  const R wc_m1 = shup(wcc, 1), wc_p1 = shdn(wcc, 1);
+  const R b15_p1 = shdn(b15, 1), vti_p1 = shdn(vti, 1);
+  const R arw = ld(m.arw, ic, act), arw_m1 = shup(arw, 1), arw_p1 = shdn(arw, 1);
+  const R tupw_m1 = shup(tupw, 1), tupw_p1 = shdn(tupw, 1);
+  double ma = 0.0, mb = 1.0, mc = 0.0, md = 0.0;
+  if (inner) {
+    const R b7 = dts * vwi;
+    const R b8 = c8 * pwf;
+    const R b21 = b7 * wc_m1;
+    const R b22 = b7 * wc_p1;
+    const R b23 = b8 * b15;
+    const R b24 = b8 * b15_p1;
+    const R b25 = c9 * vti * m.gdz_wgtm[l];
+    const R b26 = c9 * vti_p1 * m.gdz_wgtp[l];
+    ma = (double)(-arw_m1 * (b21 + b25 + b23 * tupw_m1));
+    mb = (double)(R(1) + arw * (b25 - b26 + (b23 + b24) * tupw));
+    mc = (double)(arw_p1 * (b22 + b26 - b24 * tupw_p1));
+    md = (double)delex_wm;
+  }
+  // tridiffo (tridiag_lib.f90:10-36) replaced by FP64 PCR within the wavefront
+  const double wmf = pcr_solve(ma, mb, mc, md);
+  const R wmff = (R)wmf;
+  const R wma = inner ? wmff * arw : R(0);
+  const RP vflux_thil = (RP)(wma * tupw);
+  const R vfx = wma * xupw, vfy = wma * yupw, vfz = wma * zupw;
+  if (b.istage == b.nrk && inner) b.wmasc[ic] = b.wmasc[ic] + wma;
+  // T-level updates (:818-844)
+  const R wma_m1 = shup(wma, 1);
+  const RP vft_m1 = shup(vflux_thil, 1);
+  const R vfx_m1 = shup(vfx, 1), vfy_m1 = shup(vfy, 1), vfz_m1 = shup(vfz, 1);
+  RP rho_new = rho_c;
+  R thil_new = thc;
+  if (body) {
+    rho_new = ld(b.rho0, ic, act) + (RP)dt8 * (RP)vti * (hflx + (RP)wma_m1 - (RP)wma);
+    delex_rt = delex_rt + b10 * (vft_m1 - vflux_thil);
+    b.press[ic] = b5 * (rhothil + pc3 * delex_rt);
+    thil_new = (R)((rhothil + delex_rt) / rho_new);
+    b.rho[ic] = rho_new;
+    b.vmxet_rk[ic] = tx + vfx_m1 - vfx;
+    b.vmyet_rk[ic] = ty + vfy_m1 - vfy;
+    b.vmzet_rk[ic] = tz + vfz_m1 - vfz;
+  }
+  // W-level updates (:869-882)
+  const RP rho_new_p1 = shdn(rho_new, 1);
+  R wc_new = wcc;
+  if (inner) {
+    b.wmc[ic] = wmff;
+    wc_new = wmff / (R)((RP)m.zwgt_bot8[l] * rho_new + (RP)m.zwgt_top8[l] * rho_new_p1);
+  }
+  const R wc_new_p1 = shdn(wc_new, 1);
+  const R thil_new_p1 = shdn(thil_new, 1);
+  if (k == 1) { wc_new = wc_new_p1; thil_new = thil_new_p1; }
+  if (act && k <= mza - 1) b.wc[ic] = wc_new;
+  if (act) b.thil_out[ic] = thil_new;
+}
+
+// ---------------------------------------------------------------- prog_v_begs
+// prog_wrtv_rk.f90:888-946.  At stage nrk-1 the new vmc*arv is what the last stage
+// uses as vmca, so it is added to vmasc here (:356-365).
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int acc_vmasc) {
+  const int iv = wave_item();
+  if (iv >= m.nv) return;
+  const int l = lane_id(), k = l + 1;
+  if (k < 2 || k > m.mza) return;
+  const long ldk = m.ldk, ie = (long)iv * ldk + l;
+  const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
+  const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
+  const R dts = (R)b.dt8;
+  const R pgf = (R)(b.press[i1] - b.press[i2]) * b.pvfac[ie];
+  const R v = b.vmc0[ie] + dts * (b.vmt_s[ie] + pgf +
+                                  (m.vnx[iv] * (b.vmxet_rk[i1] + b.vmxet_rk[i2]) +
+                                   m.vny[iv] * (b.vmyet_rk[i1] + b.vmyet_rk[i2]) +
+                                   m.vnz[iv] * (b.vmzet_rk[i1] + b.vmzet_rk[i2])) * m.volvi[ie]);
+  b.vmc[ie] = v;
+  b.vc[ie] = R(2) * v / (R)(b.rho[i1] + b.rho[i2]);
+  if (acc_vmasc) b.vmasc[ie] = b.vmasc[ie] + v * m.arv[ie];
+}
+
+// ---------------------------------------------------------------- vel_t3d_hex
+// vel_t3d.f90:39-108 (no closed faces on flat terrain)
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const R wcc = ld(b.wc, ic, act);
+  const R wst = shup(wcc, 1) + wcc;
+  R x = wst * (m.wnx[c] * R(0.5)), y = wst * (m.wny[c] * R(0.5)), z = wst * (m.wnz[c] * R(0.5));
+  const int np = m.w_npoly[c];
+  for (int j = 0; j < np; ++j) {
+    const R v = ld(b.vc, (long)m.w_face[c * NSLOT + j] * ldk + l, act);
+    x = x + m.w_ecx[c * NSLOT + j] * v;
+    y = y + m.w_ecy[c * NSLOT + j] * v;
+    z = z + m.w_ecz[c * NSLOT + j] * v;
+  }
+  const R x2 = shdn(x, 1), y2 = shdn(y, 1), z2 = shdn(z, 1);
+  if (k == 1) { x = x2; y = y2; z = z2; }
+  if (act) { b.vxe[ic] = x; b.vye[ic] = y; b.vze[ic] = z; }
+  (void)body;
+}

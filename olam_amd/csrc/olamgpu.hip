@@ -1,0 +1,979 @@
+// olamgpu.hip -- MI355X implementation of OLAM's RK3 dynamical core behind the C-ABI
+// declared in include/olamgpu.h.
+//
+// Host side: grid renumbering with a checked round trip, device buffers, the RK
+// drivers of prog_wrtv_rk (omodel/prog_wrtv_rk.f90:58-488) and scalar_transport_rk
+// (omodel/scalar_transport_rk.F90:30-385), the tracer adjoint, and the C entry
+// points.  Kernels live in kernels_tracer.h and kernels_dyn.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/olamgpu.h"
+#include "common.h"
+#include "kernels_dyn.h"
+#include "kernels_tracer.h"
+
+namespace {
+
+struct Err : std::runtime_error {
+  int code;
+  Err(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                               \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess)                                                                       \
+      throw Err(e_ == hipErrorOutOfMemory ? OLAMGPU_ENOMEM : OLAMGPU_EHIP,                     \
+                std::string(#x) + ": " + hipGetErrorString(e_));                                \
+  } while (0)
+
+// ------------------------------------------------------------------ device memory
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t b) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    bytes = b;
+    if (b) HIPCHK(hipMalloc(&p, b));
+  }
+  template <class T> T *as() const { return (T *)p; }
+};
+
+// ------------------------------------------------------------------ renumbering
+// 3-D Hilbert index of a point on the unit sphere (Skilling's transpose method,
+// 21 bits per axis).  Gives each XCD's contiguous block range a compact patch.
+static uint64_t hilbert3(double x, double y, double z) {
+  const int B = 21;
+  uint32_t X[3];
+  double c[3] = {x, y, z};
+  for (int i = 0; i < 3; ++i) {
+    double t = (c[i] + 1.0) * 0.5;
+    t = std::min(std::max(t, 0.0), 1.0);
+    X[i] = (uint32_t)std::min((double)((1u << B) - 1), t * (double)(1u << B));
+  }
+  uint32_t M = 1u << (B - 1), P, Q, t;
+  for (Q = M; Q > 1; Q >>= 1) {
+    P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) X[0] ^= P;
+      else { t = (X[0] ^ X[i]) & P; X[0] ^= t; X[i] ^= t; }
+    }
+  }
+  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+  t = 0;
+  for (Q = M; Q > 1; Q >>= 1) if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; ++i) X[i] ^= t;
+  uint64_t h = 0;
+  for (int b = B - 1; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+  return h;
+}
+
+struct Topo {
+  int nw, nv, nm, mza, ldk;
+  std::vector<int> perm_w, inv_w, perm_v, inv_v, perm_m, inv_m;   // perm: new -> old (0-based)
+  std::vector<int> w_npoly, w_nbr, w_face, w_back, v_iw, v_im, m_iv, m_im;
+};
+
+static int ld_k(int mza) { return ((mza + 15) / 16) * 16; }
+
+// Fortran accessors on the input descriptor (1-based, dummy index 1)
+struct In {
+  const olamgpu_mesh *m;
+  int iw(int j, int w) const { return m->w_iw[(size_t)(w + 1) * 7 + j] - 2; }   // w: 0-based real
+  int iv(int j, int w) const { return m->w_iv[(size_t)(w + 1) * 7 + j] - 2; }
+  int np(int w) const { return m->w_npoly[w + 1]; }
+  int viw(int s, int v) const { return m->v_iw[(size_t)(v + 1) * 2 + s] - 2; }
+  int vim(int s, int v) const { return m->v_im[(size_t)(v + 1) * 2 + s] - 2; }
+  int miv(int j, int i) const { return m->m_iv[(size_t)(i + 1) * 3 + j] - 2; }
+  int mim(int j, int i) const { return m->m_im[(size_t)(i + 1) * 3 + j] - 2; }
+};
+
+static Topo build_topo(const olamgpu_mesh *m) {
+  Topo t;
+  t.nw = m->mwa - 1; t.nv = m->mva - 1; t.nm = m->mma - 1; t.mza = m->mza; t.ldk = ld_k(m->mza);
+  In in{m};
+  if (t.nw < 4 || t.nv < 6 || t.nm < 4) throw Err(OLAMGPU_EINVAL, "mesh too small");
+  if (m->mza < 4 || m->mza > 64) throw Err(OLAMGPU_EINVAL, "mza must be in [4, 64]");
+  for (int i = 1; i < m->mwa; ++i)
+    if (m->lpw[i] != 2) throw Err(OLAMGPU_EINVAL, "only flat terrain (lpw = 2) is supported");
+  for (int i = 1; i < m->mva; ++i)
+    if (m->lpv[i] != 2) throw Err(OLAMGPU_EINVAL, "only flat terrain (lpv = 2) is supported");
+  for (int i = 1; i < m->mma; ++i)
+    if (m->lpm[i] != 2) throw Err(OLAMGPU_EINVAL, "only flat terrain (lpm = 2) is supported");
+  for (int w = 0; w < t.nw; ++w) {
+    int np = in.np(w);
+    if (np < 3 || np > 7) throw Err(OLAMGPU_EINVAL, "npoly out of range");
+    for (int j = 0; j < np; ++j) {
+      int a = in.iw(j, w), v = in.iv(j, w);
+      if (a < 0 || a >= t.nw || v < 0 || v >= t.nv) throw Err(OLAMGPU_EINVAL, "itab_w index out of range");
+      int p = in.viw(0, v), q = in.viw(1, v);
+      if (!((p == w && q == a) || (p == a && q == w))) throw Err(OLAMGPU_EINVAL, "itab_w/itab_v mismatch");
+    }
+  }
+  for (int v = 0; v < t.nv; ++v)
+    for (int s = 0; s < 2; ++s) {
+      int a = in.viw(s, v), b = in.vim(s, v);
+      if (a < 0 || a >= t.nw || b < 0 || b >= t.nm) throw Err(OLAMGPU_EINVAL, "itab_v index out of range");
+    }
+  for (int i = 0; i < t.nm; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int v = in.miv(j, i), o = in.mim(j, i);
+      if (v < 0 || v >= t.nv || o < 0 || o >= t.nm) throw Err(OLAMGPU_EINVAL, "itab_m index out of range");
+    }
+  // W: Hilbert order of the unit position vectors (wnx, wny, wnz)
+  std::vector<uint64_t> key(t.nw);
+  for (int w = 0; w < t.nw; ++w) key[w] = hilbert3(m->wnx[w + 1], m->wny[w + 1], m->wnz[w + 1]);
+  t.perm_w.resize(t.nw);
+  std::iota(t.perm_w.begin(), t.perm_w.end(), 0);
+  std::stable_sort(t.perm_w.begin(), t.perm_w.end(), [&](int a, int b) { return key[a] < key[b]; });
+  t.inv_w.resize(t.nw);
+  for (int c = 0; c < t.nw; ++c) t.inv_w[t.perm_w[c]] = c;
+  // V: by owning (lower-numbered) W, then the other W
+  std::vector<uint64_t> kv(t.nv);
+  for (int v = 0; v < t.nv; ++v) {
+    uint64_t a = t.inv_w[in.viw(0, v)], b = t.inv_w[in.viw(1, v)];
+    kv[v] = (std::min(a, b) << 32) | std::max(a, b);
+  }
+  t.perm_v.resize(t.nv);
+  std::iota(t.perm_v.begin(), t.perm_v.end(), 0);
+  std::stable_sort(t.perm_v.begin(), t.perm_v.end(), [&](int a, int b) { return kv[a] < kv[b]; });
+  t.inv_v.resize(t.nv);
+  for (int c = 0; c < t.nv; ++c) t.inv_v[t.perm_v[c]] = c;
+  // M: by the lowest new index of its three V points' W neighbours
+  std::vector<uint64_t> km(t.nm);
+  for (int i = 0; i < t.nm; ++i) {
+    uint64_t lo = UINT64_MAX;
+    for (int j = 0; j < 3; ++j) {
+      int v = in.miv(j, i);
+      lo = std::min<uint64_t>(lo, t.inv_v[v]);
+    }
+    km[i] = lo;
+  }
+  t.perm_m.resize(t.nm);
+  std::iota(t.perm_m.begin(), t.perm_m.end(), 0);
+  std::stable_sort(t.perm_m.begin(), t.perm_m.end(), [&](int a, int b) { return km[a] < km[b]; });
+  t.inv_m.resize(t.nm);
+  for (int c = 0; c < t.nm; ++c) t.inv_m[t.perm_m[c]] = c;
+  // renumbered tables
+  t.w_npoly.assign(t.nw, 0);
+  t.w_nbr.assign((size_t)t.nw * NSLOT, 0);
+  t.w_face.assign((size_t)t.nw * NSLOT, 0);
+  t.w_back.assign((size_t)t.nw * NSLOT, 0);
+  for (int c = 0; c < t.nw; ++c) {
+    int o = t.perm_w[c], np = in.np(o);
+    t.w_npoly[c] = np;
+    for (int j = 0; j < np; ++j) {
+      t.w_nbr[(size_t)c * NSLOT + j] = t.inv_w[in.iw(j, o)];
+      t.w_face[(size_t)c * NSLOT + j] = t.inv_v[in.iv(j, o)];
+    }
+    for (int j = np; j < NSLOT; ++j) { t.w_nbr[(size_t)c * NSLOT + j] = c; t.w_face[(size_t)c * NSLOT + j] = 0; }
+  }
+  for (int c = 0; c < t.nw; ++c)
+    for (int j = 0; j < t.w_npoly[c]; ++j) {
+      int n = t.w_nbr[(size_t)c * NSLOT + j], b = -1;
+      for (int i = 0; i < t.w_npoly[n]; ++i) if (t.w_nbr[(size_t)n * NSLOT + i] == c) b = i;
+      if (b < 0) throw Err(OLAMGPU_EINVAL, "W adjacency is not symmetric");
+      t.w_back[(size_t)c * NSLOT + j] = b;
+    }
+  t.v_iw.resize((size_t)t.nv * 2); t.v_im.resize((size_t)t.nv * 2);
+  for (int e = 0; e < t.nv; ++e) {
+    int o = t.perm_v[e];
+    for (int s = 0; s < 2; ++s) { t.v_iw[e * 2 + s] = t.inv_w[in.viw(s, o)]; t.v_im[e * 2 + s] = t.inv_m[in.vim(s, o)]; }
+  }
+  t.m_iv.assign((size_t)t.nm * 4, 0); t.m_im.assign((size_t)t.nm * 4, 0);
+  for (int i = 0; i < t.nm; ++i) {
+    int o = t.perm_m[i];
+    for (int j = 0; j < 3; ++j) { t.m_iv[i * 4 + j] = t.inv_v[in.miv(j, o)]; t.m_im[i * 4 + j] = t.inv_m[in.mim(j, o)]; }
+    t.m_iv[i * 4 + 3] = t.m_iv[i * 4]; t.m_im[i * 4 + 3] = t.m_im[i * 4];
+  }
+  // round trip: map every renumbered table back and compare with the input exactly
+  for (int c = 0; c < t.nw; ++c) {
+    int o = t.perm_w[c];
+    if (t.w_npoly[c] != in.np(o)) throw Err(OLAMGPU_EINVAL, "renumbering check failed (npoly)");
+    for (int j = 0; j < t.w_npoly[c]; ++j) {
+      if (t.perm_w[t.w_nbr[(size_t)c * NSLOT + j]] + 2 != m->w_iw[(size_t)(o + 1) * 7 + j] ||
+          t.perm_v[t.w_face[(size_t)c * NSLOT + j]] + 2 != m->w_iv[(size_t)(o + 1) * 7 + j])
+        throw Err(OLAMGPU_EINVAL, "renumbering check failed (itab_w)");
+    }
+  }
+  for (int e = 0; e < t.nv; ++e) {
+    int o = t.perm_v[e];
+    for (int s = 0; s < 2; ++s)
+      if (t.perm_w[t.v_iw[e * 2 + s]] + 2 != m->v_iw[(size_t)(o + 1) * 2 + s] ||
+          t.perm_m[t.v_im[e * 2 + s]] + 2 != m->v_im[(size_t)(o + 1) * 2 + s])
+        throw Err(OLAMGPU_EINVAL, "renumbering check failed (itab_v)");
+  }
+  for (int i = 0; i < t.nm; ++i) {
+    int o = t.perm_m[i];
+    for (int j = 0; j < 3; ++j)
+      if (t.perm_v[t.m_iv[i * 4 + j]] + 2 != m->m_iv[(size_t)(o + 1) * 3 + j] ||
+          t.perm_m[t.m_im[i * 4 + j]] + 2 != m->m_im[(size_t)(o + 1) * 3 + j])
+        throw Err(OLAMGPU_EINVAL, "renumbering check failed (itab_m)");
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------ implementation
+struct ImplBase {
+  virtual ~ImplBase() {}
+  virtual void upload_state(const olamgpu_state *) = 0;
+  virtual void download_state(olamgpu_state *) = 0;
+  virtual void upload_scalar(int, const float *, const float *) = 0;
+  virtual void download_scalar(int, float *) = 0;
+  virtual void upload_adj(int, const float *) = 0;
+  virtual void download_adj(int, float *) = 0;
+  virtual void tend0() = 0;
+  virtual void prog_wrtv_rk() = 0;
+  virtual void timeavg_momsc() = 0;
+  virtual void scalar_transport_rk(const float *) = 0;
+  virtual void scalar_transport_rk_adj() = 0;
+  virtual void mass_sums(double *) = 0;
+  virtual void cfl_report(double *) = 0;
+  virtual void copy_field(int, int, void *, int, int *) = 0;
+};
+
+template <class T> static inline T ipow(T x, int n) { T r = T(1); for (int i = 0; i < n; ++i) r *= x; return r; }
+
+template <class R, class RP>
+struct Impl : ImplBase {
+  hipStream_t s;
+  olamgpu_params prm;
+  Topo t;
+  int nw, nv, nm, mza, ldk;
+  size_t wsz, vsz, msz;        // elements per W/V/M field
+  DMesh<R> dm;
+  // mesh storage
+  DevBuf b_npoly, b_nbr, b_face, b_back, b_viw, b_vim, b_miv, b_mim, b_invw, b_invv;
+  DevBuf b_dirv, b_gxc, b_gyc, b_dxs, b_dys, b_dxn, b_dyn, b_ecx, b_ecy, b_ecz, b_xxyy, b_wn[3];
+  DevBuf b_vn[3], b_vfact, b_xxyym, b_lev[7], b_lev8[4];
+  DevBuf b_arw, b_volti, b_volwi, b_volt, b_arv, b_volvi;
+  // state
+  DevBuf thil[2], wmc, wc, vxe, vye, vze, vmc, vc, vmasc, wmasc, alpha, pwfac, pvfac, thilt, vmxet, vmyet, vmzet, vmt;
+  DevBuf rho, press, rho_old;
+  int cur_thil = 0;
+  // prog_wrtv_rk scratch
+  DevBuf rho0, rth0, wmc0, vmc0, thilt_s, vmxet_s, vmyet_s, vmzet_s, vmt_s, vmxet_rk, vmyet_rk, vmzet_rk, grads[8];
+  DevBuf div2d[2], vort2d[2], fdivh, fvorth;
+  int div_ih = 0, vort_ih = 0;
+  bool do_div = false, do_vort = false;
+  // tracers
+  int ntr;
+  long tstride;
+  DevBuf trS, trX, trY, trgx, trgy, trsct;
+  bool have_sct = false;
+  DevBuf ffimp, cflmax, nimp, masspart, massout;
+  // adjoint
+  DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy;
+  int cur_adj = 0;
+  bool have_fwd = false;
+  double cfl_out[4] = {0, 0, 0, 0};
+
+  template <class T> T *dp(DevBuf &b) { return b.as<T>(); }
+
+  void up(DevBuf &b, const void *h, size_t bytes) {
+    b.alloc(bytes);
+    HIPCHK(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  template <class T> void upv(DevBuf &b, const std::vector<T> &v) { up(b, v.data(), v.size() * sizeof(T)); }
+
+  unsigned nblk(int n) const { return (unsigned)((n + WPB - 1) / WPB); }
+
+  Impl(const olamgpu_mesh *m, const olamgpu_params *p, hipStream_t st, Topo &&topo) : s(st), prm(*p), t(std::move(topo)) {
+    nw = t.nw; nv = t.nv; nm = t.nm; mza = t.mza; ldk = t.ldk;
+    wsz = (size_t)nw * ldk; vsz = (size_t)nv * ldk; msz = (size_t)nm * ldk;
+    if (prm.nrk_wrtv < 2 || prm.nrk_wrtv > 3 || prm.nrk_scal < 2 || prm.nrk_scal > 3)
+      throw Err(OLAMGPU_EINVAL, "RK levels must be 2 or 3");
+    for (int o : {prm.thil_horiz_adv_order, prm.wind_horiz_adv_order, prm.scal_horiz_adv_order})
+      if (o < 1 || o > 2) throw Err(OLAMGPU_EINVAL, "horizontal advection order must be 1 or 2");
+    if (prm.iscal_monot != 0) throw Err(OLAMGPU_EINVAL, "iscal_monot > 0 is not supported");
+    if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
+    build_mesh(m);
+    alloc_state();
+  }
+
+  // -------------------------------------------------------------- mesh upload
+  void build_mesh(const olamgpu_mesh *m) {
+    In in{m};
+    const bool f64 = sizeof(R) == 8;
+    std::vector<R> dirv(nw * NSLOT, 0), gxc(nw * NSLOT, 0), gyc(nw * NSLOT, 0), dxs(nw * NSLOT, 0),
+        dys(nw * NSLOT, 0), dxn(nw * NSLOT, 0), dyn(nw * NSLOT, 0), ecx(nw * NSLOT, 0), ecy(nw * NSLOT, 0),
+        ecz(nw * NSLOT, 0), xxyy(nw * NSLOT, 0), wn[3] = {std::vector<R>(nw), std::vector<R>(nw), std::vector<R>(nw)};
+    for (int c = 0; c < nw; ++c) {
+      int o = t.perm_w[c];
+      size_t fo = (size_t)(o + 1) * 7;
+      for (int j = 0; j < t.w_npoly[c]; ++j) {
+        size_t i = (size_t)c * NSLOT + j;
+        dirv[i] = m->w_dirv[fo + j];
+        gxc[i] = m->gxps_coef[fo + j]; gyc[i] = m->gyps_coef[fo + j];
+        ecx[i] = m->w_ecvec_vx[fo + j]; ecy[i] = m->w_ecvec_vy[fo + j]; ecz[i] = m->w_ecvec_vz[fo + j];
+        xxyy[i] = m->xx_yy[fo + j];
+        int v = in.iv(j, o);
+        int side = (m->w_dirv[fo + j] < 0.f) ? 0 : 1;   // this cell is iw1 (dirv < 0) or iw2
+        dxs[i] = m->v_dxps[(size_t)(v + 1) * 2 + side]; dys[i] = m->v_dyps[(size_t)(v + 1) * 2 + side];
+        dxn[i] = m->v_dxps[(size_t)(v + 1) * 2 + 1 - side]; dyn[i] = m->v_dyps[(size_t)(v + 1) * 2 + 1 - side];
+      }
+      wn[0][c] = m->wnx[o + 1]; wn[1][c] = m->wny[o + 1]; wn[2][c] = m->wnz[o + 1];
+    }
+    // coefficient round trip (bitwise against the FP32 inputs)
+    for (int c = 0; c < nw; ++c) {
+      int o = t.perm_w[c];
+      size_t fo = (size_t)(o + 1) * 7;
+      for (int j = 0; j < t.w_npoly[c]; ++j) {
+        size_t i = (size_t)c * NSLOT + j;
+        if ((float)gxc[i] != m->gxps_coef[fo + j] || (float)gyc[i] != m->gyps_coef[fo + j] ||
+            (float)dirv[i] != m->w_dirv[fo + j])
+          throw Err(OLAMGPU_EINVAL, "renumbering check failed (coefficients)");
+      }
+    }
+    std::vector<R> vn[3] = {std::vector<R>(nv), std::vector<R>(nv), std::vector<R>(nv)};
+    for (int e = 0; e < nv; ++e) {
+      int o = t.perm_v[e] + 1;
+      vn[0][e] = m->vnx[o]; vn[1][e] = m->vny[o]; vn[2][e] = m->vnz[o];
+    }
+    std::vector<R> vfact(nm * 4, 0), xxyym(nm * 4, 0);
+    for (int i = 0; i < nm; ++i) {
+      int o = t.perm_m[i];
+      R arm0 = m->arm0[o + 1];
+      for (int j = 0; j < 3; ++j) {
+        int vo = in.miv(j, o);
+        R dnv = m->dnv[vo + 1];
+        // vort_div_damp.f90:282-291
+        vfact[i * 4 + j] = (in.vim(1, vo) == o) ? dnv / arm0 : -dnv / arm0;
+        xxyym[i * 4 + j] = m->xx_yy_m[(size_t)(o + 1) * 3 + j];
+      }
+    }
+    // per-level arrays (WAVE entries: every lane may read its level)
+    std::vector<R> lev[7];
+    for (auto &v : lev) v.assign(WAVE, R(0));
+    std::vector<double> lev8[4];
+    for (auto &v : lev8) v.assign(WAVE, 0.0);
+    for (int k = 1; k <= mza; ++k) {
+      int l = k - 1;
+      lev[0][l] = m->dzim[l];
+      lev[1][l] = (R)m->gdz_wgtm8[l];     // real(gdz_wgtm8), mem_grid.f90:408-410
+      lev[2][l] = (R)m->gdz_wgtp8[l];
+      lev[3][l] = m->a_w[l + mza * (0 + 2 * 0)];    // a_w(k,1,1)
+      lev[4][l] = m->a_w[l + mza * (1 + 2 * 0)];    // a_w(k,2,1)
+      lev[5][l] = m->a_w[l + mza * (0 + 2 * 1)];    // a_w(k,1,2)
+      lev[6][l] = m->a_w[l + mza * (1 + 2 * 1)];    // a_w(k,2,2)
+      lev8[0][l] = m->zwgt_top8[l]; lev8[1][l] = m->zwgt_bot8[l];
+      lev8[2][l] = m->gdz_wgtm8[l]; lev8[3][l] = m->gdz_wgtp8[l];
+    }
+    // control volumes
+    std::vector<R> arw(wsz, 0), volti(wsz, 0), volwi(wsz, 0), arv(vsz, 0), volvi(vsz, 0);
+    std::vector<double> volt(wsz, 0);
+    for (int c = 0; c < nw; ++c) {
+      size_t fo = (size_t)(t.perm_w[c] + 1) * mza;
+      for (int l = 0; l < mza; ++l) {
+        size_t i = (size_t)c * ldk + l;
+        arw[i] = m->arw[fo + l];
+        volt[i] = m->volt[fo + l];
+        if (!f64) { volti[i] = m->volti[fo + l]; volwi[i] = m->volwi[fo + l]; }
+      }
+      if (f64) {   // inverse volumes rebuilt from volt in double (see olamgpu.h)
+        for (int k = 2; k <= mza; ++k) volti[(size_t)c * ldk + k - 1] = (R)(1.0 / volt[(size_t)c * ldk + k - 1]);
+        for (int k = 2; k <= mza - 1; ++k)
+          volwi[(size_t)c * ldk + k - 1] = (R)(1.0 / (volt[(size_t)c * ldk + k - 1] + volt[(size_t)c * ldk + k]));
+      }
+    }
+    for (int e = 0; e < nv; ++e) {
+      size_t fo = (size_t)(t.perm_v[e] + 1) * mza;
+      int w1 = t.v_iw[e * 2], w2 = t.v_iw[e * 2 + 1];
+      for (int l = 0; l < mza; ++l) {
+        size_t i = (size_t)e * ldk + l;
+        arv[i] = m->arv[fo + l];
+        if (!f64) volvi[i] = m->volvi[fo + l];
+        else if (l >= 1) volvi[i] = (R)(1.0 / (volt[(size_t)w1 * ldk + l] + volt[(size_t)w2 * ldk + l]));
+      }
+    }
+    upv(b_npoly, t.w_npoly); upv(b_nbr, t.w_nbr); upv(b_face, t.w_face); upv(b_back, t.w_back);
+    upv(b_viw, t.v_iw); upv(b_vim, t.v_im); upv(b_miv, t.m_iv); upv(b_mim, t.m_im); upv(b_invw, t.inv_w); upv(b_invv, t.inv_v);
+    upv(b_dirv, dirv); upv(b_gxc, gxc); upv(b_gyc, gyc); upv(b_dxs, dxs); upv(b_dys, dys);
+    upv(b_dxn, dxn); upv(b_dyn, dyn); upv(b_ecx, ecx); upv(b_ecy, ecy); upv(b_ecz, ecz); upv(b_xxyy, xxyy);
+    for (int i = 0; i < 3; ++i) { upv(b_wn[i], wn[i]); upv(b_vn[i], vn[i]); }
+    upv(b_vfact, vfact); upv(b_xxyym, xxyym);
+    for (int i = 0; i < 7; ++i) upv(b_lev[i], lev[i]);
+    for (int i = 0; i < 4; ++i) upv(b_lev8[i], lev8[i]);
+    upv(b_arw, arw); upv(b_volti, volti); upv(b_volwi, volwi); upv(b_volt, volt); upv(b_arv, arv); upv(b_volvi, volvi);
+    // device copies of the integer tables must equal the host tables
+    verify_dev(b_nbr, t.w_nbr); verify_dev(b_face, t.w_face); verify_dev(b_viw, t.v_iw); verify_dev(b_miv, t.m_iv);
+
+    dm.nw = nw; dm.nv = nv; dm.nm = nm; dm.mza = mza; dm.ldk = ldk;
+    dm.w_npoly = dp<int>(b_npoly); dm.w_nbr = dp<int>(b_nbr); dm.w_face = dp<int>(b_face); dm.w_back = dp<int>(b_back);
+    dm.w_dirv = dp<R>(b_dirv); dm.w_gxc = dp<R>(b_gxc); dm.w_gyc = dp<R>(b_gyc);
+    dm.w_dxs = dp<R>(b_dxs); dm.w_dys = dp<R>(b_dys); dm.w_dxn = dp<R>(b_dxn); dm.w_dyn = dp<R>(b_dyn);
+    dm.w_ecx = dp<R>(b_ecx); dm.w_ecy = dp<R>(b_ecy); dm.w_ecz = dp<R>(b_ecz); dm.w_xxyy = dp<R>(b_xxyy);
+    dm.wnx = dp<R>(b_wn[0]); dm.wny = dp<R>(b_wn[1]); dm.wnz = dp<R>(b_wn[2]);
+    dm.v_iw = dp<int>(b_viw); dm.v_im = dp<int>(b_vim);
+    dm.vnx = dp<R>(b_vn[0]); dm.vny = dp<R>(b_vn[1]); dm.vnz = dp<R>(b_vn[2]);
+    dm.m_iv = dp<int>(b_miv); dm.m_im = dp<int>(b_mim); dm.m_vfact = dp<R>(b_vfact); dm.m_xxyym = dp<R>(b_xxyym);
+    dm.dzim = dp<R>(b_lev[0]); dm.gdz_wgtm = dp<R>(b_lev[1]); dm.gdz_wgtp = dp<R>(b_lev[2]);
+    dm.aw11 = dp<R>(b_lev[3]); dm.aw21 = dp<R>(b_lev[4]); dm.aw12 = dp<R>(b_lev[5]); dm.aw22 = dp<R>(b_lev[6]);
+    dm.zwgt_top8 = dp<double>(b_lev8[0]); dm.zwgt_bot8 = dp<double>(b_lev8[1]);
+    dm.gdz_wgtm8 = dp<double>(b_lev8[2]); dm.gdz_wgtp8 = dp<double>(b_lev8[3]);
+    dm.arw = dp<R>(b_arw); dm.volti = dp<R>(b_volti); dm.volwi = dp<R>(b_volwi); dm.arv = dp<R>(b_arv);
+    dm.volvi = dp<R>(b_volvi); dm.volt = dp<double>(b_volt);
+
+    // damping coefficients (first-call SAVEd factors of vort_div_damp.f90:39-119, 264-302)
+    const double dtm = prm.dtsm;
+    const int dl = prm.divh_damp_level, vl = prm.vort_damp_level;
+    do_div = prm.divh_damp_fact > 1.e-7f && dl > 0;
+    do_vort = prm.vort_damp_fact >= 1.e-7f && vl > 0;
+    if (do_div) {
+      std::vector<R> f(vsz, R(0));
+      const double facth = -std::pow(-(double)prm.divh_damp_fact, dl) / dtm;
+      const R fact2 = (R)(prm.divh_damp_fact / (R)dtm);
+      for (int e = 0; e < nv; ++e) {
+        int o = t.perm_v[e] + 1;
+        int w1o = in.viw(0, o - 1) + 1, w2o = in.viw(1, o - 1) + 1;
+        double a0 = std::min(m->arw0[w1o], m->arw0[w2o]);
+        for (int k = 2; k <= mza; ++k) {
+          double vmin = std::min(m->volt[(size_t)w1o * mza + k - 1], m->volt[(size_t)w2o * mza + k - 1]);
+          if (dl > 1) {
+            double factb = facth * std::pow(a0, dl - 1);
+            f[(size_t)e * ldk + k - 1] = (R)(factb * m->dniv[o] * m->zfacit[k - 1] * m->dzit[k - 1] * vmin);
+          } else {
+            f[(size_t)e * ldk + k - 1] =
+                (R)((double)(fact2 * (R)m->dniv[o] * (R)m->zfacit[k - 1] * (R)m->dzit[k - 1]) * vmin);
+          }
+        }
+      }
+      upv(fdivh, f);
+      div2d[0].alloc(wsz * sizeof(R)); div2d[1].alloc(wsz * sizeof(R));
+      HIPCHK(hipMemsetAsync(div2d[0].p, 0, div2d[0].bytes, s));
+      HIPCHK(hipMemsetAsync(div2d[1].p, 0, div2d[1].bytes, s));
+    }
+    if (do_vort) {
+      std::vector<R> f(nv, R(0));
+      for (int e = 0; e < nv; ++e) {
+        int o = t.perm_v[e] + 1;
+        int m1 = in.vim(0, o - 1) + 1, m2 = in.vim(1, o - 1) + 1;
+        R a = std::min((R)m->arm0[m1], (R)m->arm0[m2]);
+        f[e] = R(0.5) * (R)m->dniu[o] / (R)dtm * (R)std::pow((double)(-(R)prm.vort_damp_fact * a), vl);
+      }
+      upv(fvorth, f);
+      vort2d[0].alloc(msz * sizeof(R)); vort2d[1].alloc(msz * sizeof(R));
+      HIPCHK(hipMemsetAsync(vort2d[0].p, 0, vort2d[0].bytes, s));
+      HIPCHK(hipMemsetAsync(vort2d[1].p, 0, vort2d[1].bytes, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
+  template <class T> void verify_dev(DevBuf &b, const std::vector<T> &h) {
+    std::vector<T> back(h.size());
+    HIPCHK(hipMemcpyAsync(back.data(), b.p, h.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (std::memcmp(back.data(), h.data(), h.size() * sizeof(T)) != 0)
+      throw Err(OLAMGPU_EHIP, "device copy of a neighbour table differs from the host table");
+  }
+
+  void zero(DevBuf &b) { HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s)); }
+
+  void alloc_state() {
+    for (DevBuf *b : {&thil[0], &thil[1], &wmc, &wc, &vxe, &vye, &vze, &wmasc, &alpha, &pwfac, &thilt, &vmxet,
+                      &vmyet, &vmzet, &rho_old, &wmc0, &thilt_s, &vmxet_s, &vmyet_s, &vmzet_s, &vmxet_rk,
+                      &vmyet_rk, &vmzet_rk}) { b->alloc(wsz * sizeof(R)); zero(*b); }
+    for (DevBuf *b : {&vmc, &vc, &vmasc, &pvfac, &vmt, &vmc0, &vmt_s}) { b->alloc(vsz * sizeof(R)); zero(*b); }
+    for (DevBuf *b : {&rho, &press, &rho0, &rth0}) { b->alloc(wsz * sizeof(RP)); zero(*b); }
+    for (auto &g : grads) { g.alloc(wsz * sizeof(R)); zero(g); }
+    ntr = prm.num_scalar;
+    tstride = (long)wsz;
+    size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
+    for (DevBuf *b : {&trS, &trX, &trY, &trgx, &trgy}) { b->alloc(tb); zero(*b); }
+    ffimp.alloc((size_t)3 * nw * sizeof(R)); zero(ffimp);
+    cflmax.alloc(4 * sizeof(unsigned)); nimp.alloc(sizeof(int));
+    masspart.alloc((size_t)nblk(nw) * 65 * sizeof(double));
+    massout.alloc(65 * sizeof(double));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
+  void alloc_adj() {
+    if (adjA.p) return;
+    size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
+    for (DevBuf *b : {&adjA, &adjB, &lam0, &nu, &lpart, &lgx, &lgy}) { b->alloc(tb); zero(*b); }
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
+  // -------------------------------------------------------------- transfers
+  // Fortran (mza, n) host array with dummy column <-> device (n, ldk) renumbered
+  template <class D, class H> void h2d(DevBuf &b, const H *h, const std::vector<int> &perm) {
+    size_t n = perm.size();
+    std::vector<D> tmp(n * ldk, D(0));
+    for (size_t c = 0; c < n; ++c) {
+      const H *src = h + (size_t)(perm[c] + 1) * mza;
+      D *dst = tmp.data() + c * ldk;
+      for (int l = 0; l < mza; ++l) dst[l] = (D)src[l];
+    }
+    HIPCHK(hipMemcpyAsync(b.p, tmp.data(), tmp.size() * sizeof(D), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  template <class D, class H> void d2h(const DevBuf &b, H *h, const std::vector<int> &perm, size_t off = 0) {
+    size_t n = perm.size();
+    std::vector<D> tmp(n * ldk);
+    HIPCHK(hipMemcpyAsync(tmp.data(), (const char *)b.p + off * sizeof(D), tmp.size() * sizeof(D),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int l = 0; l < mza; ++l) h[l] = H(0);
+    for (size_t c = 0; c < n; ++c) {
+      const D *src = tmp.data() + c * ldk;
+      H *dst = h + (size_t)(perm[c] + 1) * mza;
+      for (int l = 0; l < mza; ++l) dst[l] = (H)src[l];
+    }
+  }
+  template <class D, class H> void h2d_off(DevBuf &b, size_t off, const H *h, const std::vector<int> &perm) {
+    size_t n = perm.size();
+    std::vector<D> tmp(n * ldk, D(0));
+    for (size_t c = 0; c < n; ++c) {
+      const H *src = h + (size_t)(perm[c] + 1) * mza;
+      for (int l = 0; l < mza; ++l) tmp[c * ldk + l] = (D)src[l];
+    }
+    HIPCHK(hipMemcpyAsync((char *)b.p + off * sizeof(D), tmp.data(), tmp.size() * sizeof(D), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
+  DevBuf &cur_thil_buf() { return thil[cur_thil]; }
+
+  void upload_state(const olamgpu_state *h) override {
+    auto W = [&](DevBuf &b, const float *p) { if (p) h2d<R>(b, p, t.perm_w); };
+    auto V = [&](DevBuf &b, const float *p) { if (p) h2d<R>(b, p, t.perm_v); };
+    W(cur_thil_buf(), h->thil); W(wmc, h->wmc); W(wc, h->wc); W(vxe, h->vxe); W(vye, h->vye); W(vze, h->vze);
+    if (h->rho) h2d<RP>(rho, h->rho, t.perm_w);
+    if (h->press) h2d<RP>(press, h->press, t.perm_w);
+    V(vmc, h->vmc); V(vc, h->vc); W(wmasc, h->wmasc); V(vmasc, h->vmasc);
+    W(alpha, h->alpha_press); W(pwfac, h->pwfac); V(pvfac, h->pvfac);
+    W(thilt, h->thilt); W(vmxet, h->vmxet); W(vmyet, h->vmyet); W(vmzet, h->vmzet); V(vmt, h->vmt);
+  }
+  void download_state(olamgpu_state *h) override {
+    auto W = [&](DevBuf &b, float *p) { if (p) d2h<R>(b, p, t.perm_w); };
+    auto V = [&](DevBuf &b, float *p) { if (p) d2h<R>(b, p, t.perm_v); };
+    W(cur_thil_buf(), h->thil); W(wmc, h->wmc); W(wc, h->wc); W(vxe, h->vxe); W(vye, h->vye); W(vze, h->vze);
+    if (h->rho) d2h<RP>(rho, h->rho, t.perm_w);
+    if (h->press) d2h<RP>(press, h->press, t.perm_w);
+    V(vmc, h->vmc); V(vc, h->vc); W(wmasc, h->wmasc); V(vmasc, h->vmasc);
+    W(alpha, h->alpha_press); W(pwfac, h->pwfac); V(pvfac, h->pvfac);
+    W(thilt, h->thilt); W(vmxet, h->vmxet); W(vmyet, h->vmyet); W(vmzet, h->vmzet); V(vmt, h->vmt);
+  }
+  void check_n(int n) { if (n < 1 || n > ntr) throw Err(OLAMGPU_EINVAL, "scalar index out of range"); }
+  void upload_scalar(int n, const float *p, const float *tt) override {
+    check_n(n);
+    if (p) h2d_off<R>(trS, (size_t)(n - 1) * tstride, p, t.perm_w);
+    if (tt) {
+      if (!trsct.p) { trsct.alloc(std::max<size_t>(1, ntr) * wsz * sizeof(R)); zero(trsct); }
+      h2d_off<R>(trsct, (size_t)(n - 1) * tstride, tt, t.perm_w);
+      have_sct = true;
+    } else if (have_sct) {
+      HIPCHK(hipMemsetAsync((char *)trsct.p + (size_t)(n - 1) * tstride * sizeof(R), 0, wsz * sizeof(R), s));
+    }
+  }
+  void download_scalar(int n, float *p) override { check_n(n); d2h<R>(trS, p, t.perm_w, (size_t)(n - 1) * tstride); }
+  DevBuf &cur_adj_buf() { return cur_adj ? adjB : adjA; }
+  void upload_adj(int n, const float *p) override {
+    check_n(n); alloc_adj();
+    h2d_off<R>(cur_adj_buf(), (size_t)(n - 1) * tstride, p, t.perm_w);
+  }
+  void download_adj(int n, float *p) override {
+    check_n(n); alloc_adj();
+    d2h<R>(cur_adj_buf(), p, t.perm_w, (size_t)(n - 1) * tstride);
+  }
+
+  // -------------------------------------------------------------- tend0 etc.
+  void tend0() override {
+    HIPCHK(hipMemsetAsync(vmasc.p, 0, vmasc.bytes, s));
+    HIPCHK(hipMemsetAsync(wmasc.p, 0, wmasc.bytes, s));
+    if (have_sct) HIPCHK(hipMemsetAsync(trsct.p, 0, trsct.bytes, s));
+    copy_rho_old();
+  }
+  void copy_rho_old();
+  void timeavg_momsc() override;
+
+  // -------------------------------------------------------------- prog_wrtv_rk
+  void prog_wrtv_rk() override {
+    const unsigned gw = nblk(nw), gv = nblk(nv), gm = nblk(nm);
+    hipLaunchKernelGGL((k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), 0, s, dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
+                       dp<R>(wmc), dp<R>(thilt), dp<R>(vmxet), dp<R>(vmyet), dp<R>(vmzet), dp<RP>(rth0),
+                       dp<RP>(rho0), dp<R>(wmc0), dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s));
+    // short-step damping: divergence at W and vorticity at M, then their Laplacians
+    if (do_div) {
+      hipLaunchKernelGGL((k_div<R>), dim3(gw), dim3(256), 0, s, dm, dp<R>(vmc), dp<R>(div2d[0]));
+      int ih = 0;
+      for (int nn = 1; nn <= prm.divh_damp_level - 1; ++nn) {
+        int il = (nn - 1) % 2;
+        ih = nn % 2;
+        hipLaunchKernelGGL((k_lap_w<R>), dim3(gw), dim3(256), 0, s, dm, dp<R>(div2d[il]), dp<R>(div2d[ih]));
+      }
+      div_ih = ih;
+    }
+    if (do_vort) {
+      hipLaunchKernelGGL((k_vort<R>), dim3(gm), dim3(256), 0, s, dm, dp<R>(vc), dp<R>(vort2d[0]));
+      int ih = 0;
+      for (int nn = 1; nn <= prm.vort_damp_level - 1; ++nn) {
+        int il = (nn - 1) % 2;
+        ih = nn % 2;
+        hipLaunchKernelGGL((k_lap_m<R>), dim3(gm), dim3(256), 0, s, dm, dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
+      }
+      vort_ih = ih;
+    }
+    hipLaunchKernelGGL((k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), 0, s, dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
+                       dp<R>(vmt_s), do_div ? dp<R>(div2d[div_ih]) : (R *)nullptr, dp<R>(fdivh),
+                       do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr, dp<R>(fvorth), dp<RP>(rho));
+    const int nrk = prm.nrk_wrtv;
+    for (int istage = 1; istage <= nrk; ++istage) {
+      double frac = (nrk <= 2) ? (istage == 1 ? 0.5 : 1.0)
+                               : (istage == 1 ? 1.0 / 3.0 : (istage == 2 ? 0.5 : 1.0));
+      DynBuf<R, RP> b;
+      b.thil_in = dp<R>(thil[cur_thil]);
+      b.thil_out = dp<R>(thil[1 - cur_thil]);
+      b.wmc = dp<R>(wmc); b.wc = dp<R>(wc); b.vxe = dp<R>(vxe); b.vye = dp<R>(vye); b.vze = dp<R>(vze);
+      b.vmc = dp<R>(vmc); b.vc = dp<R>(vc); b.vmasc = dp<R>(vmasc); b.wmasc = dp<R>(wmasc);
+      b.rho = dp<RP>(rho); b.press = dp<RP>(press);
+      b.alpha_press = dp<R>(alpha); b.pwfac = dp<R>(pwfac); b.pvfac = dp<R>(pvfac);
+      b.rho0 = dp<RP>(rho0); b.rth0 = dp<RP>(rth0); b.wmc0 = dp<R>(wmc0); b.vmc0 = dp<R>(vmc0);
+      b.thilt_s = dp<R>(thilt_s); b.vmxet_s = dp<R>(vmxet_s); b.vmyet_s = dp<R>(vmyet_s); b.vmzet_s = dp<R>(vmzet_s);
+      b.vmt_s = dp<R>(vmt_s);
+      b.vmxet_rk = dp<R>(vmxet_rk); b.vmyet_rk = dp<R>(vmyet_rk); b.vmzet_rk = dp<R>(vmzet_rk);
+      for (int i = 0; i < 8; ++i) b.g[i] = dp<R>(grads[i]);
+      b.thil_order = prm.thil_horiz_adv_order; b.wind_order = prm.wind_horiz_adv_order;
+      b.istage = istage; b.nrk = nrk;
+      b.dt8 = frac * prm.dtsm;
+      b.fcor = (R)prm.fcoriol;
+      int dth = prm.thil_horiz_adv_order > 1, dwn = prm.wind_horiz_adv_order > 1;
+      if (dth || dwn)
+        hipLaunchKernelGGL((k_dyn_grad<R>), dim3(gw), dim3(256), 0, s, dm, b.thil_in, b.vxe, b.vye, b.vze,
+                           dp<R>(grads[0]), dp<R>(grads[1]), dp<R>(grads[2]), dp<R>(grads[3]), dp<R>(grads[4]),
+                           dp<R>(grads[5]), dp<R>(grads[6]), dp<R>(grads[7]), dth, dwn);
+      hipLaunchKernelGGL((k_dyn_w<R, RP>), dim3(gw), dim3(256), 0, s, dm, b);
+      cur_thil = 1 - cur_thil;
+      hipLaunchKernelGGL((k_dyn_v<R, RP>), dim3(gv), dim3(256), 0, s, dm, b, istage == nrk - 1 ? 1 : 0);
+      hipLaunchKernelGGL((k_dyn_vel<R, RP>), dim3(gw), dim3(256), 0, s, dm, b);
+    }
+    HIPCHK(hipGetLastError());
+  }
+
+  // -------------------------------------------------------------- scalar_transport_rk
+  void stage_dts(R *dtlr, int nrk) const {
+    R dtl = (R)prm.dtlm;
+    const R one3 = R(1.) / R(3.);
+    if (nrk == 3) { dtlr[0] = dtl * one3; dtlr[1] = dtl * R(0.5); dtlr[2] = dtl * R(1.0); }
+    else { dtlr[0] = dtl * R(0.5); dtlr[1] = dtl * R(1.0); }
+  }
+
+  void run_cfl(bool report) {
+    const int nrk = prm.nrk_scal;
+    HIPCHK(hipMemsetAsync(cflmax.p, 0, cflmax.bytes, s));
+    HIPCHK(hipMemsetAsync(nimp.p, 0, nimp.bytes, s));
+    hipLaunchKernelGGL((k_tr_cfl<R>), dim3(nblk(nw)), dim3(256), 0, s, dm, dp<R>(vmasc), dp<R>(wmasc),
+                       dp<R>(rho_old), dp<R>(wc), (R)prm.dtlm, nrk, dp<R>(ffimp), cflmax.as<unsigned>(),
+                       nimp.as<int>());
+    if (report) {
+      unsigned mb[4];
+      int ni = 0;
+      HIPCHK(hipMemcpyAsync(mb, cflmax.p, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(&ni, nimp.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      for (int i = 0; i < 3; ++i) { float f; std::memcpy(&f, &mb[i], 4); cfl_out[i] = f; }
+      cfl_out[3] = ni;
+    }
+  }
+
+  void scalar_transport_rk(const float *rho_old_host) override {
+    if (rho_old_host) h2d<R>(rho_old, rho_old_host, t.perm_w);
+    const int nrk = prm.nrk_scal;
+    R dtlr[3];
+    stage_dts(dtlr, nrk);
+    run_cfl(false);
+    if (ntr == 0) { have_fwd = true; return; }
+    // RK3: S -> X -> Y -> X;  RK2: S -> X -> Y
+    DevBuf *in = &trS, *outs[3] = {&trX, &trY, &trX};
+    if (nrk == 2) { outs[0] = &trX; outs[1] = &trY; }
+    for (int istage = 1; istage <= nrk; ++istage) {
+      DevBuf *out = outs[istage - 1];
+      TrStage<R, RP> a;
+      a.q_in = dp<R>(*in); a.q0 = dp<R>(trS); a.sct = have_sct ? dp<R>(trsct) : nullptr;
+      a.gx = dp<R>(trgx); a.gy = dp<R>(trgy); a.q_out = dp<R>(*out);
+      a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
+      a.rho = dp<RP>(rho);
+      a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.tstride = tstride;
+      a.dtr = dtlr[istage - 1];
+      if (a.order > 1)
+        hipLaunchKernelGGL((k_tr_grad<R>), dim3(nblk(nw)), dim3(256), 0, s, dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
+                           ntr, tstride);
+      hipLaunchKernelGGL((k_tr_update<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
+      in = out;
+    }
+    // the new state is in `in`; make it the state buffer
+    std::swap(trS.p, in->p);
+    std::swap(trS.bytes, in->bytes);
+    HIPCHK(hipGetLastError());
+    have_fwd = true;
+  }
+
+  // -------------------------------------------------------------- adjoint
+  void scalar_transport_rk_adj() override {
+    if (!have_fwd) throw Err(OLAMGPU_ESTATE, "adjoint called before a forward tracer step");
+    alloc_adj();
+    if (ntr == 0) return;
+    const int nrk = prm.nrk_scal;
+    R dtlr[3];
+    stage_dts(dtlr, nrk);
+    run_cfl(false);
+    HIPCHK(hipMemsetAsync(lam0.p, 0, lam0.bytes, s));
+    for (int istage = nrk; istage >= 1; --istage) {
+      TrAdj<R, RP> a;
+      DevBuf &muB = cur_adj_buf();
+      DevBuf &lqB = cur_adj ? adjA : adjB;
+      a.mu = dp<R>(muB); a.lam0 = dp<R>(lam0); a.nu = dp<R>(nu); a.lpart = dp<R>(lpart);
+      a.lgx = dp<R>(lgx); a.lgy = dp<R>(lgy); a.lq = dp<R>(lqB);
+      a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
+      a.rho = dp<RP>(rho);
+      a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.last = istage == 1;
+      a.tstride = tstride; a.dtr = dtlr[istage - 1];
+      hipLaunchKernelGGL((k_tr_adj0<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
+      hipLaunchKernelGGL((k_tr_adj1<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
+      hipLaunchKernelGGL((k_tr_adj2<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
+      cur_adj = 1 - cur_adj;
+    }
+    HIPCHK(hipGetLastError());
+  }
+
+  // -------------------------------------------------------------- diagnostics
+  void mass_sums(double *out) override {
+    const unsigned nb = nblk(nw);
+    hipLaunchKernelGGL((k_mass_partial<R, RP>), dim3(nb), dim3(256), 0, s, dm, dp<RP>(rho), dp<R>(trS), ntr, tstride,
+                       masspart.as<double>());
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, s, masspart.as<double>(), (int)nb, 1 + ntr,
+                       massout.as<double>());
+    HIPCHK(hipMemcpyAsync(out, massout.p, (1 + ntr) * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  void cfl_report(double *o) override {
+    run_cfl(true);
+    for (int i = 0; i < 4; ++i) o[i] = cfl_out[i];
+  }
+
+  void copy_field(int id, int index, void *buf, int dir, int *eb) override;
+};
+
+// ---------------------------------------------------------------- small kernels
+template <class R, class RP>
+__global__ void k_cast_rho(const RP *rho, R *ro, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ro[i] = (R)rho[i];
+}
+template <class R>
+__global__ void k_scale(R *a, long n, R f, int mza, int ldk, int klo, int khi) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = (int)(i % ldk) + 1;
+  if (k >= klo && k <= khi) a[i] = a[i] / f;
+}
+// dense (n, mza) original order <-> device (n, ldk) renumbered
+template <class T>
+__global__ void k_perm_copy(const int *inv, T *dev, T *dense, int n, int mza, int ldk, int dir) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * mza) return;
+  long o = i / mza;
+  int l = (int)(i % mza);
+  long d = (long)inv[o] * ldk + l;
+  if (dir == 0) dense[i] = dev[d];
+  else dev[d] = dense[i];
+}
+
+template <class R, class RP>
+void Impl<R, RP>::copy_rho_old() {
+  long n = (long)wsz;
+  hipLaunchKernelGGL((k_cast_rho<R, RP>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dp<RP>(rho),
+                     dp<R>(rho_old), n);
+  HIPCHK(hipGetLastError());
+}
+
+template <class R, class RP>
+void Impl<R, RP>::timeavg_momsc() {
+  if (prm.nacoust <= 1) return;
+  R f = (R)prm.nacoust;
+  hipLaunchKernelGGL((k_scale<R>), dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, s, dp<R>(vmasc), (long)vsz, f,
+                     mza, ldk, 2, mza);
+  hipLaunchKernelGGL((k_scale<R>), dim3((unsigned)((wsz + 255) / 256)), dim3(256), 0, s, dp<R>(wmasc), (long)wsz, f,
+                     mza, ldk, 2, mza - 1);
+  HIPCHK(hipGetLastError());
+}
+
+template <class R, class RP>
+void Impl<R, RP>::copy_field(int id, int index, void *buf, int dir, int *eb) {
+  DevBuf *b = nullptr;
+  size_t off = 0;
+  bool isrp = false, isv = false;
+  switch (id) {
+    case OLAMGPU_F_SCALAR: check_n(index); b = &trS; off = (size_t)(index - 1) * tstride; break;
+    case OLAMGPU_F_SCALAR_ADJ: check_n(index); alloc_adj(); b = &cur_adj_buf(); off = (size_t)(index - 1) * tstride; break;
+    case OLAMGPU_F_RHO: b = &rho; isrp = true; break;
+    case OLAMGPU_F_PRESS: b = &press; isrp = true; break;
+    case OLAMGPU_F_RHO_OLD: b = &rho_old; break;
+    case OLAMGPU_F_VMASC: b = &vmasc; isv = true; break;
+    case OLAMGPU_F_WMASC: b = &wmasc; break;
+    case OLAMGPU_F_THIL: b = &thil[cur_thil]; break;
+    case OLAMGPU_F_VMC: b = &vmc; isv = true; break;
+    case OLAMGPU_F_WMC: b = &wmc; break;
+    default: throw Err(OLAMGPU_EINVAL, "unknown field id");
+  }
+  int n = isv ? nv : nw;
+  const int *dinv = isv ? b_invv.as<int>() : b_invw.as<int>();
+  long tot = (long)n * mza;
+  unsigned g = (unsigned)((tot + 255) / 256);
+  if (isrp) {
+    if (eb) *eb = sizeof(RP);
+    hipLaunchKernelGGL(k_perm_copy<RP>, dim3(g), dim3(256), 0, s, dinv, b->as<RP>() + off, (RP *)buf, n, mza, ldk, dir);
+  } else {
+    if (eb) *eb = sizeof(R);
+    hipLaunchKernelGGL(k_perm_copy<R>, dim3(g), dim3(256), 0, s, dinv, b->as<R>() + off, (R *)buf, n, mza, ldk, dir);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+struct olamgpu_ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  std::unique_ptr<ImplBase> impl;
+};
+
+static std::string g_err;
+
+template <class F>
+static int guarded(olamgpu_ctx *c, F &&f) {
+  if (!c) { g_err = "null context"; return OLAMGPU_EINVAL; }
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    f();
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return OLAMGPU_OK;
+  } catch (const Err &e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return OLAMGPU_EHIP;
+  }
+}
+
+extern "C" {
+
+int olamgpu_create(const olamgpu_mesh *mesh, const olamgpu_params *params, int32_t device, olamgpu_ctx **out) {
+  if (!mesh || !params || !out) { g_err = "null argument"; return OLAMGPU_EINVAL; }
+  *out = nullptr;
+  auto *c = new olamgpu_ctx;
+  try {
+    int dev = device;
+    if (dev < 0) HIPCHK(hipGetDevice(&dev));
+    c->device = dev;
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    Topo t = build_topo(mesh);
+    if (params->precision == OLAMGPU_PREC_F64)
+      c->impl.reset(new Impl<double, double>(mesh, params, c->stream, std::move(t)));
+    else if (params->precision == OLAMGPU_PREC_MIXED)
+      c->impl.reset(new Impl<float, double>(mesh, params, c->stream, std::move(t)));
+    else if (params->precision == OLAMGPU_PREC_F32)
+      c->impl.reset(new Impl<float, float>(mesh, params, c->stream, std::move(t)));
+    else
+      throw Err(OLAMGPU_EINVAL, "unknown precision mode");
+  } catch (const Err &e) {
+    g_err = e.what();
+    olamgpu_destroy(c);
+    return e.code;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    olamgpu_destroy(c);
+    return OLAMGPU_EHIP;
+  }
+  *out = c;
+  return OLAMGPU_OK;
+}
+
+int olamgpu_destroy(olamgpu_ctx *c) {
+  if (!c) return OLAMGPU_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->impl.reset();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return OLAMGPU_OK;
+}
+
+const char *olamgpu_last_error(const olamgpu_ctx *c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int olamgpu_upload_state(olamgpu_ctx *c, const olamgpu_state *h) {
+  return guarded(c, [&] { if (!h) throw Err(OLAMGPU_EINVAL, "null state"); c->impl->upload_state(h); });
+}
+int olamgpu_download_state(olamgpu_ctx *c, olamgpu_state *h) {
+  return guarded(c, [&] { if (!h) throw Err(OLAMGPU_EINVAL, "null state"); c->impl->download_state(h); });
+}
+int olamgpu_upload_scalar(olamgpu_ctx *c, int32_t n, const float *p, const float *t) {
+  return guarded(c, [&] { c->impl->upload_scalar(n, p, t); });
+}
+int olamgpu_download_scalar(olamgpu_ctx *c, int32_t n, float *p) {
+  return guarded(c, [&] { if (!p) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->download_scalar(n, p); });
+}
+int olamgpu_init_wrtv_rk(olamgpu_ctx *c) { return guarded(c, [&] {}); }
+int olamgpu_tend0(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->tend0(); }); }
+int olamgpu_prog_wrtv_rk(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->prog_wrtv_rk(); }); }
+int olamgpu_timeavg_momsc(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->timeavg_momsc(); }); }
+int olamgpu_scalar_transport_rk(olamgpu_ctx *c, const float *rho_old) {
+  return guarded(c, [&] { c->impl->scalar_transport_rk(rho_old); });
+}
+int olamgpu_scalar_transport_rk_adj(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->scalar_transport_rk_adj(); }); }
+int olamgpu_upload_scalar_adj(olamgpu_ctx *c, int32_t n, const float *p) {
+  return guarded(c, [&] { if (!p) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->upload_adj(n, p); });
+}
+int olamgpu_download_scalar_adj(olamgpu_ctx *c, int32_t n, float *p) {
+  return guarded(c, [&] { if (!p) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->download_adj(n, p); });
+}
+int olamgpu_mass_sums(olamgpu_ctx *c, double *out) {
+  return guarded(c, [&] { if (!out) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->mass_sums(out); });
+}
+int olamgpu_cfl_report(olamgpu_ctx *c, double *out) {
+  return guarded(c, [&] { if (!out) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->cfl_report(out); });
+}
+int olamgpu_copy_field(olamgpu_ctx *c, int32_t field_id, int32_t index, void *dev_buf, int32_t direction,
+                       int32_t *elem_bytes) {
+  return guarded(c, [&] {
+    if (!dev_buf || (direction != 0 && direction != 1)) throw Err(OLAMGPU_EINVAL, "bad copy arguments");
+    c->impl->copy_field(field_id, index, dev_buf, direction, elem_bytes);
+  });
+}
+int olamgpu_last_call_ms(olamgpu_ctx *c, double *ms) {
+  if (!c || !ms) return OLAMGPU_EINVAL;
+  *ms = c->last_ms;
+  return OLAMGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+"""ctypes binding of libolamgpu.so (the C-ABI in include/olamgpu.h).
+
+This is the host-side mirror of the reference interface for Python callers: the
+method names follow the Fortran procedures they replace (``prog_wrtv_rk``,
+``scalar_transport_rk``, ``tend0``, ``timeavg_momsc``, ``mass_sums``).  There is no
+CPU fallback: if the HIP library is missing or no GPU is visible, construction
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .build import LIB
+
+FIELD = {"scalar": 1, "scalar_adj": 2, "rho": 3, "rho_old": 4, "vmasc": 5, "wmasc": 6, "thil": 7,
+         "vmc": 8, "wmc": 9, "press": 10}
+
+_lib = None
+
+
+def lib():
+    """Load libolamgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build() (the HIP path has no fallback)")
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.olamgpu_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]
+        L.olamgpu_destroy.argtypes = [vp]
+        L.olamgpu_last_error.argtypes = [vp]
+        L.olamgpu_last_error.restype = C.c_char_p
+        for name in ("olamgpu_init_wrtv_rk", "olamgpu_tend0", "olamgpu_prog_wrtv_rk", "olamgpu_timeavg_momsc",
+                     "olamgpu_scalar_transport_rk_adj"):
+            getattr(L, name).argtypes = [vp]
+        L.olamgpu_upload_state.argtypes = [vp, vp]
+        L.olamgpu_download_state.argtypes = [vp, vp]
+        L.olamgpu_upload_scalar.argtypes = [vp, C.c_int32, vp, vp]
+        L.olamgpu_download_scalar.argtypes = [vp, C.c_int32, vp]
+        L.olamgpu_upload_scalar_adj.argtypes = [vp, C.c_int32, vp]
+        L.olamgpu_download_scalar_adj.argtypes = [vp, C.c_int32, vp]
+        L.olamgpu_scalar_transport_rk.argtypes = [vp, vp]
+        L.olamgpu_mass_sums.argtypes = [vp, vp]
+        L.olamgpu_cfl_report.argtypes = [vp, vp]
+        L.olamgpu_copy_field.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
+        L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_upload_state",
+           "olamgpu_download_state", "olamgpu_upload_scalar", "olamgpu_download_scalar", "olamgpu_init_wrtv_rk",
+           "olamgpu_tend0", "olamgpu_prog_wrtv_rk", "olamgpu_timeavg_momsc", "olamgpu_scalar_transport_rk",
+           "olamgpu_scalar_transport_rk_adj", "olamgpu_upload_scalar_adj", "olamgpu_download_scalar_adj",
+           "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms"]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OlamGpuError(RuntimeError):
+    pass
+
+
+class Context:
+    """One olamgpu_ctx: a grid, its device state, and the hot-path calls."""
+
+    def __init__(self, grid: abi.FortranGrid, params: abi.Params, device: int = -1):
+        self.grid = grid
+        self.params = params
+        self.h = C.c_void_p()
+        rc = lib().olamgpu_create(C.byref(grid.desc), C.byref(params), int(device), C.byref(self.h))
+        if rc:
+            raise OlamGpuError(f"olamgpu_create failed ({rc}): {lib().olamgpu_last_error(None).decode()}")
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = lib().olamgpu_last_error(self.h).decode()
+            raise OlamGpuError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self.h:
+            lib().olamgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state -------------------------------------------------------------------
+    def upload_state(self, st, only=None):
+        d = st.desc(only)
+        self._chk(lib().olamgpu_upload_state(self.h, C.byref(d)), "upload_state")
+
+    def download_state(self, st, only=None):
+        d = st.desc(only)
+        self._chk(lib().olamgpu_download_state(self.h, C.byref(d)), "download_state")
+
+    def upload_scalar(self, n, q, t=None):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        t = None if t is None else np.ascontiguousarray(t, dtype=np.float32)
+        self._chk(lib().olamgpu_upload_scalar(self.h, n, _p(q), _p(t)), "upload_scalar")
+
+    def download_scalar(self, n):
+        out = np.zeros((self.grid.mwa, self.grid.mza), np.float32)
+        self._chk(lib().olamgpu_download_scalar(self.h, n, _p(out)), "download_scalar")
+        return out
+
+    def upload_scalar_adj(self, n, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        self._chk(lib().olamgpu_upload_scalar_adj(self.h, n, _p(lam)), "upload_scalar_adj")
+
+    def download_scalar_adj(self, n):
+        out = np.zeros((self.grid.mwa, self.grid.mza), np.float32)
+        self._chk(lib().olamgpu_download_scalar_adj(self.h, n, _p(out)), "download_scalar_adj")
+        return out
+
+    # hot path ------------------------------------------------------------------
+    def init_wrtv_rk(self):
+        self._chk(lib().olamgpu_init_wrtv_rk(self.h), "init_wrtv_rk")
+
+    def tend0(self):
+        self._chk(lib().olamgpu_tend0(self.h), "tend0")
+
+    def prog_wrtv_rk(self):
+        self._chk(lib().olamgpu_prog_wrtv_rk(self.h), "prog_wrtv_rk")
+
+    def timeavg_momsc(self):
+        self._chk(lib().olamgpu_timeavg_momsc(self.h), "timeavg_momsc")
+
+    def scalar_transport_rk(self, rho_old=None):
+        r = None if rho_old is None else np.ascontiguousarray(rho_old, dtype=np.float32)
+        self._chk(lib().olamgpu_scalar_transport_rk(self.h, _p(r)), "scalar_transport_rk")
+
+    def scalar_transport_rk_adj(self):
+        self._chk(lib().olamgpu_scalar_transport_rk_adj(self.h), "scalar_transport_rk_adj")
+
+    # diagnostics -----------------------------------------------------------------
+    def mass_sums(self):
+        out = np.zeros(1 + self.params.num_scalar)
+        self._chk(lib().olamgpu_mass_sums(self.h, _p(out)), "mass_sums")
+        return out
+
+    def cfl_report(self):
+        out = np.zeros(4)
+        self._chk(lib().olamgpu_cfl_report(self.h, _p(out)), "cfl_report")
+        return out
+
+    def last_call_ms(self):
+        ms = C.c_double()
+        lib().olamgpu_last_call_ms(self.h, C.byref(ms))
+        return ms.value
+
+    # device-resident access --------------------------------------------------------
+    def copy_field(self, name, index, dev_ptr, direction):
+        eb = C.c_int32()
+        self._chk(lib().olamgpu_copy_field(self.h, FIELD[name], int(index), C.c_void_p(dev_ptr), int(direction),
+                                           C.byref(eb)), "copy_field")
+        return eb.value

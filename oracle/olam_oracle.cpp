@@ -1,0 +1,1051 @@
+// olam_oracle.cpp -- CPU restatement of OLAM's RK3 dynamical core (TEST INFRASTRUCTURE).
+//
+// This file is the oracle: a from-scratch C++ restatement of the reference
+// algorithm, written loop for loop after the Fortran so that each function can be
+// checked against the file:line it follows.  It exists only to check the MI355X
+// implementation (tests/, __graft_entry__.smoke()) and to time the CPU baseline
+// (bench.py's cpu_baseline leg).  The product library never links or calls it.
+//
+// Precision is a template policy <R, RP, RT>: R = bulk stencil type, RP = type of
+// rho/press and the prog_wrt_begs temporaries the reference keeps in real(r8),
+// RT = type of the vertical tridiagonal solve.  Modes: F32 <float,float,double>,
+// MIXED <float,double,double>, F64 <double,double,double>, and REF
+// <float,double,float> which is the reference's own type map (SURVEY appendix A of
+// the build notes; prog_wrtv_rk.f90:557-580, tridiag_lib.f90:16-21).
+//
+// Parity pins (which test checks what) are listed in the header of
+// tests/test_oracle.py.  Arrays use the reference's Fortran layout: (mza, n)
+// column-major, 1-based, dummy index 1.
+//
+// Restricted to flat terrain (lpw = lpv = lpm = 2), horizontal orders 1 and 2,
+// vertical order 3 (iorderv = 3), no Rayleigh friction / nudging / eddy diffusion
+// (their switches are off in every configuration exercised here).
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+#include "../include/olamgpu.h"
+
+namespace {
+
+template <class T>
+struct F2 {  // Fortran (m1, n) array, 1-based
+  std::vector<T> v;
+  int m1 = 0, n = 0;
+  void alloc(int m1_, int n_, T init = T(0)) { m1 = m1_; n = n_; v.assign((size_t)m1 * n, init); }
+  inline T &operator()(int k, int i) { return v[(size_t)(i - 1) * m1 + (k - 1)]; }
+  inline const T &operator()(int k, int i) const { return v[(size_t)(i - 1) * m1 + (k - 1)]; }
+  T *col(int i) { return &v[(size_t)(i - 1) * m1]; }
+};
+
+template <class T, class S>
+void load2(F2<T> &a, const S *src, int m1, int n) {
+  a.alloc(m1, n);
+  if (!src) return;
+  for (size_t i = 0; i < (size_t)m1 * n; ++i) a.v[i] = (T)src[i];
+}
+template <class T, class S>
+void load1(std::vector<T> &a, const S *src, int n) {
+  a.assign(n + 1, T(0));  // 1-based
+  if (!src) return;
+  for (int i = 0; i < n; ++i) a[i + 1] = (T)src[i];
+}
+
+template <class R_, class RP_, class RT_>
+struct Prec { using R = R_; using RP = RP_; using RT = RT_; };
+
+struct Base {
+  virtual ~Base() {}
+  virtual int upload_state(const olamgpu_state *) = 0;
+  virtual int download_state(olamgpu_state *) = 0;
+  virtual int upload_scalar(int, const float *, const float *) = 0;
+  virtual int download_scalar(int, float *) = 0;
+  virtual int upload_adj(int, const float *) = 0;
+  virtual int download_adj(int, float *) = 0;
+  virtual void tend0() = 0;
+  virtual void prog_wrtv_rk() = 0;
+  virtual void timeavg_momsc() = 0;
+  virtual void scalar_transport_rk(const float *rho_old) = 0;
+  virtual void scalar_transport_rk_adj() = 0;
+  virtual void mass_sums(double *) = 0;
+  virtual void cfl_report(double *) = 0;
+  std::string err;
+};
+
+template <class P>
+struct Model : Base {
+  using R = typename P::R;
+  using RP = typename P::RP;
+  using RT = typename P::RT;
+
+  int mza, mwa, mva, mma;
+  olamgpu_params prm;
+
+  // ---- tables (mem_ijtabs.f90:45-114) ----
+  std::vector<int> npoly;
+  F2<int> w_iv, w_iw, w_im, v_iw, v_im, m_iv, m_iw, m_im;
+  F2<R> dirv, ecx, ecy, ecz, gxc, gyc, xx_yy, dxps, dyps, xx_yy_m;
+  // ---- metrics ----
+  std::vector<R> wnx, wny, wnz, wnxo2, wnyo2, wnzo2, arw0, vnx, vny, vnz, dnu, dniu, dnv, dniv, arm0;
+  std::vector<R> dzim, dzit, zfacit, gdz_wgtm, gdz_wgtp;
+  std::vector<double> zwgt_top8, zwgt_bot8, gdz_wgtm8, gdz_wgtp8;
+  std::vector<R> aw;   // a_w(k,i,j) at aw[((j-1)*2 + (i-1))*mza + k-1]
+  F2<R> arw, volti, volwi, arv, volvi;
+  F2<double> volt;
+  // ---- state (mem_basic, mem_tend) ----
+  F2<R> thil, wmc, wc, vxe, vye, vze, vmc, vc, vmasc, wmasc, alpha_press, pwfac, pvfac;
+  F2<R> thilt, vmxet, vmyet, vmzet, vmt, rho_old;
+  F2<RP> rho, press;
+  std::vector<F2<R>> scp, sct, lam;
+  // ---- SAVEd damping state (vort_div_damp.f90:23-33,255-262) ----
+  bool div_first = true, vort_first = true;
+  F2<R> div2d[2], vort2d[2], fdivh;
+  std::vector<R> fvorth, fvortl;
+  F2<R> vfact;
+  // ---- last tracer step (for the adjoint) ----
+  bool have_tape = false;
+  F2<R> tape_vmasc, tape_wmasc, tape_rhos[3];
+  std::vector<R> tape_ffimp;  // (nrk, mwa)
+  double cfl_out[4] = {0, 0, 0, 0};
+
+  R aw_(int k, int i, int j) const { return aw[((j - 1) * 2 + (i - 1)) * mza + (k - 1)]; }
+
+  Model(const olamgpu_mesh *m, const olamgpu_params *p) {
+    prm = *p;
+    mza = m->mza; mwa = m->mwa; mva = m->mva; mma = m->mma;
+    npoly.assign(mwa + 1, 0);
+    for (int i = 0; i < mwa; ++i) npoly[i + 1] = m->w_npoly[i];
+    load2(w_iv, m->w_iv, 7, mwa); load2(w_iw, m->w_iw, 7, mwa); load2(w_im, m->w_im, 7, mwa);
+    load2(dirv, m->w_dirv, 7, mwa);
+    load2(ecx, m->w_ecvec_vx, 7, mwa); load2(ecy, m->w_ecvec_vy, 7, mwa); load2(ecz, m->w_ecvec_vz, 7, mwa);
+    load2(gxc, m->gxps_coef, 7, mwa); load2(gyc, m->gyps_coef, 7, mwa); load2(xx_yy, m->xx_yy, 7, mwa);
+    load2(v_iw, m->v_iw, 2, mva); load2(v_im, m->v_im, 2, mva);
+    load2(dxps, m->v_dxps, 2, mva); load2(dyps, m->v_dyps, 2, mva);
+    load2(m_iv, m->m_iv, 3, mma); load2(m_iw, m->m_iw, 3, mma); load2(m_im, m->m_im, 3, mma);
+    load2(xx_yy_m, m->xx_yy_m, 3, mma);
+    load1(wnx, m->wnx, mwa); load1(wny, m->wny, mwa); load1(wnz, m->wnz, mwa); load1(arw0, m->arw0, mwa);
+    wnxo2.assign(mwa + 1, 0); wnyo2.assign(mwa + 1, 0); wnzo2.assign(mwa + 1, 0);
+    for (int i = 1; i <= mwa; ++i) {  // mem_grid.f90:541-543
+      wnxo2[i] = wnx[i] * R(0.5); wnyo2[i] = wny[i] * R(0.5); wnzo2[i] = wnz[i] * R(0.5);
+    }
+    load1(vnx, m->vnx, mva); load1(vny, m->vny, mva); load1(vnz, m->vnz, mva);
+    load1(dnu, m->dnu, mva); load1(dniu, m->dniu, mva); load1(dnv, m->dnv, mva); load1(dniv, m->dniv, mva);
+    load1(arm0, m->arm0, mma);
+    load1(dzim, m->dzim, mza); load1(dzit, m->dzit, mza); load1(zfacit, m->zfacit, mza);
+    load1(zwgt_top8, m->zwgt_top8, mza); load1(zwgt_bot8, m->zwgt_bot8, mza);
+    load1(gdz_wgtm8, m->gdz_wgtm8, mza); load1(gdz_wgtp8, m->gdz_wgtp8, mza);
+    gdz_wgtm.assign(mza + 1, 0); gdz_wgtp.assign(mza + 1, 0);
+    for (int k = 1; k <= mza; ++k) { gdz_wgtm[k] = (R)gdz_wgtm8[k]; gdz_wgtp[k] = (R)gdz_wgtp8[k]; }
+    aw.assign(4 * mza, 0);
+    for (int i = 0; i < 4 * mza; ++i) aw[i] = (R)m->a_w[i];
+    load2(arw, m->arw, mza, mwa); load2(arv, m->arv, mza, mva);
+    load2(volt, m->volt, mza, mwa);
+    if (sizeof(R) == 8) {
+      // FP64 mode: inverse volumes rebuilt from volt in double (mem_grid.f90:523-556
+      // computes them in r8 and rounds to real; the FP64 oracle skips the rounding).
+      volti.alloc(mza, mwa); volwi.alloc(mza, mwa); volvi.alloc(mza, mva);
+      for (int iw = 2; iw <= mwa; ++iw) {
+        for (int k = 2; k <= mza; ++k) volti(k, iw) = (R)(1.0 / volt(k, iw));
+        for (int k = 2; k <= mza - 1; ++k) volwi(k, iw) = (R)(1.0 / (volt(k, iw) + volt(k + 1, iw)));
+      }
+      for (int iv = 2; iv <= mva; ++iv) {
+        int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+        for (int k = 2; k <= mza; ++k) volvi(k, iv) = (R)(1.0 / (volt(k, iw1) + volt(k, iw2)));
+      }
+    } else {
+      load2(volti, m->volti, mza, mwa); load2(volwi, m->volwi, mza, mwa); load2(volvi, m->volvi, mza, mva);
+    }
+    // state
+    thil.alloc(mza, mwa); wmc.alloc(mza, mwa); wc.alloc(mza, mwa); vxe.alloc(mza, mwa);
+    vye.alloc(mza, mwa); vze.alloc(mza, mwa); vmc.alloc(mza, mva); vc.alloc(mza, mva);
+    vmasc.alloc(mza, mva); wmasc.alloc(mza, mwa); alpha_press.alloc(mza, mwa); pwfac.alloc(mza, mwa);
+    pvfac.alloc(mza, mva); thilt.alloc(mza, mwa); vmxet.alloc(mza, mwa); vmyet.alloc(mza, mwa);
+    vmzet.alloc(mza, mwa); vmt.alloc(mza, mva); rho_old.alloc(mza, mwa);
+    rho.alloc(mza, mwa); press.alloc(mza, mwa);
+    int ns = std::max(prm.num_scalar, 0);
+    scp.resize(ns); sct.resize(ns); lam.resize(ns);
+    for (int n = 0; n < ns; ++n) { scp[n].alloc(mza, mwa); sct[n].alloc(mza, mwa); lam[n].alloc(mza, mwa); }
+  }
+
+  // ---------------------------------------------------------------- transfers
+  template <class T, class S>
+  static void cp_in(F2<T> &a, const S *src) { if (src) for (size_t i = 0; i < a.v.size(); ++i) a.v[i] = (T)src[i]; }
+  template <class T, class S>
+  static void cp_out(const F2<T> &a, S *dst) { if (dst) for (size_t i = 0; i < a.v.size(); ++i) dst[i] = (S)a.v[i]; }
+
+  int upload_state(const olamgpu_state *s) override {
+    cp_in(thil, s->thil); cp_in(wmc, s->wmc); cp_in(wc, s->wc); cp_in(vxe, s->vxe); cp_in(vye, s->vye);
+    cp_in(vze, s->vze); cp_in(rho, s->rho); cp_in(press, s->press); cp_in(vmc, s->vmc); cp_in(vc, s->vc);
+    cp_in(wmasc, s->wmasc); cp_in(vmasc, s->vmasc); cp_in(alpha_press, s->alpha_press);
+    cp_in(pwfac, s->pwfac); cp_in(pvfac, s->pvfac); cp_in(thilt, s->thilt); cp_in(vmxet, s->vmxet);
+    cp_in(vmyet, s->vmyet); cp_in(vmzet, s->vmzet); cp_in(vmt, s->vmt);
+    return 0;
+  }
+  int download_state(olamgpu_state *s) override {
+    cp_out(thil, s->thil); cp_out(wmc, s->wmc); cp_out(wc, s->wc); cp_out(vxe, s->vxe); cp_out(vye, s->vye);
+    cp_out(vze, s->vze); cp_out(rho, s->rho); cp_out(press, s->press); cp_out(vmc, s->vmc); cp_out(vc, s->vc);
+    cp_out(wmasc, s->wmasc); cp_out(vmasc, s->vmasc); cp_out(alpha_press, s->alpha_press);
+    cp_out(pwfac, s->pwfac); cp_out(pvfac, s->pvfac); cp_out(thilt, s->thilt); cp_out(vmxet, s->vmxet);
+    cp_out(vmyet, s->vmyet); cp_out(vmzet, s->vmzet); cp_out(vmt, s->vmt);
+    return 0;
+  }
+  int upload_scalar(int n, const float *p, const float *t) override {
+    if (n < 1 || n > (int)scp.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
+    cp_in(scp[n - 1], p);
+    if (t) cp_in(sct[n - 1], t); else std::fill(sct[n - 1].v.begin(), sct[n - 1].v.end(), R(0));
+    return 0;
+  }
+  int download_scalar(int n, float *p) override {
+    if (n < 1 || n > (int)scp.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
+    cp_out(scp[n - 1], p); return 0;
+  }
+  int upload_adj(int n, const float *p) override {
+    if (n < 1 || n > (int)lam.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
+    cp_in(lam[n - 1], p); return 0;
+  }
+  int download_adj(int n, float *p) override {
+    if (n < 1 || n > (int)lam.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
+    cp_out(lam[n - 1], p); return 0;
+  }
+
+  // ---------------------------------------------------------------- tend0
+  // timestep.f90:430-496 (tendencies of thil/momentum are physics inputs here and
+  // are left as uploaded; vmasc/wmasc/var_t are zeroed, rho_old saved).
+  void tend0() override {
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int k = 2; k <= mza; ++k) {
+        rho_old(k, iw) = (R)rho(k, iw);
+        wmasc(k, iw) = 0;
+        for (auto &t : sct) t(k, iw) = 0;
+      }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv)
+      for (int k = 2; k <= mza; ++k) vmasc(k, iv) = 0;
+  }
+
+  // timestep.f90:563-612
+  void timeavg_momsc() override {
+    if (prm.nacoust <= 1) return;
+    R f = (R)prm.nacoust;
+    for (int iv = 2; iv <= mva; ++iv) for (int k = 2; k <= mza; ++k) vmasc(k, iv) = vmasc(k, iv) / f;
+    for (int iw = 2; iw <= mwa; ++iw) for (int k = 2; k <= mza - 1; ++k) wmasc(k, iw) = wmasc(k, iw) / f;
+  }
+
+  // ---------------------------------------------------------------- grad_lib
+  // grad_t2d, grad_lib.f90:236-269
+  void grad_t2d(int iw, const F2<R> &s, R *gx, R *gy) const {
+    for (int k = 1; k <= mza; ++k) { gx[k] = 0; gy[k] = 0; }
+    for (int n = 1; n <= npoly[iw]; ++n) {
+      int iwn = w_iw(n, iw);
+      for (int k = 2; k <= mza; ++k) {
+        R d = s(k, iwn) - s(k, iw);
+        gx[k] = gx[k] + gxc(n, iw) * d;
+        gy[k] = gy[k] + gyc(n, iw) * d;
+      }
+    }
+  }
+  // grad_z_quadratic bounded, itopbc = ibotbc = 0, grad_lib.f90:166-229
+  void grad_z_quadratic(const R *s, R *sb, R *st) const {
+    std::vector<R> ds(mza + 2, R(0));
+    for (int k = 2; k <= mza - 1; ++k) ds[k] = s[k + 1] - s[k];
+    ds[1] = 0; ds[mza] = 0;
+    for (int k = 2; k <= mza; ++k) {
+      sb[k] = s[k] + ds[k] * aw_(k, 1, 1) + ds[k - 1] * aw_(k, 2, 1);
+      st[k] = s[k] + ds[k] * aw_(k, 1, 2) + ds[k - 1] * aw_(k, 2, 2);
+    }
+  }
+  // laplacian2d, grad_lib.f90:494-524
+  void laplacian2d(int iw, const F2<R> &s, R *d) const {
+    for (int k = 1; k <= mza; ++k) d[k] = 0;
+    for (int n = 1; n <= npoly[iw]; ++n) {
+      int iwn = w_iw(n, iw);
+      for (int k = 2; k <= mza; ++k) d[k] = d[k] + (s(k, iwn) - s(k, iw)) * xx_yy(n, iw);
+    }
+  }
+  // laplacian_m, grad_lib.f90:531-559
+  void laplacian_m(int im, const F2<R> &s, R *d) const {
+    for (int k = 1; k <= mza; ++k) d[k] = 0;
+    for (int j = 1; j <= 3; ++j) {
+      int imn = m_im(j, im);
+      for (int k = 2; k <= mza; ++k) d[k] = d[k] + (s(k, imn) - s(k, im)) * xx_yy_m(j, im);
+    }
+  }
+
+  // tridiffo, tridiag_lib.f90:10-36 (RT precision)
+  void tridiffo(int ka, int kz, const RT *cim1, const RT *ci, const RT *cip1, const RT *rhs, RT *soln) const {
+    std::vector<RT> scr1(kz + 2);
+    RT cji = RT(1) / ci[ka];
+    soln[ka] = rhs[ka] * cji;
+    for (int k = ka + 1; k <= kz; ++k) {
+      scr1[k] = cji * cip1[k - 1];
+      cji = RT(1) / (ci[k] - cim1[k] * scr1[k]);
+      soln[k] = cji * (rhs[k] - cim1[k] * soln[k - 1]);
+    }
+    for (int k = kz - 1; k >= ka; --k) soln[k] = soln[k] - scr1[k + 1] * soln[k + 1];
+  }
+
+  // ---------------------------------------------------------------- fluxes_wrtv
+  // prog_wrtv_rk.f90:950-1129 (iorderh 1 or 2, iorderv = 3)
+  void fluxes_wrtv(const F2<R> &s, F2<R> &upw, F2<R> &upv, int iorderh,
+                   F2<R> &gx, F2<R> &gy) {
+    if (iorderh > 1) {
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, s, gx.col(iw) - 1, gy.col(iw) - 1);
+    }
+#pragma omp parallel
+    {
+      std::vector<R> sb(mza + 2), st(mza + 2);
+#pragma omp for
+      for (int iw = 2; iw <= mwa; ++iw) {  // scalar_w_column :1085-1126
+        int kb = 2;
+        upw(kb - 1, iw) = s(kb, iw);
+        upw(mza, iw) = s(mza, iw);
+        grad_z_quadratic(&s(1, iw) - 1, sb.data(), st.data());
+        for (int k = kb; k <= mza - 1; ++k) {
+          upw(k, iw) = st[k];
+          if (wmc(k, iw) < R(0)) upw(k, iw) = sb[k + 1];
+        }
+      }
+    }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {  // scalar_v_column :1022-1081
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        R s1, s2;
+        if (iorderh == 1) { s1 = s(k, iw1); s2 = s(k, iw2); }
+        else {
+          s1 = s(k, iw1) + dxps(1, iv) * gx(k, iw1) + dyps(1, iv) * gy(k, iw1);
+          s2 = s(k, iw2) + dxps(2, iv) * gx(k, iw2) + dyps(2, iv) * gy(k, iw2);
+        }
+        upv(k, iv) = s1;
+        if (vmc(k, iv) < R(0)) upv(k, iv) = s2;
+      }
+    }
+  }
+
+  // ---------------------------------------------------------------- damping
+  // divh_damp, vort_div_damp.f90:1-230 (dorayfdiv = .false.)
+  void divh_damp(F2<R> &vmtx, double dtm) {
+    int lev = prm.divh_damp_level;
+    bool dodiv = (prm.divh_damp_fact > 1.e-7f) && lev > 0;
+    if (!dodiv) return;
+    if (div_first) {
+      div_first = false;
+      div2d[0].alloc(mza, mwa); div2d[1].alloc(mza, mwa);
+      fdivh.alloc(mza, mva);
+      if (lev > 1) {
+        double facth = -std::pow(-(double)prm.divh_damp_fact, lev) / dtm;
+        for (int iv = 2; iv <= mva; ++iv) {
+          int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+          double factb = facth * std::pow((double)std::min(arw0[iw1], arw0[iw2]), lev - 1);
+          for (int k = 2; k <= mza; ++k)
+            fdivh(k, iv) = (R)(factb * (double)(dniv[iv] * zfacit[k] * dzit[k]) *
+                               std::min(volt(k, iw1), volt(k, iw2)));
+        }
+      } else {
+        // level 1: fdivl (vort_div_damp.f90:97-103) with zf = 0
+        R fact2 = (R)(prm.divh_damp_fact / (R)dtm);
+        for (int iv = 2; iv <= mva; ++iv) {
+          int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+          for (int k = 2; k <= mza; ++k)
+            fdivh(k, iv) = (R)((double)(fact2 * dniv[iv] * zfacit[k] * dzit[k]) *
+                               std::min(volt(k, iw1), volt(k, iw2)));
+        }
+      }
+    }
+    // horizontal divergence (accumulates onto the saved div2d(:,:,1), :140-156)
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      for (int j = 1; j <= npoly[iw]; ++j) {
+        int iv = w_iv(j, iw);
+        for (int k = 2; k <= mza; ++k)
+          div2d[0](k, iw) = div2d[0](k, iw) - arv(k, iv) * dirv(j, iw) * vmc(k, iv);
+      }
+      for (int k = 2; k <= mza; ++k) div2d[0](k, iw) = div2d[0](k, iw) * volti(k, iw);
+    }
+    int ih = 0;
+    if (lev > 1) {
+      for (int nn = 1; nn <= lev - 1; ++nn) {
+        int il = (nn - 1) % 2;
+        ih = nn % 2;
+#pragma omp parallel for
+        for (int iw = 2; iw <= mwa; ++iw) laplacian2d(iw, div2d[il], div2d[ih].col(iw) - 1);
+      }
+    }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k)
+        vmtx(k, iv) = vmtx(k, iv) + fdivh(k, iv) * (div2d[ih](k, iw2) - div2d[ih](k, iw1));
+    }
+  }
+
+  // vort_damp, vort_div_damp.f90:234-445 (flat terrain: lpmmin = lpm = lpv)
+  void vort_damp(F2<R> &vmtx, double dtm) {
+    int lev = prm.vort_damp_level;
+    if (prm.vort_damp_fact < 1.e-7f || lev <= 0) return;
+    if (vort_first) {
+      vort_first = false;
+      fvorth.assign(mva + 1, 0); fvortl.assign(mva + 1, 0);
+      vfact.alloc(3, mma);
+      vort2d[0].alloc(mza, mma); vort2d[1].alloc(mza, mma);
+      for (int im = 2; im <= mma; ++im)
+        for (int jv = 1; jv <= 3; ++jv) {
+          int iv = m_iv(jv, im);
+          vfact(jv, im) = (v_im(2, iv) == im) ? dnv[iv] / arm0[im] : -dnv[iv] / arm0[im];
+        }
+      for (int iv = 2; iv <= mva; ++iv) {
+        int im1 = v_im(1, iv), im2 = v_im(2, iv);
+        R a = std::min(arm0[im1], arm0[im2]);
+        fvorth[iv] = R(0.5) * dniu[iv] / (R)dtm * (R)std::pow((double)(-prm.vort_damp_fact * a), lev);
+        if (lev > 2)
+          fvortl[iv] = R(0.5) * dniu[iv] / (R)dtm * (R)std::pow((double)(-prm.vort_damp_fact * a), lev - 1);
+      }
+    }
+#pragma omp parallel for
+    for (int im = 2; im <= mma; ++im) {
+      for (int k = 1; k <= mza; ++k) vort2d[0](k, im) = 0;
+      for (int jv = 1; jv <= 3; ++jv) {
+        int iv = m_iv(jv, im);
+        for (int k = 2; k <= mza; ++k) vort2d[0](k, im) = vort2d[0](k, im) + vc(k, iv) * vfact(jv, im);
+      }
+    }
+    int ih = 0;
+    if (lev > 1) {
+      for (int nn = 1; nn <= lev - 1; ++nn) {
+        int il = (nn - 1) % 2;
+        ih = nn % 2;
+#pragma omp parallel for
+        for (int im = 2; im <= mma; ++im) laplacian_m(im, vort2d[il], vort2d[ih].col(im) - 1);
+      }
+    }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv), im1 = v_im(1, iv), im2 = v_im(2, iv);
+      for (int k = 2; k <= mza; ++k)
+        vmtx(k, iv) = vmtx(k, iv) + fvorth[iv] * (vort2d[ih](k, im2) - vort2d[ih](k, im1)) *
+                                        (R)(rho(k, iw1) + rho(k, iw2));
+    }
+  }
+
+  // ---------------------------------------------------------------- prog_wrtv_rk
+  // Scratch of prog_wrtv_rk (automatic arrays, prog_wrtv_rk.f90:97-137)
+  F2<R> thil_upv, vxe_upv, vye_upv, vze_upv, thil_upw, vxe_upw, vye_upw, vze_upw;
+  F2<R> vmxet_rk, vmyet_rk, vmzet_rk, thilt_rk, wmc0, vmc0, thilt_s, vmxet_s, vmyet_s, vmzet_s, vmt_s, vmca;
+  F2<RP> rho0, rth0;
+  F2<R> gxs, gys;
+
+  void alloc_wrtv_scratch() {
+    if (thil_upv.n) return;
+    for (auto *a : {&thil_upv, &vxe_upv, &vye_upv, &vze_upv, &vmc0, &vmt_s, &vmca}) a->alloc(mza, mva);
+    for (auto *a : {&thil_upw, &vxe_upw, &vye_upw, &vze_upw, &vmxet_rk, &vmyet_rk, &vmzet_rk, &thilt_rk,
+                    &wmc0, &thilt_s, &vmxet_s, &vmyet_s, &vmzet_s, &gxs, &gys})
+      a->alloc(mza, mwa);
+    rho0.alloc(mza, mwa); rth0.alloc(mza, mwa);
+  }
+
+  void prog_wrtv_rk() override {
+    alloc_wrtv_scratch();
+    const double dtsm = prm.dtsm;
+    // setup loop, prog_wrtv_rk.f90:165-253 (no eddy diffusion / Rayleigh)
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      thil_upw(mza, iw) = 0; thil_upw(1, iw) = 0;
+      for (int k = 2; k <= mza; ++k) {
+        R v4 = (R)volt(k, iw);
+        rth0(k, iw) = (RP)thil(k, iw) * rho(k, iw);
+        rho0(k, iw) = rho(k, iw);
+        wmc0(k, iw) = wmc(k, iw);
+        thilt_s(k, iw) = thilt(k, iw) * v4;
+        vmxet_s(k, iw) = vmxet(k, iw) * v4;
+        vmyet_s(k, iw) = vmyet(k, iw) * v4;
+        vmzet_s(k, iw) = vmzet(k, iw) * v4;
+      }
+    }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv)
+      for (int k = 1; k <= mza; ++k) { vmc0(k, iv) = vmc(k, iv); vmt_s(k, iv) = vmt(k, iv); }
+    // short-step damping, :307-308
+    divh_damp(vmt_s, dtsm);
+    vort_damp(vmt_s, dtsm);
+
+    const double onethird = 1.0 / 3.0;
+    for (int istage = 1; istage <= prm.nrk_wrtv; ++istage) {
+      double dt8 = (prm.nrk_wrtv <= 2) ? (istage == 1 ? 0.5 : 1.0) * dtsm
+                                       : (istage == 1 ? onethird : (istage == 2 ? 0.5 : 1.0)) * dtsm;
+      R dts = (R)dt8;
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw)
+        for (int k = 2; k <= mza; ++k) {
+          vmxet_rk(k, iw) = vmxet_s(k, iw); vmyet_rk(k, iw) = vmyet_s(k, iw);
+          vmzet_rk(k, iw) = vmzet_s(k, iw); thilt_rk(k, iw) = thilt_s(k, iw);
+        }
+#pragma omp parallel for
+      for (int iv = 2; iv <= mva; ++iv) {
+        for (int k = 2; k <= mza; ++k) vmca(k, iv) = vmc(k, iv) * arv(k, iv);
+        if (istage == prm.nrk_wrtv)
+          for (int k = 2; k <= mza; ++k) vmasc(k, iv) = vmasc(k, iv) + vmca(k, iv);
+      }
+      fluxes_wrtv(thil, thil_upw, thil_upv, prm.thil_horiz_adv_order, gxs, gys);
+      fluxes_wrtv(vxe, vxe_upw, vxe_upv, prm.wind_horiz_adv_order, gxs, gys);
+      fluxes_wrtv(vye, vye_upw, vye_upv, prm.wind_horiz_adv_order, gxs, gys);
+      fluxes_wrtv(vze, vze_upw, vze_upv, prm.wind_horiz_adv_order, gxs, gys);
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) prog_wrt_begs(iw, istage, dt8);
+#pragma omp parallel for
+      for (int iv = 2; iv <= mva; ++iv) prog_v_begs(iv, dts);
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) vel_t3d_hex(iw);
+    }
+  }
+
+  // prog_wrt_begs, prog_wrtv_rk.f90:494-884 (global domain, no nudging, b20 = 1)
+  void prog_wrt_begs(int iw, int istage, double dt8) {
+    const RP fr = RP(0.70), fp2 = RP(0.70), fp3 = RP(0.70);
+    const R cpocv = (R)(1004.0 / 717.0), rocv = (R)(287.0 / 717.0);
+    const RP pc2 = fp2 * (RP)cpocv, pc3 = fp3 * (RP)cpocv;
+    const R fcor = (R)prm.fcoriol;
+    R dts = (R)dt8;
+    int ka = 2;
+    RP hflx_rho[130], delex_rho[130], delex_rhothil[130], rhothil[130], vflux_thil[130];
+    RP press_ex[130], rho_ex[130], b5[130], b10[130];
+    R delex_wm[130], wma[130], vflux_vxe[130], vflux_vye[130], vflux_vze[130];
+    R b15[130];
+    RT b31[130], b32[130], b33[130], rhs[130], wmf[130];
+    vflux_thil[ka - 1] = 0; vflux_thil[mza] = 0;
+    vflux_vxe[ka - 1] = 0; vflux_vxe[mza] = 0;
+    vflux_vye[ka - 1] = 0; vflux_vye[mza] = 0;
+    vflux_vze[ka - 1] = 0; vflux_vze[mza] = 0;
+    for (int k = ka; k <= mza; ++k) hflx_rho[k] = 0;
+    for (int jv = 1; jv <= npoly[iw]; ++jv) {  // :635-661
+      int iv = w_iv(jv, iw);
+      if (dirv(jv, iw) > R(0)) {
+        for (int k = 2; k <= mza; ++k) {
+          hflx_rho[k] = hflx_rho[k] + (RP)vmca(k, iv);
+          thilt_rk(k, iw) = thilt_rk(k, iw) + vmca(k, iv) * thil_upv(k, iv);
+          vmxet_rk(k, iw) = vmxet_rk(k, iw) + vmca(k, iv) * vxe_upv(k, iv);
+          vmyet_rk(k, iw) = vmyet_rk(k, iw) + vmca(k, iv) * vye_upv(k, iv);
+          vmzet_rk(k, iw) = vmzet_rk(k, iw) + vmca(k, iv) * vze_upv(k, iv);
+        }
+      } else {
+        for (int k = 2; k <= mza; ++k) {
+          hflx_rho[k] = hflx_rho[k] - (RP)vmca(k, iv);
+          thilt_rk(k, iw) = thilt_rk(k, iw) - vmca(k, iv) * thil_upv(k, iv);
+          vmxet_rk(k, iw) = vmxet_rk(k, iw) - vmca(k, iv) * vxe_upv(k, iv);
+          vmyet_rk(k, iw) = vmyet_rk(k, iw) - vmca(k, iv) * vye_upv(k, iv);
+          vmzet_rk(k, iw) = vmzet_rk(k, iw) - vmca(k, iv) * vze_upv(k, iv);
+        }
+      }
+    }
+    for (int k = ka; k <= mza; ++k) {  // Coriolis, global, :675-681
+      R mass = (R)(rho(k, iw) * (RP)volt(k, iw));
+      vmxet_rk(k, iw) = vmxet_rk(k, iw) + mass * fcor * vye(k, iw);
+      vmyet_rk(k, iw) = vmyet_rk(k, iw) - mass * fcor * vxe(k, iw);
+    }
+    for (int k = ka; k <= mza; ++k) {  // :687-700
+      rhothil[k] = rho(k, iw) * (RP)thil(k, iw);
+      b10[k] = (RP)dt8 * (RP)volti(k, iw);
+      b5[k] = (RP)(alpha_press(k, iw) * std::pow((R)rhothil[k], rocv));
+      b15[k] = (R)(b10[k] * b5[k]);
+      delex_rho[k] = b10[k] * hflx_rho[k];
+      delex_rhothil[k] = b10[k] * (RP)thilt_rk(k, iw);
+    }
+    if (istage > 1)
+      for (int k = ka; k <= mza; ++k) {  // :710-715
+        delex_rho[k] = delex_rho[k] + rho0(k, iw) - rho(k, iw);
+        delex_rhothil[k] = delex_rhothil[k] + rth0(k, iw) - rhothil[k];
+      }
+    for (int k = ka; k <= mza; ++k) {
+      press_ex[k] = b5[k] * (rhothil[k] + pc2 * delex_rhothil[k]);
+      rho_ex[k] = rho(k, iw) + fr * delex_rho[k];
+    }
+    for (int k = ka; k <= mza - 1; ++k)  // :727-745
+      delex_wm[k] = wmc0(k, iw) +
+          dts * ((R)(press_ex[k] - press_ex[k + 1]) * pwfac(k, iw) -
+                 (R)((RP)gdz_wgtm8[k] * rho_ex[k] + (RP)gdz_wgtp8[k] * rho_ex[k + 1]) +
+                 (wnx[iw] * (vmxet_rk(k, iw) + vmxet_rk(k + 1, iw)) +
+                  wny[iw] * (vmyet_rk(k, iw) + vmyet_rk(k + 1, iw)) +
+                  wnz[iw] * (vmzet_rk(k, iw) + vmzet_rk(k + 1, iw))) * volwi(k, iw));
+    R c8 = dts * (R)pc2;
+    R c9 = -dts * dts * (R)fr;
+    for (int k = ka; k <= mza - 1; ++k) {  // :752-769
+      R b7 = dts * volwi(k, iw);
+      R b8 = c8 * pwfac(k, iw);
+      R b21 = b7 * wc(k - 1, iw);
+      R b22 = b7 * wc(k + 1, iw);
+      R b23 = b8 * b15[k];
+      R b24 = b8 * b15[k + 1];
+      R b25 = c9 * volti(k, iw) * gdz_wgtm[k];
+      R b26 = c9 * volti(k + 1, iw) * gdz_wgtp[k];
+      b31[k] = (RT)(-arw(k - 1, iw) * (b21 + b25 + b23 * thil_upw(k - 1, iw)));
+      b32[k] = (RT)(R(1) + arw(k, iw) * (b25 - b26 + (b23 + b24) * thil_upw(k, iw)));
+      b33[k] = (RT)(arw(k + 1, iw) * (b22 + b26 - b24 * thil_upw(k + 1, iw)));
+      rhs[k] = (RT)delex_wm[k];
+    }
+    if (ka <= mza - 1) tridiffo(ka, mza - 1, b31, b32, b33, rhs, wmf);
+    wma[ka - 1] = 0; wma[mza] = 0;
+    for (int k = ka; k <= mza - 1; ++k) wma[k] = (R)wmf[k] * arw(k, iw);
+    for (int k = ka; k <= mza - 1; ++k) {
+      vflux_thil[k] = (RP)(wma[k] * thil_upw(k, iw));
+      vflux_vxe[k] = wma[k] * vxe_upw(k, iw);
+      vflux_vye[k] = wma[k] * vye_upw(k, iw);
+      vflux_vze[k] = wma[k] * vze_upw(k, iw);
+    }
+    if (istage == prm.nrk_wrtv)
+      for (int k = ka; k <= mza - 1; ++k) wmasc(k, iw) = wmasc(k, iw) + wma[k];
+    for (int k = ka; k <= mza; ++k) {  // :818-844
+      rho(k, iw) = rho0(k, iw) + (RP)dt8 * (RP)volti(k, iw) * (hflx_rho[k] + (RP)wma[k - 1] - (RP)wma[k]);
+      delex_rhothil[k] = delex_rhothil[k] + b10[k] * (vflux_thil[k - 1] - vflux_thil[k]);
+      press(k, iw) = b5[k] * (rhothil[k] + pc3 * delex_rhothil[k]);
+      thil(k, iw) = (R)((rhothil[k] + delex_rhothil[k]) / rho(k, iw));
+      vmxet_rk(k, iw) = vmxet_rk(k, iw) + vflux_vxe[k - 1] - vflux_vxe[k];
+      vmyet_rk(k, iw) = vmyet_rk(k, iw) + vflux_vye[k - 1] - vflux_vye[k];
+      vmzet_rk(k, iw) = vmzet_rk(k, iw) + vflux_vze[k - 1] - vflux_vze[k];
+    }
+    for (int k = ka; k <= mza - 1; ++k) {  // :869-872
+      wmc(k, iw) = (R)wmf[k];
+      wc(k, iw) = (R)wmf[k] / (R)((RP)zwgt_bot8[k] * rho(k, iw) + (RP)zwgt_top8[k] * rho(k + 1, iw));
+    }
+    wc(ka - 1, iw) = wc(ka, iw);
+    for (int k = 1; k <= ka - 1; ++k) thil(k, iw) = thil(ka, iw);
+  }
+
+  // prog_v_begs, prog_wrtv_rk.f90:888-946
+  void prog_v_begs(int iv, R dts) {
+    int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+    for (int k = 2; k <= mza; ++k) {
+      R pgf = (R)(press(k, iw1) - press(k, iw2)) * pvfac(k, iv);
+      vmc(k, iv) = vmc0(k, iv) + dts * (vmt_s(k, iv) + pgf +
+                                        (vnx[iv] * (vmxet_rk(k, iw1) + vmxet_rk(k, iw2)) +
+                                         vny[iv] * (vmyet_rk(k, iw1) + vmyet_rk(k, iw2)) +
+                                         vnz[iv] * (vmzet_rk(k, iw1) + vmzet_rk(k, iw2))) * volvi(k, iv));
+      vc(k, iv) = R(2) * vmc(k, iv) / (R)(rho(k, iw1) + rho(k, iw2));
+    }
+  }
+
+  // vel_t3d_hex, vel_t3d.f90:39-108 (lve2 = 0)
+  void vel_t3d_hex(int iw) {
+    int ka = 2;
+    for (int k = ka; k <= mza; ++k) {
+      R wst = wc(k - 1, iw) + wc(k, iw);
+      vxe(k, iw) = wst * wnxo2[iw];
+      vye(k, iw) = wst * wnyo2[iw];
+      vze(k, iw) = wst * wnzo2[iw];
+    }
+    for (int jv = 1; jv <= npoly[iw]; ++jv) {
+      int iv = w_iv(jv, iw);
+      for (int k = 2; k <= mza; ++k) {
+        vxe(k, iw) = vxe(k, iw) + ecx(jv, iw) * vc(k, iv);
+        vye(k, iw) = vye(k, iw) + ecy(jv, iw) * vc(k, iv);
+        vze(k, iw) = vze(k, iw) + ecz(jv, iw) * vc(k, iv);
+      }
+    }
+    vxe(1, iw) = vxe(ka, iw); vye(1, iw) = vye(ka, iw); vze(1, iw) = vze(ka, iw);
+  }
+
+  // ---------------------------------------------------------------- scalar transport
+  F2<R> scp0, gxt, gyt, upv;
+  std::vector<R> ffimp;   // ff_implic(n, iw) at (iw-1)*nrk + n-1
+  F2<R> rhos[3];
+
+  // check_cfls, scalar_transport_rk.F90:1476-1712 (diagnostic maxima kept for the report)
+  void check_cfls(const F2<R> &rho_o, int nrk) {
+    const R dt = (R)prm.dtlm;
+    double ctot = 0, chor = 0, wmax = 0;
+    ffimp.assign((size_t)mwa * nrk, R(0));
+#pragma omp parallel for reduction(max : ctot, chor, wmax)
+    for (int iw = 2; iw <= mwa; ++iw) {
+      R cflh[130], cfl[130], fimp[3] = {0, 0, 0};
+      for (int k = 1; k <= mza; ++k) cflh[k] = 0;
+      for (int n = 1; n <= npoly[iw]; ++n) {
+        int iv = w_iv(n, iw);
+        for (int k = 2; k <= mza; ++k) cflh[k] = cflh[k] - std::min(dirv(n, iw) * vmasc(k, iv), R(0));
+      }
+      for (int k = 2; k <= mza; ++k) {
+        R dovr = dt * volti(k, iw) / rho_o(k, iw);
+        cflh[k] = cflh[k] * dovr;
+        R cflw = (std::max(wmasc(k, iw), R(0)) - std::min(wmasc(k - 1, iw), R(0))) * dovr;
+        cfl[k] = cflh[k] + cflw;
+        cflw = std::max(cflw, R(0.1));
+        fimp[nrk - 1] = std::max(fimp[nrk - 1], (cfl[k] - R(0.98)) / cflw);
+        fimp[nrk - 2] = std::max(fimp[nrk - 2], (cfl[k] - R(1.98)) / cflw);
+        if (nrk == 3) fimp[0] = std::max(fimp[0], (cfl[k] - R(2.98)) / cflw);
+      }
+      for (int n = 0; n < nrk; ++n) {
+        R f = std::min(R(1), fimp[n]);
+        if (f < R(0.02)) f = 0;
+        ffimp[(size_t)(iw - 1) * nrk + n] = f;
+      }
+      for (int k = 2; k <= mza; ++k) {
+        ctot = std::max(ctot, (double)cfl[k]);
+        chor = std::max(chor, (double)cflh[k]);
+        wmax = std::max(wmax, (double)std::fabs(wc(k, iw)));
+      }
+    }
+    cfl_out[0] = ctot; cfl_out[1] = chor; cfl_out[2] = wmax;
+  }
+
+  // update_scalar, scalar_transport_rk.F90:389-615 (iorderv = 3).  q is updated
+  // from its own old values; rhosf is the final-stage density (:328-337).
+  void update_scalar(F2<R> &q, const F2<R> &q0, const F2<R> &st, const F2<R> &rhosf, R dtr,
+                     int iorder, int istage, int nrk) {
+    if (iorder > 1) {
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, q, gxt.col(iw) - 1, gyt.col(iw) - 1);
+    }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {  // scalar_v_column :474-533
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        R s1, s2;
+        if (iorder == 1) { s1 = q(k, iw1); s2 = q(k, iw2); }
+        else {
+          s1 = q(k, iw1) + dxps(1, iv) * gxt(k, iw1) + dyps(1, iv) * gyt(k, iw1);
+          s2 = q(k, iw2) + dxps(2, iv) * gxt(k, iw2) + dyps(2, iv) * gyt(k, iw2);
+        }
+        upv(k, iv) = s1;
+        if (vmasc(k, iv) < R(0)) upv(k, iv) = s2;
+      }
+    }
+#pragma omp parallel
+    {
+      std::vector<R> sb(mza + 2), stp(mza + 2), upw(mza + 2), fd(mza + 2);
+#pragma omp for
+      for (int iw = 2; iw <= mwa; ++iw) {  // scalar_w_column :537-611
+        int kb = 2;
+        R fimpl = ffimp[(size_t)(iw - 1) * nrk + istage - 1];
+        R fexpl = R(1) - fimpl;
+        upw[kb - 1] = q(kb, iw);
+        upw[mza] = q(mza, iw);
+        grad_z_quadratic(&q(1, iw) - 1, sb.data(), stp.data());
+        for (int k = kb; k <= mza - 1; ++k) {
+          upw[k] = stp[k];
+          if (wmasc(k, iw) < R(0)) upw[k] = sb[k + 1];
+        }
+        for (int k = 1; k <= mza; ++k) fd[k] = 0;
+        for (int jv = 1; jv <= npoly[iw]; ++jv) {
+          int iv = w_iv(jv, iw);
+          for (int k = 2; k <= mza; ++k)
+            fd[k] = fd[k] + dirv(jv, iw) * vmasc(k, iv) * (upv(k, iv) - q0(k, iw));
+        }
+        for (int k = kb; k <= mza; ++k) {
+          fd[k] = fd[k] + fexpl * (wmasc(k - 1, iw) * (upw[k - 1] - q0(k, iw)) -
+                                   wmasc(k, iw) * (upw[k] - q0(k, iw)));
+          q(k, iw) = q0(k, iw) + dtr * (st(k, iw) + volti(k, iw) * fd[k]) / rhosf(k, iw);
+        }
+        for (int k = 1; k <= kb - 1; ++k) q(k, iw) = q(kb, iw);
+      }
+    }
+  }
+
+  // implicit vertical coefficients, scalar_transport_rk.F90:186-217 with
+  // tridif_prep (tridiag_lib.f90:321-340); solve with tridif_fini (:344-364).
+  void implicit_column(F2<R> &q, const F2<R> &q0, int iw, int istage, R dtr, const R *dtlr, int nrk) {
+    R ff = ffimp[(size_t)(iw - 1) * nrk + istage - 1];
+    if (!(ff > R(0.001))) return;
+    int kb = 2;
+    R wup[130], wdn[130], b1[130], b2a[130], b3a[130], scr1[130], cji[130], dscp0[130], delex[130], del[130];
+    wup[mza] = 0; wdn[mza] = 0; wup[kb - 1] = 0; wdn[kb - 1] = 0;
+    for (int k = kb; k <= mza - 1; ++k) { wup[k] = std::max(wmasc(k, iw), R(0)); wdn[k] = std::min(wmasc(k, iw), R(0)); }
+    R c1 = ff * dtlr[istage - 1];
+    for (int k = kb; k <= mza; ++k) {
+      R c2 = c1 * volti(k, iw) / rhos[istage - 1](k, iw);
+      b1[k] = -c2 * wup[k - 1];
+      b2a[k] = R(1) + c2 * (wup[k] - wdn[k - 1]);
+      b3a[k] = c2 * wdn[k];
+    }
+    cji[kb] = R(1) / b2a[kb];
+    for (int k = kb + 1; k <= mza; ++k) {
+      scr1[k] = cji[k - 1] * b3a[k - 1];
+      cji[k] = R(1) / (b2a[k] - b1[k] * scr1[k]);
+    }
+    dscp0[mza] = 0; dscp0[kb - 1] = 0;
+    for (int k = kb; k <= mza - 1; ++k) dscp0[k] = q0(k + 1, iw) - q0(k, iw);
+    for (int k = kb; k <= mza; ++k) {
+      R cc = ff * dtr * volti(k, iw) / rhos[istage - 1](k, iw);
+      delex[k] = q(k, iw) - q0(k, iw) - cc * (std::min(wmasc(k, iw), R(0)) * dscp0[k] +
+                                              std::max(wmasc(k - 1, iw), R(0)) * dscp0[k - 1]);
+    }
+    del[kb] = delex[kb] * cji[kb];
+    for (int k = kb + 1; k <= mza; ++k) del[k] = cji[k] * (delex[k] - b1[k] * del[k - 1]);
+    for (int k = mza - 1; k >= kb; --k) del[k] = del[k] - scr1[k + 1] * del[k + 1];
+    for (int k = kb; k <= mza; ++k) q(k, iw) = q0(k, iw) + del[k];
+  }
+
+  void stage_densities(const F2<R> &rho_o, int nrk) {
+    for (int n = 0; n < nrk; ++n) if (!rhos[n].n) rhos[n].alloc(mza, mwa);
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int k = 2; k <= mza; ++k) {  // scalar_transport_rk.F90:131-150
+        R rf = (R)rho(k, iw);
+        if (nrk == 3) {
+          rhos[0](k, iw) = R(2.) / R(3.) * rho_o(k, iw) + R(1.) / R(3.) * rf;
+          rhos[1](k, iw) = R(0.5) * (rho_o(k, iw) + rf);
+          rhos[2](k, iw) = rf;
+        } else {
+          rhos[0](k, iw) = R(0.5) * (rho_o(k, iw) + rf);
+          rhos[1](k, iw) = rf;
+        }
+      }
+  }
+
+  void stage_dts(R *dtlr, int nrk) const {
+    R dtl = (R)prm.dtlm;
+    const R one3 = R(1.) / R(3.);
+    if (nrk == 3) { dtlr[0] = dtl * one3; dtlr[1] = dtl * R(0.5); dtlr[2] = dtl * R(1.0); }
+    else { dtlr[0] = dtl * R(0.5); dtlr[1] = dtl * R(1.0); }
+  }
+
+  // scalar_transport_rk, scalar_transport_rk.F90:30-385 (no sgs mixing, no limiter)
+  void scalar_transport_rk(const float *rho_old_host) override {
+    if (!scp0.n) { scp0.alloc(mza, mwa); gxt.alloc(mza, mwa); gyt.alloc(mza, mwa); upv.alloc(mza, mva); }
+    F2<R> rho_o;
+    const F2<R> *ro = &rho_old;
+    if (rho_old_host) { load2(rho_o, rho_old_host, mza, mwa); ro = &rho_o; }
+    const int nrk = prm.nrk_scal;
+    R dtlr[3];
+    stage_dts(dtlr, nrk);
+    check_cfls(*ro, nrk);
+    stage_densities(*ro, nrk);
+    int nimp = 0;
+    for (int iw = 2; iw <= mwa; ++iw) {
+      bool any = false;
+      for (int n = 0; n < nrk; ++n) any = any || ffimp[(size_t)(iw - 1) * nrk + n] > R(0.001);
+      nimp += any;
+    }
+    cfl_out[3] = nimp;
+    for (size_t n = 0; n < scp.size(); ++n) {
+      F2<R> &q = scp[n];
+      scp0 = q;
+      for (int istage = 1; istage <= nrk; ++istage) {
+        update_scalar(q, scp0, sct[n], rhos[nrk - 1], dtlr[istage - 1], prm.scal_horiz_adv_order, istage, nrk);
+        if (nimp) {
+#pragma omp parallel for
+          for (int iw = 2; iw <= mwa; ++iw) implicit_column(q, scp0, iw, istage, dtlr[istage - 1], dtlr, nrk);
+        }
+      }
+    }
+    // tape for the adjoint: winds and stage quantities of this step
+    tape_vmasc = vmasc; tape_wmasc = wmasc; tape_ffimp = ffimp;
+    for (int n = 0; n < nrk; ++n) tape_rhos[n] = rhos[n];
+    have_tape = true;
+  }
+
+  // ---------------------------------------------------------------- adjoint
+  // Transpose of update_scalar (+ implicit column) for fixed winds.  Derivation:
+  // q_new = q0 + D (sct + volti F), F linear in (q, q0) with upwind choices frozen;
+  // see the formulae in the comments below.  lq = adjoint of the stage input q,
+  // lq0 accumulates the adjoint of scp0.
+  void update_scalar_adj(const F2<R> &mu_in, F2<R> &lq, F2<R> &lq0, const F2<R> &rhosf, R dtr,
+                         int iorder, int istage, int nrk, const R *dtlr) {
+    const F2<R> &vm = tape_vmasc, &wm = tape_wmasc;
+    F2<R> nu; nu.alloc(mza, mwa);
+    F2<R> lqpart; lqpart.alloc(mza, mwa);
+    F2<R> lgx; lgx.alloc(mza, mwa);
+    F2<R> lgy; lgy.alloc(mza, mwa);
+    // (1) implicit columns: mu -> lambda of the explicit result; (2) nu = D volti mu_e
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      int kb = 2;
+      R mu[130];
+      for (int k = 1; k <= mza; ++k) mu[k] = mu_in(k, iw);
+      R ff = tape_ffimp[(size_t)(iw - 1) * nrk + istage - 1];
+      bool imp = ff > R(0.001);
+      R fexpl = R(1) - ff;
+      // level 1 copy: explicit q(1) = q(kb); implicit branch leaves q(1) at the explicit value
+      R mu1 = mu[1];
+      mu[1] = 0;
+      if (imp) {
+        // u = q0 + T^-1 (e - q0 - c M q0); adjoint: le = T^-T mu, lq0 += mu - le - M^T(c le)
+        R wup[130], wdn[130], b1[130], b2a[130], b3a[130], le[130], c[130];
+        wup[mza] = 0; wdn[mza] = 0; wup[kb - 1] = 0; wdn[kb - 1] = 0;
+        for (int k = kb; k <= mza - 1; ++k) { wup[k] = std::max(wm(k, iw), R(0)); wdn[k] = std::min(wm(k, iw), R(0)); }
+        R c1 = ff * dtlr[istage - 1];
+        for (int k = kb; k <= mza; ++k) {
+          R c2 = c1 * volti(k, iw) / tape_rhos[istage - 1](k, iw);
+          b1[k] = -c2 * wup[k - 1]; b2a[k] = R(1) + c2 * (wup[k] - wdn[k - 1]); b3a[k] = c2 * wdn[k];
+          c[k] = ff * dtr * volti(k, iw) / tape_rhos[istage - 1](k, iw);
+        }
+        // transpose system: sub-diagonal of T^T is b3a(k-1), super-diagonal b1(k+1)
+        R scr[130], cj[130];
+        cj[kb] = R(1) / b2a[kb];
+        le[kb] = mu[kb] * cj[kb];
+        for (int k = kb + 1; k <= mza; ++k) {
+          scr[k] = cj[k - 1] * b1[k];
+          cj[k] = R(1) / (b2a[k] - b3a[k - 1] * scr[k]);
+          le[k] = cj[k] * (mu[k] - b3a[k - 1] * le[k - 1]);
+        }
+        for (int k = mza - 1; k >= kb; --k) le[k] = le[k] - scr[k + 1] * le[k + 1];
+        le[kb] = le[kb] + mu1;  // q(1) kept the explicit value q_e(kb)
+        for (int k = kb; k <= mza; ++k) lq0(k, iw) = lq0(k, iw) + mu[k] - le[k];
+        // - c(k) (min(wm(k),0) dq0(k) + max(wm(k-1),0) dq0(k-1)), dq0(k) = q0(k+1) - q0(k)
+        for (int k = kb; k <= mza; ++k) {
+          R a = -c[k] * le[k];
+          if (k <= mza - 1) { R t = a * std::min(wm(k, iw), R(0)); lq0(k + 1, iw) += t; lq0(k, iw) -= t; }
+          if (k - 1 >= kb) { R t = a * std::max(wm(k - 1, iw), R(0)); lq0(k, iw) += t; lq0(k - 1, iw) -= t; }
+        }
+        for (int k = kb; k <= mza; ++k) mu[k] = le[k];
+      } else {
+        mu[kb] = mu[kb] + mu1;
+      }
+      // explicit: q = q0 + dtr (sct + volti F)/rhos
+      R gsum[130];
+      for (int k = 1; k <= mza; ++k) gsum[k] = 0;
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw);
+        for (int k = 2; k <= mza; ++k) gsum[k] = gsum[k] + dirv(jv, iw) * vm(k, iv);
+      }
+      for (int k = kb; k <= mza; ++k) {
+        R v = mu[k] * dtr * volti(k, iw) / rhosf(k, iw);
+        nu(k, iw) = v;
+        R cc = gsum[k] + fexpl * (wm(k - 1, iw) - wm(k, iw));
+        lq0(k, iw) = lq0(k, iw) + mu[k] - v * cc;
+      }
+      // vertical part: F += fexpl (wm(k-1) upw(k-1) - wm(k) upw(k))
+      R lupw[130], lds[130];
+      for (int k = 0; k <= mza + 1 && k < 130; ++k) { lupw[k] = 0; lds[k] = 0; }
+      for (int k = kb; k <= mza; ++k) {
+        lupw[k - 1] = lupw[k - 1] + nu(k, iw) * fexpl * wm(k - 1, iw);
+        lupw[k] = lupw[k] - nu(k, iw) * fexpl * wm(k, iw);
+      }
+      R lqv[130];
+      for (int k = 1; k <= mza; ++k) lqv[k] = 0;
+      lqv[kb] = lqv[kb] + lupw[kb - 1];
+      lqv[mza] = lqv[mza] + lupw[mza];
+      for (int k = kb; k <= mza - 1; ++k) {
+        if (wm(k, iw) < R(0)) {  // upw(k) = scpb(k+1)
+          R l = lupw[k];
+          lqv[k + 1] += l;
+          lds[k + 1] += l * aw_(k + 1, 1, 1);
+          lds[k] += l * aw_(k + 1, 2, 1);
+        } else {                 // upw(k) = scpt(k)
+          R l = lupw[k];
+          lqv[k] += l;
+          lds[k] += l * aw_(k, 1, 2);
+          lds[k - 1] += l * aw_(k, 2, 2);
+        }
+      }
+      for (int k = kb; k <= mza - 1; ++k) { lqv[k + 1] += lds[k]; lqv[k] -= lds[k]; }
+      for (int k = 1; k <= mza; ++k) lqpart(k, iw) = lqv[k];
+    }
+    // (3) face adjoint fluxes phi = vm (nu(w2) - nu(w1)) routed to the upwind side
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw);
+        int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+        int side = (iw == iw1) ? 1 : 2;
+        for (int k = 2; k <= mza; ++k) {
+          R phi = vm(k, iv) * (nu(k, iw2) - nu(k, iw1));
+          bool up2 = vm(k, iv) < R(0);
+          if ((side == 2) == up2) {
+            lqpart(k, iw) += phi;
+            if (iorder > 1) {
+              lgx(k, iw) += dxps(side, iv) * phi;
+              lgy(k, iw) += dyps(side, iv) * phi;
+            }
+          }
+        }
+      }
+    }
+    // (4) gradient transpose: gx(c) = sum_n gxc(n,c) (q(n) - q(c))
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      for (int k = 1; k <= mza; ++k) lq(k, iw) = lqpart(k, iw);
+      if (iorder > 1) {
+        for (int n = 1; n <= npoly[iw]; ++n) {
+          int iwn = w_iw(n, iw);
+          // slot of iw in iwn's list
+          int back = 0;
+          for (int m = 1; m <= npoly[iwn]; ++m) if (w_iw(m, iwn) == iw) back = m;
+          for (int k = 2; k <= mza; ++k) {
+            lq(k, iw) += gxc(back, iwn) * lgx(k, iwn) + gyc(back, iwn) * lgy(k, iwn);
+            lq(k, iw) -= gxc(n, iw) * lgx(k, iw) + gyc(n, iw) * lgy(k, iw);
+          }
+        }
+      }
+    }
+  }
+
+  void scalar_transport_rk_adj() override {
+    if (!have_tape) { err = "adjoint called before a forward tracer step"; throw 1; }
+    const int nrk = prm.nrk_scal;
+    R dtlr[3];
+    stage_dts(dtlr, nrk);
+    F2<R> lq, lq0, mu;
+    lq.alloc(mza, mwa); lq0.alloc(mza, mwa);
+    for (size_t n = 0; n < lam.size(); ++n) {
+      mu = lam[n];
+      std::fill(lq0.v.begin(), lq0.v.end(), R(0));
+      for (int istage = nrk; istage >= 1; --istage) {
+        std::fill(lq.v.begin(), lq.v.end(), R(0));
+        update_scalar_adj(mu, lq, lq0, tape_rhos[nrk - 1], dtlr[istage - 1], prm.scal_horiz_adv_order,
+                          istage, nrk, dtlr);
+        mu = lq;
+      }
+      // stage 1 read q = q0: both adjoints land on the initial tracer
+      for (size_t i = 0; i < lq0.v.size(); ++i) lam[n].v[i] = lq0.v[i] + mu.v[i];
+    }
+  }
+
+  // ---------------------------------------------------------------- mass sums
+  // compute_mass_sums, check_nans.F90:207-262 (FP64 accumulation)
+  void mass_sums(double *out) override {
+    double dry = 0;
+    std::vector<double> tr(scp.size(), 0.0);
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int k = 2; k <= mza; ++k) {
+        double m = (double)rho(k, iw) * volt(k, iw);
+        dry += m;
+        for (size_t n = 0; n < scp.size(); ++n) tr[n] += m * (double)scp[n](k, iw);
+      }
+    out[0] = dry;
+    for (size_t n = 0; n < scp.size(); ++n) out[1 + n] = tr[n];
+  }
+  void cfl_report(double *o) override { for (int i = 0; i < 4; ++i) o[i] = cfl_out[i]; }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------- C API
+extern "C" {
+
+struct oracle_ctx { Base *m; std::string err; };
+
+int oracle_create(const olamgpu_mesh *mesh, const olamgpu_params *p, int32_t ref_types, void **out) {
+  auto *c = new oracle_ctx;
+  if (mesh->mza > 128) { delete c; return OLAMGPU_EINVAL; }
+  if (ref_types) c->m = new Model<Prec<float, double, float>>(mesh, p);
+  else if (p->precision == OLAMGPU_PREC_F64) c->m = new Model<Prec<double, double, double>>(mesh, p);
+  else if (p->precision == OLAMGPU_PREC_MIXED) c->m = new Model<Prec<float, double, double>>(mesh, p);
+  else c->m = new Model<Prec<float, float, double>>(mesh, p);
+  *out = c;
+  return 0;
+}
+int oracle_destroy(void *h) { auto *c = (oracle_ctx *)h; delete c->m; delete c; return 0; }
+int oracle_upload_state(void *h, const olamgpu_state *s) { return ((oracle_ctx *)h)->m->upload_state(s); }
+int oracle_download_state(void *h, olamgpu_state *s) { return ((oracle_ctx *)h)->m->download_state(s); }
+int oracle_upload_scalar(void *h, int32_t n, const float *p, const float *t) { return ((oracle_ctx *)h)->m->upload_scalar(n, p, t); }
+int oracle_download_scalar(void *h, int32_t n, float *p) { return ((oracle_ctx *)h)->m->download_scalar(n, p); }
+int oracle_upload_scalar_adj(void *h, int32_t n, const float *p) { return ((oracle_ctx *)h)->m->upload_adj(n, p); }
+int oracle_download_scalar_adj(void *h, int32_t n, float *p) { return ((oracle_ctx *)h)->m->download_adj(n, p); }
+int oracle_tend0(void *h) { ((oracle_ctx *)h)->m->tend0(); return 0; }
+int oracle_prog_wrtv_rk(void *h) { ((oracle_ctx *)h)->m->prog_wrtv_rk(); return 0; }
+int oracle_timeavg_momsc(void *h) { ((oracle_ctx *)h)->m->timeavg_momsc(); return 0; }
+int oracle_scalar_transport_rk(void *h, const float *rho_old) { ((oracle_ctx *)h)->m->scalar_transport_rk(rho_old); return 0; }
+int oracle_scalar_transport_rk_adj(void *h) {
+  try { ((oracle_ctx *)h)->m->scalar_transport_rk_adj(); } catch (...) { return OLAMGPU_ESTATE; }
+  return 0;
+}
+int oracle_mass_sums(void *h, double *out) { ((oracle_ctx *)h)->m->mass_sums(out); return 0; }
+int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(out); return 0; }
+int oracle_set_threads(int32_t n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
+
+}  // extern "C"

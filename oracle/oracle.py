@@ -1,0 +1,135 @@
+"""ctypes loader of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module; the product path (olam_amd.gpu) never imports it.  The oracle takes the same
+olamgpu_mesh / olamgpu_params / olamgpu_state descriptors as the product C-ABI, so a
+test can feed identical inputs to both.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.oracle_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]
+        for name in ("oracle_destroy", "oracle_tend0", "oracle_prog_wrtv_rk", "oracle_timeavg_momsc",
+                     "oracle_scalar_transport_rk_adj"):
+            getattr(L, name).argtypes = [vp]
+        L.oracle_upload_state.argtypes = [vp, vp]
+        L.oracle_download_state.argtypes = [vp, vp]
+        L.oracle_upload_scalar.argtypes = [vp, C.c_int32, vp, vp]
+        L.oracle_download_scalar.argtypes = [vp, C.c_int32, vp]
+        L.oracle_upload_scalar_adj.argtypes = [vp, C.c_int32, vp]
+        L.oracle_download_scalar_adj.argtypes = [vp, C.c_int32, vp]
+        L.oracle_scalar_transport_rk.argtypes = [vp, vp]
+        L.oracle_mass_sums.argtypes = [vp, vp]
+        L.oracle_cfl_report.argtypes = [vp, vp]
+        L.oracle_set_threads.argtypes = [C.c_int32]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Same call surface as olam_amd.gpu.Context, backed by the CPU restatement."""
+
+    def __init__(self, grid, params, ref_types=False):
+        self.grid = grid
+        self.params = params
+        self.h = C.c_void_p()
+        rc = lib().oracle_create(C.byref(grid.desc), C.byref(params), int(ref_types), C.byref(self.h))
+        if rc:
+            raise RuntimeError(f"oracle_create failed ({rc})")
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def upload_state(self, st):
+        d = st.desc()
+        lib().oracle_upload_state(self.h, C.byref(d))
+
+    def download_state(self, st):
+        d = st.desc()
+        lib().oracle_download_state(self.h, C.byref(d))
+
+    def upload_scalar(self, n, q, t=None):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        t = None if t is None else np.ascontiguousarray(t, dtype=np.float32)
+        rc = lib().oracle_upload_scalar(self.h, n, _p(q), _p(t))
+        assert rc == 0
+
+    def download_scalar(self, n):
+        out = np.zeros((self.grid.mwa, self.grid.mza), np.float32)
+        lib().oracle_download_scalar(self.h, n, _p(out))
+        return out
+
+    def upload_scalar_adj(self, n, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        lib().oracle_upload_scalar_adj(self.h, n, _p(lam))
+
+    def download_scalar_adj(self, n):
+        out = np.zeros((self.grid.mwa, self.grid.mza), np.float32)
+        lib().oracle_download_scalar_adj(self.h, n, _p(out))
+        return out
+
+    def tend0(self):
+        lib().oracle_tend0(self.h)
+
+    def prog_wrtv_rk(self):
+        lib().oracle_prog_wrtv_rk(self.h)
+
+    def timeavg_momsc(self):
+        lib().oracle_timeavg_momsc(self.h)
+
+    def scalar_transport_rk(self, rho_old=None):
+        r = None if rho_old is None else np.ascontiguousarray(rho_old, dtype=np.float32)
+        lib().oracle_scalar_transport_rk(self.h, _p(r))
+
+    def scalar_transport_rk_adj(self):
+        rc = lib().oracle_scalar_transport_rk_adj(self.h)
+        if rc:
+            raise RuntimeError("oracle adjoint failed")
+
+    def mass_sums(self):
+        out = np.zeros(1 + self.params.num_scalar)
+        lib().oracle_mass_sums(self.h, _p(out))
+        return out
+
+    def cfl_report(self):
+        out = np.zeros(4)
+        lib().oracle_cfl_report(self.h, _p(out))
+        return out
+
+
+def set_threads(n):
+    return lib().oracle_set_threads(int(n))

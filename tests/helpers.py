@@ -1,0 +1,123 @@
+"""Shared case builders and error norms for the parity tests."""
+
+import numpy as np
+
+from olam_amd import abi, cases
+
+
+def tracer_case(grid, dt=600.0, ntr=3, seed=1, u0=40.0, noise=1.0, wnoise=0.01):
+    """Prescribed winds + [constant, cosine bell, log-normal, seeded ...] tracers."""
+    w = cases.transport_winds(grid, dt, u0=u0, noise=noise, wnoise=wnoise, seed=seed)
+    st = abi.FortranState(grid)
+    st.arrays["vmasc"][:] = w["vmasc"]
+    st.arrays["wmasc"][:] = w["wmasc"]
+    st.arrays["rho"][:] = w["rho"]
+    tr = [np.full((grid.mwa, grid.mza), 1.0, np.float32), cases.cosine_bell(grid)]
+    tr += cases.lognormal_tracers(grid, 1, seed=seed)
+    k = 0
+    while len(tr) < ntr:
+        tr.append(cases.seeded_field(grid, seed, 500 + k))
+        k += 1
+    return w, st, tr[:ntr]
+
+
+def load_tracer_case(be, w, st, tr):
+    be.upload_state(st)
+    for n, q in enumerate(tr, start=1):
+        be.upload_scalar(n, q)
+
+
+def rel_linf(a, b):
+    """max |a - b| over real points and levels 2..mza, relative to max |b|."""
+    a = np.asarray(a, dtype=np.float64)[1:, 1:]
+    b = np.asarray(b, dtype=np.float64)[1:, 1:]
+    den = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / den)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)[1:, 1:]
+    b = np.asarray(b, dtype=np.float64)[1:, 1:]
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+DYN_FIELDS = ["thil", "rho", "press", "wmc", "wc", "vmc", "vc", "vxe", "vye", "vze", "vmasc", "wmasc"]
+
+
+def dyn_setup(be, grid, jet):
+    be.upload_state(jet)
+    be.init_wrtv_rk() if hasattr(be, "init_wrtv_rk") else None
+    be.tend0()
+
+
+def dyn_fields(be, grid):
+    st = abi.FortranState(grid)
+    be.download_state(st)
+    return st.arrays
+
+
+class PermutedGrid:
+    """The same grid with W, V and M points relabelled by seeded permutations.
+
+    Only labels change: every point keeps its neighbour order and every coefficient
+    keeps its bits, so any correct implementation must give bit-identical results
+    (mapped back through the permutation)."""
+
+    W_ROWS = ["w_npoly", "w_iv", "w_iw", "w_im", "w_dirv", "w_ecvec_vx", "w_ecvec_vy", "w_ecvec_vz",
+              "gxps_coef", "gyps_coef", "xx_yy", "wnx", "wny", "wnz", "arw0", "lpw",
+              "arw", "volti", "volwi", "volt"]
+    V_ROWS = ["v_iw", "v_im", "v_dxps", "v_dyps", "vnx", "vny", "vnz", "dnu", "dniu", "dnv", "dniv",
+              "lpv", "arv", "volvi"]
+    M_ROWS = ["m_iv", "m_iw", "m_im", "xx_yy_m", "arm0", "lpm"]
+
+    def __init__(self, grid, seed=0):
+        rng = np.random.default_rng(seed)
+        self.base = grid
+        self.mza, self.mwa, self.mva, self.mma = grid.mza, grid.mwa, grid.mva, grid.mma
+        nw, nv, nm = grid.mwa - 1, grid.mva - 1, grid.mma - 1
+        self.pw, self.pv, self.pm = rng.permutation(nw), rng.permutation(nv), rng.permutation(nm)
+        src = grid.arrays
+        out = {k: v.copy() for k, v in src.items()}
+
+        def rows(a, p):
+            b = a.copy()
+            b[p + 1] = a[1:]
+            return b
+
+        for k in self.W_ROWS:
+            out[k] = rows(src[k], self.pw)
+        for k in self.V_ROWS:
+            out[k] = rows(src[k], self.pv)
+        for k in self.M_ROWS:
+            out[k] = rows(src[k], self.pm)
+
+        def relabel(a, p):
+            b = a.copy()
+            m = b >= 2
+            b[m] = p[b[m] - 2] + 2
+            return b
+
+        for k in ("w_iw", "v_iw", "m_iw"):
+            out[k] = relabel(out[k], self.pw)
+        for k in ("w_iv", "m_iv"):
+            out[k] = relabel(out[k], self.pv)
+        for k in ("w_im", "v_im", "m_im"):
+            out[k] = relabel(out[k], self.pm)
+        self.arrays = out
+        d = abi.MeshDesc()
+        d.mza, d.mwa, d.mva, d.mma = self.mza, self.mwa, self.mva, self.mma
+        for fname, ftype in abi.MeshDesc._fields_[4:]:
+            setattr(d, fname, out[fname].ctypes.data_as(abi.C.POINTER(ftype._type_)))
+        self.desc = d
+
+    def fwd(self, a, kind="w"):
+        p = self.pw if kind == "w" else self.pv
+        b = a.copy()
+        b[p + 1] = a[1:]
+        return b
+
+    def back(self, a, kind="w"):
+        p = self.pw if kind == "w" else self.pv
+        b = a.copy()
+        b[1:] = a[p + 1]
+        return b
