@@ -1,0 +1,212 @@
+"""Benchmark of the RK3 dynamical core on one MI355X.
+
+One step = one long step of the hot path with nacoust = 1 (so dtsm = dtlm):
+  tend0 -> prog_wrtv_rk (acoustic RK3: W/V momentum, rho, rho*thil, press, damping,
+  vel_t3d) -> timeavg_momsc -> scalar_transport_rk (RK3, all tracers) ->
+  scalar_transport_rk_adj (adjoint of that tracer step, all tracers).
+Workload: global icosahedral-hexagonal mesh NXP=320 (1,024,002 columns), 60 levels,
+10 tracers with log-normal mixing ratios, dt = 30 s, synthetic jet state (no
+datasets).  Metric: cell-updates/s = columns x levels / seconds per step.
+
+Prints one JSON line.  --gpus N > 1 exits with a message: this build targets one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
+
+
+def alg_bytes(ntr, prec):
+    """Algorithmic bytes per cell-level per RK3 step (dynamics, tracer fwd, tracer adj).
+
+    Dynamics 772 B (FP32 bulk) or 856 B (FP64 rho/press); tracers 120 + 48 n each for
+    the forward and the adjoint (per-stage shared 24 B + 16 B per tracer, 3 stages,
+    plus 48 B of setup); all-FP64 doubles everything."""
+    dyn = 856 if prec == "mixed" else 772
+    tr = 120 + 48 * ntr
+    if prec == "f64":
+        return 2 * dyn, 2 * tr, 2 * tr
+    return dyn, tr, tr
+
+
+def kernel_alg_bytes(name, ntr, prec):
+    """Algorithmic bytes per cell-level for one launch of the named kernel."""
+    w = 8 if prec == "f64" else 4
+    if name == "k_tr_update":          # per stage: rhos, volti, wmasc, 3 vmasc + 4 per tracer
+        return (6 + 4 * ntr) * w
+    if name == "k_dyn_w":              # P1 of the dynamics stage
+        return 132 * w // 4 + (20 if prec == "mixed" else 0)
+    if name == "k_dyn_v":              # P2
+        return 92 * w // 4 + (8 if prec == "mixed" else 0)
+    if name == "k_dyn_vel":            # P3
+        return 28 * w // 4
+    return None
+
+
+def cpu_baseline(nxp, nlev, ntr, prec, dt, threads, steps=2):
+    """Oracle (CPU restatement, OpenMP) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from olam_amd import abi, cases, mesh
+    m = mesh.build_mesh(nxp)
+    grid = abi.FortranGrid(m, mesh.vertical_grid(nlev))
+    jet = cases.jet_state(grid)
+    tr = cases.lognormal_tracers(grid, ntr)
+    used = O.set_threads(threads)
+    prm = abi.make_params(prec, num_scalar=ntr, dtsm=dt)
+    o = O.Oracle(grid, prm)
+    o.upload_state(jet)
+    for n in range(ntr):
+        o.upload_scalar(n + 1, tr[n])
+        o.upload_scalar_adj(n + 1, tr[n])
+
+    def step():
+        o.tend0()
+        o.prog_wrtv_rk()
+        o.timeavg_momsc()
+        o.scalar_transport_rk()
+        o.scalar_transport_rk_adj()
+
+    step()                                   # warm-up (allocates scratch)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sec = (time.perf_counter() - t0) / steps
+    cells = m.nw * nlev
+    return {"value": cells / sec, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": f"oracle (C++ restatement, OpenMP) NXP={nxp} ({m.nw} columns), L={nlev}, {ntr} tracers, "
+                      f"{prec}, 1 warm-up + {steps} timed steps of dynamics+tracer fwd+adjoint; "
+                      f"{sec:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nxp", type=int, default=320)
+    ap.add_argument("--nlev", type=int, default=60)
+    ap.add_argument("--ntr", type=int, default=10)
+    ap.add_argument("--dt", type=float, default=30.0)
+    ap.add_argument("--prec", default="f32", choices=["f32", "mixed", "f64"])
+    ap.add_argument("--cpu-nxp", type=int, default=160)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy", action="store_true", help="skip the achievable-copy-bandwidth probe")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU decomposition is described, not built)"}))
+        sys.exit(2)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    from olam_amd import abi, cases, mesh
+    from olam_amd.build import COPY
+    from olam_amd.gpu import Context
+
+    copy_bw = None
+    if not args.no_copy and os.path.exists(COPY):
+        # achievable HBM copy bandwidth, measured by a child process before this one
+        # touches the GPU
+        import subprocess
+        r = subprocess.run([COPY, str(1 << 30), "10"], capture_output=True, text=True, timeout=120)
+        if r.returncode == 0:
+            copy_bw = json.loads(r.stdout.strip().splitlines()[-1])["copy_GBps"]
+            log(f"[bench] achievable copy bandwidth {copy_bw:.0f} GB/s")
+
+    t = time.perf_counter()
+    m = mesh.build_mesh(args.nxp)
+    vg = mesh.vertical_grid(args.nlev)
+    grid = abi.FortranGrid(m, vg)
+    log(f"[bench] mesh NXP={args.nxp}: {m.nw} columns, {m.nv} faces, {time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    jet = cases.jet_state(grid)
+    tr = cases.lognormal_tracers(grid, args.ntr)
+    log(f"[bench] synthetic state + {args.ntr} tracers: {time.perf_counter() - t:.1f}s")
+    prm = abi.make_params(args.prec, num_scalar=args.ntr, dtsm=args.dt)
+    t = time.perf_counter()
+    ctx = Context(grid, prm, device=0)
+    ctx.upload_state(jet)
+    for n in range(args.ntr):
+        ctx.upload_scalar(n + 1, tr[n])
+        ctx.upload_scalar_adj(n + 1, tr[n])
+    log(f"[bench] device setup (renumbering, checks, uploads): {time.perf_counter() - t:.1f}s")
+
+    def step():
+        ctx.tend0()
+        ctx.prog_wrtv_rk()
+        ctx.timeavg_momsc()
+        ctx.scalar_transport_rk()
+        ctx.scalar_transport_rk_adj()
+
+    for _ in range(args.warmup):
+        step()
+    log("[bench] warm-up done")
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()          # every C-ABI call returns after synchronising its stream
+    sec = (time.perf_counter() - t0) / args.steps
+    prof = ctx.kernel_profile()
+    ctx.set_profiling(False)
+    cells = m.nw * args.nlev
+    value = cells / sec
+    log(f"[bench] {sec * 1e3:.2f} ms/step, {value:.3e} cell-updates/s")
+    kern_ms = {k: v[1] / args.steps for k, v in prof.items()}
+    for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        log(f"[bench]   {k:16s} {n // args.steps:3d} launches/step  {ms / args.steps:8.3f} ms/step  "
+            f"{ms / n:8.3f} ms/launch")
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    n_launch, tot_ms = prof[dom]
+    per_launch_s = tot_ms / n_launch * 1e-3
+    ab = kernel_alg_bytes(dom, args.ntr, args.prec)
+    roof = None
+    if ab is not None:
+        ach = ab * cells / per_launch_s
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "traffic": None,
+                "frac_of_copy": (ach / (copy_bw * 1e9)) if copy_bw else None,
+                "alg_bytes_per_launch": ab * cells, "avg_launch_ms": per_launch_s * 1e3}
+    bd, bf, ba = alg_bytes(args.ntr, args.prec)
+    step_frac = (bd + bf + ba) * cells / sec / PEAK_HBM
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cpu = cpu_baseline(args.cpu_nxp, args.nlev, args.ntr, args.prec, args.dt, args.cpu_threads)
+        log(f"[bench] CPU {cpu['value']:.3e} cell-updates/s on {cpu['cores']} threads")
+    out = {
+        "metric": "cell-updates/s per RK3 step (fwd+adjoint) at NXP=320, L=60; % HBM roofline",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"f32": "fp32 (fp64 column solve and mass sums)", "mixed": "fp32 + fp64 rho/press",
+                  "f64": "fp64"}[args.prec],
+        "data": "synthetic (seeded jet state, log-normal tracers)",
+        "config": {"workload": f"prog_wrtv_rk + scalar_transport_rk + adjoint, NXP={args.nxp} "
+                               f"({m.nw} columns), L={args.nlev}, {args.ntr} tracers, dt={args.dt:g}s, RK3",
+                   "columns": m.nw, "levels": args.nlev, "tracers": args.ntr, "precision": args.prec,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"alg_bytes_per_cell_level": bd + bf + ba, "frac": step_frac,
+                          "achieved_GBps": (bd + bf + ba) * cells / sec / 1e9,
+                          "copy_GBps": copy_bw,
+                          "frac_of_copy": ((bd + bf + ba) * cells / sec / (copy_bw * 1e9)) if copy_bw else None},
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in kern_ms.items()},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -25,8 +25,9 @@
  *    the ABI.
  *  - Calls are ordered on the context's HIP stream and return after the stream is
  *    synchronised.  One host thread per context.
- *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere) and mza <= 128 are supported;
- *    create() rejects anything else with OLAMGPU_EINVAL.
+ *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere), mza <= 64, horizontal
+ *    advection orders 1 and 2 and iscal_monot = 0 are supported; create() rejects
+ *    anything else with OLAMGPU_EINVAL.
  */
 #ifndef OLAMGPU_H
 #define OLAMGPU_H
@@ -95,7 +96,7 @@ typedef struct olamgpu_params {
   int32_t thil_horiz_adv_order;      /* 1 or 2 */
   int32_t wind_horiz_adv_order;      /* 1 or 2 */
   int32_t scal_horiz_adv_order;      /* 1 or 2 */
-  int32_t iscal_monot;               /* 0 = none, 1 = monotone final stage */
+  int32_t iscal_monot;               /* must be 0 (the limiters are not built) */
   int32_t divh_damp_level;           /* 0 disables; 1..3 */
   int32_t vort_damp_level;           /* 0 disables; 1..3 */
   float   divh_damp_fact;
@@ -212,6 +213,14 @@ int olamgpu_copy_field(olamgpu_ctx *ctx, int32_t field_id, int32_t index,
 /* Synchronise the context's stream and report the time of the last call's
  * kernels in milliseconds (HIP events around the call). */
 int olamgpu_last_call_ms(olamgpu_ctx *ctx, double *ms);
+
+/* ---- kernel timing (no counterpart in the reference) ------------------------ */
+
+/* on != 0: record HIP events around every kernel launch on the context's stream
+ * and accumulate per-kernel counts and times; resets the counters either way. */
+int olamgpu_set_profiling(olamgpu_ctx *ctx, int32_t on);
+/* Writes one line "kernel_name launches total_ms" per kernel into text[len]. */
+int olamgpu_kernel_profile(olamgpu_ctx *ctx, char *text, int32_t len);
 
 #ifdef __cplusplus
 }

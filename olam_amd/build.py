@@ -41,6 +41,20 @@ def build_hip(verbose=False):
     return LIB
 
 
+COPY = os.path.join(LIBDIR, "hbm_copy")
+
+
+def build_copy(verbose=False):
+    """The plain-copy program used for bandwidth and counter calibration."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-o", COPY, os.path.join(CSRC, "hbm_copy.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return COPY
+
+
 def build_oracle():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     return os.path.join(ROOT, "oracle", "build", "liboracle.so")
@@ -73,6 +87,7 @@ def build_fortran():
 
 def build_all(verbose=False):
     build_hip(verbose)
+    build_copy(verbose)
     build_oracle()
     if os.path.exists(os.path.join(FORTRAN, "olamgpu_iface.f90")):
         build_fortran()

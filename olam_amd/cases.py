@@ -40,16 +40,14 @@ def _splitmix(x):
 
 
 def counter_normal(seed, stream, ids, nlev):
-    """N(0,1) draws of shape (len(ids), nlev) keyed by (seed, stream, id, level)."""
-    with np.errstate(over="ignore"):
-        ids = np.asarray(ids, dtype=np.uint64)[:, None]
-        lev = np.arange(nlev, dtype=np.uint64)[None, :]
-        base = np.uint64((seed * 1000003 + stream * 7919) & _MASK64)
-        key = _splitmix(base ^ (ids * np.uint64(0x100000001B3)) ^ (lev * np.uint64(0xC2B2AE3D27D4EB4F)))
-        k2 = _splitmix(key ^ np.uint64(0xD6E8FEB86659FD93))
-    u1 = ((key >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
-    u2 = ((k2 >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
-    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+    """N(0,1) draws of shape (len(ids), nlev) keyed by (seed, stream) and the canonical
+    point id: draws are made for canonical ids 0..max(ids) in order (Philox stream)
+    and then picked by id, so they do not depend on how points are numbered."""
+    ids = np.asarray(ids, dtype=np.int64)
+    key = (seed * 1000003 + stream * 7919) & _MASK64
+    gen = np.random.Generator(np.random.Philox(key=key))
+    z = gen.standard_normal((int(ids.max()) + 1, nlev))
+    return z[ids]
 
 
 def _latlon_w(mesh):
@@ -92,11 +90,12 @@ def transport_winds(grid: FortranGrid, dt: float, u0: float = 40.0, noise: float
     rho_old = np.zeros((mza + 1, nw))
     # rho_old is FP32 in the reference (timestep.f90:38); build rho from its FP32 value
     rho_old[2:] = rho_prof[2:, None].astype(np.float32).astype(np.float64)
-    div = np.zeros((mza + 1, nw))
+    vT = vmasc.T
+    ivs = np.where(np.arange(7)[None, :] < mesh.npoly[:, None], mesh.w_iv, 0)
+    divT = np.zeros((nw, mza + 1))
     for j in range(7):
-        act = j < mesh.npoly
-        iv = mesh.w_iv[act, j]
-        div[:, act] += mesh.dirv[act, j][None, :] * vmasc[:, iv]
+        divT += mesh.dirv[:, j, None] * vT[ivs[:, j]]
+    div = divT.T.copy()
     div[2:] += wmasc[1:mza] - wmasc[2:mza + 1]
     rho = np.zeros_like(rho_old)
     rho[2:] = rho_old[2:] + dt / cm["volt"][2:] * div[2:]
@@ -179,7 +178,7 @@ def jet_state(grid: FortranGrid, seed: int = 3, u_jet: float = 35.0, bump: float
     for k in range(2, mza):
         pk, rk = pr[k], rh[k]
         pn = pk * np.exp(-dzm[k] * M.GRAV / (M.RDRY * 250.0))
-        for _ in range(30):
+        for _ in range(8):
             rn = rho_of(pn, theta[k + 1])
             f = pk - pn - dzm[k] * M.GRAV * (zb[k] * rk + ztp[k] * rn)
             dfdp = -1.0 - dzm[k] * M.GRAV * ztp[k] * rn / (M.CPOCV * pn)
@@ -204,14 +203,16 @@ def jet_state(grid: FortranGrid, seed: int = 3, u_jet: float = 35.0, bump: float
         u = u_jet * np.sin(2.0 * latv) ** 2 * F
         vc[k] = (u * en).astype(np.float32)
         vmc[k] = vc[k] * 0.5 * (rh[k, iw1] + rh[k, iw2])
-    # cell velocities from vc (vel_t3d_hex with wc = 0)
-    vxe = np.zeros((mza + 1, nw)); vye = np.zeros_like(vxe); vze = np.zeros_like(vxe)
+    # cell velocities from vc (vel_t3d_hex with wc = 0); padded slots have zero weight
+    vcT = vc.T                                                    # (nv, mza+1)
+    ivs = np.where(np.arange(7)[None, :] < mesh.npoly[:, None], mesh.w_iv, 0)
+    vxe = np.zeros((nw, mza + 1)); vye = np.zeros_like(vxe); vze = np.zeros_like(vxe)
     for j in range(7):
-        act = j < mesh.npoly
-        iv = mesh.w_iv[act, j]
-        vxe[:, act] += g["ecvec_vx"][act, j][None, :] * vc[:, iv]
-        vye[:, act] += g["ecvec_vy"][act, j][None, :] * vc[:, iv]
-        vze[:, act] += g["ecvec_vz"][act, j][None, :] * vc[:, iv]
+        col = vcT[ivs[:, j]]
+        vxe += g["ecvec_vx"][:, j, None] * col
+        vye += g["ecvec_vy"][:, j, None] * col
+        vze += g["ecvec_vz"][:, j, None] * col
+    vxe, vye, vze = vxe.T.copy(), vye.T.copy(), vze.T.copy()
     for a in (vxe, vye, vze, theta):
         a[1] = a[2]
     st = FortranState(grid)

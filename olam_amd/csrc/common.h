@@ -53,6 +53,21 @@ __device__ __forceinline__ int wave_item() {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Work item of a tracer kernel: column c and the tracer range [n0, n1) handled by this
+// wave.  The tpw tracers of one group share a wave; the groups of one column are
+// consecutive items, so they run in the same block and share the column's gathers.
+struct TrItem { int c, n0, n1; };
+__device__ __forceinline__ TrItem tr_item(int nw, int ntr, int tpw) {
+  const int ng = (ntr + tpw - 1) / tpw;
+  const int it = wave_item();
+  TrItem t;
+  t.c = it / ng;
+  t.n0 = (it - t.c * ng) * tpw;
+  t.n1 = min(ntr, t.n0 + tpw);
+  if (t.c >= nw) t.c = nw;
+  return t;
+}
+
 template <class T>
 __device__ __forceinline__ T shdn(T v, int d) { return __shfl_down(v, d, WAVE); }
 template <class T>
@@ -83,3 +98,13 @@ __device__ __forceinline__ double pcr_solve(double a, double b, double c, double
 
 template <class T>
 __device__ __forceinline__ T ld(const T *p, long i, bool ok) { return ok ? p[i] : T(0); }
+
+// Neighbour slots: slots 0..5 are always processed, so every gather of a column is
+// issued without a branch in between (pentagons pad slot 5 with the column itself and
+// zero weights, which adds exact zeros); slot 6 exists only for heptagons.
+template <class F>
+__device__ __forceinline__ void for_slots(int np, F &&f) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) f(j);
+  if (np > 6) f(6);
+}

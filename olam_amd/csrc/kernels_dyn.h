@@ -10,6 +10,11 @@
 #pragma once
 #include "common.h"
 
+// All dynamics kernels evaluate their expressions exactly as written (no FMA
+// contraction): the W-momentum balance subtracts nearly equal pressure-gradient and
+// buoyancy terms, so every ulp upstream of it shows in w.  The oracle does the same.
+#define NOCONTRACT _Pragma("clang fp contract(off)")
+
 template <class R, class RP>
 struct DynBuf {
   // prognostic state
@@ -36,6 +41,7 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, 
                                                      const R *thilt, const R *vmxet, const R *vmyet,
                                                      const R *vmzet, RP *rth0, RP *rho0, R *wmc0, R *thilt_s,
                                                      R *vmxet_s, R *vmyet_s, R *vmzet_s) {
+  NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1;
@@ -57,6 +63,7 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, 
 // call left there (the last Laplacian for level 3); reproduced as is.
 template <class R>
 __global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, R *div) {
+  NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1;
@@ -64,16 +71,17 @@ __global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, R *div) {
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
   R d = div[ic];
-  for (int j = 0; j < np; ++j) {
+  for_slots(np, [&](int j) {
     const long iv = (long)m.w_face[c * NSLOT + j] * ldk + l;
     d = d - m.arv[iv] * m.w_dirv[c * NSLOT + j] * vmc[iv];
-  }
+  });
   div[ic] = d * m.volti[ic];
 }
 
 // laplacian2d, grad_lib.f90:494-524
 template <class R>
 __global__ __launch_bounds__(256) void k_lap_w(DMesh<R> m, const R *src, R *dst) {
+  NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1;
@@ -83,8 +91,7 @@ __global__ __launch_bounds__(256) void k_lap_w(DMesh<R> m, const R *src, R *dst)
   R d = R(0);
   if (k >= 2) {
     const R s0 = src[ic];
-    for (int j = 0; j < np; ++j)
-      d = d + (src[(long)m.w_nbr[c * NSLOT + j] * ldk + l] - s0) * m.w_xxyy[c * NSLOT + j];
+    for_slots(np, [&](int j) { d = d + (src[(long)m.w_nbr[c * NSLOT + j] * ldk + l] - s0) * m.w_xxyy[c * NSLOT + j]; });
   }
   dst[ic] = d;
 }
@@ -92,6 +99,7 @@ __global__ __launch_bounds__(256) void k_lap_w(DMesh<R> m, const R *src, R *dst)
 // vort_damp vorticity at M points, vort_div_damp.f90:304-336 (flat terrain)
 template <class R>
 __global__ __launch_bounds__(256) void k_vort(DMesh<R> m, const R *vc, R *vort) {
+  NOCONTRACT
   const int im = wave_item();
   if (im >= m.nm) return;
   const int l = lane_id(), k = l + 1;
@@ -106,6 +114,7 @@ __global__ __launch_bounds__(256) void k_vort(DMesh<R> m, const R *vc, R *vort) 
 // laplacian_m, grad_lib.f90:531-559
 template <class R>
 __global__ __launch_bounds__(256) void k_lap_m(DMesh<R> m, const R *src, R *dst) {
+  NOCONTRACT
   const int im = wave_item();
   if (im >= m.nm) return;
   const int l = lane_id(), k = l + 1;
@@ -126,6 +135,7 @@ template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, const R *vmt, R *vmc0, R *vmt_s,
                                                      const R *div, const R *fdivh, const R *vort,
                                                      const R *fvorth, const RP *rho) {
+  NOCONTRACT
   const int iv = wave_item();
   if (iv >= m.nv) return;
   const int l = lane_id(), k = l + 1;
@@ -151,6 +161,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, const R *vxe, const R *vye,
                                                   const R *vze, R *g0, R *g1, R *g2, R *g3, R *g4, R *g5, R *g6,
                                                   R *g7, int do_thil, int do_wind) {
+  NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1;
@@ -164,14 +175,11 @@ __global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, con
     if ((q == 0 && !do_thil) || (q > 0 && !do_wind)) continue;
     const R s0 = ld(f[q], ic, act);
     R ax = R(0), ay = R(0);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      if (j < np) {
-        R d = ld(f[q], (long)m.w_nbr[c * NSLOT + j] * ldk + l, act) - s0;
-        ax = ax + m.w_gxc[c * NSLOT + j] * d;
-        ay = ay + m.w_gyc[c * NSLOT + j] * d;
-      }
-    }
+    for_slots(np, [&](int j) {
+      const R d = ld(f[q], (long)m.w_nbr[c * NSLOT + j] * ldk + l, act) - s0;
+      ax = ax + m.w_gxc[c * NSLOT + j] * d;
+      ay = ay + m.w_gyc[c * NSLOT + j] * d;
+    });
     if (!body) { ax = R(0); ay = R(0); }
     if (act) { gx[q][ic] = ax; gy[q][ic] = ay; }
   }
@@ -181,6 +189,7 @@ __global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, con
 // (fluxes_wrtv scalar_w_column :1085-1126 with grad_z_quadratic, bounded)
 template <class R>
 __device__ __forceinline__ R vert_upw(R q, R wsign, int k, int mza, R a11, R a21, R a12, R a22) {
+  NOCONTRACT
   const bool inner = k >= 2 && k <= mza - 1;
   const R qp1 = shdn(q, 1);
   const R ds = inner ? qp1 - q : R(0);
@@ -197,8 +206,14 @@ __device__ __forceinline__ R vert_upw(R q, R wsign, int k, int mza, R a11, R a21
 // ---------------------------------------------------------------- prog_wrt_begs
 // prog_wrtv_rk.f90:494-884 (global domain, no nudging, b20 = 1) fused with the face
 // values of fluxes_wrtv's scalar_v_column (:1022-1081) and scalar_w_column.
+// R-precision power evaluated through an FP64 pow and rounded once, so the FP32
+// result does not depend on which math library computes it (the oracle does the same).
+__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ double pow_r(double x, double y) { return pow(x, y); }
+
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
+  NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1, mza = m.mza;
@@ -225,7 +240,8 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   RP hflx = RP(0);
   R tt = ld(b.thilt_s, ic, act), tx = ld(b.vmxet_s, ic, act), ty = ld(b.vmyet_s, ic, act),
     tz = ld(b.vmzet_s, ic, act);
-  for (int j = 0; j < np; ++j) {
+  for_slots(np, [&](int j) {
+    const bool pad = j >= np;
     const int iv = m.w_face[c * NSLOT + j];
     const int nbr = m.w_nbr[c * NSLOT + j];
     const R dv = m.w_dirv[c * NSLOT + j];
@@ -233,7 +249,7 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     const R dxn = m.w_dxn[c * NSLOT + j], dyn = m.w_dyn[c * NSLOT + j];
     const long ie = (long)iv * ldk + l, in = (long)nbr * ldk + l;
     const R vmcv = ld(b.vmc, ie, act);
-    const R vmca = vmcv * ld(m.arv, ie, act);
+    const R vmca = pad ? R(0) : vmcv * ld(m.arv, ie, act);
     const bool sel = (dv < R(0)) ? (vmcv < R(0)) : !(vmcv < R(0));
     R f[4];
     for (int q = 0; q < 4; ++q) {
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
       hflx = hflx - (RP)vmca;
       tt = tt - vmca * f[0]; tx = tx - vmca * f[1]; ty = ty - vmca * f[2]; tz = tz - vmca * f[3];
     }
-  }
+  });
   // Coriolis, global (:675-681)
   const RP rho_c = act ? b.rho[ic] : RP(0);
   const double vol = act ? m.volt[ic] : 0.0;
@@ -275,7 +291,7 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R alp = ld(b.alpha_press, ic, act);
   const RP rhothil = rho_c * (RP)thc;
   const RP b10 = (RP)dt8 * (RP)vti;
-  const RP b5 = body ? (RP)(alp * (R)pow((R)rhothil, rocv)) : RP(0);
+  const RP b5 = body ? (RP)(alp * pow_r((R)rhothil, rocv)) : RP(0);
   const R b15 = (R)(b10 * b5);
   RP delex_rho = b10 * hflx;
   RP delex_rt = b10 * (RP)tt;
@@ -360,6 +376,7 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
 // uses as vmca, so it is added to vmasc here (:356-365).
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int acc_vmasc) {
+  NOCONTRACT
   const int iv = wave_item();
   if (iv >= m.nv) return;
   const int l = lane_id(), k = l + 1;
@@ -382,6 +399,7 @@ __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int 
 // vel_t3d.f90:39-108 (no closed faces on flat terrain)
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
+  NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1, mza = m.mza;
@@ -391,12 +409,12 @@ __global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
   const R wst = shup(wcc, 1) + wcc;
   R x = wst * (m.wnx[c] * R(0.5)), y = wst * (m.wny[c] * R(0.5)), z = wst * (m.wnz[c] * R(0.5));
   const int np = m.w_npoly[c];
-  for (int j = 0; j < np; ++j) {
+  for_slots(np, [&](int j) {
     const R v = ld(b.vc, (long)m.w_face[c * NSLOT + j] * ldk + l, act);
     x = x + m.w_ecx[c * NSLOT + j] * v;
     y = y + m.w_ecy[c * NSLOT + j] * v;
     z = z + m.w_ecz[c * NSLOT + j] * v;
-  }
+  });
   const R x2 = shdn(x, 1), y2 = shdn(y, 1), z2 = shdn(z, 1);
   if (k == 1) { x = x2; y = y2; z = z2; }
   if (act) { b.vxe[ic] = x; b.vye[ic] = y; b.vze[ic] = z; }

@@ -4,7 +4,9 @@
 // holds level k = l+1.  The horizontal reconstruction is materialised per stage as
 // the gradient pair (gx, gy) of grad_t2d (grad_lib.f90:236-269); the update kernel
 // then rebuilds each V-face value from the upwind cell's (q, gx, gy), exactly the
-// scalar_v_column formula (:474-533), so no V-face array is ever written.
+// scalar_v_column formula (:474-533), so no V-face array is ever written.  Columns
+// whose vertical CFL needs the implicit branch (ff_implic > 0.001, rare) are finished
+// by separate kernels that run only when such columns exist.
 #pragma once
 #include "common.h"
 
@@ -14,7 +16,7 @@ struct TrStage {
   R *q_out;
   const R *vmasc, *wmasc, *rho_old, *ffimp;
   const RP *rho;
-  int ntr, nrk, istage, order;
+  int ntr, nrk, istage, order, tpw;
   long tstride;
   R dtr;
 };
@@ -22,8 +24,9 @@ struct TrStage {
 // ---------------------------------------------------------------- grad_t2d
 template <class R>
 __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict__ q, R *__restrict__ gx,
-                                                 R *__restrict__ gy, int ntr, long tstride) {
-  const int c = wave_item();
+                                                 R *__restrict__ gy, int ntr, long tstride, int tpw) {
+  const TrItem ti = tr_item(m.nw, ntr, tpw);
+  const int c = ti.c;
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
@@ -37,18 +40,15 @@ __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict
     cy[j] = m.w_gyc[c * NSLOT + j];
   }
   const long ldk = m.ldk;
-  for (int n = 0; n < ntr; ++n) {
+  for (int n = ti.n0; n < ti.n1; ++n) {
     const R *qn = q + n * tstride;
-    R qc = ld(qn, c * ldk + l, act);
+    const R qc = ld(qn, c * ldk + l, act);
     R ax = R(0), ay = R(0);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      if (j < np) {
-        R d = ld(qn, nb[j] * ldk + l, act) - qc;
-        ax = ax + cx[j] * d;
-        ay = ay + cy[j] * d;
-      }
-    }
+    for_slots(np, [&](int j) {
+      const R d = ld(qn, nb[j] * ldk + l, act) - qc;
+      ax = ax + cx[j] * d;
+      ay = ay + cy[j] * d;
+    });
     if (!body) { ax = R(0); ay = R(0); }
     if (act) {
       gx[n * tstride + c * ldk + l] = ax;
@@ -69,11 +69,12 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 }
 
 // ---------------------------------------------------------------- update_scalar
-// One RK stage of all tracers: scalar_v_column + scalar_w_column (:474-611) and the
-// implicit vertical branch (:186-217, :347-375) for columns with ff_implic > 0.001.
+// One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
+// (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
-  const int c = wave_item();
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
@@ -91,38 +92,19 @@ __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a)
     const R dv = m.w_dirv[c * NSLOT + j];
     dxs[j] = m.w_dxs[c * NSLOT + j]; dys[j] = m.w_dys[c * NSLOT + j];
     dxn[j] = m.w_dxn[c * NSLOT + j]; dyn[j] = m.w_dyn[c * NSLOT + j];
-    R vm = (j < np) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
+    const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
     dvm[j] = dv * vm;
     // neighbour is upwind: this cell is iw1 (dirv < 0) and vm < 0, or iw2 and vm >= 0
     sel[j] = (dv < R(0)) ? (vm < R(0)) : !(vm < R(0));
   }
   const R wm = ld(a.wmasc, ic, act);
   const R wm_m1 = shup(wm, 1);
-  const R fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
-  const R fexpl = R(1) - fimpl;
-  const bool implicit = fimpl > R(0.001);
+  const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
   const R rf = act ? (R)a.rho[ic] : R(1);
   const R rinv = R(1) / (body ? rf : R(1));
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
-  // implicit tridiagonal of this column and stage (tridif_prep inputs)
-  double ib1 = 0.0, ib2 = 1.0, ib3 = 0.0;
-  R cimp = R(0);
-  if (implicit) {
-    R ro = ld(a.rho_old, ic, act);
-    R rs = stage_rho(ro, rf, a.istage, a.nrk);
-    R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
-    R wdn = inner ? (wm < R(0) ? wm : R(0)) : R(0);
-    R wup_m1 = shup(wup, 1), wdn_m1 = shup(wdn, 1);
-    R c2 = fimpl * a.dtr * vt / (body ? rs : R(1));
-    if (body) {
-      ib1 = (double)(-c2 * wup_m1);
-      ib2 = (double)(R(1) + c2 * (wup - wdn_m1));
-      ib3 = (double)(c2 * wdn);
-    }
-    cimp = c2;
-  }
-  for (int n = 0; n < a.ntr; ++n) {
+  for (int n = ti.n0; n < ti.n1; ++n) {
     const long off = n * a.tstride;
     const R *qi = a.q_in + off;
     const R qc = ld(qi, ic, act);
@@ -131,20 +113,17 @@ __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a)
     R gxc = R(0), gyc = R(0);
     if (a.order > 1) { gxc = ld(a.gx + off, ic, act); gyc = ld(a.gy + off, ic, act); }
     R F = R(0);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      if (j < np) {
-        const long in = (long)nb[j] * ldk + l;
-        R qn = ld(qi, in, act);
-        R fs = qc, fn = qn;
-        if (a.order > 1) {
-          fs = qc + dxs[j] * gxc + dys[j] * gyc;
-          fn = qn + dxn[j] * ld(a.gx + off, in, act) + dyn[j] * ld(a.gy + off, in, act);
-        }
-        R f = sel[j] ? fn : fs;
-        F = F + dvm[j] * (f - q0);
+    for_slots(np, [&](int j) {
+      const long in = (long)nb[j] * ldk + l;
+      const R qn = ld(qi, in, act);
+      R fs = qc, fn = qn;
+      if (a.order > 1) {
+        fs = qc + dxs[j] * gxc + dys[j] * gyc;
+        fn = qn + dxn[j] * ld(a.gx + off, in, act) + dyn[j] * ld(a.gy + off, in, act);
       }
-    }
+      const R f = sel[j] ? fn : fs;
+      F = F + dvm[j] * (f - q0);
+    });
     // vertical: grad_z_quadratic (bounded, zero-gradient ends) and upwinding
     const R qp1 = shdn(qc, 1);
     const R ds = inner ? qp1 - qc : R(0);
@@ -158,106 +137,212 @@ __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a)
     const R upw_m1 = shup(upw, 1);
     F = F + fexpl * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
     R qn_new = q0 + a.dtr * (st + vt * F) * rinv;
-    R lev1 = shdn(qn_new, 1);             // explicit value of level 2 for level 1
-    if (implicit) {
-      const R q0p1 = shdn(q0, 1);
-      const R dq0 = inner ? q0p1 - q0 : R(0);
-      const R dq0_m1 = shup(dq0, 1);
-      R delex = qn_new - q0 - cimp * ((wm < R(0) ? wm : R(0)) * dq0 + (wm_m1 > R(0) ? wm_m1 : R(0)) * dq0_m1);
-      double del = pcr_solve(ib1, ib2, ib3, body ? (double)delex : 0.0);
-      if (body) qn_new = q0 + (R)del;
-    }
+    const R lev1 = shdn(qn_new, 1);             // level 1 copies level 2
     if (k == 1) qn_new = lev1;
     if (act) a.q_out[off + ic] = qn_new;
   }
 }
 
+// Implicit vertical transport of one stage for columns with ff_implic > 0.001:
+// the tridif_prep/tridif_fini pair (tridiag_lib.f90:321-364) applied after the
+// explicit update (scalar_transport_rk.F90:186-217, :347-375), solved in FP64 by
+// PCR.  Level 1 keeps the explicit value, as in the reference.
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_tr_implicit(DMesh<R> m, TrStage<R, RP> a) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const R fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
+  if (!(fimpl > R(0.001))) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const R wm = ld(a.wmasc, ic, act);
+  const R wm_m1 = shup(wm, 1);
+  const R vt = ld(m.volti, ic, act);
+  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R ro = ld(a.rho_old, ic, act);
+  const R rs = stage_rho(ro, rf, a.istage, a.nrk);
+  const R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
+  const R wdn = inner ? (wm < R(0) ? wm : R(0)) : R(0);
+  const R wup_m1 = shup(wup, 1), wdn_m1 = shup(wdn, 1);
+  const R c2 = fimpl * a.dtr * vt / (body ? rs : R(1));
+  double ib1 = 0.0, ib2 = 1.0, ib3 = 0.0;
+  if (body) {
+    ib1 = (double)(-c2 * wup_m1);
+    ib2 = (double)(R(1) + c2 * (wup - wdn_m1));
+    ib3 = (double)(c2 * wdn);
+  }
+  for (int n = 0; n < a.ntr; ++n) {
+    const long off = n * a.tstride;
+    const R q0 = ld(a.q0 + off, ic, act);
+    const R qe = ld(a.q_out + off, ic, act);
+    const R q0p1 = shdn(q0, 1);
+    const R dq0 = inner ? q0p1 - q0 : R(0);
+    const R dq0_m1 = shup(dq0, 1);
+    const R delex = qe - q0 - c2 * ((wm < R(0) ? wm : R(0)) * dq0 + (wm_m1 > R(0) ? wm_m1 : R(0)) * dq0_m1);
+    const double del = pcr_solve(ib1, ib2, ib3, body ? (double)delex : 0.0);
+    if (body) a.q_out[off + ic] = q0 + (R)del;
+  }
+}
+
 // ---------------------------------------------------------------- check_cfls
-// scalar_transport_rk.F90:1476-1712: implicit fractions per stage and the maxima of
-// total CFL, horizontal CFL and |wc| (unsigned bit patterns of non-negative floats).
+// scalar_transport_rk.F90:1476-1712: implicit fractions per stage; with report != 0
+// also the maxima of total CFL, horizontal CFL and |wc| (unsigned bit patterns of
+// non-negative floats, one atomic per block) and the number of implicit columns.
+// any_imp is set (plain store) when some column needs the implicit branch.
 template <class R>
 __global__ __launch_bounds__(256) void k_tr_cfl(DMesh<R> m, const R *vmasc, const R *wmasc, const R *rho_o,
                                                 const R *wc, R dt, int nrk, R *ffimp, unsigned *maxbits,
-                                                int *nimp) {
+                                                int *nimp, int *any_imp, int report) {
+  __shared__ float sh[WPB][4];
   const int c = wave_item();
-  if (c >= m.nw) return;
+  const int w = threadIdx.x >> 6;
   const int l = lane_id(), k = l + 1, mza = m.mza;
-  const bool act = k <= mza, body = act && k >= 2;
-  const long ldk = m.ldk, ic = c * ldk + l;
-  const int np = m.w_npoly[c];
-  R cflh = R(0);
-  for (int j = 0; j < np; ++j) {
-    const int iv = m.w_face[c * NSLOT + j];
-    const R dv = m.w_dirv[c * NSLOT + j];
-    R x = dv * ld(vmasc, (long)iv * ldk + l, act);
-    cflh = cflh - (x < R(0) ? x : R(0));
-  }
-  const R wm = ld(wmasc, ic, act), wm_m1 = shup(wm, 1);
-  R fimp[3] = {R(0), R(0), R(0)};
-  R cfl = R(0);
-  if (body) {
-    R dovr = dt * m.volti[ic] / rho_o[ic];
-    cflh = cflh * dovr;
-    R cflw = ((wm > R(0) ? wm : R(0)) - (wm_m1 < R(0) ? wm_m1 : R(0))) * dovr;
-    cfl = cflh + cflw;
-    cflw = cflw > R(0.1) ? cflw : R(0.1);
-    fimp[nrk - 1] = (cfl - R(0.98)) / cflw;
-    fimp[nrk - 2] = (cfl - R(1.98)) / cflw;
-    if (nrk == 3) fimp[0] = (cfl - R(2.98)) / cflw;
-  } else {
-    cflh = R(0);
-    fimp[0] = fimp[1] = fimp[2] = R(-1e30);
-  }
+  float v0 = 0.f, v1 = 0.f, wmx = 0.f;
   bool any = false;
-  for (int n = 0; n < nrk; ++n) {
-    R v = fimp[n];
-    for (int s = 32; s >= 1; s >>= 1) { R o = __shfl_xor(v, s, WAVE); v = v > o ? v : o; }
-    v = v > R(0) ? v : R(0);         // fimp starts at 0 (:1558)
-    R f = v < R(1) ? v : R(1);
-    if (f < R(0.02)) f = R(0);
-    if (l == 0) ffimp[(long)n * m.nw + c] = f;
-    any = any || f > R(0.001);
+  if (c < m.nw) {
+    const bool act = k <= mza, body = act && k >= 2;
+    const long ldk = m.ldk, ic = c * ldk + l;
+    const int np = m.w_npoly[c];
+    R cflh = R(0);
+    for_slots(np, [&](int j) {
+      const int iv = m.w_face[c * NSLOT + j];
+      const R dv = m.w_dirv[c * NSLOT + j];
+      const R x = dv * ld(vmasc, (long)iv * ldk + l, act);
+      cflh = cflh - (x < R(0) ? x : R(0));
+    });
+    const R wm = ld(wmasc, ic, act), wm_m1 = shup(wm, 1);
+    R fimp[3] = {R(0), R(0), R(0)};
+    R cfl = R(0);
+    if (body) {
+      const R dovr = dt * m.volti[ic] / rho_o[ic];
+      cflh = cflh * dovr;
+      R cflw = ((wm > R(0) ? wm : R(0)) - (wm_m1 < R(0) ? wm_m1 : R(0))) * dovr;
+      cfl = cflh + cflw;
+      cflw = cflw > R(0.1) ? cflw : R(0.1);
+      fimp[nrk - 1] = (cfl - R(0.98)) / cflw;
+      fimp[nrk - 2] = (cfl - R(1.98)) / cflw;
+      if (nrk == 3) fimp[0] = (cfl - R(2.98)) / cflw;
+    } else {
+      cflh = R(0);
+      fimp[0] = fimp[1] = fimp[2] = R(-1e30);
+    }
+    for (int n = 0; n < nrk; ++n) {
+      R v = fimp[n];
+      for (int s = 32; s >= 1; s >>= 1) { R o = __shfl_xor(v, s, WAVE); v = v > o ? v : o; }
+      v = v > R(0) ? v : R(0);         // fimp starts at 0 (:1558)
+      R f = v < R(1) ? v : R(1);
+      if (f < R(0.02)) f = R(0);
+      if (l == 0) ffimp[(long)n * m.nw + c] = f;
+      any = any || f > R(0.001);
+    }
+    if (report) {
+      wmx = body ? fabsf((float)wc[ic]) : 0.f;
+      v0 = (float)cfl;
+      v1 = (float)cflh;
+      for (int s = 32; s >= 1; s >>= 1) {
+        v0 = fmaxf(v0, __shfl_xor(v0, s, WAVE));
+        v1 = fmaxf(v1, __shfl_xor(v1, s, WAVE));
+        wmx = fmaxf(wmx, __shfl_xor(wmx, s, WAVE));
+      }
+    }
   }
-  float w = act && body ? fabsf((float)wc[ic]) : 0.f;
-  float v0 = (float)cfl, v1 = (float)cflh;
-  for (int s = 32; s >= 1; s >>= 1) {
-    v0 = fmaxf(v0, __shfl_xor(v0, s, WAVE));
-    v1 = fmaxf(v1, __shfl_xor(v1, s, WAVE));
-    w = fmaxf(w, __shfl_xor(w, s, WAVE));
-  }
-  if (l == 0) {
-    atomicMax(&maxbits[0], __float_as_uint(fmaxf(v0, 0.f)));
-    atomicMax(&maxbits[1], __float_as_uint(fmaxf(v1, 0.f)));
-    atomicMax(&maxbits[2], __float_as_uint(w));
-    if (any) atomicAdd(nimp, 1);
+  if (any && l == 0) *any_imp = 1;
+  if (!report) return;
+  if (l == 0) { sh[w][0] = v0; sh[w][1] = v1; sh[w][2] = wmx; sh[w][3] = any ? 1.f : 0.f; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    int cnt = 0;
+    for (int i = 0; i < WPB; ++i) {
+      a0 = fmaxf(a0, sh[i][0]); a1 = fmaxf(a1, sh[i][1]); a2 = fmaxf(a2, sh[i][2]);
+      cnt += sh[i][3] > 0.f;
+    }
+    atomicMax(&maxbits[0], __float_as_uint(fmaxf(a0, 0.f)));
+    atomicMax(&maxbits[1], __float_as_uint(fmaxf(a1, 0.f)));
+    atomicMax(&maxbits[2], __float_as_uint(a2));
+    if (cnt) atomicAdd(nimp, cnt);
   }
 }
 
 // ---------------------------------------------------------------- adjoint
-// Transpose of k_tr_update for fixed winds (upwind choices frozen).  Three passes
-// per stage in reverse order:
-//   A0 (per column): undo the implicit solve (transposed tridiagonal), nu = D mu,
-//      the scp0 terms into lam0 and the vertical reconstruction transpose -> lpart.
+// Transpose of k_tr_update (+ k_tr_implicit) for fixed winds (upwind choices
+// frozen).  Per stage, in reverse order:
+//   k_tr_adj_implicit (implicit columns only): mu -> T^-T mu, with the scp0 terms of
+//      the implicit branch into lam0;
+//   A0 (per column): nu = D mu, the scp0 terms into lam0 and the transpose of the
+//      vertical reconstruction -> lpart;
 //   A1 (per column): face adjoint fluxes phi = vm (nu(iw2) - nu(iw1)) routed to the
-//      upwind side: lpart += phi, (Gx, Gy) += (dxps, dyps) phi.
+//      upwind side: lpart += phi, (Gx, Gy) = sum (dxps, dyps) phi;
 //   A2 (per column): transpose of grad_t2d: lq = lpart + sum over neighbours of
 //      their (Gx, Gy) weighted by the coefficient that points back to this column.
 template <class R, class RP>
 struct TrAdj {
-  const R *mu;       // adjoint of the stage output
+  R *mu;             // adjoint of the stage output (overwritten by the implicit pass)
   R *lam0;           // accumulates the adjoint of scp0
   R *nu, *lpart, *lgx, *lgy;
   R *lq;             // adjoint of the stage input
   const R *vmasc, *wmasc, *rho_old, *ffimp;
   const RP *rho;
-  int ntr, nrk, istage, order, last;
+  int ntr, nrk, istage, order, last, tpw;
   long tstride;
   R dtr;
 };
 
 template <class R, class RP>
-__global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
+__global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP> a) {
   const int c = wave_item();
+  if (c >= m.nw) return;
+  const R fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
+  if (!(fimpl > R(0.001))) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
+  const R vt = ld(m.volti, ic, act);
+  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R ro = ld(a.rho_old, ic, act);
+  const R rs = stage_rho(ro, rf, a.istage, a.nrk);
+  const R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
+  const R wdn = inner ? (wm < R(0) ? wm : R(0)) : R(0);
+  const R wup_m1 = shup(wup, 1), wdn_m1 = shup(wdn, 1);
+  const R c2 = fimpl * a.dtr * vt / (body ? rs : R(1));
+  const R b1 = body ? -c2 * wup_m1 : R(0);
+  const R b2 = body ? R(1) + c2 * (wup - wdn_m1) : R(1);
+  const R b3 = body ? c2 * wdn : R(0);
+  // transposed system: sub-diagonal b3(k-1), diagonal b2(k), super-diagonal b1(k+1)
+  const R b3_m1 = shup(b3, 1), b1_p1 = shdn(b1, 1);
+  double tb1 = 0.0, tb2 = 1.0, tb3 = 0.0;
+  if (body) { tb1 = (double)b3_m1; tb2 = (double)b2; tb3 = (double)(k < mza ? b1_p1 : R(0)); }
+  for (int n = 0; n < a.ntr; ++n) {
+    const long off = n * a.tstride;
+    R mu = ld(a.mu + off, ic, act);
+    const R mu1 = __shfl(mu, 0, WAVE);
+    if (k == 1) mu = R(0);
+    R l0 = ld(a.lam0 + off, ic, act);
+    const double le = pcr_solve(tb1, tb2, tb3, body ? (double)mu : 0.0);
+    R lef = body ? (R)le : R(0);
+    if (k == 2) lef = lef + mu1;              // level 1 kept the explicit level-2 value
+    l0 = l0 + mu - lef;
+    // -c2 (min(wm(k),0) dq0(k) + max(wm(k-1),0) dq0(k-1)),  dq0(k) = q0(k+1) - q0(k)
+    const R av = -c2 * lef;
+    const R t1 = inner ? av * (wm < R(0) ? wm : R(0)) : R(0);
+    const R t2 = (body && k - 1 >= 2) ? av * (wm_m1 > R(0) ? wm_m1 : R(0)) : R(0);
+    const R t1_m1 = shup(t1, 1), t2_p1 = shdn(t2, 1);
+    if (body) l0 = l0 - t1 + (k - 1 >= 2 ? t1_m1 : R(0)) + t2 - (k + 1 <= mza ? t2_p1 : R(0));
+    if (act) {
+      a.lam0[off + ic] = l0;
+      a.mu[off + ic] = lef;                   // lef = 0 on level 1
+    }
+  }
+}
+
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
@@ -265,73 +350,40 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
   R csum = R(0);
-  for (int j = 0; j < np; ++j) {
+  for_slots(np, [&](int j) {
     const int iv = m.w_face[c * NSLOT + j];
     const R dv = m.w_dirv[c * NSLOT + j];
     csum = csum + dv * ld(a.vmasc, (long)iv * ldk + l, act);
-  }
+  });
   const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
-  const R fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
-  const R fexpl = R(1) - fimpl;
-  const bool implicit = fimpl > R(0.001);
+  const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
   const R rf = act ? (R)a.rho[ic] : R(1);
   const R rinv = R(1) / (body ? rf : R(1));
   const R cc = csum + fexpl * (wm_m1 - wm);
-  const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
-  const R a11_p1 = shdn(a11, 1), a21_p1 = shdn(a21, 1), a22_p1 = shdn(a22, 1);
+  const R a11 = m.aw11[l], a12 = m.aw12[l];
+  const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
   const bool neg = wm < R(0);
-  // transposed implicit system: sub = b3a(k-1), diag = b2a(k), super = b1(k+1)
-  double tb1 = 0.0, tb2 = 1.0, tb3 = 0.0;
-  R cimp = R(0);
-  if (implicit) {
-    R ro = ld(a.rho_old, ic, act);
-    R rs = stage_rho(ro, rf, a.istage, a.nrk);
-    R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
-    R wdn = inner ? (wm < R(0) ? wm : R(0)) : R(0);
-    R wup_m1 = shup(wup, 1), wdn_m1 = shup(wdn, 1);
-    R c2 = fimpl * a.dtr * vt / (body ? rs : R(1));
-    R b1 = body ? -c2 * wup_m1 : R(0);
-    R b2 = body ? R(1) + c2 * (wup - wdn_m1) : R(1);
-    R b3 = body ? c2 * wdn : R(0);
-    R b3_m1 = shup(b3, 1), b1_p1 = shdn(b1, 1);
-    if (body) { tb1 = (double)b3_m1; tb2 = (double)b2; tb3 = (double)(k < mza ? b1_p1 : R(0)); }
-    cimp = c2;
-  }
-  for (int n = 0; n < a.ntr; ++n) {
+  const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
+  for (int n = ti.n0; n < ti.n1; ++n) {
     const long off = n * a.tstride;
     R mu = ld(a.mu + off, ic, act);
     const R mu1 = __shfl(mu, 0, WAVE);
     if (k == 1) mu = R(0);
+    if (k == 2) mu = mu + mu1;                 // level 1 copied level 2 (explicit)
     R l0 = ld(a.lam0 + off, ic, act);
-    R mue = mu;
-    if (implicit) {
-      double le = pcr_solve(tb1, tb2, tb3, body ? (double)mu : 0.0);
-      R lef = body ? (R)le : R(0);
-      if (k == 2) lef = lef + mu1;
-      l0 = l0 + mu - lef;
-      R av = -cimp * lef;
-      R t1 = inner ? av * (wm < R(0) ? wm : R(0)) : R(0);
-      R t2 = (body && k - 1 >= 2) ? av * (wm_m1 > R(0) ? wm_m1 : R(0)) : R(0);
-      R t1_m1 = shup(t1, 1), t2_p1 = shdn(t2, 1);
-      if (body) l0 = l0 - t1 + (k - 1 >= 2 ? t1_m1 : R(0)) + t2 - (k + 1 <= mza ? t2_p1 : R(0));
-      mue = lef;
-    } else if (k == 2) {
-      mue = mu + mu1;
-    }
-    R nu = body ? mue * a.dtr * vt * rinv : R(0);
-    if (body) l0 = l0 + mue - nu * cc;
-    // vertical transpose
+    const R nu = body ? mu * a.dtr * vt * rinv : R(0);
+    if (body) l0 = l0 + mu - nu * cc;
+    // vertical transpose: upw(j) feeds F(j+1) with +fexpl wm(j), F(j) with -fexpl wm(j)
     const R nu_p1 = shdn(nu, 1);
-    R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
+    const R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
     const R lupw_m1 = shup(lupw, 1), lupw_p1 = shdn(lupw, 1);
-    const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
     R lqv = R(0), lds = R(0);
     if (body) {
       if (inner && !neg) lqv = lqv + lupw;
       if (k - 1 >= 2 && neg_m1) lqv = lqv + lupw_m1;
-      // lds(k): from W level k-1 (neg) via aw11(k), from W level k (neg) via aw21(k+1),
-      // from W level k (not neg) via aw12(k), from W level k+1 (not neg) via aw22(k+1)
+      // lds(k): from W level k-1 (neg) via aw11(k), W level k (neg) via aw21(k+1),
+      // W level k (not neg) via aw12(k), W level k+1 (not neg) via aw22(k+1)
       if (k - 1 >= 2 && neg_m1) lds = lds + lupw_m1 * a11;
       if (inner && neg) lds = lds + lupw * a21_p1;
       if (inner && !neg) lds = lds + lupw * a12;
@@ -340,7 +392,6 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
     if (!inner) lds = R(0);
     const R lds_m1 = shup(lds, 1);
     if (body) lqv = lqv + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
-    (void)a11_p1;
     if (act) {
       a.lam0[off + ic] = l0;
       a.nu[off + ic] = nu;
@@ -351,7 +402,8 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
-  const int c = wave_item();
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
@@ -366,27 +418,23 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
     const int iv = m.w_face[c * NSLOT + j];
     const R dv = m.w_dirv[c * NSLOT + j];
     dxs[j] = m.w_dxs[c * NSLOT + j]; dys[j] = m.w_dys[c * NSLOT + j];
-    R vm = (j < np) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
-    vmj[j] = vm;
+    const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
+    vmj[j] = vm * dv * dv;                       // zero on padded slots
     iw1[j] = dv < R(0);
-    up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));   // this column is the upwind side
+    up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));  // this column is the upwind side
   }
-  for (int n = 0; n < a.ntr; ++n) {
+  for (int n = ti.n0; n < ti.n1; ++n) {
     const long off = n * a.tstride;
     const R nuc = ld(a.nu + off, ic, act);
     R P = R(0), Gx = R(0), Gy = R(0);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      if (j < np) {
-        const R nun = ld(a.nu + off, (long)nb[j] * ldk + l, act);
-        const R phi = iw1[j] ? vmj[j] * (nun - nuc) : vmj[j] * (nuc - nun);
-        if (up[j]) {
-          P = P + phi;
-          Gx = Gx + dxs[j] * phi;
-          Gy = Gy + dys[j] * phi;
-        }
-      }
-    }
+    for_slots(np, [&](int j) {
+      const R nun = ld(a.nu + off, (long)nb[j] * ldk + l, act);
+      const R phi = iw1[j] ? vmj[j] * (nun - nuc) : vmj[j] * (nuc - nun);
+      const R ph = up[j] ? phi : R(0);
+      P = P + ph;
+      Gx = Gx + dxs[j] * ph;
+      Gy = Gy + dys[j] * ph;
+    });
     if (!body) { P = R(0); Gx = R(0); Gy = R(0); }
     if (act) {
       a.lpart[off + ic] = a.lpart[off + ic] + P;
@@ -397,7 +445,8 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
-  const int c = wave_item();
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
@@ -413,19 +462,16 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     bx[j] = (j < np) ? m.w_gxc[nb[j] * NSLOT + b] : R(0);
     by[j] = (j < np) ? m.w_gyc[nb[j] * NSLOT + b] : R(0);
   }
-  for (int n = 0; n < a.ntr; ++n) {
+  for (int n = ti.n0; n < ti.n1; ++n) {
     const long off = n * a.tstride;
     R lq = ld(a.lpart + off, ic, act);
     if (a.order > 1) {
       const R gxs = ld(a.lgx + off, ic, act), gys = ld(a.lgy + off, ic, act);
-#pragma unroll
-      for (int j = 0; j < 7; ++j) {
-        if (j < np) {
-          const long in = (long)nb[j] * ldk + l;
-          lq = lq + (bx[j] * ld(a.lgx + off, in, act) + by[j] * ld(a.lgy + off, in, act));
-          lq = lq - (cx[j] * gxs + cy[j] * gys);
-        }
-      }
+      for_slots(np, [&](int j) {
+        const long in = (long)nb[j] * ldk + l;
+        lq = lq + (bx[j] * ld(a.lgx + off, in, act) + by[j] * ld(a.lgy + off, in, act));
+        lq = lq - (cx[j] * gxs + cy[j] * gys);
+      });
     }
     if (!body) lq = R(0);
     if (a.last) lq = lq + ld(a.lam0 + off, ic, act);   // stage 1 read q = scp0

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -52,6 +53,56 @@ struct DevBuf {
   }
   template <class T> T *as() const { return (T *)p; }
 };
+
+// ------------------------------------------------------------------ kernel timing
+// Optional per-kernel timing with HIP events recorded on the context's stream
+// around each launch (olamgpu_set_profiling).  Elapsed times are collected when the
+// API call has synchronised its stream.
+struct Prof {
+  bool on = false;
+  struct Rec { const char *name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> names;
+  std::vector<long> count;
+  std::vector<double> ms;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  void flush() {
+    for (auto &r : pending) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      size_t i = 0;
+      while (i < names.size() && names[i] != r.name) ++i;
+      if (i == names.size()) { names.push_back(r.name); count.push_back(0); ms.push_back(0.0); }
+      count[i] += 1;
+      ms[i] += t;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  void reset() { pending.clear(); names.clear(); count.clear(); ms.clear(); }
+  ~Prof() { for (auto e : pool) (void)hipEventDestroy(e); }
+};
+
+#define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...)                                  \
+  do {                                                                          \
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;                                    \
+    if (prof && prof->on) {                                                     \
+      e0_ = prof->get(); e1_ = prof->get();                                     \
+      HIPCHK(hipEventRecord(e0_, s));                                           \
+    }                                                                           \
+    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, s, __VA_ARGS__);                 \
+    if (e0_) {                                                                  \
+      HIPCHK(hipEventRecord(e1_, s));                                           \
+      prof->pending.push_back({NAME, e0_, e1_});                                \
+    }                                                                           \
+  } while (0)
 
 // ------------------------------------------------------------------ renumbering
 // 3-D Hilbert index of a point on the unit sphere (Skilling's transpose method,
@@ -252,6 +303,7 @@ template <class T> static inline T ipow(T x, int n) { T r = T(1); for (int i = 0
 template <class R, class RP>
 struct Impl : ImplBase {
   hipStream_t s;
+  Prof *prof = nullptr;
   olamgpu_params prm;
   Topo t;
   int nw, nv, nm, mza, ldk;
@@ -276,7 +328,7 @@ struct Impl : ImplBase {
   long tstride;
   DevBuf trS, trX, trY, trgx, trgy, trsct;
   bool have_sct = false;
-  DevBuf ffimp, cflmax, nimp, masspart, massout;
+  DevBuf ffimp, cflmax, nimp, anyimp, masspart, massout;
   // adjoint
   DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy;
   int cur_adj = 0;
@@ -293,8 +345,12 @@ struct Impl : ImplBase {
   template <class T> void upv(DevBuf &b, const std::vector<T> &v) { up(b, v.data(), v.size() * sizeof(T)); }
 
   unsigned nblk(int n) const { return (unsigned)((n + WPB - 1) / WPB); }
+  // tracers per wave in the tracer kernels (OLAMGPU_TPW overrides, for tuning)
+  int tpw = 2;
+  unsigned nblk_tr() const { int ng = (std::max(ntr, 1) + tpw - 1) / tpw; return nblk(nw * ng); }
 
-  Impl(const olamgpu_mesh *m, const olamgpu_params *p, hipStream_t st, Topo &&topo) : s(st), prm(*p), t(std::move(topo)) {
+  Impl(const olamgpu_mesh *m, const olamgpu_params *p, hipStream_t st, Prof *pr, Topo &&topo)
+      : s(st), prof(pr), prm(*p), t(std::move(topo)) {
     nw = t.nw; nv = t.nv; nm = t.nm; mza = t.mza; ldk = t.ldk;
     wsz = (size_t)nw * ldk; vsz = (size_t)nv * ldk; msz = (size_t)nm * ldk;
     if (prm.nrk_wrtv < 2 || prm.nrk_wrtv > 3 || prm.nrk_scal < 2 || prm.nrk_scal > 3)
@@ -305,6 +361,7 @@ struct Impl : ImplBase {
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
     build_mesh(m);
     alloc_state();
+    if (const char *e = std::getenv("OLAMGPU_TPW")) tpw = std::max(1, std::atoi(e));
   }
 
   // -------------------------------------------------------------- mesh upload
@@ -497,7 +554,7 @@ struct Impl : ImplBase {
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
     for (DevBuf *b : {&trS, &trX, &trY, &trgx, &trgy}) { b->alloc(tb); zero(*b); }
     ffimp.alloc((size_t)3 * nw * sizeof(R)); zero(ffimp);
-    cflmax.alloc(4 * sizeof(unsigned)); nimp.alloc(sizeof(int));
+    cflmax.alloc(4 * sizeof(unsigned)); nimp.alloc(sizeof(int)); anyimp.alloc(sizeof(int));
     masspart.alloc((size_t)nblk(nw) * 65 * sizeof(double));
     massout.alloc(65 * sizeof(double));
     HIPCHK(hipStreamSynchronize(s));
@@ -605,31 +662,31 @@ struct Impl : ImplBase {
   // -------------------------------------------------------------- prog_wrtv_rk
   void prog_wrtv_rk() override {
     const unsigned gw = nblk(nw), gv = nblk(nv), gm = nblk(nm);
-    hipLaunchKernelGGL((k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), 0, s, dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
+    LAUNCH("k_dyn_setup_w", (k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
                        dp<R>(wmc), dp<R>(thilt), dp<R>(vmxet), dp<R>(vmyet), dp<R>(vmzet), dp<RP>(rth0),
                        dp<RP>(rho0), dp<R>(wmc0), dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s));
     // short-step damping: divergence at W and vorticity at M, then their Laplacians
     if (do_div) {
-      hipLaunchKernelGGL((k_div<R>), dim3(gw), dim3(256), 0, s, dm, dp<R>(vmc), dp<R>(div2d[0]));
+      LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]));
       int ih = 0;
       for (int nn = 1; nn <= prm.divh_damp_level - 1; ++nn) {
         int il = (nn - 1) % 2;
         ih = nn % 2;
-        hipLaunchKernelGGL((k_lap_w<R>), dim3(gw), dim3(256), 0, s, dm, dp<R>(div2d[il]), dp<R>(div2d[ih]));
+        LAUNCH("k_lap_w", (k_lap_w<R>), dim3(gw), dim3(256), dm, dp<R>(div2d[il]), dp<R>(div2d[ih]));
       }
       div_ih = ih;
     }
     if (do_vort) {
-      hipLaunchKernelGGL((k_vort<R>), dim3(gm), dim3(256), 0, s, dm, dp<R>(vc), dp<R>(vort2d[0]));
+      LAUNCH("k_vort", (k_vort<R>), dim3(gm), dim3(256), dm, dp<R>(vc), dp<R>(vort2d[0]));
       int ih = 0;
       for (int nn = 1; nn <= prm.vort_damp_level - 1; ++nn) {
         int il = (nn - 1) % 2;
         ih = nn % 2;
-        hipLaunchKernelGGL((k_lap_m<R>), dim3(gm), dim3(256), 0, s, dm, dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
+        LAUNCH("k_lap_m", (k_lap_m<R>), dim3(gm), dim3(256), dm, dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
       }
       vort_ih = ih;
     }
-    hipLaunchKernelGGL((k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), 0, s, dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
+    LAUNCH("k_dyn_setup_v", (k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
                        dp<R>(vmt_s), do_div ? dp<R>(div2d[div_ih]) : (R *)nullptr, dp<R>(fdivh),
                        do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr, dp<R>(fvorth), dp<RP>(rho));
     const int nrk = prm.nrk_wrtv;
@@ -654,13 +711,13 @@ struct Impl : ImplBase {
       b.fcor = (R)prm.fcoriol;
       int dth = prm.thil_horiz_adv_order > 1, dwn = prm.wind_horiz_adv_order > 1;
       if (dth || dwn)
-        hipLaunchKernelGGL((k_dyn_grad<R>), dim3(gw), dim3(256), 0, s, dm, b.thil_in, b.vxe, b.vye, b.vze,
+        LAUNCH("k_dyn_grad", (k_dyn_grad<R>), dim3(gw), dim3(256), dm, b.thil_in, b.vxe, b.vye, b.vze,
                            dp<R>(grads[0]), dp<R>(grads[1]), dp<R>(grads[2]), dp<R>(grads[3]), dp<R>(grads[4]),
                            dp<R>(grads[5]), dp<R>(grads[6]), dp<R>(grads[7]), dth, dwn);
-      hipLaunchKernelGGL((k_dyn_w<R, RP>), dim3(gw), dim3(256), 0, s, dm, b);
+      LAUNCH("k_dyn_w", (k_dyn_w<R, RP>), dim3(gw), dim3(256), dm, b);
       cur_thil = 1 - cur_thil;
-      hipLaunchKernelGGL((k_dyn_v<R, RP>), dim3(gv), dim3(256), 0, s, dm, b, istage == nrk - 1 ? 1 : 0);
-      hipLaunchKernelGGL((k_dyn_vel<R, RP>), dim3(gw), dim3(256), 0, s, dm, b);
+      LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
+      LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);
     }
     HIPCHK(hipGetLastError());
   }
@@ -673,13 +730,20 @@ struct Impl : ImplBase {
     else { dtlr[0] = dtl * R(0.5); dtlr[1] = dtl * R(1.0); }
   }
 
-  void run_cfl(bool report) {
+  // check_cfls: implicit fractions (ffimp) for this step; returns whether any column
+  // needs the implicit branch (one small device-to-host copy).
+  bool run_cfl(bool report) {
     const int nrk = prm.nrk_scal;
-    HIPCHK(hipMemsetAsync(cflmax.p, 0, cflmax.bytes, s));
-    HIPCHK(hipMemsetAsync(nimp.p, 0, nimp.bytes, s));
-    hipLaunchKernelGGL((k_tr_cfl<R>), dim3(nblk(nw)), dim3(256), 0, s, dm, dp<R>(vmasc), dp<R>(wmasc),
-                       dp<R>(rho_old), dp<R>(wc), (R)prm.dtlm, nrk, dp<R>(ffimp), cflmax.as<unsigned>(),
-                       nimp.as<int>());
+    HIPCHK(hipMemsetAsync(anyimp.p, 0, anyimp.bytes, s));
+    if (report) {
+      HIPCHK(hipMemsetAsync(cflmax.p, 0, cflmax.bytes, s));
+      HIPCHK(hipMemsetAsync(nimp.p, 0, nimp.bytes, s));
+    }
+    LAUNCH("k_tr_cfl", (k_tr_cfl<R>), dim3(nblk(nw)), dim3(256), dm, dp<R>(vmasc), dp<R>(wmasc),
+           dp<R>(rho_old), dp<R>(wc), (R)prm.dtlm, nrk, dp<R>(ffimp), cflmax.as<unsigned>(), nimp.as<int>(),
+           anyimp.as<int>(), report ? 1 : 0);
+    int any = 0;
+    HIPCHK(hipMemcpyAsync(&any, anyimp.p, sizeof(int), hipMemcpyDeviceToHost, s));
     if (report) {
       unsigned mb[4];
       int ni = 0;
@@ -689,6 +753,8 @@ struct Impl : ImplBase {
       for (int i = 0; i < 3; ++i) { float f; std::memcpy(&f, &mb[i], 4); cfl_out[i] = f; }
       cfl_out[3] = ni;
     }
+    HIPCHK(hipStreamSynchronize(s));
+    return any != 0;
   }
 
   void scalar_transport_rk(const float *rho_old_host) override {
@@ -696,7 +762,7 @@ struct Impl : ImplBase {
     const int nrk = prm.nrk_scal;
     R dtlr[3];
     stage_dts(dtlr, nrk);
-    run_cfl(false);
+    const bool any_imp = run_cfl(false);
     if (ntr == 0) { have_fwd = true; return; }
     // RK3: S -> X -> Y -> X;  RK2: S -> X -> Y
     DevBuf *in = &trS, *outs[3] = {&trX, &trY, &trX};
@@ -710,10 +776,12 @@ struct Impl : ImplBase {
       a.rho = dp<RP>(rho);
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.tstride = tstride;
       a.dtr = dtlr[istage - 1];
+      a.tpw = tpw;
       if (a.order > 1)
-        hipLaunchKernelGGL((k_tr_grad<R>), dim3(nblk(nw)), dim3(256), 0, s, dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
-                           ntr, tstride);
-      hipLaunchKernelGGL((k_tr_update<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
+        LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
+               ntr, tstride, tpw);
+      LAUNCH("k_tr_update", (k_tr_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (any_imp) LAUNCH("k_tr_implicit", (k_tr_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
       in = out;
     }
     // the new state is in `in`; make it the state buffer
@@ -731,7 +799,7 @@ struct Impl : ImplBase {
     const int nrk = prm.nrk_scal;
     R dtlr[3];
     stage_dts(dtlr, nrk);
-    run_cfl(false);
+    const bool any_imp = run_cfl(false);
     HIPCHK(hipMemsetAsync(lam0.p, 0, lam0.bytes, s));
     for (int istage = nrk; istage >= 1; --istage) {
       TrAdj<R, RP> a;
@@ -743,9 +811,11 @@ struct Impl : ImplBase {
       a.rho = dp<RP>(rho);
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.last = istage == 1;
       a.tstride = tstride; a.dtr = dtlr[istage - 1];
-      hipLaunchKernelGGL((k_tr_adj0<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
-      hipLaunchKernelGGL((k_tr_adj1<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
-      hipLaunchKernelGGL((k_tr_adj2<R, RP>), dim3(nblk(nw)), dim3(256), 0, s, dm, a);
+      a.tpw = tpw;
+      if (any_imp) LAUNCH("k_tr_adj_implicit", (k_tr_adj_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
+      LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
       cur_adj = 1 - cur_adj;
     }
     HIPCHK(hipGetLastError());
@@ -754,9 +824,9 @@ struct Impl : ImplBase {
   // -------------------------------------------------------------- diagnostics
   void mass_sums(double *out) override {
     const unsigned nb = nblk(nw);
-    hipLaunchKernelGGL((k_mass_partial<R, RP>), dim3(nb), dim3(256), 0, s, dm, dp<RP>(rho), dp<R>(trS), ntr, tstride,
+    LAUNCH("k_mass_partial", (k_mass_partial<R, RP>), dim3(nb), dim3(256), dm, dp<RP>(rho), dp<R>(trS), ntr, tstride,
                        masspart.as<double>());
-    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, s, masspart.as<double>(), (int)nb, 1 + ntr,
+    LAUNCH("k_mass_final", k_mass_final, dim3(1), dim3(256), masspart.as<double>(), (int)nb, 1 + ntr,
                        massout.as<double>());
     HIPCHK(hipMemcpyAsync(out, massout.p, (1 + ntr) * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -797,7 +867,7 @@ __global__ void k_perm_copy(const int *inv, T *dev, T *dense, int n, int mza, in
 template <class R, class RP>
 void Impl<R, RP>::copy_rho_old() {
   long n = (long)wsz;
-  hipLaunchKernelGGL((k_cast_rho<R, RP>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dp<RP>(rho),
+  LAUNCH("k_cast_rho", (k_cast_rho<R, RP>), dim3((unsigned)((n + 255) / 256)), dim3(256), dp<RP>(rho),
                      dp<R>(rho_old), n);
   HIPCHK(hipGetLastError());
 }
@@ -806,9 +876,9 @@ template <class R, class RP>
 void Impl<R, RP>::timeavg_momsc() {
   if (prm.nacoust <= 1) return;
   R f = (R)prm.nacoust;
-  hipLaunchKernelGGL((k_scale<R>), dim3((unsigned)((vsz + 255) / 256)), dim3(256), 0, s, dp<R>(vmasc), (long)vsz, f,
+  LAUNCH("k_scale", (k_scale<R>), dim3((unsigned)((vsz + 255) / 256)), dim3(256), dp<R>(vmasc), (long)vsz, f,
                      mza, ldk, 2, mza);
-  hipLaunchKernelGGL((k_scale<R>), dim3((unsigned)((wsz + 255) / 256)), dim3(256), 0, s, dp<R>(wmasc), (long)wsz, f,
+  LAUNCH("k_scale", (k_scale<R>), dim3((unsigned)((wsz + 255) / 256)), dim3(256), dp<R>(wmasc), (long)wsz, f,
                      mza, ldk, 2, mza - 1);
   HIPCHK(hipGetLastError());
 }
@@ -837,10 +907,10 @@ void Impl<R, RP>::copy_field(int id, int index, void *buf, int dir, int *eb) {
   unsigned g = (unsigned)((tot + 255) / 256);
   if (isrp) {
     if (eb) *eb = sizeof(RP);
-    hipLaunchKernelGGL(k_perm_copy<RP>, dim3(g), dim3(256), 0, s, dinv, b->as<RP>() + off, (RP *)buf, n, mza, ldk, dir);
+    LAUNCH("k_perm_copy", k_perm_copy<RP>, dim3(g), dim3(256), dinv, b->as<RP>() + off, (RP *)buf, n, mza, ldk, dir);
   } else {
     if (eb) *eb = sizeof(R);
-    hipLaunchKernelGGL(k_perm_copy<R>, dim3(g), dim3(256), 0, s, dinv, b->as<R>() + off, (R *)buf, n, mza, ldk, dir);
+    LAUNCH("k_perm_copy", k_perm_copy<R>, dim3(g), dim3(256), dinv, b->as<R>() + off, (R *)buf, n, mza, ldk, dir);
   }
   HIPCHK(hipGetLastError());
 }
@@ -854,6 +924,7 @@ struct olamgpu_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  Prof prof;
   std::unique_ptr<ImplBase> impl;
 };
 
@@ -871,6 +942,7 @@ static int guarded(olamgpu_ctx *c, F &&f) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->prof.flush();
     return OLAMGPU_OK;
   } catch (const Err &e) {
     c->err = e.what();
@@ -897,11 +969,11 @@ int olamgpu_create(const olamgpu_mesh *mesh, const olamgpu_params *params, int32
     HIPCHK(hipEventCreate(&c->ev1));
     Topo t = build_topo(mesh);
     if (params->precision == OLAMGPU_PREC_F64)
-      c->impl.reset(new Impl<double, double>(mesh, params, c->stream, std::move(t)));
+      c->impl.reset(new Impl<double, double>(mesh, params, c->stream, &c->prof, std::move(t)));
     else if (params->precision == OLAMGPU_PREC_MIXED)
-      c->impl.reset(new Impl<float, double>(mesh, params, c->stream, std::move(t)));
+      c->impl.reset(new Impl<float, double>(mesh, params, c->stream, &c->prof, std::move(t)));
     else if (params->precision == OLAMGPU_PREC_F32)
-      c->impl.reset(new Impl<float, float>(mesh, params, c->stream, std::move(t)));
+      c->impl.reset(new Impl<float, float>(mesh, params, c->stream, &c->prof, std::move(t)));
     else
       throw Err(OLAMGPU_EINVAL, "unknown precision mode");
   } catch (const Err &e) {
@@ -969,6 +1041,23 @@ int olamgpu_copy_field(olamgpu_ctx *c, int32_t field_id, int32_t index, void *de
     if (!dev_buf || (direction != 0 && direction != 1)) throw Err(OLAMGPU_EINVAL, "bad copy arguments");
     c->impl->copy_field(field_id, index, dev_buf, direction, elem_bytes);
   });
+}
+int olamgpu_set_profiling(olamgpu_ctx *c, int32_t on) {
+  if (!c) return OLAMGPU_EINVAL;
+  c->prof.reset();
+  c->prof.on = on != 0;
+  return OLAMGPU_OK;
+}
+int olamgpu_kernel_profile(olamgpu_ctx *c, char *text, int32_t len) {
+  if (!c || !text || len <= 0) return OLAMGPU_EINVAL;
+  std::string out;
+  char line[256];
+  for (size_t i = 0; i < c->prof.names.size(); ++i) {
+    std::snprintf(line, sizeof line, "%s %ld %.6f\n", c->prof.names[i].c_str(), c->prof.count[i], c->prof.ms[i]);
+    out += line;
+  }
+  std::snprintf(text, (size_t)len, "%s", out.c_str());
+  return (int)out.size() < len ? OLAMGPU_OK : OLAMGPU_EINVAL;
 }
 int olamgpu_last_call_ms(olamgpu_ctx *c, double *ms) {
   if (!c || !ms) return OLAMGPU_EINVAL;

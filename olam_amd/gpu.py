@@ -49,6 +49,8 @@ def lib():
         L.olamgpu_cfl_report.argtypes = [vp, vp]
         L.olamgpu_copy_field.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        L.olamgpu_set_profiling.argtypes = [vp, C.c_int32]
+        L.olamgpu_kernel_profile.argtypes = [vp, C.c_char_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -57,7 +59,8 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_download_state", "olamgpu_upload_scalar", "olamgpu_download_scalar", "olamgpu_init_wrtv_rk",
            "olamgpu_tend0", "olamgpu_prog_wrtv_rk", "olamgpu_timeavg_momsc", "olamgpu_scalar_transport_rk",
            "olamgpu_scalar_transport_rk_adj", "olamgpu_upload_scalar_adj", "olamgpu_download_scalar_adj",
-           "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms"]
+           "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
+           "olamgpu_set_profiling", "olamgpu_kernel_profile"]
 
 
 def _p(a):
@@ -158,6 +161,19 @@ class Context:
         ms = C.c_double()
         lib().olamgpu_last_call_ms(self.h, C.byref(ms))
         return ms.value
+
+    def set_profiling(self, on=True):
+        lib().olamgpu_set_profiling(self.h, int(bool(on)))
+
+    def kernel_profile(self):
+        """{kernel: (launches, total_ms)} accumulated since set_profiling(True)."""
+        buf = C.create_string_buffer(1 << 16)
+        lib().olamgpu_kernel_profile(self.h, buf, len(buf))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
 
     # device-resident access --------------------------------------------------------
     def copy_field(self, name, index, dev_ptr, direction):

@@ -60,6 +60,11 @@ void load1(std::vector<T> &a, const S *src, int n) {
 template <class R_, class RP_, class RT_>
 struct Prec { using R = R_; using RP = RP_; using RT = RT_; };
 
+// FP32 power through an FP64 pow, rounded once (the reference's real ** real,
+// prog_wrtv_rk.f90:691, evaluated without a math-library dependent last bit)
+inline float pow_r(float x, float y) { return (float)std::pow((double)x, (double)y); }
+inline double pow_r(double x, double y) { return std::pow(x, y); }
+
 struct Base {
   virtual ~Base() {}
   virtual int upload_state(const olamgpu_state *) = 0;
@@ -552,7 +557,7 @@ struct Model : Base {
     for (int k = ka; k <= mza; ++k) {  // :687-700
       rhothil[k] = rho(k, iw) * (RP)thil(k, iw);
       b10[k] = (RP)dt8 * (RP)volti(k, iw);
-      b5[k] = (RP)(alpha_press(k, iw) * std::pow((R)rhothil[k], rocv));
+      b5[k] = (RP)(alpha_press(k, iw) * pow_r((R)rhothil[k], rocv));
       b15[k] = (R)(b10[k] * b5[k]);
       delex_rho[k] = b10[k] * hflx_rho[k];
       delex_rhothil[k] = b10[k] * (RP)thilt_rk(k, iw);
@@ -839,6 +844,7 @@ struct Model : Base {
   }
 
   // ---------------------------------------------------------------- adjoint
+  F2<R> a_nu, a_lqpart, a_lgx, a_lgy;
   // Transpose of update_scalar (+ implicit column) for fixed winds.  Derivation:
   // q_new = q0 + D (sct + volti F), F linear in (q, q0) with upwind choices frozen;
   // see the formulae in the comments below.  lq = adjoint of the stage input q,
@@ -846,10 +852,11 @@ struct Model : Base {
   void update_scalar_adj(const F2<R> &mu_in, F2<R> &lq, F2<R> &lq0, const F2<R> &rhosf, R dtr,
                          int iorder, int istage, int nrk, const R *dtlr) {
     const F2<R> &vm = tape_vmasc, &wm = tape_wmasc;
-    F2<R> nu; nu.alloc(mza, mwa);
-    F2<R> lqpart; lqpart.alloc(mza, mwa);
-    F2<R> lgx; lgx.alloc(mza, mwa);
-    F2<R> lgy; lgy.alloc(mza, mwa);
+    if (!a_nu.n) { a_nu.alloc(mza, mwa); a_lqpart.alloc(mza, mwa); a_lgx.alloc(mza, mwa); a_lgy.alloc(mza, mwa); }
+    F2<R> &nu = a_nu, &lqpart = a_lqpart, &lgx = a_lgx, &lgy = a_lgy;
+#pragma omp parallel for
+    for (int iw = 1; iw <= mwa; ++iw)
+      for (int k = 1; k <= mza; ++k) { nu(k, iw) = 0; lqpart(k, iw) = 0; lgx(k, iw) = 0; lgy(k, iw) = 0; }
     // (1) implicit columns: mu -> lambda of the explicit result; (2) nu = D volti mu_e
 #pragma omp parallel for
     for (int iw = 2; iw <= mwa; ++iw) {
