@@ -65,6 +65,21 @@ struct Prec { using R = R_; using RP = RP_; using RT = RT_; };
 inline float pow_r(float x, float y) { return (float)std::pow((double)x, (double)y); }
 inline double pow_r(double x, double y) { return std::pow(x, y); }
 
+// tridiffo, tridiag_lib.f90:10-36: Thomas algorithm on rows ka..kz (arrays indexed by
+// row), cim1 = sub-diagonal, ci = diagonal, cip1 = super-diagonal.
+template <class RT>
+void tridiffo_t(int ka, int kz, const RT *cim1, const RT *ci, const RT *cip1, const RT *rhs, RT *soln) {
+  std::vector<RT> scr1(kz + 2);
+  RT cji = RT(1) / ci[ka];
+  soln[ka] = rhs[ka] * cji;
+  for (int k = ka + 1; k <= kz; ++k) {
+    scr1[k] = cji * cip1[k - 1];
+    cji = RT(1) / (ci[k] - cim1[k] * scr1[k]);
+    soln[k] = cji * (rhs[k] - cim1[k] * soln[k - 1]);
+  }
+  for (int k = kz - 1; k >= ka; --k) soln[k] = soln[k] - scr1[k + 1] * soln[k + 1];
+}
+
 struct Base {
   virtual ~Base() {}
   virtual int upload_state(const olamgpu_state *) = 0;
@@ -285,15 +300,7 @@ struct Model : Base {
 
   // tridiffo, tridiag_lib.f90:10-36 (RT precision)
   void tridiffo(int ka, int kz, const RT *cim1, const RT *ci, const RT *cip1, const RT *rhs, RT *soln) const {
-    std::vector<RT> scr1(kz + 2);
-    RT cji = RT(1) / ci[ka];
-    soln[ka] = rhs[ka] * cji;
-    for (int k = ka + 1; k <= kz; ++k) {
-      scr1[k] = cji * cip1[k - 1];
-      cji = RT(1) / (ci[k] - cim1[k] * scr1[k]);
-      soln[k] = cji * (rhs[k] - cim1[k] * soln[k - 1]);
-    }
-    for (int k = kz - 1; k >= ka; --k) soln[k] = soln[k] - scr1[k + 1] * soln[k + 1];
+    tridiffo_t<RT>(ka, kz, cim1, ci, cip1, rhs, soln);
   }
 
   // ---------------------------------------------------------------- fluxes_wrtv
@@ -1054,5 +1061,15 @@ int oracle_scalar_transport_rk_adj(void *h) {
 int oracle_mass_sums(void *h, double *out) { ((oracle_ctx *)h)->m->mass_sums(out); return 0; }
 int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(out); return 0; }
 int oracle_set_threads(int32_t n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
+
+// tridiffo (tridiag_lib.f90:10-36) on arrays indexed 1..n, for the unit test of the
+// column solve: a = sub-diagonal, b = diagonal, c = super-diagonal.
+int oracle_tridiffo(int32_t n, const double *a, const double *b, const double *c, const double *d, double *x) {
+  std::vector<double> A(n + 2), B(n + 2), Cc(n + 2), D(n + 2), X(n + 2);
+  for (int i = 0; i < n; ++i) { A[i + 1] = a[i]; B[i + 1] = b[i]; Cc[i + 1] = c[i]; D[i + 1] = d[i]; }
+  tridiffo_t<double>(1, n, A.data(), B.data(), Cc.data(), D.data(), X.data());
+  for (int i = 0; i < n; ++i) x[i] = X[i + 1];
+  return 0;
+}
 
 }  // extern "C"

@@ -48,6 +48,7 @@ def lib():
         L.oracle_mass_sums.argtypes = [vp, vp]
         L.oracle_cfl_report.argtypes = [vp, vp]
         L.oracle_set_threads.argtypes = [C.c_int32]
+        L.oracle_tridiffo.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -129,6 +130,14 @@ class Oracle:
         out = np.zeros(4)
         lib().oracle_cfl_report(self.h, _p(out))
         return out
+
+
+def tridiffo(a, b, c, d):
+    """The oracle's Thomas solve of a x(i-1) + b x(i) + c x(i+1) = d."""
+    a, b, c, d = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, d))
+    x = np.zeros_like(d)
+    lib().oracle_tridiffo(len(d), _p(a), _p(b), _p(c), _p(d), _p(x))
+    return x
 
 
 def set_threads(n):
