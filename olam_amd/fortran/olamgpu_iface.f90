@@ -1,0 +1,231 @@
+! olamgpu_iface.f90 -- ISO_C_BINDING interfaces to libolamgpu (include/olamgpu.h).
+!
+! A Fortran host (OLAM's model driver) uses this module to call the MI355X dynamical
+! core in place of the module procedures it replaces:
+!   prog_wrtv_rk()              omodel/prog_wrtv_rk.f90:58  (called at timestep.f90:205)
+!   scalar_transport_rk(rho_old) omodel/scalar_transport_rk.F90:30 (timestep.f90:254)
+!   tend0 / timeavg_momsc       omodel/timestep.f90:430 / :563
+!   compute_mass_sums           outils/check_nans.F90:207
+! The derived types below mirror the C structs member by member (bind(C) gives them
+! the C layout).  Pointer members are filled with c_loc() of the host's arrays, which
+! must be contiguous and keep the reference's (mza, n) layout with the dummy index 1.
+
+module olamgpu_iface
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: OLAMGPU_OK = 0, OLAMGPU_EINVAL = -1, &
+       OLAMGPU_EHIP = -2, OLAMGPU_ENOMEM = -3, OLAMGPU_ESTATE = -4
+  integer(c_int32_t), parameter, public :: OLAMGPU_PREC_F32 = 0, OLAMGPU_PREC_MIXED = 1, &
+       OLAMGPU_PREC_F64 = 2
+
+  type, bind(C), public :: olamgpu_mesh_t
+     integer(c_int32_t) :: mza = 0, mwa = 0, mva = 0, mma = 0
+     type(c_ptr) :: w_npoly = c_null_ptr, w_iv = c_null_ptr, w_iw = c_null_ptr, w_im = c_null_ptr
+     type(c_ptr) :: w_dirv = c_null_ptr
+     type(c_ptr) :: w_ecvec_vx = c_null_ptr, w_ecvec_vy = c_null_ptr, w_ecvec_vz = c_null_ptr
+     type(c_ptr) :: gxps_coef = c_null_ptr, gyps_coef = c_null_ptr, xx_yy = c_null_ptr
+     type(c_ptr) :: v_iw = c_null_ptr, v_im = c_null_ptr, v_dxps = c_null_ptr, v_dyps = c_null_ptr
+     type(c_ptr) :: m_iv = c_null_ptr, m_iw = c_null_ptr, m_im = c_null_ptr, xx_yy_m = c_null_ptr
+     type(c_ptr) :: lpw = c_null_ptr, lpv = c_null_ptr, lpm = c_null_ptr
+     type(c_ptr) :: wnx = c_null_ptr, wny = c_null_ptr, wnz = c_null_ptr, arw0 = c_null_ptr
+     type(c_ptr) :: vnx = c_null_ptr, vny = c_null_ptr, vnz = c_null_ptr
+     type(c_ptr) :: dnu = c_null_ptr, dniu = c_null_ptr, dnv = c_null_ptr, dniv = c_null_ptr
+     type(c_ptr) :: arm0 = c_null_ptr
+     type(c_ptr) :: dzim = c_null_ptr, dzit = c_null_ptr, zfacit = c_null_ptr
+     type(c_ptr) :: zwgt_top8 = c_null_ptr, zwgt_bot8 = c_null_ptr
+     type(c_ptr) :: gdz_wgtm8 = c_null_ptr, gdz_wgtp8 = c_null_ptr
+     type(c_ptr) :: a_w = c_null_ptr, a_wu = c_null_ptr
+     type(c_ptr) :: arw = c_null_ptr, volti = c_null_ptr, volwi = c_null_ptr, volt = c_null_ptr
+     type(c_ptr) :: arv = c_null_ptr, volvi = c_null_ptr
+  end type olamgpu_mesh_t
+
+  type, bind(C), public :: olamgpu_params_t
+     integer(c_int32_t) :: precision = OLAMGPU_PREC_F32
+     integer(c_int32_t) :: num_scalar = 0
+     integer(c_int32_t) :: nrk_wrtv = 3, nrk_scal = 3
+     integer(c_int32_t) :: thil_horiz_adv_order = 2, wind_horiz_adv_order = 2
+     integer(c_int32_t) :: scal_horiz_adv_order = 2
+     integer(c_int32_t) :: iscal_monot = 0
+     integer(c_int32_t) :: divh_damp_level = 3, vort_damp_level = 3
+     real(c_float)      :: divh_damp_fact = 0.1, vort_damp_fact = 0.075
+     real(c_double)     :: dtsm = 0.0d0, dtlm = 0.0d0
+     integer(c_int32_t) :: nacoust = 1
+     integer(c_int32_t) :: adjoint_tape = 0
+     real(c_double)     :: fcoriol = 0.0d0
+  end type olamgpu_params_t
+
+  type, bind(C), public :: olamgpu_state_t
+     type(c_ptr) :: thil = c_null_ptr, wmc = c_null_ptr, wc = c_null_ptr
+     type(c_ptr) :: vxe = c_null_ptr, vye = c_null_ptr, vze = c_null_ptr
+     type(c_ptr) :: rho = c_null_ptr, press = c_null_ptr
+     type(c_ptr) :: vmc = c_null_ptr, vc = c_null_ptr
+     type(c_ptr) :: wmasc = c_null_ptr, vmasc = c_null_ptr
+     type(c_ptr) :: alpha_press = c_null_ptr, pwfac = c_null_ptr, pvfac = c_null_ptr
+     type(c_ptr) :: thilt = c_null_ptr, vmxet = c_null_ptr, vmyet = c_null_ptr, vmzet = c_null_ptr
+     type(c_ptr) :: vmt = c_null_ptr
+  end type olamgpu_state_t
+
+  public :: olamgpu_create, olamgpu_destroy, olamgpu_last_error, olamgpu_upload_state, &
+       olamgpu_download_state, olamgpu_upload_scalar, olamgpu_download_scalar, &
+       olamgpu_init_wrtv_rk, olamgpu_tend0, olamgpu_prog_wrtv_rk, olamgpu_timeavg_momsc, &
+       olamgpu_scalar_transport_rk, olamgpu_scalar_transport_rk_adj, &
+       olamgpu_upload_scalar_adj, olamgpu_download_scalar_adj, olamgpu_mass_sums, &
+       olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
+       olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_error_message
+
+  interface
+     integer(c_int) function olamgpu_create(mesh, params, device, ctx) bind(C, name="olamgpu_create")
+       import :: c_int, c_int32_t, c_ptr, olamgpu_mesh_t, olamgpu_params_t
+       type(olamgpu_mesh_t), intent(in) :: mesh
+       type(olamgpu_params_t), intent(in) :: params
+       integer(c_int32_t), value :: device
+       type(c_ptr), intent(out) :: ctx
+     end function olamgpu_create
+
+     integer(c_int) function olamgpu_destroy(ctx) bind(C, name="olamgpu_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_destroy
+
+     type(c_ptr) function olamgpu_last_error(ctx) bind(C, name="olamgpu_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_last_error
+
+     integer(c_int) function olamgpu_upload_state(ctx, host) bind(C, name="olamgpu_upload_state")
+       import :: c_int, c_ptr, olamgpu_state_t
+       type(c_ptr), value :: ctx
+       type(olamgpu_state_t), intent(in) :: host
+     end function olamgpu_upload_state
+
+     integer(c_int) function olamgpu_download_state(ctx, host) bind(C, name="olamgpu_download_state")
+       import :: c_int, c_ptr, olamgpu_state_t
+       type(c_ptr), value :: ctx
+       type(olamgpu_state_t), intent(in) :: host
+     end function olamgpu_download_state
+
+     integer(c_int) function olamgpu_upload_scalar(ctx, n, var_p, var_t) bind(C, name="olamgpu_upload_scalar")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: var_p, var_t
+     end function olamgpu_upload_scalar
+
+     integer(c_int) function olamgpu_download_scalar(ctx, n, var_p) bind(C, name="olamgpu_download_scalar")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: var_p
+     end function olamgpu_download_scalar
+
+     integer(c_int) function olamgpu_init_wrtv_rk(ctx) bind(C, name="olamgpu_init_wrtv_rk")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_init_wrtv_rk
+
+     integer(c_int) function olamgpu_tend0(ctx) bind(C, name="olamgpu_tend0")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_tend0
+
+     integer(c_int) function olamgpu_prog_wrtv_rk(ctx) bind(C, name="olamgpu_prog_wrtv_rk")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_prog_wrtv_rk
+
+     integer(c_int) function olamgpu_timeavg_momsc(ctx) bind(C, name="olamgpu_timeavg_momsc")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_timeavg_momsc
+
+     ! rho_old: c_loc of a real(4) (mza, mwa) array, or c_null_ptr for the copy saved by tend0
+     integer(c_int) function olamgpu_scalar_transport_rk(ctx, rho_old) &
+          bind(C, name="olamgpu_scalar_transport_rk")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, rho_old
+     end function olamgpu_scalar_transport_rk
+
+     integer(c_int) function olamgpu_scalar_transport_rk_adj(ctx) bind(C, name="olamgpu_scalar_transport_rk_adj")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function olamgpu_scalar_transport_rk_adj
+
+     integer(c_int) function olamgpu_upload_scalar_adj(ctx, n, lambda) bind(C, name="olamgpu_upload_scalar_adj")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: lambda
+     end function olamgpu_upload_scalar_adj
+
+     integer(c_int) function olamgpu_download_scalar_adj(ctx, n, lambda) &
+          bind(C, name="olamgpu_download_scalar_adj")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       type(c_ptr), value :: lambda
+     end function olamgpu_download_scalar_adj
+
+     integer(c_int) function olamgpu_mass_sums(ctx, out) bind(C, name="olamgpu_mass_sums")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(*)
+     end function olamgpu_mass_sums
+
+     integer(c_int) function olamgpu_cfl_report(ctx, out) bind(C, name="olamgpu_cfl_report")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(4)
+     end function olamgpu_cfl_report
+
+     integer(c_int) function olamgpu_copy_field(ctx, field_id, index, dev_buf, direction, elem_bytes) &
+          bind(C, name="olamgpu_copy_field")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: field_id, index
+       type(c_ptr), value :: dev_buf
+       integer(c_int32_t), value :: direction
+       integer(c_int32_t), intent(out) :: elem_bytes
+     end function olamgpu_copy_field
+
+     integer(c_int) function olamgpu_last_call_ms(ctx, ms) bind(C, name="olamgpu_last_call_ms")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms
+     end function olamgpu_last_call_ms
+
+     integer(c_int) function olamgpu_set_profiling(ctx, on) bind(C, name="olamgpu_set_profiling")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function olamgpu_set_profiling
+
+     integer(c_int) function olamgpu_kernel_profile(ctx, text, len) bind(C, name="olamgpu_kernel_profile")
+       import :: c_int, c_int32_t, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(out) :: text(*)
+       integer(c_int32_t), value :: len
+     end function olamgpu_kernel_profile
+  end interface
+
+contains
+
+  ! The message of olamgpu_last_error as a Fortran string (blank when there is none).
+  function olamgpu_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=512) :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ' '
+    p = olamgpu_last_error(ctx)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [512])
+    do i = 1, 512
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    end do
+  end function olamgpu_error_message
+
+end module olamgpu_iface

@@ -121,3 +121,38 @@ class PermutedGrid:
         b = a.copy()
         b[1:] = a[p + 1]
         return b
+
+
+def write_fortran_case(path, grid, params, state, scalars, nsteps):
+    """Files read by the Fortran host olam_amd/lib/olamgpu_fdriver: params.txt,
+    fields.txt and one raw .bin per array (Fortran layout, native byte order)."""
+    import os
+    os.makedirs(path, exist_ok=True)
+    p = params
+    with open(os.path.join(path, "params.txt"), "w") as f:
+        f.write(" ".join(str(v) for v in (
+            p.precision, p.num_scalar, p.nrk_wrtv, p.nrk_scal, p.thil_horiz_adv_order,
+            p.wind_horiz_adv_order, p.scal_horiz_adv_order, p.iscal_monot, p.divh_damp_level,
+            p.vort_damp_level, repr(float(p.divh_damp_fact)), repr(float(p.vort_damp_fact)),
+            repr(p.dtsm), repr(p.dtlm), p.nacoust, p.adjoint_tape, repr(p.fcoriol), nsteps)) + "\n")
+        f.write(f"{grid.mza} {grid.mwa} {grid.mva} {grid.mma}\n")
+    lines = []
+
+    def put(group, name, a):
+        a = np.ascontiguousarray(a)
+        a.tofile(os.path.join(path, name + ".bin"))
+        lines.append(f"{group} {name} {a.nbytes}")
+
+    for name, a in grid.arrays.items():
+        put("mesh", name, a)
+    for name, a in state.arrays.items():
+        put("state", name, a)
+    for n, q in enumerate(scalars):
+        put("scalar", f"scalar{n + 1}", np.asarray(q, np.float32))
+    with open(os.path.join(path, "fields.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_fortran_output(path, name, like):
+    import os
+    return np.fromfile(os.path.join(path, f"out_{name}.bin"), dtype=like.dtype).reshape(like.shape)

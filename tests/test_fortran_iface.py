@@ -1,0 +1,124 @@
+"""The Fortran ISO_C_BINDING boundary (olam_amd/fortran/olamgpu_iface.f90).
+
+CPU: the bind(C) derived types have the size and member offsets of the C structs in
+include/olamgpu.h (checked against ctypes, whose layout follows the C ABI), and every
+bind(C) interface names a symbol the library exports.
+GPU: a Fortran host program driving whole long steps (timestep.f90 call order) gives
+results bit-identical to the same calls made from Python through ctypes.
+"""
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from olam_amd import abi, build, cases, mesh as M
+from helpers import read_fortran_output, write_fortran_case
+
+FC = build.flang()
+IFACE = os.path.join(build.FORTRAN, "olamgpu_iface.f90")
+needs_fc = pytest.mark.skipif(FC is None, reason="no Fortran compiler (amdflang) in this image")
+
+PROBE = """
+program probe
+  use iso_c_binding
+  use olamgpu_iface
+  implicit none
+  type(olamgpu_mesh_t), target :: m
+  type(olamgpu_params_t), target :: p
+  type(olamgpu_state_t), target :: s
+  print '(a,1x,i0)', 'mesh', c_sizeof(m)
+  print '(a,1x,i0)', 'params', c_sizeof(p)
+  print '(a,1x,i0)', 'state', c_sizeof(s)
+  print '(a,1x,i0)', 'mesh.w_npoly', off(c_loc(m%w_npoly), c_loc(m))
+  print '(a,1x,i0)', 'mesh.volvi', off(c_loc(m%volvi), c_loc(m))
+  print '(a,1x,i0)', 'mesh.zwgt_top8', off(c_loc(m%zwgt_top8), c_loc(m))
+  print '(a,1x,i0)', 'params.dtsm', off(c_loc(p%dtsm), c_loc(p))
+  print '(a,1x,i0)', 'params.nacoust', off(c_loc(p%nacoust), c_loc(p))
+  print '(a,1x,i0)', 'params.fcoriol', off(c_loc(p%fcoriol), c_loc(p))
+  print '(a,1x,i0)', 'params.vort_damp_fact', off(c_loc(p%vort_damp_fact), c_loc(p))
+  print '(a,1x,i0)', 'state.rho', off(c_loc(s%rho), c_loc(s))
+  print '(a,1x,i0)', 'state.vmt', off(c_loc(s%vmt), c_loc(s))
+contains
+  integer function off(a, b)
+    type(c_ptr), intent(in) :: a, b
+    off = int(transfer(a, 0_c_intptr_t) - transfer(b, 0_c_intptr_t))
+  end function off
+end program probe
+"""
+
+
+@needs_fc
+@pytest.mark.skipif(not os.path.exists(build.LIB), reason="libolamgpu.so not built")
+def test_bindc_types_match_c_layout(tmp_path):
+    mod = tmp_path / "mod"
+    mod.mkdir()
+    subprocess.run([FC, "-c", "-J", str(mod), "-o", str(tmp_path / "iface.o"), IFACE], check=True)
+    src = tmp_path / "probe.f90"
+    src.write_text(PROBE)
+    exe = tmp_path / "probe"
+    # the module's helper function references olamgpu_last_error: link the library
+    subprocess.run([FC, "-I", str(mod), "-o", str(exe), str(src), str(tmp_path / "iface.o"),
+                    "-L", build.LIBDIR, "-lolamgpu", f"-Wl,-rpath,{build.LIBDIR}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {k: int(v) for k, v in (ln.split() for ln in out.strip().splitlines())}
+    ty = {"mesh": abi.MeshDesc, "params": abi.Params, "state": abi.StateDesc}
+    for key, val in got.items():
+        if "." in key:
+            t, f = key.split(".")
+            assert val == getattr(ty[t], f).offset, key
+        else:
+            assert val == C.sizeof(ty[key]), key
+
+
+def test_interfaces_name_exported_symbols():
+    text = open(IFACE).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(olamgpu_\w+)"\)', text))
+    header = open(os.path.join(build.ROOT, "include", "olamgpu.h")).read()
+    declared = set(re.findall(r"\b(olamgpu_\w+)\s*\(", header))
+    assert bound == declared
+
+
+@pytest.mark.gpu
+def test_fortran_host_matches_ctypes(tmp_path):
+    from olam_amd.gpu import Context
+    exe = os.path.join(build.LIBDIR, "olamgpu_fdriver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no amdflang at build time)")
+    grid = abi.FortranGrid(M.build_mesh(10), M.vertical_grid(20))
+    st = cases.jet_state(grid)
+    tr = cases.lognormal_tracers(grid, 3)
+    prm = abi.make_params("f32", num_scalar=3, dtsm=120.0, nacoust=2, dtlm=240.0)
+    nsteps = 3
+    write_fortran_case(str(tmp_path), grid, prm, st, tr, nsteps)
+    print("fortran driver: nxp=10 L=20 ntr=3 f32 nsteps=3", flush=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    ctx = Context(grid, prm, device=0)
+    ctx.init_wrtv_rk()
+    ctx.upload_state(st)
+    for n in range(3):
+        ctx.upload_scalar(n + 1, tr[n])
+    for _ in range(nsteps):
+        ctx.tend0()
+        for _ in range(prm.nacoust):
+            ctx.prog_wrtv_rk()
+        ctx.timeavg_momsc()
+        ctx.scalar_transport_rk()
+    mass = ctx.mass_sums()
+    out = abi.FortranState(grid)
+    ctx.download_state(out)
+    for name in ("thil", "rho", "press", "wmc", "vmc", "vc", "wc", "vmasc", "wmasc"):
+        f = read_fortran_output(str(tmp_path), name, out.arrays[name])
+        assert np.array_equal(f[1:], out.arrays[name][1:]), name
+    for n in range(3):
+        q = ctx.download_scalar(n + 1)
+        f = read_fortran_output(str(tmp_path), f"scalar{n + 1}", q)
+        assert np.array_equal(f[1:], q[1:]), n
+    fm = np.fromfile(str(tmp_path / "out_mass.bin"), dtype=np.float64)
+    assert np.array_equal(fm, mass)
+    ctx.close()
