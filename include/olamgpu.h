@@ -210,6 +210,24 @@ int olamgpu_cfl_report(olamgpu_ctx *ctx, double *out);
 int olamgpu_copy_field(olamgpu_ctx *ctx, int32_t field_id, int32_t index,
                        void *dev_buf, int32_t direction, int32_t *elem_bytes);
 
+/* ---- checkpoints for multi-step adjoint windows (no counterpart in the reference) --- */
+
+#define OLAMGPU_CKPT_STATE  1   /* fields the hot-path calls modify (prognostic state,
+                                   vmasc, wmasc, rho_old); host-set inputs excluded */
+#define OLAMGPU_CKPT_TAPE   2   /* vmasc, wmasc, rho, rho_old: what a tracer step and its
+                                   adjoint read besides the tracers */
+
+/* Size in bytes of a checkpoint of the given kind. */
+int olamgpu_checkpoint_bytes(olamgpu_ctx *ctx, int32_t kind, int64_t *bytes);
+/* Device-to-device copy of the context's state into / out of a caller-owned device
+ * buffer of checkpoint_bytes bytes.  Restoring a STATE checkpoint makes the following
+ * calls reproduce bit-identically what followed the save (given the same host-set
+ * inputs and tracers); restoring a TAPE makes
+ * olamgpu_scalar_transport_rk_adj valid for the tracer step that the tape was saved
+ * after (the adjoint depends on the tracers only linearly). */
+int olamgpu_checkpoint_save(olamgpu_ctx *ctx, int32_t kind, void *dev_buf);
+int olamgpu_checkpoint_restore(olamgpu_ctx *ctx, int32_t kind, const void *dev_buf);
+
 /* Synchronise the context's stream and report the time of the last call's
  * kernels in milliseconds (HIP events around the call). */
 int olamgpu_last_call_ms(olamgpu_ctx *ctx, double *ms);

@@ -296,6 +296,8 @@ struct ImplBase {
   virtual void mass_sums(double *) = 0;
   virtual void cfl_report(double *) = 0;
   virtual void copy_field(int, int, void *, int, int *) = 0;
+  virtual size_t checkpoint_bytes(int) = 0;
+  virtual void checkpoint(int, void *, int) = 0;
 };
 
 template <class T> static inline T ipow(T x, int n) { T r = T(1); for (int i = 0; i < n; ++i) r *= x; return r; }
@@ -345,8 +347,10 @@ struct Impl : ImplBase {
   template <class T> void upv(DevBuf &b, const std::vector<T> &v) { up(b, v.data(), v.size() * sizeof(T)); }
 
   unsigned nblk(int n) const { return (unsigned)((n + WPB - 1) / WPB); }
-  // tracers per wave in the tracer kernels (OLAMGPU_TPW overrides, for tuning)
-  int tpw = 2;
+  // tracers per wave in the tracer kernels: all of them up to 16 (measured at NXP=320,
+  // L=60, 10 tracers: 1 -> 151, 2 -> 121, 5 -> 107, 10 -> 104 ms per step, because the
+  // per-column gathers of winds and metrics are shared); OLAMGPU_TPW overrides
+  int tpw = 16;
   unsigned nblk_tr() const { int ng = (std::max(ntr, 1) + tpw - 1) / tpw; return nblk(nw * ng); }
 
   Impl(const olamgpu_mesh *m, const olamgpu_params *p, hipStream_t st, Prof *pr, Topo &&topo)
@@ -837,6 +841,36 @@ struct Impl : ImplBase {
   }
 
   void copy_field(int id, int index, void *buf, int dir, int *eb) override;
+
+  // -------------------------------------------------------------- checkpoints
+  // STATE: every field that the library's own calls modify and read back later
+  // (prognostic fields, accumulated mass fluxes, rho_old), so a restored context
+  // continues bit-identically as long as the host-set inputs (alpha_press, pwfac,
+  // pvfac, long-step tendencies, tracer tendencies) are the same.  TAPE: what a tracer
+  // step and its adjoint read besides the tracers (vmasc, wmasc, rho, rho_old).
+  std::vector<DevBuf *> ck_list(int kind) {
+    if (kind == OLAMGPU_CKPT_STATE)
+      return {&cur_thil_buf(), &wmc, &wc, &vxe, &vye, &vze, &vmc, &vc, &rho, &press, &vmasc, &wmasc, &rho_old};
+    if (kind == OLAMGPU_CKPT_TAPE) return {&vmasc, &wmasc, &rho, &rho_old};
+    throw Err(OLAMGPU_EINVAL, "unknown checkpoint kind");
+  }
+  static size_t ck_round(size_t b) { return (b + 255) & ~(size_t)255; }
+  size_t checkpoint_bytes(int kind) override {
+    size_t tot = 0;
+    for (DevBuf *b : ck_list(kind)) tot += ck_round(b->bytes);
+    return tot;
+  }
+  void checkpoint(int kind, void *buf, int restore) override {
+    size_t off = 0;
+    for (DevBuf *b : ck_list(kind)) {
+      char *d = (char *)buf + off;
+      if (restore) HIPCHK(hipMemcpyAsync(b->p, d, b->bytes, hipMemcpyDeviceToDevice, s));
+      else HIPCHK(hipMemcpyAsync(d, b->p, b->bytes, hipMemcpyDeviceToDevice, s));
+      off += ck_round(b->bytes);
+    }
+    // the adjoint of a tracer step depends only on the taped winds and densities
+    if (restore && kind == OLAMGPU_CKPT_TAPE) have_fwd = true;
+  }
 };
 
 // ---------------------------------------------------------------- small kernels
@@ -1040,6 +1074,24 @@ int olamgpu_copy_field(olamgpu_ctx *c, int32_t field_id, int32_t index, void *de
   return guarded(c, [&] {
     if (!dev_buf || (direction != 0 && direction != 1)) throw Err(OLAMGPU_EINVAL, "bad copy arguments");
     c->impl->copy_field(field_id, index, dev_buf, direction, elem_bytes);
+  });
+}
+int olamgpu_checkpoint_bytes(olamgpu_ctx *c, int32_t kind, int64_t *bytes) {
+  return guarded(c, [&] {
+    if (!bytes) throw Err(OLAMGPU_EINVAL, "null pointer");
+    *bytes = (int64_t)c->impl->checkpoint_bytes(kind);
+  });
+}
+int olamgpu_checkpoint_save(olamgpu_ctx *c, int32_t kind, void *dev_buf) {
+  return guarded(c, [&] {
+    if (!dev_buf) throw Err(OLAMGPU_EINVAL, "null buffer");
+    c->impl->checkpoint(kind, dev_buf, 0);
+  });
+}
+int olamgpu_checkpoint_restore(olamgpu_ctx *c, int32_t kind, const void *dev_buf) {
+  return guarded(c, [&] {
+    if (!dev_buf) throw Err(OLAMGPU_EINVAL, "null buffer");
+    c->impl->checkpoint(kind, const_cast<void *>(dev_buf), 1);
   });
 }
 int olamgpu_set_profiling(olamgpu_ctx *c, int32_t on) {

@@ -19,6 +19,7 @@ module olamgpu_iface
        OLAMGPU_EHIP = -2, OLAMGPU_ENOMEM = -3, OLAMGPU_ESTATE = -4
   integer(c_int32_t), parameter, public :: OLAMGPU_PREC_F32 = 0, OLAMGPU_PREC_MIXED = 1, &
        OLAMGPU_PREC_F64 = 2
+  integer(c_int32_t), parameter, public :: OLAMGPU_CKPT_STATE = 1, OLAMGPU_CKPT_TAPE = 2
 
   type, bind(C), public :: olamgpu_mesh_t
      integer(c_int32_t) :: mza = 0, mwa = 0, mva = 0, mma = 0
@@ -73,7 +74,8 @@ module olamgpu_iface
        olamgpu_scalar_transport_rk, olamgpu_scalar_transport_rk_adj, &
        olamgpu_upload_scalar_adj, olamgpu_download_scalar_adj, olamgpu_mass_sums, &
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
-       olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_error_message
+       olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_checkpoint_bytes, &
+       olamgpu_checkpoint_save, olamgpu_checkpoint_restore, olamgpu_error_message
 
   interface
      integer(c_int) function olamgpu_create(mesh, params, device, ctx) bind(C, name="olamgpu_create")
@@ -207,6 +209,29 @@ module olamgpu_iface
        character(kind=c_char), intent(out) :: text(*)
        integer(c_int32_t), value :: len
      end function olamgpu_kernel_profile
+
+     integer(c_int) function olamgpu_checkpoint_bytes(ctx, kind, bytes) bind(C, name="olamgpu_checkpoint_bytes")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: kind
+       integer(c_int64_t), intent(out) :: bytes
+     end function olamgpu_checkpoint_bytes
+
+     ! dev_buf: a device pointer (e.g. from hipMalloc) of checkpoint_bytes bytes
+     integer(c_int) function olamgpu_checkpoint_save(ctx, kind, dev_buf) bind(C, name="olamgpu_checkpoint_save")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: kind
+       type(c_ptr), value :: dev_buf
+     end function olamgpu_checkpoint_save
+
+     integer(c_int) function olamgpu_checkpoint_restore(ctx, kind, dev_buf) &
+          bind(C, name="olamgpu_checkpoint_restore")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: kind
+       type(c_ptr), value :: dev_buf
+     end function olamgpu_checkpoint_restore
   end interface
 
 contains

@@ -49,6 +49,9 @@ def lib():
         L.olamgpu_cfl_report.argtypes = [vp, vp]
         L.olamgpu_copy_field.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        L.olamgpu_checkpoint_bytes.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
+        L.olamgpu_checkpoint_save.argtypes = [vp, C.c_int32, vp]
+        L.olamgpu_checkpoint_restore.argtypes = [vp, C.c_int32, vp]
         L.olamgpu_set_profiling.argtypes = [vp, C.c_int32]
         L.olamgpu_kernel_profile.argtypes = [vp, C.c_char_p, C.c_int32]
         _lib = L
@@ -60,7 +63,10 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_tend0", "olamgpu_prog_wrtv_rk", "olamgpu_timeavg_momsc", "olamgpu_scalar_transport_rk",
            "olamgpu_scalar_transport_rk_adj", "olamgpu_upload_scalar_adj", "olamgpu_download_scalar_adj",
            "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
-           "olamgpu_set_profiling", "olamgpu_kernel_profile"]
+           "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
+           "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore"]
+
+CKPT_STATE, CKPT_TAPE = 1, 2
 
 
 def _p(a):
@@ -174,6 +180,18 @@ class Context:
             name, n, ms = line.split()
             out[name] = (int(n), float(ms))
         return out
+
+    # checkpoints (device buffers owned by the caller) ------------------------------
+    def checkpoint_bytes(self, kind):
+        n = C.c_int64()
+        self._chk(lib().olamgpu_checkpoint_bytes(self.h, int(kind), C.byref(n)), "checkpoint_bytes")
+        return n.value
+
+    def checkpoint_save(self, kind, dev_ptr):
+        self._chk(lib().olamgpu_checkpoint_save(self.h, int(kind), C.c_void_p(dev_ptr)), "checkpoint_save")
+
+    def checkpoint_restore(self, kind, dev_ptr):
+        self._chk(lib().olamgpu_checkpoint_restore(self.h, int(kind), C.c_void_p(dev_ptr)), "checkpoint_restore")
 
     # device-resident access --------------------------------------------------------
     def copy_field(self, name, index, dev_ptr, direction):
