@@ -143,6 +143,217 @@ __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a)
   }
 }
 
+// ---------------------------------------------------------------- LDS-staged stage
+// Patch tables (built on the host): see Patches in olamgpu.hip.
+struct PatchTab {
+  const int *cells, *cnt;
+  const uint16_t *nbr;
+  int pmax, rmax;
+};
+
+// Stencil record of one patch cell, staged in LDS once per block: the patch-local
+// indices of its NSLOT neighbour slots, its grad_t2d weights and npoly.  Laid out so
+// that each group is one or two 128-bit LDS reads.
+template <class R>
+struct __align__(16) PCell {
+  uint16_t nl[NSLOT];
+  R cx[NSLOT], cy[NSLOT];
+  int np, pad_[3];
+};
+
+// Fill the stencil records of the owned and ring-1 cells of patch p.
+template <class R>
+__device__ __forceinline__ void stage_pcells(const DMesh<R> &m, const PatchTab &pt, int p, int nr, PCell<R> *sc) {
+  const int *cells = pt.cells + (long)p * pt.pmax;
+  const uint16_t *lnb = pt.nbr + (long)p * pt.rmax * NSLOT;
+  for (int e = threadIdx.x; e < nr * NSLOT; e += blockDim.x) {
+    const int i = e / NSLOT, j = e - i * NSLOT;
+    const int g = cells[i];
+    sc[i].nl[j] = lnb[e];
+    sc[i].cx[j] = m.w_gxc[g * NSLOT + j];
+    sc[i].cy[j] = m.w_gyc[g * NSLOT + j];
+    if (j == 0) sc[i].np = m.w_npoly[g];
+  }
+}
+
+// Register copy of a stencil record (wide LDS reads of a wave-uniform address).
+template <class R>
+struct PRec {
+  int nl[NSLOT];
+  R cx[NSLOT], cy[NSLOT];
+  int np;
+  __device__ __forceinline__ void load(const PCell<R> &c) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(c.nl);
+    nl[0] = v.x & 0xffff; nl[1] = v.x >> 16; nl[2] = v.y & 0xffff; nl[3] = v.y >> 16;
+    nl[4] = v.z & 0xffff; nl[5] = v.z >> 16; nl[6] = v.w & 0xffff; nl[7] = v.w >> 16;
+#pragma unroll
+    for (int j = 0; j < NSLOT; ++j) { cx[j] = c.cx[j]; cy[j] = c.cy[j]; }
+    np = c.np;
+  }
+};
+
+// One explicit RK stage of all tracers for one patch of columns, with the gradients
+// computed in the block instead of materialised.  The block stages the stencil
+// records of its owned and ring-1 cells in LDS once, and each wave keeps the
+// tracer-independent data of its CPW owned columns (face mass fluxes, upwind choices,
+// wmasc, volti, 1/rho) in registers.  Per tracer, the stage-input column of every
+// patch cell (owned, ring 1, ring 2) is staged in LDS, one row of 64 levels per cell;
+// each face value is then rebuilt from LDS: the column's own grad_t2d for outflow
+// faces and the upwind neighbour's grad_t2d (from its neighbours, ring 2 at the patch
+// edge) for inflow faces, skipped when no level of the face takes it.  The next
+// tracer's rows and scp0 are loaded into registers while the current one is computed.
+// The arithmetic and its order are those of k_tr_grad + k_tr_update.
+template <class R, class RP, int CPW, int RPW>
+__global__ __launch_bounds__(512) void k_tr_patch(DMesh<R> m, TrStage<R, RP> a, PatchTab pt) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int p = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x));
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int NWV = 8;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk;
+  const int n_own = pt.cnt[p * 4], n_r1 = pt.cnt[p * 4 + 1], n_tot = pt.cnt[p * 4 + 2];
+  const int *cells = pt.cells + (long)p * pt.pmax;
+  PCell<R> *sc = reinterpret_cast<PCell<R> *>(smem_raw);
+  R *sq = reinterpret_cast<R *>(smem_raw + (size_t)pt.rmax * sizeof(PCell<R>));
+  stage_pcells(m, pt, p, n_own + n_r1, sc);
+  const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
+  // tracer-independent data of this wave's owned columns
+  R dvm[CPW][7], wmr[CPW], vtr[CPW], rinvr[CPW], fexr[CPW];
+  unsigned selm[CPW];
+  long icr[CPW];
+#pragma unroll
+  for (int t = 0; t < CPW; ++t) {
+    const int i = w + t * NWV;
+    selm[t] = 0; wmr[t] = R(0); vtr[t] = R(0); rinvr[t] = R(1); fexr[t] = R(0); icr[t] = l;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) dvm[t][j] = R(0);
+    if (i < n_own) {
+      const int c = cells[i];
+      const long ic = (long)c * ldk + l;
+      icr[t] = ic;
+      const int np = m.w_npoly[c];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int iv = m.w_face[c * NSLOT + j];
+        const R dv = m.w_dirv[c * NSLOT + j];
+        const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
+        dvm[t][j] = dv * vm;
+        const bool sel = (dv < R(0)) ? (vm < R(0)) : !(vm < R(0));
+        selm[t] |= (sel ? 1u : 0u) << j;
+      }
+      wmr[t] = ld(a.wmasc, ic, act);
+      fexr[t] = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
+      vtr[t] = ld(m.volti, ic, act);
+      const R rf = act ? (R)a.rho[ic] : R(1);
+      rinvr[t] = R(1) / (body ? rf : R(1));
+    }
+  }
+  // prefetch of tracer 0
+  R rq[RPW], q0r[CPW], str[CPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int i = w + r * NWV;
+    rq[r] = (i < n_tot) ? ld(a.q_in, (long)cells[i] * ldk + l, act) : R(0);
+  }
+#pragma unroll
+  for (int t = 0; t < CPW; ++t) {
+    const bool own = w + t * NWV < n_own;
+    q0r[t] = own ? ld(a.q0, icr[t], act) : R(0);
+    str[t] = (own && a.sct) ? ld(a.sct, icr[t], act) : R(0);
+  }
+  for (int n = 0; n < a.ntr; ++n) {
+    const long off = n * a.tstride;
+    __syncthreads();                                  // records staged / previous tracer done
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int i = w + r * NWV;
+      if (i < n_tot) sq[i * WAVE + l] = rq[r];
+    }
+    __syncthreads();
+    R q0c[CPW], stc[CPW];
+#pragma unroll
+    for (int t = 0; t < CPW; ++t) { q0c[t] = q0r[t]; stc[t] = str[t]; }
+    if (n + 1 < a.ntr) {                              // next tracer, in flight during compute
+      const long offn = off + a.tstride;
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        const int i = w + r * NWV;
+        rq[r] = (i < n_tot) ? ld(a.q_in + offn, (long)cells[i] * ldk + l, act) : R(0);
+      }
+#pragma unroll
+      for (int t = 0; t < CPW; ++t) {
+        const bool own = w + t * NWV < n_own;
+        q0r[t] = own ? ld(a.q0 + offn, icr[t], act) : R(0);
+        str[t] = (own && a.sct) ? ld(a.sct + offn, icr[t], act) : R(0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < CPW; ++t) {
+      const int i = w + t * NWV;
+      if (i >= n_own) continue;
+      const int c = cells[i];
+      PRec<R> me;
+      me.load(sc[i]);
+      const int np = me.np;
+      const R qc = sq[i * WAVE + l];
+      const R q0 = q0c[t];
+      R gxc = R(0), gyc = R(0);
+      if (a.order > 1) {
+        R ax = R(0), ay = R(0);
+        for_slots(np, [&](int j) {
+          const R d = sq[me.nl[j] * WAVE + l] - qc;
+          ax = ax + me.cx[j] * d;
+          ay = ay + me.cy[j] * d;
+        });
+        if (body) { gxc = ax; gyc = ay; }
+      }
+      R F = R(0);
+      for_slots(np, [&](int j) {
+        const int u = me.nl[j];
+        const R qn = sq[u * WAVE + l];
+        const bool sel = (selm[t] >> j) & 1u;
+        R fs = qc, fn = qn;
+        if (a.order > 1) {
+          fs = qc + m.w_dxs[c * NSLOT + j] * gxc + m.w_dys[c * NSLOT + j] * gyc;
+          if (__ballot(sel && act) != 0) {
+            PRec<R> nb;
+            nb.load(sc[u]);
+            R bx = R(0), by = R(0);
+            for_slots(nb.np, [&](int i2) {
+              const R d = sq[nb.nl[i2] * WAVE + l] - qn;
+              bx = bx + nb.cx[i2] * d;
+              by = by + nb.cy[i2] * d;
+            });
+            if (!body) { bx = R(0); by = R(0); }
+            fn = qn + m.w_dxn[c * NSLOT + j] * bx + m.w_dyn[c * NSLOT + j] * by;
+          }
+        }
+        const R f = sel ? fn : fs;
+        F = F + dvm[t][j] * (f - q0);
+      });
+      const R wm = wmr[t];
+      const R wm_m1 = shup(wm, 1);
+      const R qp1 = shdn(qc, 1);
+      const R ds = inner ? qp1 - qc : R(0);
+      const R dsm1 = shup(ds, 1);
+      const R scpb = qc + ds * a11 + dsm1 * a21;
+      const R scpt = qc + ds * a12 + dsm1 * a22;
+      const R scpb_p1 = shdn(scpb, 1);
+      R upw = (wm < R(0)) ? scpb_p1 : scpt;
+      if (k == 1) upw = qp1;
+      if (k == mza) upw = qc;
+      const R upw_m1 = shup(upw, 1);
+      F = F + fexr[t] * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
+      R qn_new = q0 + a.dtr * (stc[t] + vtr[t] * F) * rinvr[t];
+      const R lev1 = shdn(qn_new, 1);
+      if (k == 1) qn_new = lev1;
+      if (act) a.q_out[off + icr[t]] = qn_new;
+    }
+  }
+}
+
 // Implicit vertical transport of one stage for columns with ff_implic > 0.001:
 // the tridif_prep/tridif_fini pair (tridiag_lib.f90:321-364) applied after the
 // explicit update (scalar_transport_rk.F90:186-217, :347-375), solved in FP64 by
@@ -476,6 +687,189 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     if (!body) lq = R(0);
     if (a.last) lq = lq + ld(a.lam0 + off, ic, act);   // stage 1 read q = scp0
     if (act) a.lq[off + ic] = lq;
+  }
+}
+
+// ---------------------------------------------------------------- LDS-staged adjoint
+// One adjoint RK stage of all tracers for one patch: k_tr_adj0, k_tr_adj1 and
+// k_tr_adj2 in one pass, with nu, the face-flux sums and the gradient adjoints kept
+// in LDS instead of global memory.  Per tracer:
+//   A. nu = mu dtr volti / rho of every patch cell (owned, rings 1 and 2) -> LDS;
+//   B. for owned and ring-1 cells, the face adjoints phi of the faces where the cell
+//      is upwind, summed into the gradient adjoints (Gx, Gy) -> LDS; for owned cells
+//      also the scp0 terms, the vertical transpose and the face sum P (registers);
+//   C. for owned cells, the grad_t2d transpose from the (Gx, Gy) of the column and of
+//      its neighbours, and the stage-input adjoint.
+// The arithmetic and its order are those of the three separate kernels.
+template <class R, class RP, int CPW>
+__global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a, PatchTab pt) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int p = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x));
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nwv = blockDim.x >> 6;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk;
+  const int n_own = pt.cnt[p * 4], n_r1 = pt.cnt[p * 4 + 1], n_tot = pt.cnt[p * 4 + 2];
+  const int nr = n_own + n_r1;
+  const int *cells = pt.cells + (long)p * pt.pmax;
+  PCell<R> *sc = reinterpret_cast<PCell<R> *>(smem_raw);
+  R *snu = reinterpret_cast<R *>(smem_raw + (size_t)pt.rmax * sizeof(PCell<R>));
+  R *sgx = snu + (size_t)pt.pmax * WAVE;
+  R *sgy = sgx + (size_t)pt.rmax * WAVE;
+  stage_pcells(m, pt, p, nr, sc);
+  const R a11 = m.aw11[l], a12 = m.aw12[l];
+  const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
+  // tracer-independent data of this wave's owned columns
+  R vmj[CPW][7], cc[CPW], wmr[CPW], fexr[CPW];
+  unsigned upm[CPW], iw1m[CPW];
+#pragma unroll
+  for (int t = 0; t < CPW; ++t) {
+    const int i = w + t * nwv;
+    upm[t] = 0; iw1m[t] = 0; cc[t] = R(0); wmr[t] = R(0); fexr[t] = R(0);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) vmj[t][j] = R(0);
+    if (i < n_own) {
+      const int c = cells[i];
+      const long ic = (long)c * ldk + l;
+      const int np = sc[i].np;
+      R csum = R(0);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int iv = m.w_face[c * NSLOT + j];
+        const R dv = m.w_dirv[c * NSLOT + j];
+        const R vmf = ld(a.vmasc, (long)iv * ldk + l, act);
+        if (j < 6 || np > 6) csum = csum + dv * vmf;
+        const R vm = (j < 6 || np > 6) ? vmf : R(0);
+        vmj[t][j] = vm * dv * dv;
+        const bool iw1 = dv < R(0);
+        const bool up = iw1 ? !(vm < R(0)) : (vm < R(0));
+        iw1m[t] |= (iw1 ? 1u : 0u) << j;
+        upm[t] |= (up ? 1u : 0u) << j;
+      }
+      wmr[t] = ld(a.wmasc, ic, act);
+      fexr[t] = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
+      cc[t] = csum + fexr[t] * (shup(wmr[t], 1) - wmr[t]);
+    }
+  }
+  for (int n = 0; n < a.ntr; ++n) {
+    const long off = n * a.tstride;
+    __syncthreads();                                  // records staged / previous tracer done
+    // A: nu of every patch cell
+    for (int i = w; i < n_tot; i += nwv) {
+      const long ix = (long)cells[i] * ldk + l;
+      R mu = ld(a.mu + off, ix, act);
+      const R mu1 = __shfl(mu, 0, WAVE);
+      if (k == 1) mu = R(0);
+      if (k == 2) mu = mu + mu1;
+      const R vt = ld(m.volti, ix, act);
+      const R rf = act ? (R)a.rho[ix] : R(1);
+      const R rinv = R(1) / (body ? rf : R(1));
+      snu[i * WAVE + l] = body ? mu * a.dtr * vt * rinv : R(0);
+    }
+    __syncthreads();
+    // B: owned columns
+    R lpart[CPW], l0r[CPW];
+#pragma unroll
+    for (int t = 0; t < CPW; ++t) {
+      lpart[t] = R(0); l0r[t] = R(0);
+      const int i = w + t * nwv;
+      if (i >= n_own) continue;
+      const int c = cells[i];
+      const long ic = (long)c * ldk + l;
+      const PCell<R> &me = sc[i];
+      R mu = ld(a.mu + off, ic, act);
+      const R mu1 = __shfl(mu, 0, WAVE);
+      if (k == 1) mu = R(0);
+      if (k == 2) mu = mu + mu1;
+      R l0 = ld(a.lam0 + off, ic, act);
+      const R nu = snu[i * WAVE + l];
+      if (body) l0 = l0 + mu - nu * cc[t];
+      const R wm = wmr[t], fexpl = fexr[t];
+      const bool neg = wm < R(0);
+      const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
+      const R nu_p1 = shdn(nu, 1);
+      const R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
+      const R lupw_m1 = shup(lupw, 1), lupw_p1 = shdn(lupw, 1);
+      R lqv = R(0), lds = R(0);
+      if (body) {
+        if (inner && !neg) lqv = lqv + lupw;
+        if (k - 1 >= 2 && neg_m1) lqv = lqv + lupw_m1;
+        if (k - 1 >= 2 && neg_m1) lds = lds + lupw_m1 * a11;
+        if (inner && neg) lds = lds + lupw * a21_p1;
+        if (inner && !neg) lds = lds + lupw * a12;
+        if (k + 1 <= mza - 1 && !neg_p1) lds = lds + lupw_p1 * a22_p1;
+      }
+      if (!inner) lds = R(0);
+      const R lds_m1 = shup(lds, 1);
+      if (body) lqv = lqv + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
+      R P = R(0), Gx = R(0), Gy = R(0);
+      for_slots(me.np, [&](int j) {
+        const R nun = snu[me.nl[j] * WAVE + l];
+        const R phi = ((iw1m[t] >> j) & 1u) ? vmj[t][j] * (nun - nu) : vmj[t][j] * (nu - nun);
+        const R ph = ((upm[t] >> j) & 1u) ? phi : R(0);
+        P = P + ph;
+        Gx = Gx + m.w_dxs[c * NSLOT + j] * ph;
+        Gy = Gy + m.w_dys[c * NSLOT + j] * ph;
+      });
+      if (!body) { P = R(0); Gx = R(0); Gy = R(0); }
+      lpart[t] = lqv + P;
+      l0r[t] = l0;
+      if (act) a.lam0[off + ic] = l0;
+      sgx[i * WAVE + l] = Gx;
+      sgy[i * WAVE + l] = Gy;
+    }
+    // B: ring-1 cells (gradient adjoints only)
+    if (a.order > 1) {
+      for (int i = n_own + w; i < nr; i += nwv) {
+        const int x = cells[i];
+        const PCell<R> &me = sc[i];
+        const R nu = snu[i * WAVE + l];
+        R Gx = R(0), Gy = R(0);
+        for_slots(me.np, [&](int j) {
+          const int iv = m.w_face[x * NSLOT + j];
+          const R dv = m.w_dirv[x * NSLOT + j];
+          const R vm = ld(a.vmasc, (long)iv * ldk + l, act);
+          const R vj = vm * dv * dv;
+          const bool iw1 = dv < R(0);
+          const bool up = iw1 ? !(vm < R(0)) : (vm < R(0));
+          const R nun = snu[me.nl[j] * WAVE + l];
+          const R phi = iw1 ? vj * (nun - nu) : vj * (nu - nun);
+          const R ph = up ? phi : R(0);
+          Gx = Gx + m.w_dxs[x * NSLOT + j] * ph;
+          Gy = Gy + m.w_dys[x * NSLOT + j] * ph;
+        });
+        if (!body) { Gx = R(0); Gy = R(0); }
+        sgx[i * WAVE + l] = Gx;
+        sgy[i * WAVE + l] = Gy;
+      }
+    }
+    __syncthreads();
+    // C: grad_t2d transpose and the stage-input adjoint of owned columns
+#pragma unroll
+    for (int t = 0; t < CPW; ++t) {
+      const int i = w + t * nwv;
+      if (i >= n_own) continue;
+      const int c = cells[i];
+      const long ic = (long)c * ldk + l;
+      const PCell<R> &me = sc[i];
+      R lq = lpart[t];
+      if (a.order > 1) {
+        const R gxs = sgx[i * WAVE + l], gys = sgy[i * WAVE + l];
+        for_slots(me.np, [&](int j) {
+          const int u = me.nl[j];
+          const int b = m.w_back[c * NSLOT + j];
+          const R bx = (j < me.np) ? sc[u].cx[b] : R(0);
+          const R by = (j < me.np) ? sc[u].cy[b] : R(0);
+          lq = lq + (bx * sgx[u * WAVE + l] + by * sgy[u * WAVE + l]);
+          lq = lq - (me.cx[j] * gxs + me.cy[j] * gys);
+        });
+      }
+      if (!body) lq = R(0);
+      if (a.last) lq = lq + l0r[t];
+      if (act) a.lq[off + ic] = lq;
+    }
   }
 }
 

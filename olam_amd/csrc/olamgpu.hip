@@ -90,49 +90,68 @@ struct Prof {
   ~Prof() { for (auto e : pool) (void)hipEventDestroy(e); }
 };
 
-#define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...)                                  \
+#define LAUNCH_SH(NAME, KERNEL, GRID, BLOCK, SHMEM, ...)                       \
   do {                                                                          \
     hipEvent_t e0_ = nullptr, e1_ = nullptr;                                    \
     if (prof && prof->on) {                                                     \
       e0_ = prof->get(); e1_ = prof->get();                                     \
       HIPCHK(hipEventRecord(e0_, s));                                           \
     }                                                                           \
-    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, s, __VA_ARGS__);                 \
+    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHMEM, s, __VA_ARGS__);             \
     if (e0_) {                                                                  \
       HIPCHK(hipEventRecord(e1_, s));                                           \
       prof->pending.push_back({NAME, e0_, e1_});                                \
     }                                                                           \
   } while (0)
+#define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...) LAUNCH_SH(NAME, KERNEL, GRID, BLOCK, 0, __VA_ARGS__)
 
 // ------------------------------------------------------------------ renumbering
 // 3-D Hilbert index of a point on the unit sphere (Skilling's transpose method,
 // 21 bits per axis).  Gives each XCD's contiguous block range a compact patch.
-static uint64_t hilbert3(double x, double y, double z) {
-  const int B = 21;
-  uint32_t X[3];
-  double c[3] = {x, y, z};
-  for (int i = 0; i < 3; ++i) {
-    double t = (c[i] + 1.0) * 0.5;
-    t = std::min(std::max(t, 0.0), 1.0);
-    X[i] = (uint32_t)std::min((double)((1u << B) - 1), t * (double)(1u << B));
-  }
-  uint32_t M = 1u << (B - 1), P, Q, t;
-  for (Q = M; Q > 1; Q >>= 1) {
-    P = Q - 1;
-    for (int i = 0; i < 3; ++i) {
-      if (X[i] & Q) X[0] ^= P;
-      else { t = (X[0] ^ X[i]) & P; X[0] ^= t; X[i] ^= t; }
+// Locality key of a point on the unit sphere: the cube face it projects to (largest
+// component), then the 2-D Hilbert index of its equal-angle face coordinates.  Runs of
+// consecutive keys form compact patches on the sphere, which keeps the halo of a block
+// of consecutive columns small.
+static uint64_t hilbert2(uint32_t x, uint32_t y, int bits) {
+  const uint32_t n = 1u << bits;
+  uint64_t d = 0;
+  for (uint32_t s = n >> 1; s > 0; s >>= 1) {
+    uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += (uint64_t)s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) { x = n - 1 - x; y = n - 1 - y; }
+      std::swap(x, y);
     }
   }
-  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
-  t = 0;
-  for (Q = M; Q > 1; Q >>= 1) if (X[2] & Q) t ^= Q - 1;
-  for (int i = 0; i < 3; ++i) X[i] ^= t;
-  uint64_t h = 0;
-  for (int b = B - 1; b >= 0; --b)
-    for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
-  return h;
+  return d;
 }
+static uint64_t sphere_key(double x, double y, double z) {
+  const double c[3] = {x, y, z};
+  int ax = 0;
+  for (int i = 1; i < 3; ++i) if (std::fabs(c[i]) > std::fabs(c[ax])) ax = i;
+  const int face = ax * 2 + (c[ax] > 0.0 ? 1 : 0);
+  const double den = std::fabs(c[ax]) > 0.0 ? std::fabs(c[ax]) : 1.0;
+  const int bits = 20;
+  auto coord = [&](double v) {
+    double a = (std::atan(v / den) / (0.25 * M_PI) + 1.0) * 0.5;
+    a = std::min(std::max(a, 0.0), 1.0);
+    return (uint32_t)std::min((double)((1u << bits) - 1), a * (double)(1u << bits));
+  };
+  const uint32_t u = coord(c[(ax + 1) % 3]), v = coord(c[(ax + 2) % 3]);
+  return ((uint64_t)face << 41) | hilbert2(u, v, bits);
+}
+
+// Column patches for the LDS-staged tracer kernels.  Patch p owns the consecutive
+// columns [p*P, p*P + n_own); its cell list is the owned columns, then ring 1 (their
+// neighbours outside the patch), then ring 2 (neighbours of ring 1 outside both).  For
+// every owned and ring-1 cell, nbr holds the patch-local index of each of its NSLOT
+// neighbour slots (padded slots point at the cell itself, like w_nbr).
+struct Patches {
+  int P = 0, np = 0, pmax = 0, rmax = 0;
+  std::vector<int> cells;        // np * pmax global column ids (unused entries: 0)
+  std::vector<int> cnt;          // np * 4: n_own, n_r1, n_tot, 0
+  std::vector<uint16_t> nbr;     // np * rmax * NSLOT
+};
 
 struct Topo {
   int nw, nv, nm, mza, ldk;
@@ -186,9 +205,9 @@ static Topo build_topo(const olamgpu_mesh *m) {
       int v = in.miv(j, i), o = in.mim(j, i);
       if (v < 0 || v >= t.nv || o < 0 || o >= t.nm) throw Err(OLAMGPU_EINVAL, "itab_m index out of range");
     }
-  // W: Hilbert order of the unit position vectors (wnx, wny, wnz)
+  // W: locality order of the unit position vectors (wnx, wny, wnz)
   std::vector<uint64_t> key(t.nw);
-  for (int w = 0; w < t.nw; ++w) key[w] = hilbert3(m->wnx[w + 1], m->wny[w + 1], m->wnz[w + 1]);
+  for (int w = 0; w < t.nw; ++w) key[w] = sphere_key(m->wnx[w + 1], m->wny[w + 1], m->wnz[w + 1]);
   t.perm_w.resize(t.nw);
   std::iota(t.perm_w.begin(), t.perm_w.end(), 0);
   std::stable_sort(t.perm_w.begin(), t.perm_w.end(), [&](int a, int b) { return key[a] < key[b]; });
@@ -279,6 +298,49 @@ static Topo build_topo(const olamgpu_mesh *m) {
   return t;
 }
 
+static Patches build_patches(const Topo &t, int P) {
+  Patches pt;
+  pt.P = P;
+  pt.np = (t.nw + P - 1) / P;
+  std::vector<int> stamp(t.nw, -1), loc(t.nw, 0);
+  std::vector<std::vector<int>> lists(pt.np);
+  for (int p = 0; p < pt.np; ++p) {
+    std::vector<int> &L = lists[p];
+    const int c0 = p * P, c1 = std::min(t.nw, c0 + P);
+    for (int c = c0; c < c1; ++c) { stamp[c] = p; loc[c] = (int)L.size(); L.push_back(c); }
+    const int n_own = (int)L.size();
+    for (int i = 0; i < n_own; ++i)
+      for (int j = 0; j < t.w_npoly[L[i]]; ++j) {
+        int n = t.w_nbr[(size_t)L[i] * NSLOT + j];
+        if (stamp[n] != p) { stamp[n] = p; loc[n] = (int)L.size(); L.push_back(n); }
+      }
+    const int n_r1 = (int)L.size() - n_own;
+    for (int i = n_own; i < n_own + n_r1; ++i)
+      for (int j = 0; j < t.w_npoly[L[i]]; ++j) {
+        int n = t.w_nbr[(size_t)L[i] * NSLOT + j];
+        if (stamp[n] != p) { stamp[n] = p; loc[n] = (int)L.size(); L.push_back(n); }
+      }
+    pt.pmax = std::max(pt.pmax, (int)L.size());
+    pt.rmax = std::max(pt.rmax, n_own + n_r1);
+    pt.cnt.push_back(n_own); pt.cnt.push_back(n_r1); pt.cnt.push_back((int)L.size()); pt.cnt.push_back(0);
+  }
+  if (pt.pmax > 65535) throw Err(OLAMGPU_EINVAL, "patch too large");
+  pt.cells.assign((size_t)pt.np * pt.pmax, 0);
+  pt.nbr.assign((size_t)pt.np * pt.rmax * NSLOT, 0);
+  for (int p = 0; p < pt.np; ++p) {
+    const std::vector<int> &L = lists[p];
+    for (size_t i = 0; i < L.size(); ++i) { stamp[L[i]] = p; loc[L[i]] = (int)i; pt.cells[(size_t)p * pt.pmax + i] = L[i]; }
+    const int nr = pt.cnt[p * 4] + pt.cnt[p * 4 + 1];
+    for (int i = 0; i < nr; ++i)
+      for (int j = 0; j < NSLOT; ++j) {
+        int n = t.w_nbr[(size_t)L[i] * NSLOT + j];
+        if (stamp[n] != p) throw Err(OLAMGPU_EINVAL, "patch construction failed");
+        pt.nbr[((size_t)p * pt.rmax + i) * NSLOT + j] = (uint16_t)loc[n];
+      }
+  }
+  return pt;
+}
+
 // ------------------------------------------------------------------ implementation
 struct ImplBase {
   virtual ~ImplBase() {}
@@ -316,6 +378,15 @@ struct Impl : ImplBase {
   DevBuf b_dirv, b_gxc, b_gyc, b_dxs, b_dys, b_dxn, b_dyn, b_ecx, b_ecy, b_ecz, b_xxyy, b_wn[3];
   DevBuf b_vn[3], b_vfact, b_xxyym, b_lev[7], b_lev8[4];
   DevBuf b_arw, b_volti, b_volwi, b_volt, b_arv, b_volvi;
+  // column patches of the LDS-staged tracer kernels (forward and adjoint)
+  struct PatchSet {
+    Patches h;
+    DevBuf cells, cnt, nbr;
+    PatchTab tab{};
+    int cpw = 1;
+    size_t lds = 0;
+  } pf, pa;
+  bool use_patch = true;
   // state
   DevBuf thil[2], wmc, wc, vxe, vye, vze, vmc, vc, vmasc, wmasc, alpha, pwfac, pvfac, thilt, vmxet, vmyet, vmzet, vmt;
   DevBuf rho, press, rho_old;
@@ -366,6 +437,50 @@ struct Impl : ImplBase {
     build_mesh(m);
     alloc_state();
     if (const char *e = std::getenv("OLAMGPU_TPW")) tpw = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("OLAMGPU_TRACER")) use_patch = std::strcmp(e, "gather") != 0;
+    build_patch_tables();
+  }
+
+  // Patches of 8 * cpw columns (one block of 8 waves, cpw owned columns per wave),
+  // halved from the largest size until the LDS of the kernel fits the device's limit
+  // per workgroup.  Forward: the stencil records of owned + ring-1 cells and one staged
+  // tracer row per patch cell.  Adjoint: in addition the gradient adjoints (Gx, Gy) of
+  // owned + ring-1 cells.
+  size_t lds_limit = 65536;
+  // staged rows per wave that the forward kernel prefetches into registers
+  static constexpr int rows_per_wave(int cpw) { return cpw == 4 ? 20 : (cpw == 2 ? 16 : 12); }
+  size_t lds_fwd(const Patches &h) const { return (size_t)h.rmax * sizeof(PCell<R>) + (size_t)h.pmax * WAVE * sizeof(R); }
+  size_t lds_adj(const Patches &h) const { return lds_fwd(h) + (size_t)h.rmax * 2 * WAVE * sizeof(R); }
+  void build_patch_set(PatchSet &ps, bool adj) {
+    int cpw = 4;
+    // OLAMGPU_PATCH_CPW (forward) / OLAMGPU_ADJ_CPW (adjoint): 1, 2 or 4, for tuning
+    if (const char *e = std::getenv(adj ? "OLAMGPU_ADJ_CPW" : "OLAMGPU_PATCH_CPW")) {
+      const int v = std::atoi(e);
+      if (v == 1 || v == 2 || v == 4) cpw = v;
+    }
+    for (;;) {
+      ps.h = build_patches(t, 8 * cpw);
+      ps.lds = adj ? lds_adj(ps.h) : lds_fwd(ps.h);
+      if ((ps.lds <= lds_limit && ps.h.pmax <= 8 * rows_per_wave(cpw)) || cpw == 1) break;
+      cpw /= 2;
+    }
+    if (ps.lds > lds_limit || ps.h.pmax > 8 * rows_per_wave(cpw))
+      throw Err(OLAMGPU_EINVAL, "mesh neighbourhoods too large for the LDS patches");
+    ps.cpw = cpw;
+    upv(ps.cells, ps.h.cells); upv(ps.cnt, ps.h.cnt);
+    ps.nbr.alloc(ps.h.nbr.size() * sizeof(uint16_t));
+    HIPCHK(hipMemcpyAsync(ps.nbr.p, ps.h.nbr.data(), ps.nbr.bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ps.tab.cells = (const int *)ps.cells.p; ps.tab.cnt = (const int *)ps.cnt.p; ps.tab.nbr = (const uint16_t *)ps.nbr.p;
+    ps.tab.pmax = ps.h.pmax; ps.tab.rmax = ps.h.rmax;
+  }
+  void build_patch_tables() {
+    int dev = 0, v = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      lds_limit = (size_t)v;
+    build_patch_set(pf, false);
+    build_patch_set(pa, true);
   }
 
   // -------------------------------------------------------------- mesh upload
@@ -781,10 +896,17 @@ struct Impl : ImplBase {
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.tstride = tstride;
       a.dtr = dtlr[istage - 1];
       a.tpw = tpw;
-      if (a.order > 1)
-        LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
-               ntr, tstride, tpw);
-      LAUNCH("k_tr_update", (k_tr_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (use_patch) {
+        const dim3 g(pf.h.np), b(512);
+        if (pf.cpw == 4) LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 4, rows_per_wave(4)>), g, b, pf.lds, dm, a, pf.tab);
+        else if (pf.cpw == 2) LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 2, rows_per_wave(2)>), g, b, pf.lds, dm, a, pf.tab);
+        else LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 1, rows_per_wave(1)>), g, b, pf.lds, dm, a, pf.tab);
+      } else {
+        if (a.order > 1)
+          LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
+                 ntr, tstride, tpw);
+        LAUNCH("k_tr_update", (k_tr_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      }
       if (any_imp) LAUNCH("k_tr_implicit", (k_tr_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
       in = out;
     }
@@ -817,9 +939,16 @@ struct Impl : ImplBase {
       a.tstride = tstride; a.dtr = dtlr[istage - 1];
       a.tpw = tpw;
       if (any_imp) LAUNCH("k_tr_adj_implicit", (k_tr_adj_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
-      LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-      LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-      LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (use_patch) {
+        const dim3 g(pa.h.np), b(512);
+        if (pa.cpw == 4) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 4>), g, b, pa.lds, dm, a, pa.tab);
+        else if (pa.cpw == 2) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 2>), g, b, pa.lds, dm, a, pa.tab);
+        else LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 1>), g, b, pa.lds, dm, a, pa.tab);
+      } else {
+        LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      }
       cur_adj = 1 - cur_adj;
     }
     HIPCHK(hipGetLastError());

@@ -95,6 +95,8 @@ struct Base {
   virtual void scalar_transport_rk_adj() = 0;
   virtual void mass_sums(double *) = 0;
   virtual void cfl_report(double *) = 0;
+  virtual int set_pdef(int, int) = 0;
+  virtual void set_frozen(int) = 0;
   std::string err;
 };
 
@@ -116,6 +118,7 @@ struct Model : Base {
   std::vector<R> dzim, dzit, zfacit, gdz_wgtm, gdz_wgtp;
   std::vector<double> zwgt_top8, zwgt_bot8, gdz_wgtm8, gdz_wgtp8;
   std::vector<R> aw;   // a_w(k,i,j) at aw[((j-1)*2 + (i-1))*mza + k-1]
+  std::vector<R> awu;  // a_wu, same layout (unbounded vertical quadratic)
   F2<R> arw, volti, volwi, arv, volvi;
   F2<double> volt;
   // ---- state (mem_basic, mem_tend) ----
@@ -123,6 +126,11 @@ struct Model : Base {
   F2<R> thilt, vmxet, vmyet, vmzet, vmt, rho_old;
   F2<RP> rho, press;
   std::vector<F2<R>> scp, sct, lam;
+  std::vector<int> pdef;                    // scalar_tab(n)%pdef (var_tables.f90:58)
+  // frozen limiter scales of the last forward step's final stage, per tracer
+  std::vector<F2<R>> tape_sv, tape_sw;      // per V face / per W level
+  std::vector<char> tape_lim;               // 0 none, 1 monotone, 2 positive definite
+  bool frozen_lim = false;                  // test hook: replay the taped scales
   // ---- SAVEd damping state (vort_div_damp.f90:23-33,255-262) ----
   bool div_first = true, vort_first = true;
   F2<R> div2d[2], vort2d[2], fdivh;
@@ -135,6 +143,7 @@ struct Model : Base {
   double cfl_out[4] = {0, 0, 0, 0};
 
   R aw_(int k, int i, int j) const { return aw[((j - 1) * 2 + (i - 1)) * mza + (k - 1)]; }
+  R awu_(int k, int i, int j) const { return awu[((j - 1) * 2 + (i - 1)) * mza + (k - 1)]; }
 
   Model(const olamgpu_mesh *m, const olamgpu_params *p) {
     prm = *p;
@@ -164,6 +173,8 @@ struct Model : Base {
     for (int k = 1; k <= mza; ++k) { gdz_wgtm[k] = (R)gdz_wgtm8[k]; gdz_wgtp[k] = (R)gdz_wgtp8[k]; }
     aw.assign(4 * mza, 0);
     for (int i = 0; i < 4 * mza; ++i) aw[i] = (R)m->a_w[i];
+    awu.assign(4 * mza, 0);
+    for (int i = 0; i < 4 * mza; ++i) awu[i] = (R)m->a_wu[i];
     load2(arw, m->arw, mza, mwa); load2(arv, m->arv, mza, mva);
     load2(volt, m->volt, mza, mwa);
     if (sizeof(R) == 8) {
@@ -191,6 +202,14 @@ struct Model : Base {
     int ns = std::max(prm.num_scalar, 0);
     scp.resize(ns); sct.resize(ns); lam.resize(ns);
     for (int n = 0; n < ns; ++n) { scp[n].alloc(mza, mwa); sct[n].alloc(mza, mwa); lam[n].alloc(mza, mwa); }
+    pdef.assign(ns, 1);
+    tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
+  }
+  void set_frozen(int v) override { frozen_lim = v != 0; }
+  int set_pdef(int n, int v) override {
+    if (n < 1 || n > (int)pdef.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
+    pdef[n - 1] = v != 0;
+    return 0;
   }
 
   // ---------------------------------------------------------------- transfers
@@ -836,8 +855,25 @@ struct Model : Base {
     for (size_t n = 0; n < scp.size(); ++n) {
       F2<R> &q = scp[n];
       scp0 = q;
+      const int imonot = prm.iscal_monot;
+      tape_lim[n] = 0;
       for (int istage = 1; istage <= nrk; ++istage) {
-        update_scalar(q, scp0, sct[n], rhos[nrk - 1], dtlr[istage - 1], prm.scal_horiz_adv_order, istage, nrk);
+        const R dtr = dtlr[istage - 1];
+        // final stage of the monotone / positive-definite options: low-order update of
+        // scp0 by the tendency first (scalar_transport_rk.F90:297-313)
+        if (istage == nrk && imonot > 0 && pdef[n]) {
+#pragma omp parallel for
+          for (int iw = 2; iw <= mwa; ++iw) {
+            for (int k = 2; k <= mza; ++k) scp0(k, iw) = scp0(k, iw) + dtr * sct[n](k, iw) / (*ro)(k, iw);
+            scp0(1, iw) = scp0(2, iw);
+          }
+        }
+        if (istage == nrk && imonot == 1)
+          update_scalar_lim(n, q, scp0, rhos[nrk - 1], dtr, prm.scal_horiz_adv_order, istage, nrk, 1);
+        else if (istage == nrk && imonot == 2 && pdef[n])
+          update_scalar_lim(n, q, scp0, rhos[nrk - 1], dtr, prm.scal_horiz_adv_order, istage, nrk, 2);
+        else
+          update_scalar(q, scp0, sct[n], rhos[nrk - 1], dtr, prm.scal_horiz_adv_order, istage, nrk);
         if (nimp) {
 #pragma omp parallel for
           for (int iw = 2; iw <= mwa; ++iw) implicit_column(q, scp0, iw, istage, dtlr[istage - 1], dtlr, nrk);
@@ -848,6 +884,335 @@ struct Model : Base {
     tape_vmasc = vmasc; tape_wmasc = wmasc; tape_ffimp = ffimp;
     for (int n = 0; n < nrk; ++n) tape_rhos[n] = rhos[n];
     have_tape = true;
+  }
+
+  // ---------------------------------------------------------------- limiters
+  // grad_z_quadratic (grad_lib.f90:166-229) with its options: itop/ibot = 1 extend the
+  // end differences (constant gradient), bounded selects a_w or the unbounded a_wu.
+  void grad_z_q(const R *s, R *sb, R *st, int itop, int ibot, bool bounded) const {
+    R ds[130];
+    const int ka = 2;
+    for (int k = ka; k <= mza - 1; ++k) ds[k] = s[k + 1] - s[k];
+    ds[ka - 1] = ibot ? ds[ka] : R(0);
+    ds[mza] = itop ? ds[mza - 1] : R(0);
+    for (int k = ka; k <= mza; ++k) {
+      if (bounded) {
+        sb[k] = s[k] + ds[k] * aw_(k, 1, 1) + ds[k - 1] * aw_(k, 2, 1);
+        st[k] = s[k] + ds[k] * aw_(k, 1, 2) + ds[k - 1] * aw_(k, 2, 2);
+      } else {
+        sb[k] = s[k] + ds[k] * awu_(k, 1, 1) + ds[k - 1] * awu_(k, 2, 1);
+        st[k] = s[k] + ds[k] * awu_(k, 1, 2) + ds[k - 1] * awu_(k, 2, 2);
+      }
+    }
+  }
+
+  F2<R> l_sfv, l_upv, l_hiv, l_s1, l_s2, l_upw, l_hiw, l_sfw;
+  // update_scalar_monot (mode 1, scalar_transport_rk.F90:619-1081) and
+  // update_scalar_pd (mode 2, :1085-1472): the final-stage flux renormalisation of
+  // the high-order corrections (scp_hiv, scp_hiw) to the low-order upwind fluxes from
+  // scp0, with iorderv = 3 and centered_monot = .false. (:20-21).  The applied face and
+  // vertical scales are kept in tape_sv / tape_sw for the frozen-limiter adjoint.
+  void update_scalar_lim(int n, F2<R> &q, const F2<R> &q0, const F2<R> &rhosf, R dtr, int iorder, int istage,
+                         int nrk, int mode) {
+    const R eps0 = R(1.e-32), eps1 = R(3.e-07), onep = R(1) + eps1, onem = R(1) - eps1;
+    if (!l_sfv.n) {
+      l_sfv.alloc(mza, mva); l_upv.alloc(mza, mva); l_hiv.alloc(mza, mva);
+      l_s1.alloc(mza, mwa); l_s2.alloc(mza, mwa); l_upw.alloc(mza + 1, mwa); l_hiw.alloc(mza + 1, mwa);
+      l_sfw.alloc(mza + 1, mwa);
+    }
+    if (!tape_sv[n].n) { tape_sv[n].alloc(mza, mva); tape_sw[n].alloc(mza, mwa); }
+    tape_lim[n] = (char)mode;
+    const bool frozen = frozen_lim;   // the linear map with the last forward's scales
+    if (iorder > 1) {
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, q, gxt.col(iw) - 1, gyt.col(iw) - 1);
+    }
+    // scalar_vflux_monot (:748-827) / scalar_vflux_pd (:1227-1290)
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        R s1, s2;
+        if (iorder == 1) { s1 = q(k, iw1); s2 = (mode == 2) ? q(k, iw1) : q(k, iw2); }
+        else {
+          s1 = q(k, iw1) + dxps(1, iv) * gxt(k, iw1) + dyps(1, iv) * gyt(k, iw1);
+          s2 = q(k, iw2) + dxps(2, iv) * gxt(k, iw2) + dyps(2, iv) * gyt(k, iw2);
+        }
+        R up = q0(k, iw1);
+        if (vmasc(k, iv) < R(0)) up = q0(k, iw2);
+        R hi = s1;
+        if (vmasc(k, iv) < R(0)) hi = s2;
+        l_upv(k, iv) = up;
+        l_hiv(k, iv) = hi - up;
+        l_sfv(k, iv) = vmasc(k, iv) * l_hiv(k, iv);
+      }
+    }
+    // scalar_wflux_and_scales_monot (:830-1010) / scalar_wflux_and_scale_pd (:1293-1402)
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      const int kb = 2;
+      const R fimpl = ffimp[(size_t)(iw - 1) * nrk + istage - 1];
+      const R fexpl = R(1) - fimpl;
+      R sb[130], stp[130], smax[130], smin[130], fl[130], fin[130], fout[130];
+      R *upw = &l_upw(1, iw) - 1, *hiw = &l_hiw(1, iw) - 1, *sfw = &l_sfw(1, iw) - 1;
+      if (mode == 1) { upw[kb - 1] = q0(kb, iw); upw[mza] = q0(mza, iw); }
+      else { upw[kb - 1] = q(kb, iw); upw[mza] = q(mza, iw); }
+      for (int k = kb; k <= mza - 1; ++k) {
+        upw[k] = q0(k, iw);
+        if (wmasc(k, iw) < R(0)) upw[k] = q0(k + 1, iw);
+      }
+      if (mode == 1) grad_z_q(&q(1, iw) - 1, sb, stp, 1, 1, false);
+      else grad_z_q(&q(1, iw) - 1, sb, stp, 0, 0, true);
+      sfw[kb - 1] = 0; sfw[mza] = 0; hiw[kb - 1] = 0; hiw[mza] = 0;
+      for (int k = kb; k <= mza - 1; ++k) {
+        hiw[k] = stp[k];
+        if (wmasc(k, iw) < R(0)) hiw[k] = sb[k + 1];
+        hiw[k] = hiw[k] - upw[k];
+        sfw[k] = fexpl * wmasc(k, iw) * hiw[k];
+      }
+      for (int k = 1; k <= mza; ++k) { fl[k] = 0; fin[k] = 0; fout[k] = 0; }
+      if (mode == 1) {
+        for (int k = kb; k <= mza; ++k) { smax[k] = q0(k, iw); smin[k] = q0(k, iw); }
+        for (int k = kb + 1; k <= mza; ++k)
+          if (wmasc(k - 1, iw) > R(0)) {
+            smax[k] = std::max(smax[k], q0(k - 1, iw));
+            smin[k] = std::min(smin[k], q0(k - 1, iw));
+          }
+        for (int k = kb; k <= mza - 1; ++k)
+          if (wmasc(k, iw) < R(0)) {
+            smax[k] = std::max(smax[k], q0(k + 1, iw));
+            smin[k] = std::min(smin[k], q0(k + 1, iw));
+          }
+      }
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw), iwn = w_iw(jv, iw);
+        for (int k = 2; k <= mza; ++k) {
+          fl[k] = fl[k] + dirv(jv, iw) * vmasc(k, iv) * (l_upv(k, iv) - q0(k, iw));
+          if (mode == 1) fin[k] = fin[k] + std::max(dirv(jv, iw) * l_sfv(k, iv), R(0));
+          fout[k] = fout[k] + std::min(dirv(jv, iw) * l_sfv(k, iv), R(0));
+          if (mode == 1 && dirv(jv, iw) * vmasc(k, iv) > R(0)) {
+            smax[k] = std::max(smax[k], q0(k, iwn));
+            smin[k] = std::min(smin[k], q0(k, iwn));
+          }
+        }
+      }
+      if (fimpl > R(.001)) {
+        for (int k = kb; k <= mza; ++k) {
+          if (mode == 1) {
+            R dwi = fimpl * (wmasc(k, iw) - wmasc(k - 1, iw));
+            fl[k] = fl[k] + fexpl * (wmasc(k - 1, iw) * (upw[k - 1] - q0(k, iw)) - wmasc(k, iw) * (upw[k] - q0(k, iw)))
+                    + q0(k, iw) * dwi;
+            fin[k] = fin[k] + std::max(sfw[k - 1], R(0)) - std::min(sfw[k], R(0));
+            fout[k] = fout[k] + std::min(sfw[k - 1], R(0)) - std::max(sfw[k], R(0));
+            R imp = rhosf(k, iw) + dtr * volti(k, iw) * dwi;
+            smax[k] = imp * smax[k];
+            smin[k] = imp * smin[k];
+          } else {
+            fl[k] = fl[k] + fexpl * (wmasc(k - 1, iw) * (upw[k - 1] - q0(k, iw)) - wmasc(k, iw) * (upw[k] - q0(k, iw)))
+                    + fimpl * q0(k, iw) * (wmasc(k, iw) - wmasc(k - 1, iw));
+            fout[k] = fout[k] + std::min(sfw[k - 1], R(0)) - std::max(sfw[k], R(0));
+          }
+        }
+      } else {
+        for (int k = kb; k <= mza; ++k) {
+          fl[k] = fl[k] + wmasc(k - 1, iw) * (upw[k - 1] - q0(k, iw)) - wmasc(k, iw) * (upw[k] - q0(k, iw));
+          if (mode == 1) fin[k] = fin[k] + std::max(sfw[k - 1], R(0)) - std::min(sfw[k], R(0));
+          fout[k] = fout[k] + std::min(sfw[k - 1], R(0)) - std::max(sfw[k], R(0));
+          if (mode == 1) { smax[k] = rhosf(k, iw) * smax[k]; smin[k] = rhosf(k, iw) * smin[k]; }
+        }
+      }
+      for (int k = kb; k <= mza; ++k) {
+        R rlow = q0(k, iw) * rhosf(k, iw) + dtr * volti(k, iw) * fl[k];
+        if (mode == 1) {
+          R dtrp = onep * dtr;
+          R flux_in = std::max(dtrp * volti(k, iw) * fin[k], eps0);
+          R flux_out = std::min(dtrp * volti(k, iw) * fout[k], -eps0);
+          R epsr = eps1 * std::fabs(rlow);
+          R sc_in = std::max(smax[k] - (rlow + epsr) - eps0, R(0));
+          R sc_out = std::min(smin[k] - (rlow - epsr) + eps0, R(0));
+          l_s1(k, iw) = std::min(R(1), sc_in / std::max(flux_in, R(0.01) * sc_in));
+          l_s2(k, iw) = std::min(R(1), sc_out / std::min(flux_out, R(0.01) * sc_out));
+        } else {
+          R mdtrp = -onep * dtr;
+          R flux_out = std::max(mdtrp * volti(k, iw) * fout[k], eps0);
+          R sc_out = std::max(onem * rlow - eps0, R(0));
+          l_s1(k, iw) = std::min(R(1), sc_out / std::max(flux_out, R(0.01) * sc_out));
+        }
+      }
+      // vertical renormalisation (:1003-1008, :1396-1401)
+      for (int k = kb; k <= mza - 1; ++k) {
+        R scale;
+        if (mode == 1) {
+          scale = std::min(l_s1(k + 1, iw), l_s2(k, iw));
+          if (sfw[k] < R(0)) scale = std::min(l_s1(k, iw), l_s2(k + 1, iw));
+        } else {
+          scale = l_s1(k, iw);
+          if (sfw[k] < R(0)) scale = l_s1(k + 1, iw);
+        }
+        if (frozen) scale = tape_sw[n](k, iw);
+        else tape_sw[n](k, iw) = scale;
+        upw[k] = upw[k] + hiw[k] * scale;
+      }
+    }
+    // horizontal renormalisation (:704-716, :1183-1193)
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        R scale;
+        if (mode == 1) {
+          scale = std::min(l_s1(k, iw2), l_s2(k, iw1));
+          if (l_sfv(k, iv) < R(0)) scale = std::min(l_s1(k, iw1), l_s2(k, iw2));
+        } else {
+          scale = l_s1(k, iw1);
+          if (l_sfv(k, iv) < R(0)) scale = l_s1(k, iw2);
+        }
+        if (frozen) scale = tape_sv[n](k, iv);
+        else tape_sv[n](k, iv) = scale;
+        l_upv(k, iv) = l_upv(k, iv) + l_hiv(k, iv) * scale;
+      }
+    }
+    // scalar_tend_monot (:1013-1078) / scalar_tend_pd (:1405-1469); level 1 is left
+    // as it was (the copy scp(2:kb-1) is empty for kb = 2)
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      const int kb = 2;
+      const R fimpl = ffimp[(size_t)(iw - 1) * nrk + istage - 1];
+      R fd[130];
+      const R *upw = &l_upw(1, iw) - 1;
+      for (int k = 1; k <= mza; ++k) fd[k] = 0;
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw);
+        for (int k = 2; k <= mza; ++k) fd[k] = fd[k] + dirv(jv, iw) * vmasc(k, iv) * (l_upv(k, iv) - q0(k, iw));
+      }
+      if (fimpl > R(.001)) {
+        R fexpl = R(1) - fimpl;
+        for (int k = kb; k <= mza; ++k) {
+          fd[k] = fd[k] + fexpl * (wmasc(k - 1, iw) * (upw[k - 1] - q0(k, iw)) - wmasc(k, iw) * (upw[k] - q0(k, iw)));
+          q(k, iw) = q0(k, iw) + dtr * volti(k, iw) * fd[k] / rhosf(k, iw);
+        }
+      } else {
+        for (int k = kb; k <= mza; ++k) {
+          fd[k] = fd[k] + wmasc(k - 1, iw) * (upw[k - 1] - q0(k, iw)) - wmasc(k, iw) * (upw[k] - q0(k, iw));
+          q(k, iw) = q0(k, iw) + dtr * volti(k, iw) * fd[k] / rhosf(k, iw);
+        }
+      }
+    }
+  }
+
+  // Adjoint of update_scalar_lim with the renormalisation scales frozen at the values
+  // the forward step applied (tape_sv, tape_sw).  Per face, the flux value is
+  // (1 - s) q0(upwind) + s * reconstruction(q), so the face adjoint phi is split into
+  // (1 - s) phi to the adjoint of scp0 and s phi to that of q (through grad_t2d or the
+  // vertical quadratic), exactly as in the forward selections.
+  void update_scalar_lim_adj(int n, const F2<R> &mu_in, F2<R> &lq, F2<R> &lq0, const F2<R> &rhosf, R dtr,
+                             int iorder, int istage, int nrk, const R *dtlr) {
+    const F2<R> &vm = tape_vmasc, &wm = tape_wmasc;
+    const int mode = tape_lim[n];
+    if (!a_nu.n) { a_nu.alloc(mza, mwa); a_lqpart.alloc(mza, mwa); a_lgx.alloc(mza, mwa); a_lgy.alloc(mza, mwa); }
+    F2<R> &nu = a_nu, &lgx = a_lgx, &lgy = a_lgy;
+    for (auto *a : {&nu, &lgx, &lgy}) std::fill(a->v.begin(), a->v.end(), R(0));
+    for (int iw = 2; iw <= mwa; ++iw) {
+      const int kb = 2;
+      R mu[130];
+      for (int k = 1; k <= mza; ++k) mu[k] = mu_in(k, iw);
+      const R ff = tape_ffimp[(size_t)(iw - 1) * nrk + istage - 1];
+      const bool imp = ff > R(0.001);
+      const R fexp = imp ? R(1) - ff : R(1);
+      lq(1, iw) += mu[1];                      // level 1 passes through unchanged
+      mu[1] = 0;
+      if (imp) {
+        R wup[130], wdn[130], b1[130], b2a[130], b3a[130], le[130], c[130], scr[130], cj[130];
+        wup[mza] = 0; wdn[mza] = 0; wup[kb - 1] = 0; wdn[kb - 1] = 0;
+        for (int k = kb; k <= mza - 1; ++k) { wup[k] = std::max(wm(k, iw), R(0)); wdn[k] = std::min(wm(k, iw), R(0)); }
+        R c1 = ff * dtlr[istage - 1];
+        for (int k = kb; k <= mza; ++k) {
+          R c2 = c1 * volti(k, iw) / tape_rhos[istage - 1](k, iw);
+          b1[k] = -c2 * wup[k - 1]; b2a[k] = R(1) + c2 * (wup[k] - wdn[k - 1]); b3a[k] = c2 * wdn[k];
+          c[k] = ff * dtr * volti(k, iw) / tape_rhos[istage - 1](k, iw);
+        }
+        cj[kb] = R(1) / b2a[kb];
+        le[kb] = mu[kb] * cj[kb];
+        for (int k = kb + 1; k <= mza; ++k) {
+          scr[k] = cj[k - 1] * b1[k];
+          cj[k] = R(1) / (b2a[k] - b3a[k - 1] * scr[k]);
+          le[k] = cj[k] * (mu[k] - b3a[k - 1] * le[k - 1]);
+        }
+        for (int k = mza - 1; k >= kb; --k) le[k] = le[k] - scr[k + 1] * le[k + 1];
+        for (int k = kb; k <= mza; ++k) lq0(k, iw) = lq0(k, iw) + mu[k] - le[k];
+        for (int k = kb; k <= mza; ++k) {
+          R a = -c[k] * le[k];
+          if (k <= mza - 1) { R t = a * std::min(wm(k, iw), R(0)); lq0(k + 1, iw) += t; lq0(k, iw) -= t; }
+          if (k - 1 >= kb) { R t = a * std::max(wm(k - 1, iw), R(0)); lq0(k, iw) += t; lq0(k - 1, iw) -= t; }
+        }
+        for (int k = kb; k <= mza; ++k) mu[k] = le[k];
+      }
+      R gsum[130];
+      for (int k = 1; k <= mza; ++k) gsum[k] = 0;
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw);
+        for (int k = 2; k <= mza; ++k) gsum[k] = gsum[k] + dirv(jv, iw) * vm(k, iv);
+      }
+      for (int k = kb; k <= mza; ++k) {
+        R v = mu[k] * dtr * volti(k, iw) / rhosf(k, iw);
+        nu(k, iw) = v;
+        lq0(k, iw) = lq0(k, iw) + mu[k] - v * (gsum[k] + fexp * (wm(k - 1, iw) - wm(k, iw)));
+      }
+      // vertical faces: upw(k) feeds fluxdiv(k+1) with +fexp wm(k), fluxdiv(k) with -fexp wm(k)
+      R psi[131], lds[131];
+      for (int k = 0; k <= mza; ++k) { psi[k] = 0; lds[k] = 0; }
+      for (int k = kb - 1; k <= mza; ++k) {
+        R nup = (k + 1 <= mza) ? nu(k + 1, iw) : R(0);
+        R nuk = (k >= kb) ? nu(k, iw) : R(0);
+        psi[k] = fexp * wm(k, iw) * (nup - nuk);
+      }
+      if (mode == 1) { lq0(kb, iw) += psi[kb - 1]; lq0(mza, iw) += psi[mza]; }
+      else { lq(kb, iw) += psi[kb - 1]; lq(mza, iw) += psi[mza]; }
+      for (int k = kb; k <= mza - 1; ++k) {
+        const R s = tape_sw[n](k, iw);
+        const bool neg = wm(k, iw) < R(0);
+        lq0(neg ? k + 1 : k, iw) += (R(1) - s) * psi[k];
+        const R h = s * psi[k];
+        if (neg) {        // scpb(k+1) = q(k+1) + ds(k+1) A(k+1,1,1) + ds(k) A(k+1,2,1)
+          lq(k + 1, iw) += h;
+          lds[k + 1] += h * (mode == 1 ? awu_(k + 1, 1, 1) : aw_(k + 1, 1, 1));
+          lds[k] += h * (mode == 1 ? awu_(k + 1, 2, 1) : aw_(k + 1, 2, 1));
+        } else {          // scpt(k) = q(k) + ds(k) A(k,1,2) + ds(k-1) A(k,2,2)
+          lq(k, iw) += h;
+          lds[k] += h * (mode == 1 ? awu_(k, 1, 2) : aw_(k, 1, 2));
+          lds[k - 1] += h * (mode == 1 ? awu_(k, 2, 2) : aw_(k, 2, 2));
+        }
+      }
+      // ds(kb-1) and ds(mza): constant-gradient ends (monotone) copy ds(kb), ds(mza-1)
+      if (mode == 1) { lds[kb] += lds[kb - 1]; lds[mza - 1] += lds[mza]; }
+      for (int k = kb; k <= mza - 1; ++k) { lq(k + 1, iw) += lds[k]; lq(k, iw) -= lds[k]; }
+    }
+    // horizontal faces
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        const R phi = vm(k, iv) * (nu(k, iw2) - nu(k, iw1));
+        const R s = tape_sv[n](k, iv);
+        const bool neg = vm(k, iv) < R(0);
+        lq0(k, neg ? iw2 : iw1) += (R(1) - s) * phi;
+        const R h = s * phi;
+        int side = neg ? 2 : 1;
+        if (iorder == 1 && mode == 2) side = 1;       // scalar_vflux_pd order-1 form
+        const int iwh = side == 1 ? iw1 : iw2;
+        lq(k, iwh) += h;
+        if (iorder > 1) { lgx(k, iwh) += dxps(side, iv) * h; lgy(k, iwh) += dyps(side, iv) * h; }
+      }
+    }
+    if (iorder > 1) {
+      for (int iw = 2; iw <= mwa; ++iw)
+        for (int m = 1; m <= npoly[iw]; ++m) {
+          int iwn = w_iw(m, iw);
+          for (int k = 2; k <= mza; ++k) {
+            lq(k, iwn) += gxc(m, iw) * lgx(k, iw) + gyc(m, iw) * lgy(k, iw);
+            lq(k, iw) -= gxc(m, iw) * lgx(k, iw) + gyc(m, iw) * lgy(k, iw);
+          }
+        }
+    }
   }
 
   // ---------------------------------------------------------------- adjoint
@@ -1000,8 +1365,12 @@ struct Model : Base {
       std::fill(lq0.v.begin(), lq0.v.end(), R(0));
       for (int istage = nrk; istage >= 1; --istage) {
         std::fill(lq.v.begin(), lq.v.end(), R(0));
-        update_scalar_adj(mu, lq, lq0, tape_rhos[nrk - 1], dtlr[istage - 1], prm.scal_horiz_adv_order,
-                          istage, nrk, dtlr);
+        if (istage == nrk && tape_lim[n])
+          update_scalar_lim_adj(n, mu, lq, lq0, tape_rhos[nrk - 1], dtlr[istage - 1], prm.scal_horiz_adv_order,
+                                istage, nrk, dtlr);
+        else
+          update_scalar_adj(mu, lq, lq0, tape_rhos[nrk - 1], dtlr[istage - 1], prm.scal_horiz_adv_order,
+                            istage, nrk, dtlr);
         mu = lq;
       }
       // stage 1 read q = q0: both adjoints land on the initial tracer
@@ -1060,6 +1429,8 @@ int oracle_scalar_transport_rk_adj(void *h) {
 }
 int oracle_mass_sums(void *h, double *out) { ((oracle_ctx *)h)->m->mass_sums(out); return 0; }
 int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(out); return 0; }
+int oracle_set_scalar_pdef(void *h, int32_t n, int32_t pdef) { return ((oracle_ctx *)h)->m->set_pdef(n, pdef); }
+int oracle_set_frozen_limiter(void *h, int32_t on) { ((oracle_ctx *)h)->m->set_frozen(on); return 0; }
 int oracle_set_threads(int32_t n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
 
 // tridiffo (tridiag_lib.f90:10-36) on arrays indexed 1..n, for the unit test of the

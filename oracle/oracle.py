@@ -49,6 +49,8 @@ def lib():
         L.oracle_cfl_report.argtypes = [vp, vp]
         L.oracle_set_threads.argtypes = [C.c_int32]
         L.oracle_tridiffo.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
+        L.oracle_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
+        L.oracle_set_frozen_limiter.argtypes = [vp, C.c_int32]
         _lib = L
     return _lib
 
@@ -130,6 +132,13 @@ class Oracle:
         out = np.zeros(4)
         lib().oracle_cfl_report(self.h, _p(out))
         return out
+
+    def set_scalar_pdef(self, n, pdef):
+        assert lib().oracle_set_scalar_pdef(self.h, int(n), int(bool(pdef))) == 0
+
+    def set_frozen_limiter(self, on):
+        """Test hook: later forward steps apply the limiter scales of the last step."""
+        lib().oracle_set_frozen_limiter(self.h, int(bool(on)))
 
 
 def tridiffo(a, b, c, d):
