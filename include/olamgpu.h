@@ -25,9 +25,9 @@
  *    the ABI.
  *  - Calls are ordered on the context's HIP stream and return after the stream is
  *    synchronised.  One host thread per context.
- *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere), mza <= 64, horizontal
- *    advection orders 1 and 2 and iscal_monot = 0 are supported; create() rejects
- *    anything else with OLAMGPU_EINVAL.
+ *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere), mza <= 64 and horizontal
+ *    advection orders 1 and 2 are supported; create() rejects anything else with
+ *    OLAMGPU_EINVAL.
  */
 #ifndef OLAMGPU_H
 #define OLAMGPU_H
@@ -96,7 +96,7 @@ typedef struct olamgpu_params {
   int32_t thil_horiz_adv_order;      /* 1 or 2 */
   int32_t wind_horiz_adv_order;      /* 1 or 2 */
   int32_t scal_horiz_adv_order;      /* 1 or 2 */
-  int32_t iscal_monot;               /* must be 0 (the limiters are not built) */
+  int32_t iscal_monot;               /* 0 none, 1 monotone, 2 positive definite (final stage) */
   int32_t divh_damp_level;           /* 0 disables; 1..3 */
   int32_t vort_damp_level;           /* 0 disables; 1..3 */
   float   divh_damp_fact;
@@ -143,6 +143,10 @@ int olamgpu_download_state(olamgpu_ctx *ctx, olamgpu_state *host);
  * var_t may be NULL (treated as zero tendency). */
 int olamgpu_upload_scalar(olamgpu_ctx *ctx, int32_t n, const float *var_p, const float *var_t);
 int olamgpu_download_scalar(olamgpu_ctx *ctx, int32_t n, float *var_p);
+/* scalar_tab(n)%pdef (var_tables.f90:58, default true): with iscal_monot = 2 only pdef
+ * tracers are limited, and with iscal_monot > 0 only pdef tracers get the low-order
+ * tendency update of scp0 before the final stage (scalar_transport_rk.F90:297-337). */
+int olamgpu_set_scalar_pdef(olamgpu_ctx *ctx, int32_t n, int32_t pdef);
 
 /* ---- the hot path ---------------------------------------------------------- */
 
@@ -170,7 +174,8 @@ int olamgpu_scalar_transport_rk(olamgpu_ctx *ctx, const float *rho_old);
 /* ---- adjoint (no counterpart in the reference) ------------------------------ */
 
 /* Adjoint of the last olamgpu_scalar_transport_rk call with the upwind choices
- * and limiter scales frozen from that forward pass (params.adjoint_tape > 0).
+ * and limiter scales frozen from that forward pass (params.adjoint_tape > 0 records
+ * the scales of a limited final stage).
  * On entry the device adjoint buffers hold dJ/dq(after the step); on return they
  * hold dJ/dq(before the step).  Buffers are set/read with the calls below. */
 int olamgpu_scalar_transport_rk_adj(olamgpu_ctx *ctx);
@@ -215,7 +220,9 @@ int olamgpu_copy_field(olamgpu_ctx *ctx, int32_t field_id, int32_t index,
 #define OLAMGPU_CKPT_STATE  1   /* fields the hot-path calls modify (prognostic state,
                                    vmasc, wmasc, rho_old); host-set inputs excluded */
 #define OLAMGPU_CKPT_TAPE   2   /* vmasc, wmasc, rho, rho_old: what a tracer step and its
-                                   adjoint read besides the tracers */
+                                   adjoint read besides the tracers (with iscal_monot > 0
+                                   also the taped limiter scales; STATE then also holds
+                                   the tracers) */
 
 /* Size in bytes of a checkpoint of the given kind. */
 int olamgpu_checkpoint_bytes(olamgpu_ctx *ctx, int32_t kind, int64_t *bytes);

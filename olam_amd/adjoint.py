@@ -7,7 +7,10 @@ the mass fluxes vmasc/wmasc and densities a tracer step reads), then
 scalar_transport_rk.  For fixed dynamics a tracer step is linear in the tracers (the
 upwind choices depend on the winds only), so the window's tracer map L is linear and
 its adjoint is the product of the per-step adjoints in reverse order, each run with
-that step's winds (the "tape": vmasc, wmasc, rho, rho_old).
+that step's winds (the "tape": vmasc, wmasc, rho, rho_old).  With a final-stage
+limiter (iscal_monot > 0) the step is linear only with the limiter's scales frozen at
+the forward trajectory; the tape then also holds those scales, the checkpoints hold
+the tracers, and the backward pass re-runs each tracer step to regenerate them.
 
 Memory: the dynamics state is checkpointed every ``every`` steps during the forward
 pass (default ceil(sqrt(N))); the backward pass restores each segment's checkpoint,
@@ -44,6 +47,9 @@ class TransportWindow:
         self.nw = ctx.grid.mwa - 1
         self.mza = ctx.grid.mza
         self.dtype = torch.float64 if ctx.params.precision == 2 else torch.float32
+        # with a final-stage limiter the tracer map is linear only with the scales
+        # frozen at the forward trajectory: the backward pass re-runs the tracer steps
+        self.limited = ctx.params.iscal_monot > 0
         nseg = math.ceil(nsteps / self.every)
         sb = ctx.checkpoint_bytes(CKPT_STATE)
         tb = ctx.checkpoint_bytes(CKPT_TAPE)
@@ -107,6 +113,10 @@ class TransportWindow:
             c.checkpoint_restore(CKPT_STATE, self.ckpts[s].data_ptr())
             for i in range(i0, i1):
                 self._dyn()
+                if self.limited:
+                    # the limiter scales depend on the tracers: re-run the tracer step
+                    # (the checkpoint holds the tracers) to tape them
+                    c.scalar_transport_rk()
                 c.checkpoint_save(CKPT_TAPE, self.tapes[i - i0].data_ptr())
                 self.recomputed_steps += 1
             for i in reversed(range(i0, i1)):

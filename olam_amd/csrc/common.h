@@ -12,6 +12,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Expressions are evaluated as written, without fusing a*b+c into an FMA, in the
+// kernels whose results must track the oracle (compiled with -ffp-contract=off) to the
+// last bit: the dynamics, where small differences of large terms amplify rounding, and
+// the limiters, whose scale factors switch on comparisons of such differences.
+#define NOCONTRACT _Pragma("clang fp contract(off)")
+
 #define WAVE 64
 #define WPB 4          // waves per 256-thread block
 #define NSLOT 8        // padded neighbour slots per W point
@@ -32,6 +38,7 @@ struct DMesh {
   const R *m_vfact, *m_xxyym;
   // per level (ldk)
   const R *dzim, *gdz_wgtm, *gdz_wgtp, *aw11, *aw21, *aw12, *aw22;
+  const R *awu11, *awu21, *awu12, *awu22;      // unbounded vertical quadratic (a_wu)
   const double *zwgt_top8, *zwgt_bot8, *gdz_wgtm8, *gdz_wgtp8;
   // control volumes (n x ldk)
   const R *arw, *volti, *volwi, *arv, *volvi;

@@ -13,7 +13,6 @@
 // All dynamics kernels evaluate their expressions exactly as written (no FMA
 // contraction): the W-momentum balance subtracts nearly equal pressure-gradient and
 // buoyancy terms, so every ulp upstream of it shows in w.  The oracle does the same.
-#define NOCONTRACT _Pragma("clang fp contract(off)")
 
 template <class R, class RP>
 struct DynBuf {

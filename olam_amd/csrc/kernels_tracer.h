@@ -19,6 +19,9 @@ struct TrStage {
   int ntr, nrk, istage, order, tpw;
   long tstride;
   R dtr;
+  const int *tmode;  // final stage: tracers with tmode[n] != 0 go through the limiter
+  int lim;           // this stage is a limited final stage
+  __device__ __forceinline__ bool limited(int n) const { return lim && tmode[n] != 0; }
 };
 
 // ---------------------------------------------------------------- grad_t2d
@@ -72,7 +75,7 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 // One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
 // (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
 template <class R, class RP>
-__global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
+__global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
   if (c >= m.nw) return;
@@ -104,22 +107,47 @@ __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a)
   const R rf = act ? (R)a.rho[ic] : R(1);
   const R rinv = R(1) / (body ? rf : R(1));
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
-  for (int n = ti.n0; n < ti.n1; ++n) {
+  // a neighbour's gradient is read only if some level takes that face from it
+  unsigned need = 0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) need |= (__ballot(sel[j] && act) != 0 ? 1u : 0u) << j;
+  // the next tracer's loads are issued before the current one is computed
+  struct Ld { R qc, q0, st, gxc, gyc, qn[7], gxn[7], gyn[7]; };
+  auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
     const R *qi = a.q_in + off;
-    const R qc = ld(qi, ic, act);
-    const R q0 = ld(a.q0 + off, ic, act);
-    const R st = a.sct ? ld(a.sct + off, ic, act) : R(0);
-    R gxc = R(0), gyc = R(0);
-    if (a.order > 1) { gxc = ld(a.gx + off, ic, act); gyc = ld(a.gy + off, ic, act); }
+    L.qc = ld(qi, ic, act);
+    L.q0 = ld(a.q0 + off, ic, act);
+    L.st = a.sct ? ld(a.sct + off, ic, act) : R(0);
+    L.gxc = R(0); L.gyc = R(0);
+    if (a.order > 1) { L.gxc = ld(a.gx + off, ic, act); L.gyc = ld(a.gy + off, ic, act); }
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      L.qn[j] = R(0); L.gxn[j] = R(0); L.gyn[j] = R(0);
+      if (j < 6 || np > 6) {
+        const long in = (long)nb[j] * ldk + l;
+        L.qn[j] = ld(qi, in, act);
+        if (a.order > 1 && ((need >> j) & 1u)) {
+          L.gxn[j] = ld(a.gx + off, in, act);
+          L.gyn[j] = ld(a.gy + off, in, act);
+        }
+      }
+    }
+  };
+  Ld nx;
+  if (ti.n0 < ti.n1) load(ti.n0, nx);
+  for (int n = ti.n0; n < ti.n1; ++n) {
+    const Ld L = nx;
+    if (n + 1 < ti.n1) load(n + 1, nx);
+    if (a.limited(n)) continue;
+    const long off = n * a.tstride;
+    const R qc = L.qc, q0 = L.q0;
     R F = R(0);
     for_slots(np, [&](int j) {
-      const long in = (long)nb[j] * ldk + l;
-      const R qn = ld(qi, in, act);
-      R fs = qc, fn = qn;
+      R fs = qc, fn = L.qn[j];
       if (a.order > 1) {
-        fs = qc + dxs[j] * gxc + dys[j] * gyc;
-        fn = qn + dxn[j] * ld(a.gx + off, in, act) + dyn[j] * ld(a.gy + off, in, act);
+        fs = qc + dxs[j] * L.gxc + dys[j] * L.gyc;
+        fn = L.qn[j] + dxn[j] * L.gxn[j] + dyn[j] * L.gyn[j];
       }
       const R f = sel[j] ? fn : fs;
       F = F + dvm[j] * (f - q0);
@@ -136,7 +164,7 @@ __global__ __launch_bounds__(256) void k_tr_update(DMesh<R> m, TrStage<R, RP> a)
     if (k == mza) upw = qc;
     const R upw_m1 = shup(upw, 1);
     F = F + fexpl * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
-    R qn_new = q0 + a.dtr * (st + vt * F) * rinv;
+    R qn_new = q0 + a.dtr * (L.st + vt * F) * rinv;
     const R lev1 = shdn(qn_new, 1);             // level 1 copies level 2
     if (k == 1) qn_new = lev1;
     if (act) a.q_out[off + ic] = qn_new;
@@ -499,6 +527,9 @@ struct TrAdj {
   int ntr, nrk, istage, order, last, tpw;
   long tstride;
   R dtr;
+  const int *tmode;  // as in TrStage
+  int lim;
+  __device__ __forceinline__ bool limited(int n) const { return lim && tmode[n] != 0; }
 };
 
 template <class R, class RP>
@@ -529,13 +560,13 @@ __global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP
   if (body) { tb1 = (double)b3_m1; tb2 = (double)b2; tb3 = (double)(k < mza ? b1_p1 : R(0)); }
   for (int n = 0; n < a.ntr; ++n) {
     const long off = n * a.tstride;
+    const bool limn = a.limited(n);
     R mu = ld(a.mu + off, ic, act);
     const R mu1 = __shfl(mu, 0, WAVE);
     if (k == 1) mu = R(0);
     R l0 = ld(a.lam0 + off, ic, act);
     const double le = pcr_solve(tb1, tb2, tb3, body ? (double)mu : 0.0);
     R lef = body ? (R)le : R(0);
-    if (k == 2) lef = lef + mu1;              // level 1 kept the explicit level-2 value
     l0 = l0 + mu - lef;
     // -c2 (min(wm(k),0) dq0(k) + max(wm(k-1),0) dq0(k-1)),  dq0(k) = q0(k+1) - q0(k)
     const R av = -c2 * lef;
@@ -543,9 +574,14 @@ __global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP
     const R t2 = (body && k - 1 >= 2) ? av * (wm_m1 > R(0) ? wm_m1 : R(0)) : R(0);
     const R t1_m1 = shup(t1, 1), t2_p1 = shdn(t2, 1);
     if (body) l0 = l0 - t1 + (k - 1 >= 2 ? t1_m1 : R(0)) + t2 - (k + 1 <= mza ? t2_p1 : R(0));
+    // level 1 kept the explicit level-2 value, so its adjoint joins that of the explicit
+    // result at level 2 (on a limited stage level 1 is the stage input: its adjoint
+    // stays in place for the limited explicit pass)
+    if (k == 2 && !limn) lef = lef + mu1;
+    if (k == 1 && limn) lef = mu1;
     if (act) {
       a.lam0[off + ic] = l0;
-      a.mu[off + ic] = lef;                   // lef = 0 on level 1
+      a.mu[off + ic] = lef;                   // level 1: 0, or mu(1) on a limited stage
     }
   }
 }
@@ -577,6 +613,7 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
   const bool neg = wm < R(0);
   const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
   for (int n = ti.n0; n < ti.n1; ++n) {
+    if (a.limited(n)) continue;
     const long off = n * a.tstride;
     R mu = ld(a.mu + off, ic, act);
     const R mu1 = __shfl(mu, 0, WAVE);
@@ -635,6 +672,7 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
     up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));  // this column is the upwind side
   }
   for (int n = ti.n0; n < ti.n1; ++n) {
+    if (a.limited(n)) continue;
     const long off = n * a.tstride;
     const R nuc = ld(a.nu + off, ic, act);
     R P = R(0), Gx = R(0), Gy = R(0);
@@ -673,18 +711,40 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     bx[j] = (j < np) ? m.w_gxc[nb[j] * NSLOT + b] : R(0);
     by[j] = (j < np) ? m.w_gyc[nb[j] * NSLOT + b] : R(0);
   }
-  for (int n = ti.n0; n < ti.n1; ++n) {
+  // the next tracer's loads are issued before the current one is reduced
+  struct Ld { R lp, gxs, gys, gxn[7], gyn[7]; };
+  auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
-    R lq = ld(a.lpart + off, ic, act);
+    L.lp = ld(a.lpart + off, ic, act);
+    L.gxs = R(0); L.gys = R(0);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) { L.gxn[j] = R(0); L.gyn[j] = R(0); }
     if (a.order > 1) {
-      const R gxs = ld(a.lgx + off, ic, act), gys = ld(a.lgy + off, ic, act);
+      L.gxs = ld(a.lgx + off, ic, act); L.gys = ld(a.lgy + off, ic, act);
+#pragma unroll
+      for (int j = 0; j < 7; ++j)
+        if (j < 6 || np > 6) {
+          const long in = (long)nb[j] * ldk + l;
+          L.gxn[j] = ld(a.lgx + off, in, act);
+          L.gyn[j] = ld(a.lgy + off, in, act);
+        }
+    }
+  };
+  Ld nx;
+  if (ti.n0 < ti.n1) load(ti.n0, nx);
+  for (int n = ti.n0; n < ti.n1; ++n) {
+    const Ld L = nx;
+    if (n + 1 < ti.n1) load(n + 1, nx);
+    const long off = n * a.tstride;
+    R lq = L.lp;
+    const R lev1 = lq;                                  // limited stage: mu(1) passes through
+    if (a.order > 1) {
       for_slots(np, [&](int j) {
-        const long in = (long)nb[j] * ldk + l;
-        lq = lq + (bx[j] * ld(a.lgx + off, in, act) + by[j] * ld(a.lgy + off, in, act));
-        lq = lq - (cx[j] * gxs + cy[j] * gys);
+        lq = lq + (bx[j] * L.gxn[j] + by[j] * L.gyn[j]);
+        lq = lq - (cx[j] * L.gxs + cy[j] * L.gys);
       });
     }
-    if (!body) lq = R(0);
+    if (!body) lq = a.limited(n) ? lev1 : R(0);
     if (a.last) lq = lq + ld(a.lam0 + off, ic, act);   // stage 1 read q = scp0
     if (act) a.lq[off + ic] = lq;
   }

@@ -22,6 +22,7 @@
 #include "common.h"
 #include "kernels_dyn.h"
 #include "kernels_tracer.h"
+#include "kernels_limiter.h"
 
 namespace {
 
@@ -360,6 +361,7 @@ struct ImplBase {
   virtual void copy_field(int, int, void *, int, int *) = 0;
   virtual size_t checkpoint_bytes(int) = 0;
   virtual void checkpoint(int, void *, int) = 0;
+  virtual void set_pdef(int, int) = 0;
 };
 
 template <class T> static inline T ipow(T x, int n) { T r = T(1); for (int i = 0; i < n; ++i) r *= x; return r; }
@@ -376,7 +378,7 @@ struct Impl : ImplBase {
   // mesh storage
   DevBuf b_npoly, b_nbr, b_face, b_back, b_viw, b_vim, b_miv, b_mim, b_invw, b_invv;
   DevBuf b_dirv, b_gxc, b_gyc, b_dxs, b_dys, b_dxn, b_dyn, b_ecx, b_ecy, b_ecz, b_xxyy, b_wn[3];
-  DevBuf b_vn[3], b_vfact, b_xxyym, b_lev[7], b_lev8[4];
+  DevBuf b_vn[3], b_vfact, b_xxyym, b_lev[11], b_lev8[4];
   DevBuf b_arw, b_volti, b_volwi, b_volt, b_arv, b_volvi;
   // column patches of the LDS-staged tracer kernels (forward and adjoint)
   struct PatchSet {
@@ -386,7 +388,7 @@ struct Impl : ImplBase {
     int cpw = 1;
     size_t lds = 0;
   } pf, pa;
-  bool use_patch = true;
+  bool use_patch = false;   // OLAMGPU_TRACER=patch selects the LDS-staged tracer kernels
   // state
   DevBuf thil[2], wmc, wc, vxe, vye, vze, vmc, vc, vmasc, wmasc, alpha, pwfac, pvfac, thilt, vmxet, vmyet, vmzet, vmt;
   DevBuf rho, press, rho_old;
@@ -401,6 +403,10 @@ struct Impl : ImplBase {
   long tstride;
   DevBuf trS, trX, trY, trgx, trgy, trsct;
   bool have_sct = false;
+  // final-stage limiters (iscal_monot = 1 monotone, 2 positive definite)
+  std::vector<int> pdef;            // scalar_tab(n)%pdef, default .true. (var_tables.f90:58)
+  DevBuf lim_mode, lim_pre, lim_s1, lim_s2, lim_tsv, lim_tsw;
+  bool any_lim = false, any_pre = false, taped_lim = false;
   DevBuf ffimp, cflmax, nimp, anyimp, masspart, massout;
   // adjoint
   DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy;
@@ -432,12 +438,12 @@ struct Impl : ImplBase {
       throw Err(OLAMGPU_EINVAL, "RK levels must be 2 or 3");
     for (int o : {prm.thil_horiz_adv_order, prm.wind_horiz_adv_order, prm.scal_horiz_adv_order})
       if (o < 1 || o > 2) throw Err(OLAMGPU_EINVAL, "horizontal advection order must be 1 or 2");
-    if (prm.iscal_monot != 0) throw Err(OLAMGPU_EINVAL, "iscal_monot > 0 is not supported");
+    if (prm.iscal_monot < 0 || prm.iscal_monot > 2) throw Err(OLAMGPU_EINVAL, "iscal_monot must be 0, 1 or 2");
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
     build_mesh(m);
     alloc_state();
     if (const char *e = std::getenv("OLAMGPU_TPW")) tpw = std::max(1, std::atoi(e));
-    if (const char *e = std::getenv("OLAMGPU_TRACER")) use_patch = std::strcmp(e, "gather") != 0;
+    if (const char *e = std::getenv("OLAMGPU_TRACER")) use_patch = std::strcmp(e, "patch") == 0;
     build_patch_tables();
   }
 
@@ -535,7 +541,7 @@ struct Impl : ImplBase {
       }
     }
     // per-level arrays (WAVE entries: every lane may read its level)
-    std::vector<R> lev[7];
+    std::vector<R> lev[11];
     for (auto &v : lev) v.assign(WAVE, R(0));
     std::vector<double> lev8[4];
     for (auto &v : lev8) v.assign(WAVE, 0.0);
@@ -548,6 +554,10 @@ struct Impl : ImplBase {
       lev[4][l] = m->a_w[l + mza * (1 + 2 * 0)];    // a_w(k,2,1)
       lev[5][l] = m->a_w[l + mza * (0 + 2 * 1)];    // a_w(k,1,2)
       lev[6][l] = m->a_w[l + mza * (1 + 2 * 1)];    // a_w(k,2,2)
+      lev[7][l] = m->a_wu[l + mza * (0 + 2 * 0)];   // a_wu(k,1,1) (unbounded)
+      lev[8][l] = m->a_wu[l + mza * (1 + 2 * 0)];   // a_wu(k,2,1)
+      lev[9][l] = m->a_wu[l + mza * (0 + 2 * 1)];   // a_wu(k,1,2)
+      lev[10][l] = m->a_wu[l + mza * (1 + 2 * 1)];  // a_wu(k,2,2)
       lev8[0][l] = m->zwgt_top8[l]; lev8[1][l] = m->zwgt_bot8[l];
       lev8[2][l] = m->gdz_wgtm8[l]; lev8[3][l] = m->gdz_wgtp8[l];
     }
@@ -584,7 +594,7 @@ struct Impl : ImplBase {
     upv(b_dxn, dxn); upv(b_dyn, dyn); upv(b_ecx, ecx); upv(b_ecy, ecy); upv(b_ecz, ecz); upv(b_xxyy, xxyy);
     for (int i = 0; i < 3; ++i) { upv(b_wn[i], wn[i]); upv(b_vn[i], vn[i]); }
     upv(b_vfact, vfact); upv(b_xxyym, xxyym);
-    for (int i = 0; i < 7; ++i) upv(b_lev[i], lev[i]);
+    for (int i = 0; i < 11; ++i) upv(b_lev[i], lev[i]);
     for (int i = 0; i < 4; ++i) upv(b_lev8[i], lev8[i]);
     upv(b_arw, arw); upv(b_volti, volti); upv(b_volwi, volwi); upv(b_volt, volt); upv(b_arv, arv); upv(b_volvi, volvi);
     // device copies of the integer tables must equal the host tables
@@ -601,6 +611,7 @@ struct Impl : ImplBase {
     dm.m_iv = dp<int>(b_miv); dm.m_im = dp<int>(b_mim); dm.m_vfact = dp<R>(b_vfact); dm.m_xxyym = dp<R>(b_xxyym);
     dm.dzim = dp<R>(b_lev[0]); dm.gdz_wgtm = dp<R>(b_lev[1]); dm.gdz_wgtp = dp<R>(b_lev[2]);
     dm.aw11 = dp<R>(b_lev[3]); dm.aw21 = dp<R>(b_lev[4]); dm.aw12 = dp<R>(b_lev[5]); dm.aw22 = dp<R>(b_lev[6]);
+    dm.awu11 = dp<R>(b_lev[7]); dm.awu21 = dp<R>(b_lev[8]); dm.awu12 = dp<R>(b_lev[9]); dm.awu22 = dp<R>(b_lev[10]);
     dm.zwgt_top8 = dp<double>(b_lev8[0]); dm.zwgt_bot8 = dp<double>(b_lev8[1]);
     dm.gdz_wgtm8 = dp<double>(b_lev8[2]); dm.gdz_wgtp8 = dp<double>(b_lev8[3]);
     dm.arw = dp<R>(b_arw); dm.volti = dp<R>(b_volti); dm.volwi = dp<R>(b_volwi); dm.arv = dp<R>(b_arv);
@@ -672,6 +683,15 @@ struct Impl : ImplBase {
     tstride = (long)wsz;
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
     for (DevBuf *b : {&trS, &trX, &trY, &trgx, &trgy}) { b->alloc(tb); zero(*b); }
+    pdef.assign(std::max(ntr, 1), 1);
+    if (prm.iscal_monot > 0) {
+      for (DevBuf *b : {&lim_s1, &lim_s2}) { b->alloc(tb); zero(*b); }
+      lim_mode.alloc(std::max(ntr, 1) * sizeof(int)); lim_pre.alloc(std::max(ntr, 1) * sizeof(int));
+      if (prm.adjoint_tape > 0) {
+        lim_tsv.alloc(std::max<size_t>(1, (size_t)ntr) * vsz * sizeof(R)); zero(lim_tsv);
+        lim_tsw.alloc(tb); zero(lim_tsw);
+      }
+    }
     ffimp.alloc((size_t)3 * nw * sizeof(R)); zero(ffimp);
     cflmax.alloc(4 * sizeof(unsigned)); nimp.alloc(sizeof(int)); anyimp.alloc(sizeof(int));
     masspart.alloc((size_t)nblk(nw) * 65 * sizeof(double));
@@ -876,6 +896,35 @@ struct Impl : ImplBase {
     return any != 0;
   }
 
+  void set_pdef(int n, int v) override { check_n(n); pdef[n - 1] = v != 0; }
+
+  // Per-tracer final-stage treatment (scalar_transport_rk.F90:297-337): mode 1 with
+  // iscal_monot = 1, mode 2 with iscal_monot = 2 for pdef tracers, else 0 (regular
+  // update); scp0 gets the low-order tendency update when iscal_monot > 0 and pdef.
+  void lim_setup() {
+    any_lim = any_pre = false;
+    if (prm.iscal_monot == 0 || ntr == 0) return;
+    std::vector<int> mode(ntr), pre(ntr);
+    for (int n = 0; n < ntr; ++n) {
+      mode[n] = prm.iscal_monot == 1 ? 1 : (pdef[n] ? 2 : 0);
+      pre[n] = pdef[n] ? 1 : 0;
+      any_lim = any_lim || mode[n] != 0;
+      any_pre = any_pre || (pre[n] && have_sct);
+    }
+    HIPCHK(hipMemcpyAsync(lim_mode.p, mode.data(), ntr * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(lim_pre.p, pre.data(), ntr * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  TrLim<R> lim_args(bool tape) {
+    TrLim<R> L;
+    L.tmode = lim_mode.as<int>();
+    L.s1 = dp<R>(lim_s1); L.s2 = dp<R>(lim_s2);
+    L.tsv = tape && lim_tsv.p ? dp<R>(lim_tsv) : nullptr;
+    L.tsw = tape && lim_tsw.p ? dp<R>(lim_tsw) : nullptr;
+    L.vstride = (long)vsz;
+    return L;
+  }
+
   void scalar_transport_rk(const float *rho_old_host) override {
     if (rho_old_host) h2d<R>(rho_old, rho_old_host, t.perm_w);
     const int nrk = prm.nrk_scal;
@@ -883,6 +932,8 @@ struct Impl : ImplBase {
     stage_dts(dtlr, nrk);
     const bool any_imp = run_cfl(false);
     if (ntr == 0) { have_fwd = true; return; }
+    lim_setup();
+    taped_lim = false;
     // RK3: S -> X -> Y -> X;  RK2: S -> X -> Y
     DevBuf *in = &trS, *outs[3] = {&trX, &trY, &trX};
     if (nrk == 2) { outs[0] = &trX; outs[1] = &trY; }
@@ -896,7 +947,23 @@ struct Impl : ImplBase {
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.tstride = tstride;
       a.dtr = dtlr[istage - 1];
       a.tpw = tpw;
-      if (use_patch) {
+      a.tmode = lim_mode.as<int>();
+      a.lim = (istage == nrk && any_lim) ? 1 : 0;
+      if (istage == nrk && any_pre)
+        LAUNCH("k_tr_lim_pre", (k_tr_lim_pre<R>), dim3(nblk(nw)), dim3(256), dm, dp<R>(trS), dp<R>(trsct),
+               dp<R>(rho_old), lim_pre.as<int>(), ntr, tstride, a.dtr);
+      if (a.lim) {
+        // limited final stage: gradients, regular update of the other tracers, then the
+        // two limiter passes
+        const TrLim<R> L = lim_args(prm.adjoint_tape > 0);
+        if (a.order > 1)
+          LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
+                 ntr, tstride, tpw);
+        LAUNCH("k_tr_update", (k_tr_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        LAUNCH("k_tr_lim_scales", (k_tr_lim_scales<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
+        LAUNCH("k_tr_lim_update", (k_tr_lim_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
+        taped_lim = prm.adjoint_tape > 0;
+      } else if (use_patch) {
         const dim3 g(pf.h.np), b(512);
         if (pf.cpw == 4) LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 4, rows_per_wave(4)>), g, b, pf.lds, dm, a, pf.tab);
         else if (pf.cpw == 2) LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 2, rows_per_wave(2)>), g, b, pf.lds, dm, a, pf.tab);
@@ -922,6 +989,9 @@ struct Impl : ImplBase {
     if (!have_fwd) throw Err(OLAMGPU_ESTATE, "adjoint called before a forward tracer step");
     alloc_adj();
     if (ntr == 0) return;
+    lim_setup();
+    if (any_lim && !taped_lim)
+      throw Err(OLAMGPU_ESTATE, "limited step has no taped scales (adjoint_tape = 0 at create)");
     const int nrk = prm.nrk_scal;
     R dtlr[3];
     stage_dts(dtlr, nrk);
@@ -938,8 +1008,17 @@ struct Impl : ImplBase {
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.last = istage == 1;
       a.tstride = tstride; a.dtr = dtlr[istage - 1];
       a.tpw = tpw;
+      a.tmode = lim_mode.as<int>();
+      a.lim = (istage == nrk && any_lim) ? 1 : 0;
       if (any_imp) LAUNCH("k_tr_adj_implicit", (k_tr_adj_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
-      if (use_patch) {
+      if (a.lim) {
+        const TrLim<R> L = lim_args(true);
+        LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        LAUNCH("k_tr_lim_adj0", (k_tr_lim_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
+        LAUNCH("k_tr_lim_adj1", (k_tr_lim_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
+        LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      } else if (use_patch) {
         const dim3 g(pa.h.np), b(512);
         if (pa.cpw == 4) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 4>), g, b, pa.lds, dm, a, pa.tab);
         else if (pa.cpw == 2) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 2>), g, b, pa.lds, dm, a, pa.tab);
@@ -978,9 +1057,20 @@ struct Impl : ImplBase {
   // pvfac, long-step tendencies, tracer tendencies) are the same.  TAPE: what a tracer
   // step and its adjoint read besides the tracers (vmasc, wmasc, rho, rho_old).
   std::vector<DevBuf *> ck_list(int kind) {
-    if (kind == OLAMGPU_CKPT_STATE)
-      return {&cur_thil_buf(), &wmc, &wc, &vxe, &vye, &vze, &vmc, &vc, &rho, &press, &vmasc, &wmasc, &rho_old};
-    if (kind == OLAMGPU_CKPT_TAPE) return {&vmasc, &wmasc, &rho, &rho_old};
+    // With a final-stage limiter the tracers themselves are part of both: the scales
+    // depend on them, so a recomputation needs them and the adjoint needs the scales.
+    const bool lim = prm.iscal_monot > 0;
+    if (kind == OLAMGPU_CKPT_STATE) {
+      std::vector<DevBuf *> v = {&cur_thil_buf(), &wmc, &wc, &vxe, &vye, &vze, &vmc, &vc, &rho, &press, &vmasc,
+                                 &wmasc, &rho_old};
+      if (lim) v.push_back(&trS);
+      return v;
+    }
+    if (kind == OLAMGPU_CKPT_TAPE) {
+      std::vector<DevBuf *> v = {&vmasc, &wmasc, &rho, &rho_old};
+      if (lim && lim_tsv.p) { v.push_back(&lim_tsv); v.push_back(&lim_tsw); }
+      return v;
+    }
     throw Err(OLAMGPU_EINVAL, "unknown checkpoint kind");
   }
   static size_t ck_round(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -998,7 +1088,10 @@ struct Impl : ImplBase {
       off += ck_round(b->bytes);
     }
     // the adjoint of a tracer step depends only on the taped winds and densities
-    if (restore && kind == OLAMGPU_CKPT_TAPE) have_fwd = true;
+    if (restore && kind == OLAMGPU_CKPT_TAPE) {
+      have_fwd = true;
+      taped_lim = prm.iscal_monot > 0 && lim_tsv.p;
+    }
   }
 };
 
@@ -1204,6 +1297,9 @@ int olamgpu_copy_field(olamgpu_ctx *c, int32_t field_id, int32_t index, void *de
     if (!dev_buf || (direction != 0 && direction != 1)) throw Err(OLAMGPU_EINVAL, "bad copy arguments");
     c->impl->copy_field(field_id, index, dev_buf, direction, elem_bytes);
   });
+}
+int olamgpu_set_scalar_pdef(olamgpu_ctx *c, int32_t n, int32_t pdef) {
+  return guarded(c, [&] { c->impl->set_pdef(n, pdef); });
 }
 int olamgpu_checkpoint_bytes(olamgpu_ctx *c, int32_t kind, int64_t *bytes) {
   return guarded(c, [&] {

@@ -75,7 +75,8 @@ module olamgpu_iface
        olamgpu_upload_scalar_adj, olamgpu_download_scalar_adj, olamgpu_mass_sums, &
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
        olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_checkpoint_bytes, &
-       olamgpu_checkpoint_save, olamgpu_checkpoint_restore, olamgpu_error_message
+       olamgpu_checkpoint_save, olamgpu_checkpoint_restore, olamgpu_set_scalar_pdef, &
+       olamgpu_error_message
 
   interface
      integer(c_int) function olamgpu_create(mesh, params, device, ctx) bind(C, name="olamgpu_create")
@@ -121,6 +122,13 @@ module olamgpu_iface
        integer(c_int32_t), value :: n
        type(c_ptr), value :: var_p
      end function olamgpu_download_scalar
+
+     ! scalar_tab(n)%pdef of the limiter options (default true)
+     integer(c_int) function olamgpu_set_scalar_pdef(ctx, n, pdef) bind(C, name="olamgpu_set_scalar_pdef")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n, pdef
+     end function olamgpu_set_scalar_pdef
 
      integer(c_int) function olamgpu_init_wrtv_rk(ctx) bind(C, name="olamgpu_init_wrtv_rk")
        import :: c_int, c_ptr

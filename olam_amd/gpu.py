@@ -49,6 +49,7 @@ def lib():
         L.olamgpu_cfl_report.argtypes = [vp, vp]
         L.olamgpu_copy_field.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+        L.olamgpu_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
         L.olamgpu_checkpoint_bytes.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.olamgpu_checkpoint_save.argtypes = [vp, C.c_int32, vp]
         L.olamgpu_checkpoint_restore.argtypes = [vp, C.c_int32, vp]
@@ -64,7 +65,7 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_scalar_transport_rk_adj", "olamgpu_upload_scalar_adj", "olamgpu_download_scalar_adj",
            "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
-           "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore"]
+           "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
@@ -122,6 +123,9 @@ class Context:
         out = np.zeros((self.grid.mwa, self.grid.mza), np.float32)
         self._chk(lib().olamgpu_download_scalar(self.h, n, _p(out)), "download_scalar")
         return out
+
+    def set_scalar_pdef(self, n, pdef):
+        self._chk(lib().olamgpu_set_scalar_pdef(self.h, int(n), int(bool(pdef))), "set_scalar_pdef")
 
     def upload_scalar_adj(self, n, lam):
         lam = np.ascontiguousarray(lam, dtype=np.float32)

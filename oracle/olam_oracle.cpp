@@ -863,10 +863,8 @@ struct Model : Base {
         // scp0 by the tendency first (scalar_transport_rk.F90:297-313)
         if (istage == nrk && imonot > 0 && pdef[n]) {
 #pragma omp parallel for
-          for (int iw = 2; iw <= mwa; ++iw) {
+          for (int iw = 2; iw <= mwa; ++iw)      // (the copy scp0(2:lpw-1) is empty for lpw = 2)
             for (int k = 2; k <= mza; ++k) scp0(k, iw) = scp0(k, iw) + dtr * sct[n](k, iw) / (*ro)(k, iw);
-            scp0(1, iw) = scp0(2, iw);
-          }
         }
         if (istage == nrk && imonot == 1)
           update_scalar_lim(n, q, scp0, rhos[nrk - 1], dtr, prm.scal_horiz_adv_order, istage, nrk, 1);
@@ -1262,7 +1260,6 @@ struct Model : Base {
           le[k] = cj[k] * (mu[k] - b3a[k - 1] * le[k - 1]);
         }
         for (int k = mza - 1; k >= kb; --k) le[k] = le[k] - scr[k + 1] * le[k + 1];
-        le[kb] = le[kb] + mu1;  // q(1) kept the explicit value q_e(kb)
         for (int k = kb; k <= mza; ++k) lq0(k, iw) = lq0(k, iw) + mu[k] - le[k];
         // - c(k) (min(wm(k),0) dq0(k) + max(wm(k-1),0) dq0(k-1)), dq0(k) = q0(k+1) - q0(k)
         for (int k = kb; k <= mza; ++k) {
@@ -1270,6 +1267,7 @@ struct Model : Base {
           if (k <= mza - 1) { R t = a * std::min(wm(k, iw), R(0)); lq0(k + 1, iw) += t; lq0(k, iw) -= t; }
           if (k - 1 >= kb) { R t = a * std::max(wm(k - 1, iw), R(0)); lq0(k, iw) += t; lq0(k - 1, iw) -= t; }
         }
+        le[kb] = le[kb] + mu1;  // q(1) kept the explicit value q_e(kb)
         for (int k = kb; k <= mza; ++k) mu[k] = le[k];
       } else {
         mu[kb] = mu[kb] + mu1;

@@ -196,6 +196,39 @@ def test_linearity_and_adjoint_dot_product(grid_factory):
     assert abs(a - b) / abs(a) < 1e-6
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_adjoint_dot_product_with_implicit_columns(grid_factory, mode):
+    """Vertical Courant numbers far above 1 put every column through the implicit
+    branch (tridif_prep/tridif_fini); level 1 keeps the explicit value there, which the
+    adjoint must route to the explicit level 2 only after the implicit scp0 terms.  With
+    mode > 0 the final stage is limited and its scales frozen (replayed forward)."""
+    grid = grid_factory(8, 16)
+    w, st, tr = tracer_case(grid, ntr=1, wnoise=40.0)
+    o = Oracle(grid, abi.make_params("f64", num_scalar=1, dtsm=600.0, iscal_monot=mode))
+    o.upload_state(st)
+    o.upload_scalar(1, tr[0])
+    o.scalar_transport_rk(w["rho_old"])
+    assert o.cfl_report()[3] > 0.9 * (grid.mwa - 1)
+    if mode:
+        o.set_frozen_limiter(True)
+    rng = np.random.default_rng(2)
+    dq = rng.standard_normal((grid.mwa, grid.mza)).astype(np.float32)
+    lam = rng.standard_normal((grid.mwa, grid.mza)).astype(np.float32)
+    dq[0] = 0
+    lam[0] = 0
+    o.upload_scalar(1, dq)
+    o.scalar_transport_rk(w["rho_old"])
+    Ldq = o.download_scalar(1)
+    o.upload_scalar_adj(1, lam)
+    o.scalar_transport_rk_adj()
+    LT = o.download_scalar_adj(1)
+    a = float(np.dot(Ldq[1:].ravel().astype(np.float64), lam[1:].ravel()))
+    b = float(np.dot(dq[1:].ravel().astype(np.float64), LT[1:].ravel()))
+    scale = float(np.linalg.norm(Ldq[1:].astype(np.float64)) * np.linalg.norm(lam[1:].astype(np.float64)))
+    print(f"mode {mode}: mismatch / (|L dq| |lam|) {abs(a - b) / scale:.2e}")
+    assert abs(a - b) / scale < 5e-9          # FP32 host arrays; a misplaced term gives 1e-6
+
+
 def test_implicit_branch_conserves(grid_factory):
     grid = grid_factory(8, 16)
     w, st, tr = tracer_case(grid, ntr=2, wnoise=40.0)
