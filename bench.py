@@ -39,17 +39,40 @@ def alg_bytes(ntr, prec):
 
 
 def kernel_alg_bytes(name, ntr, prec):
-    """Algorithmic bytes per cell-level for one launch of the named kernel."""
+    """Algorithmic bytes per cell-level for one launch of the named kernel: every array
+    the kernel must read or write, counted once (V-face fields as 3 faces per cell)."""
     w = 8 if prec == "f64" else 4
-    if name == "k_tr_update":          # per stage: rhos, volti, wmasc, 3 vmasc + 4 per tracer
-        return (6 + 4 * ntr) * w
-    if name == "k_dyn_w":              # P1 of the dynamics stage
-        return 132 * w // 4 + (20 if prec == "mixed" else 0)
-    if name == "k_dyn_v":              # P2
-        return 92 * w // 4 + (8 if prec == "mixed" else 0)
-    if name == "k_dyn_vel":            # P3
-        return 28 * w // 4
-    return None
+    per = {
+        "k_tr_grad": 3 * ntr,                 # q -> gx, gy
+        "k_tr_update": 6 + 5 * ntr,           # rho, volti, wmasc, vmasc; q, q0, gx, gy -> q
+        "k_tr_adj0": 6 + 5 * ntr,             # vmasc, wmasc, volti, rho; mu, lam0 -> lam0, nu, lpart
+        "k_tr_adj1": 3 + 5 * ntr,             # vmasc; nu, lpart -> lpart, lgx, lgy
+        "k_tr_adj2": 4 * ntr,                 # lpart, lgx, lgy -> lq
+        "k_dyn_w": 33,                        # P1 of the dynamics stage
+        "k_dyn_v": 23,                        # P2
+        "k_dyn_vel": 7,                       # P3
+    }.get(name)
+    if per is None:
+        return None
+    extra = {"k_dyn_w": 20, "k_dyn_v": 8}.get(name, 0) if prec == "mixed" else 0
+    return per * w + extra
+
+
+def committed_traffic(kernel, nxp, nlev, ntr, prec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of
+    this workload (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated), if there is one."""
+    path = os.path.join(ROOT, "profiles", f"hbm_counters_nxp{nxp}_L{nlev}_tr{ntr}_{prec}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == kernel:
+            try:
+                read_gb, write_gb = float(cells[2]), float(cells[4])
+            except ValueError:
+                continue
+            return (read_gb + write_gb) * 1e9, os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(nxp, nlev, ntr, prec, dt, threads, steps=2):
@@ -174,8 +197,10 @@ def main():
     roof = None
     if ab is not None:
         ach = ab * cells / per_launch_s
+        traffic, tsrc = committed_traffic(dom, args.nxp, args.nlev, args.ntr, args.prec)
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": ach / PEAK_HBM, "traffic": None,
+                "frac": ach / PEAK_HBM, "traffic": traffic,
+                "traffic_source": tsrc,
                 "frac_of_copy": (ach / (copy_bw * 1e9)) if copy_bw else None,
                 "alg_bytes_per_launch": ab * cells, "avg_launch_ms": per_launch_s * 1e3}
     bd, bf, ba = alg_bytes(args.ntr, args.prec)

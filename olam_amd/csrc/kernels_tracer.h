@@ -612,14 +612,21 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
   const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
   const bool neg = wm < R(0);
   const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
+  // the next tracer's loads are issued before the current one is computed (all of a
+  // tracer's loads are in the prefetch, so no later load waits on them)
+  R nmu = R(0), nl0 = R(0);
+  if (ti.n0 < ti.n1) { nmu = ld(a.mu + ti.n0 * a.tstride, ic, act); nl0 = ld(a.lam0 + ti.n0 * a.tstride, ic, act); }
   for (int n = ti.n0; n < ti.n1; ++n) {
+    R mu = nmu, l0 = nl0;
+    if (n + 1 < ti.n1) {
+      nmu = ld(a.mu + (n + 1) * a.tstride, ic, act);
+      nl0 = ld(a.lam0 + (n + 1) * a.tstride, ic, act);
+    }
     if (a.limited(n)) continue;
     const long off = n * a.tstride;
-    R mu = ld(a.mu + off, ic, act);
     const R mu1 = __shfl(mu, 0, WAVE);
     if (k == 1) mu = R(0);
     if (k == 2) mu = mu + mu1;                 // level 1 copied level 2 (explicit)
-    R l0 = ld(a.lam0 + off, ic, act);
     const R nu = body ? mu * a.dtr * vt * rinv : R(0);
     if (body) l0 = l0 + mu - nu * cc;
     // vertical transpose: upw(j) feeds F(j+1) with +fexpl wm(j), F(j) with -fexpl wm(j)
@@ -671,13 +678,27 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
     iw1[j] = dv < R(0);
     up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));  // this column is the upwind side
   }
+  // the next tracer's loads (nu of the column and its neighbours, lpart) are issued
+  // before the current one is reduced; all of a tracer's loads are in the prefetch
+  struct Ld { R nuc, lp, nun[7]; };
+  auto load = [&](int n, Ld &L) {
+    const long off = n * a.tstride;
+    L.nuc = ld(a.nu + off, ic, act);
+    L.lp = ld(a.lpart + off, ic, act);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) L.nun[j] = (j < 6 || np > 6) ? ld(a.nu + off, (long)nb[j] * ldk + l, act) : R(0);
+  };
+  Ld nx;
+  if (ti.n0 < ti.n1) load(ti.n0, nx);
   for (int n = ti.n0; n < ti.n1; ++n) {
+    const Ld L = nx;
+    if (n + 1 < ti.n1) load(n + 1, nx);
     if (a.limited(n)) continue;
     const long off = n * a.tstride;
-    const R nuc = ld(a.nu + off, ic, act);
+    const R nuc = L.nuc;
     R P = R(0), Gx = R(0), Gy = R(0);
     for_slots(np, [&](int j) {
-      const R nun = ld(a.nu + off, (long)nb[j] * ldk + l, act);
+      const R nun = L.nun[j];
       const R phi = iw1[j] ? vmj[j] * (nun - nuc) : vmj[j] * (nuc - nun);
       const R ph = up[j] ? phi : R(0);
       P = P + ph;
@@ -686,7 +707,7 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
     });
     if (!body) { P = R(0); Gx = R(0); Gy = R(0); }
     if (act) {
-      a.lpart[off + ic] = a.lpart[off + ic] + P;
+      a.lpart[off + ic] = L.lp + P;
       if (a.order > 1) { a.lgx[off + ic] = Gx; a.lgy[off + ic] = Gy; }
     }
   }
@@ -712,10 +733,11 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     by[j] = (j < np) ? m.w_gyc[nb[j] * NSLOT + b] : R(0);
   }
   // the next tracer's loads are issued before the current one is reduced
-  struct Ld { R lp, gxs, gys, gxn[7], gyn[7]; };
+  struct Ld { R lp, l0, gxs, gys, gxn[7], gyn[7]; };
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
     L.lp = ld(a.lpart + off, ic, act);
+    L.l0 = a.last ? ld(a.lam0 + off, ic, act) : R(0);
     L.gxs = R(0); L.gys = R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j) { L.gxn[j] = R(0); L.gyn[j] = R(0); }
@@ -745,7 +767,7 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
       });
     }
     if (!body) lq = a.limited(n) ? lev1 : R(0);
-    if (a.last) lq = lq + ld(a.lam0 + off, ic, act);   // stage 1 read q = scp0
+    if (a.last) lq = lq + L.l0;                        // stage 1 read q = scp0
     if (act) a.lq[off + ic] = lq;
   }
 }
