@@ -75,6 +75,8 @@ __device__ __forceinline__ TrItem tr_item(int nw, int ntr, int tpw) {
   return t;
 }
 
+// Cross-lane moves.  Call them with every lane of the wave active (never inside a
+// per-lane branch or a per-lane ?: operand): a lane that does not take part supplies 0.
 template <class T>
 __device__ __forceinline__ T shdn(T v, int d) { return __shfl_down(v, d, WAVE); }
 template <class T>

@@ -311,7 +311,10 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     R l0 = ld(a.lam0 + off, ic, act);
     const R nu = body ? mu * a.dtr * vt / rf : R(0);
     if (body) l0 = l0 + mu - nu * (gsum + fexp * (wm_m1 - wm));
-    const R nu_p1 = (k + 1 <= mza) ? shdn(nu, 1) : R(0);
+    // every lane takes part in a shuffle: a lane that skipped it would hand 0 to its
+    // neighbour (here lane mza - 1 would lose nu(mza))
+    const R nu_dn = shdn(nu, 1);
+    const R nu_p1 = (k + 1 <= mza) ? nu_dn : R(0);
     const R psi = act ? fexp * wm * (nu_p1 - nu) : R(0);          // adjoint of upw(k)
     const R psi_m1 = shup(psi, 1);
     const R s = inner ? ld(T.tsw + off, ic, act) : R(0);
@@ -319,12 +322,13 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     const R lowB = neg ? low : R(0), lowA = neg ? R(0) : low;
     const R h = inner ? s * psi : R(0);
     const R hneg = neg ? h : R(0), hpos = neg ? R(0) : h;
+    const R lowB_m1 = shup(lowB, 1);
     if (body) {
       if (mode == 1) {
         if (k == 2) l0 = l0 + psi_m1;          // upw(1) = scp0(2)
         if (k == mza) l0 = l0 + psi;           // upw(mza) = scp0(mza)
       }
-      l0 = l0 + shup(lowB, 1) + lowA;
+      l0 = l0 + lowB_m1 + lowA;
     }
     // vertical quadratic transpose
     const R A11 = mode == 1 ? m.awu11[l] : m.aw11[l];

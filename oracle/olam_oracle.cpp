@@ -97,6 +97,7 @@ struct Base {
   virtual void cfl_report(double *) = 0;
   virtual int set_pdef(int, int) = 0;
   virtual void set_frozen(int) = 0;
+  virtual int tape_scales(int, int, double *) = 0;
   std::string err;
 };
 
@@ -206,6 +207,13 @@ struct Model : Base {
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
   }
   void set_frozen(int v) override { frozen_lim = v != 0; }
+  // test hook: the taped limiter scales of tracer n (which 0: faces (mza, mva), 1: W levels (mza, mwa))
+  int tape_scales(int n, int which, double *out) override {
+    if (n < 1 || n > (int)tape_sv.size() || !tape_lim[n - 1]) { err = "no taped scales"; return OLAMGPU_ESTATE; }
+    const F2<R> &t = which ? tape_sw[n - 1] : tape_sv[n - 1];
+    for (size_t i = 0; i < t.v.size(); ++i) out[i] = (double)t.v[i];
+    return 0;
+  }
   int set_pdef(int n, int v) override {
     if (n < 1 || n > (int)pdef.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
     pdef[n - 1] = v != 0;
@@ -1429,6 +1437,7 @@ int oracle_mass_sums(void *h, double *out) { ((oracle_ctx *)h)->m->mass_sums(out
 int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(out); return 0; }
 int oracle_set_scalar_pdef(void *h, int32_t n, int32_t pdef) { return ((oracle_ctx *)h)->m->set_pdef(n, pdef); }
 int oracle_set_frozen_limiter(void *h, int32_t on) { ((oracle_ctx *)h)->m->set_frozen(on); return 0; }
+int oracle_tape_scales(void *h, int32_t n, int32_t which, double *out) { return ((oracle_ctx *)h)->m->tape_scales(n, which, out); }
 int oracle_set_threads(int32_t n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
 
 // tridiffo (tridiag_lib.f90:10-36) on arrays indexed 1..n, for the unit test of the

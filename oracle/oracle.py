@@ -51,6 +51,7 @@ def lib():
         L.oracle_tridiffo.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
         L.oracle_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
         L.oracle_set_frozen_limiter.argtypes = [vp, C.c_int32]
+        L.oracle_tape_scales.argtypes = [vp, C.c_int32, C.c_int32, vp]
         _lib = L
     return _lib
 
@@ -135,6 +136,13 @@ class Oracle:
 
     def set_scalar_pdef(self, n, pdef):
         assert lib().oracle_set_scalar_pdef(self.h, int(n), int(bool(pdef))) == 0
+
+    def tape_scales(self, n, which):
+        """Test hook: taped limiter scales of tracer n, which 0 = faces (mva, mza), 1 = W levels."""
+        rows = self.grid.mva if which == 0 else self.grid.mwa
+        out = np.zeros((rows, self.grid.mza))
+        assert lib().oracle_tape_scales(self.h, int(n), int(which), _p(out)) == 0
+        return out
 
     def set_frozen_limiter(self, on):
         """Test hook: later forward steps apply the limiter scales of the last step."""
