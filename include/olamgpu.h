@@ -84,6 +84,11 @@ typedef struct olamgpu_mesh {
   const float  *arw, *volti, *volwi;         /* (mza, mwa) */
   const double *volt;                        /* (mza, mwa) FP64 as mem_grid.f90:66 */
   const float  *arv, *volvi;                 /* (mza, mva) */
+  /* horizontal quadratic reconstruction (order 3), mem_adv.f90:11-17,161-354.  Read
+   * only when an *_horiz_adv_order is 3; may be NULL otherwise. */
+  const float  *xy_h, *xy_hu;                /* (5, 7, mwa) bounded / unbounded LSQ   */
+  const float  *xx0_v, *xy0_v, *yy0_v;       /* (2, mva) face offsets, bounded        */
+  const float  *xx0_vu, *xy0_vu, *yy0_vu;    /* (2, mva) face offsets, unbounded      */
 } olamgpu_mesh;
 
 /* Namelist options and time steps the path reads (oname_coms.f90:190-238,
@@ -93,9 +98,9 @@ typedef struct olamgpu_params {
   int32_t num_scalar;                /* tracers carried by scalar_transport_rk */
   int32_t nrk_wrtv;                  /* acoust_timestep_level: 2 or 3 */
   int32_t nrk_scal;                  /* scalar_timestep_level: 2 or 3 */
-  int32_t thil_horiz_adv_order;      /* 1 or 2 */
-  int32_t wind_horiz_adv_order;      /* 1 or 2 */
-  int32_t scal_horiz_adv_order;      /* 1 or 2 */
+  int32_t thil_horiz_adv_order;      /* 1, 2 or 3 */
+  int32_t wind_horiz_adv_order;      /* 1, 2 or 3 */
+  int32_t scal_horiz_adv_order;      /* 1, 2 or 3 */
   int32_t iscal_monot;               /* 0 none, 1 monotone, 2 positive definite (final stage) */
   int32_t divh_damp_level;           /* 0 disables; 1..3 */
   int32_t vort_damp_level;           /* 0 disables; 1..3 */

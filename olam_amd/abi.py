@@ -41,6 +41,9 @@ class MeshDesc(C.Structure):
         ("a_w", _fp), ("a_wu", _fp),
         ("arw", _fp), ("volti", _fp), ("volwi", _fp), ("volt", _dp),
         ("arv", _fp), ("volvi", _fp),
+        ("xy_h", _fp), ("xy_hu", _fp),
+        ("xx0_v", _fp), ("xy0_v", _fp), ("yy0_v", _fp),
+        ("xx0_vu", _fp), ("xy0_vu", _fp), ("yy0_vu", _fp),
     ]
 
 
@@ -170,6 +173,11 @@ class FortranGrid:
         keep["volt"] = col(cm["volt"], np.float64)
         keep["arv"] = col(cm["arv"])
         keep["volvi"] = col(cm["volvi"])
+        # xy_h(5,7,mwa) Fortran -> numpy [iw, n, term]; xx0_v(2,mva) -> [iv, side]
+        for name in ("xy_h", "xy_hu"):
+            keep[name] = per(g[name].reshape(mesh.nw, 35)).reshape(self.mwa, 7, 5)
+        for name in ("xx0_v", "xy0_v", "yy0_v", "xx0_vu", "xy0_vu", "yy0_vu"):
+            keep[name] = per(g[name])
         self.arrays = keep
         d = MeshDesc()
         d.mza, d.mwa, d.mva, d.mma = mza, self.mwa, self.mva, self.mma

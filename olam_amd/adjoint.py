@@ -96,6 +96,11 @@ class TransportWindow:
         c = self.ctx
         c.checkpoint_restore(CKPT_STATE, self.ckpts[0].data_ptr())
         self._put("scalar", q0)
+        if self.limited:
+            # the state checkpoints also hold the tracers (the backward pass re-runs the
+            # tracer steps): the first one must hold q(0)
+            torch.cuda.synchronize(self.device)
+            c.checkpoint_save(CKPT_STATE, self.ckpts[0].data_ptr())
         for i in range(self.nsteps):
             if i > 0 and i % self.every == 0:
                 c.checkpoint_save(CKPT_STATE, self.ckpts[i // self.every].data_ptr())

@@ -29,6 +29,10 @@ struct DMesh {
   const int *w_npoly, *w_nbr, *w_face, *w_back;
   const R *w_dirv, *w_gxc, *w_gyc, *w_dxs, *w_dys, *w_dxn, *w_dyn;
   const R *w_ecx, *w_ecy, *w_ecz, *w_xxyy;
+  // order-3 reconstruction per slot: the least-squares terms xy_h ([0]) / xy_hu ([1])
+  // of neighbour j, and the quadratic face offsets (xx, xy, yy) of this cell's side
+  // (w_os) and of the neighbour's side (w_on), bounded ([0], xx0_v) / unbounded ([1])
+  const R *w_q[2][5], *w_os[2][3], *w_on[2][3];
   const R *wnx, *wny, *wnz;
   // V records (nv x 2)
   const int *v_iw, *v_im;

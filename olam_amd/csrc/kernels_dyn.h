@@ -26,8 +26,10 @@ struct DynBuf {
   const RP *rho0, *rth0;
   const R *wmc0, *vmc0, *thilt_s, *vmxet_s, *vmyet_s, *vmzet_s, *vmt_s;
   R *vmxet_rk, *vmyet_rk, *vmzet_rk;
-  // horizontal gradients: [thil, vxe, vye, vze] x [gx, gy]
+  // horizontal gradients: [thil, vxe, vye, vze] x [gx, gy]; order 3 adds the quadratic
+  // terms [thil, vxe, vye, vze] x [xx, xy, yy]
   const R *g[8];
+  const R *gq[4][3];
   int thil_order, wind_order, istage, nrk;
   double dt8;
   R fcor;
@@ -184,6 +186,39 @@ __global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, con
   }
 }
 
+// grad_t2d_quadratic with bounded=.false. (prog_wrtv_rk.f90:983-984) of the order-3
+// fields: the five least-squares terms per level, written to (gx, gy) and gq
+template <class R>
+struct DynGrad3 { R *g[4][5]; };
+
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_grad3(DMesh<R> m, const R *thil, const R *vxe, const R *vye,
+                                                   const R *vze, DynGrad3<R> out, int do_thil, int do_wind) {
+  NOCONTRACT
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza, body = act && k >= 2;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const int np = m.w_npoly[c];
+  const R *f[4] = {thil, vxe, vye, vze};
+  for (int q = 0; q < 4; ++q) {
+    if ((q == 0 && !do_thil) || (q > 0 && !do_wind)) continue;
+    const R s0 = ld(f[q], ic, act);
+    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
+    for_slots(np, [&](int j) {
+      const R sc = ld(f[q], (long)m.w_nbr[c * NSLOT + j] * ldk + l, act) - s0;
+#pragma unroll
+      for (int t = 0; t < 5; ++t) acc[t] = acc[t] + sc * m.w_q[1][t][c * NSLOT + j];
+    });
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      if (!body) acc[t] = R(0);
+      if (act) out.g[q][t][ic] = acc[t];
+    }
+  }
+}
+
 // vertical upwinded face value at W level k of a column held one level per lane
 // (fluxes_wrtv scalar_w_column :1085-1126 with grad_z_quadratic, bounded)
 template <class R>
@@ -230,9 +265,12 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R thc = ld(b.thil_in, ic, act), vxc = ld(b.vxe, ic, act), vyc = ld(b.vye, ic, act),
           vzc = ld(b.vze, ic, act);
   R gxs[4] = {R(0), R(0), R(0), R(0)}, gys[4] = {R(0), R(0), R(0), R(0)};
+  R gqs[4][3] = {};
   for (int q = 0; q < 4; ++q) {
     const int ord = q == 0 ? b.thil_order : b.wind_order;
     if (ord > 1) { gxs[q] = ld(b.g[2 * q], ic, act); gys[q] = ld(b.g[2 * q + 1], ic, act); }
+    if (ord == 3)
+      for (int t = 0; t < 3; ++t) gqs[q][t] = ld(b.gq[q][t], ic, act);
   }
   const R qc[4] = {thc, vxc, vyc, vzc};
   // horizontal advective fluxes (:632-661)
@@ -259,6 +297,12 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
       if (ord > 1) {
         fs = qc[q] + dxs * gxs[q] + dys * gys[q];
         fn = qn + dxn * ld(b.g[2 * q], in, act) + dyn * ld(b.g[2 * q + 1], in, act);
+      }
+      if (ord == 3) {   // the unbounded fit at the bounded offsets xx0_v (:1057-1071)
+        const long e = (long)c * NSLOT + j;
+        fs = fs + m.w_os[0][0][e] * gqs[q][0] + m.w_os[0][1][e] * gqs[q][1] + m.w_os[0][2][e] * gqs[q][2];
+        fn = fn + m.w_on[0][0][e] * ld(b.gq[q][0], in, act) + m.w_on[0][1][e] * ld(b.gq[q][1], in, act) +
+             m.w_on[0][2][e] * ld(b.gq[q][2], in, act);
       }
       f[q] = sel ? fn : fs;
     }

@@ -89,11 +89,14 @@ __device__ __forceinline__ void lim_col(const DMesh<R> &m, const TrStage<R, RP> 
 
 // Face j of column c for tracer data at offset off: the low-order value (scp0 of the
 // upwind cell), the high-order correction and its flux (scalar_vflux_monot :748-827,
-// scalar_vflux_pd :1227-1290).  up_c: this column is the upwind one.
+// scalar_vflux_pd :1227-1290).  up_c: this column is the upwind one.  At order 3 the
+// quadratic terms gqc (this column's) are added with the unbounded face offsets for the
+// monotone form (xx0_vu, :793-807) and the bounded ones for the positive-definite form
+// (xx0_v, :1252-1266).
 template <class R, class RP>
 __device__ __forceinline__ void lim_face(const DMesh<R> &m, const TrStage<R, RP> &a, const LimCol<R> &L, int j,
-                                         long off, int l, bool act, int mode, R qc, R q0c, R gxc, R gyc, R &low,
-                                         R &hiv, R &sf, R &q0n) {
+                                         long off, int l, bool act, int mode, R qc, R q0c, R gxc, R gyc,
+                                         const R *gqc, R &low, R &hiv, R &sf, R &q0n) {
   NOCONTRACT
   const long ldk = m.ldk, in = (long)L.nb[j] * ldk + l;
   const R vm = L.vm[j];
@@ -108,8 +111,19 @@ __device__ __forceinline__ void lim_face(const DMesh<R> &m, const TrStage<R, RP>
     high = hi_c ? qc : qn;
   } else if (hi_c) {
     high = qc + L.dxs[j] * gxc + L.dys[j] * gyc;
+    if (a.order == 3) {
+      const int set = mode == 1 ? 1 : 0;
+      const long e = (long)L.c * NSLOT + j;
+      high = high + m.w_os[set][0][e] * gqc[0] + m.w_os[set][1][e] * gqc[1] + m.w_os[set][2][e] * gqc[2];
+    }
   } else {
     high = qn + L.dxn[j] * ld(a.gx + off, in, act) + L.dyn[j] * ld(a.gy + off, in, act);
+    if (a.order == 3) {
+      const int set = mode == 1 ? 1 : 0;
+      const long e = (long)L.c * NSLOT + j;
+      high = high + m.w_on[set][0][e] * ld(a.gq[0] + off, in, act) + m.w_on[set][1][e] * ld(a.gq[1] + off, in, act) +
+             m.w_on[set][2][e] * ld(a.gq[2] + off, in, act);
+    }
   }
   hiv = high - low;
   sf = vm * hiv;
@@ -134,8 +148,10 @@ __global__ __launch_bounds__(256) void k_tr_lim_scales(DMesh<R> m, TrStage<R, RP
     if (mode == 0) continue;
     const long off = n * a.tstride;
     const R qc = ld(a.q_in + off, ic, act), q0c = ld(a.q0 + off, ic, act);
-    R gxc = R(0), gyc = R(0);
+    R gxc = R(0), gyc = R(0), gqc[3] = {R(0), R(0), R(0)};
     if (a.order > 1) { gxc = ld(a.gx + off, ic, act); gyc = ld(a.gy + off, ic, act); }
+    if (a.order == 3)
+      for (int t = 0; t < 3; ++t) gqc[t] = ld(a.gq[t] + off, ic, act);
     // vertical: low-order upwind values from scp0, high-order from the stage input
     const R q0p1 = shdn(q0c, 1), qp1 = shdn(qc, 1);
     R upwl = (L.wm < R(0)) ? q0p1 : q0c;
@@ -157,7 +173,7 @@ __global__ __launch_bounds__(256) void k_tr_lim_scales(DMesh<R> m, TrStage<R, RP
     R fl = R(0), fin = R(0), fout = R(0);
     for_slots(L.np, [&](int j) {
       R low, hiv, sf, q0n;
-      lim_face(m, a, L, j, off, l, act, mode, qc, q0c, gxc, gyc, low, hiv, sf, q0n);
+      lim_face(m, a, L, j, off, l, act, mode, qc, q0c, gxc, gyc, gqc, low, hiv, sf, q0n);
       const R d = L.dv[j];
       fl = fl + d * L.vm[j] * (low - q0c);
       if (mode == 1) fin = fin + fmax(d * sf, R(0));
@@ -224,15 +240,17 @@ __global__ __launch_bounds__(256) void k_tr_lim_update(DMesh<R> m, TrStage<R, RP
     if (mode == 0) continue;
     const long off = n * a.tstride;
     const R qc = ld(a.q_in + off, ic, act), q0c = ld(a.q0 + off, ic, act);
-    R gxc = R(0), gyc = R(0);
+    R gxc = R(0), gyc = R(0), gqc[3] = {R(0), R(0), R(0)};
     if (a.order > 1) { gxc = ld(a.gx + off, ic, act); gyc = ld(a.gy + off, ic, act); }
+    if (a.order == 3)
+      for (int t = 0; t < 3; ++t) gqc[t] = ld(a.gq[t] + off, ic, act);
     const R s1c = body ? T.s1[off + ic] : R(1);
     const R s2c = (body && mode == 1) ? T.s2[off + ic] : R(1);
     // horizontal: renormalised face values (:704-716, :1183-1193)
     R fd = R(0);
     for_slots(L.np, [&](int j) {
       R low, hiv, sf, q0n;
-      lim_face(m, a, L, j, off, l, act, mode, qc, q0c, gxc, gyc, low, hiv, sf, q0n);
+      lim_face(m, a, L, j, off, l, act, mode, qc, q0c, gxc, gyc, gqc, low, hiv, sf, q0n);
       const long in = (long)L.nb[j] * ldk + l;
       const bool iw1side = L.dv[j] < R(0);
       const R s1n = body ? T.s1[off + in] : R(1);
@@ -390,7 +408,8 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj1(DMesh<R> m, TrAdj<R, RP> a,
     if (mode == 0) continue;
     const long off = n * a.tstride;
     const R nuc = ld(a.nu + off, ic, act);
-    R L0 = R(0), P = R(0), Gx = R(0), Gy = R(0);
+    const int set = mode == 1 ? 1 : 0;
+    R L0 = R(0), P = R(0), Gx = R(0), Gy = R(0), Gq[3] = {R(0), R(0), R(0)};
     for_slots(np, [&](int j) {
       const R nun = ld(a.nu + off, (long)nb[j] * ldk + l, act);
       const R phi = iw1[j] ? vmj[j] * (nun - nuc) : vmj[j] * (nuc - nun);
@@ -401,12 +420,16 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj1(DMesh<R> m, TrAdj<R, RP> a,
       P = P + hh;
       Gx = Gx + dxs[j] * hh;
       Gy = Gy + dys[j] * hh;
+      if (a.order == 3)
+        for (int t = 0; t < 3; ++t) Gq[t] = Gq[t] + m.w_os[set][t][c * NSLOT + j] * hh;
     });
-    if (!body) { L0 = R(0); P = R(0); Gx = R(0); Gy = R(0); }
+    if (!body) { L0 = R(0); P = R(0); Gx = R(0); Gy = R(0); Gq[0] = Gq[1] = Gq[2] = R(0); }
     if (act) {
       a.lam0[off + ic] = a.lam0[off + ic] + L0;
       a.lpart[off + ic] = a.lpart[off + ic] + P;
       if (a.order > 1) { a.lgx[off + ic] = Gx; a.lgy[off + ic] = Gy; }
+      if (a.order == 3)
+        for (int t = 0; t < 3; ++t) a.lgq[t][off + ic] = Gq[t];
     }
   }
 }

@@ -2,9 +2,10 @@
 //
 // Reference: omodel/scalar_transport_rk.F90.  One wavefront per W column, lane l
 // holds level k = l+1.  The horizontal reconstruction is materialised per stage as
-// the gradient pair (gx, gy) of grad_t2d (grad_lib.f90:236-269); the update kernel
-// then rebuilds each V-face value from the upwind cell's (q, gx, gy), exactly the
-// scalar_v_column formula (:474-533), so no V-face array is ever written.  Columns
+// the gradient pair (gx, gy) of grad_t2d (grad_lib.f90:236-269), or at order 3 the five
+// terms of grad_t2d_quadratic (:278-341: gx, gy and the quadratic terms gq[0..2]); the
+// update kernel then rebuilds each V-face value from the upwind cell's coefficients,
+// exactly the scalar_v_column formula (:474-533), so no V-face array is ever written.  Columns
 // whose vertical CFL needs the implicit branch (ff_implic > 0.001, rare) are finished
 // by separate kernels that run only when such columns exist.
 #pragma once
@@ -13,6 +14,7 @@
 template <class R, class RP>
 struct TrStage {
   const R *q_in, *q0, *sct, *gx, *gy;
+  const R *gq[3];    // order 3: the quadratic terms (xx, xy, yy) of the reconstruction
   R *q_out;
   const R *vmasc, *wmasc, *rho_old, *ffimp;
   const RP *rho;
@@ -60,6 +62,46 @@ __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict
   }
 }
 
+// Output pointers of the order-3 reconstruction (five terms per tracer).
+template <class R>
+struct Q5 { R *p[5]; };
+
+// ---------------------------------------------------------------- grad_t2d_quadratic
+// grad_lib.f90:278-341 for all tracers: the five least-squares terms (x, y, xx, xy,
+// yy) from the neighbour differences.  The bounded set (xy_h) serves the unlimited
+// stages and the positive-definite final stage (scalar_transport_rk.F90:429-430,
+// 1129-1130), the unbounded set (xy_hu) the monotone final stage (:664-665).
+template <class R>
+__global__ __launch_bounds__(256) void k_tr_grad3(DMesh<R> m, const R *__restrict__ q, Q5<R> g, int ntr,
+                                                  long tstride, int tpw, const int *tmode, int lim) {
+  const TrItem ti = tr_item(m.nw, ntr, tpw);
+  const int c = ti.c;
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza, body = act && k >= 2;
+  const int np = m.w_npoly[c];
+  int nb[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) nb[j] = m.w_nbr[c * NSLOT + j];
+  const long ldk = m.ldk;
+  for (int n = ti.n0; n < ti.n1; ++n) {
+    const int set = (lim && tmode[n] == 1) ? 1 : 0;
+    const R *qn = q + n * tstride;
+    const R qc = ld(qn, c * ldk + l, act);
+    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
+    for_slots(np, [&](int j) {
+      const R d = ld(qn, nb[j] * ldk + l, act) - qc;
+#pragma unroll
+      for (int t = 0; t < 5; ++t) acc[t] = acc[t] + d * m.w_q[set][t][c * NSLOT + j];
+    });
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      if (!body) acc[t] = R(0);
+      if (act) g.p[t][n * tstride + c * ldk + l] = acc[t];
+    }
+  }
+}
+
 // stage density used by the implicit branch, scalar_transport_rk.F90:139-150
 template <class R>
 __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
@@ -74,8 +116,8 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 // ---------------------------------------------------------------- update_scalar
 // One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
 // (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
-template <class R, class RP>
-__global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
+template <class R, class RP, bool Q3>
+__global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
   if (c >= m.nw) return;
@@ -85,8 +127,9 @@ __global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP>
   const long ldk = m.ldk;
   const long ic = c * ldk + l;
   const int np = m.w_npoly[c];
+  constexpr int NQ = Q3 ? 3 : 1;
   int nb[7];
-  R dvm[7], dxs[7], dys[7], dxn[7], dyn[7];
+  R dvm[7], dxs[7], dys[7], dxn[7], dyn[7], oqs[7][NQ], oqn[7][NQ];
   bool sel[7];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
@@ -95,6 +138,10 @@ __global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP>
     const R dv = m.w_dirv[c * NSLOT + j];
     dxs[j] = m.w_dxs[c * NSLOT + j]; dys[j] = m.w_dys[c * NSLOT + j];
     dxn[j] = m.w_dxn[c * NSLOT + j]; dyn[j] = m.w_dyn[c * NSLOT + j];
+    if constexpr (Q3) {
+#pragma unroll
+      for (int t = 0; t < 3; ++t) { oqs[j][t] = m.w_os[0][t][c * NSLOT + j]; oqn[j][t] = m.w_on[0][t][c * NSLOT + j]; }
+    }
     const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
     dvm[j] = dv * vm;
     // neighbour is upwind: this cell is iw1 (dirv < 0) and vm < 0, or iw2 and vm >= 0
@@ -112,7 +159,7 @@ __global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP>
 #pragma unroll
   for (int j = 0; j < 7; ++j) need |= (__ballot(sel[j] && act) != 0 ? 1u : 0u) << j;
   // the next tracer's loads are issued before the current one is computed
-  struct Ld { R qc, q0, st, gxc, gyc, qn[7], gxn[7], gyn[7]; };
+  struct Ld { R qc, q0, st, gxc, gyc, qn[7], gxn[7], gyn[7], gqc[NQ], gqn[7][NQ]; };
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
     const R *qi = a.q_in + off;
@@ -121,15 +168,27 @@ __global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP>
     L.st = a.sct ? ld(a.sct + off, ic, act) : R(0);
     L.gxc = R(0); L.gyc = R(0);
     if (a.order > 1) { L.gxc = ld(a.gx + off, ic, act); L.gyc = ld(a.gy + off, ic, act); }
+    if constexpr (Q3) {
+#pragma unroll
+      for (int t = 0; t < 3; ++t) L.gqc[t] = ld(a.gq[t] + off, ic, act);
+    }
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
       L.qn[j] = R(0); L.gxn[j] = R(0); L.gyn[j] = R(0);
+      if constexpr (Q3) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) L.gqn[j][t] = R(0);
+      }
       if (j < 6 || np > 6) {
         const long in = (long)nb[j] * ldk + l;
         L.qn[j] = ld(qi, in, act);
         if (a.order > 1 && ((need >> j) & 1u)) {
           L.gxn[j] = ld(a.gx + off, in, act);
           L.gyn[j] = ld(a.gy + off, in, act);
+          if constexpr (Q3) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) L.gqn[j][t] = ld(a.gq[t] + off, in, act);
+          }
         }
       }
     }
@@ -148,6 +207,10 @@ __global__ __launch_bounds__(256, 4) void k_tr_update(DMesh<R> m, TrStage<R, RP>
       if (a.order > 1) {
         fs = qc + dxs[j] * L.gxc + dys[j] * L.gyc;
         fn = L.qn[j] + dxn[j] * L.gxn[j] + dyn[j] * L.gyn[j];
+      }
+      if constexpr (Q3) {   // bounded offsets xx0_v (:515-524)
+        fs = fs + oqs[j][0] * L.gqc[0] + oqs[j][1] * L.gqc[1] + oqs[j][2] * L.gqc[2];
+        fn = fn + oqn[j][0] * L.gqn[j][0] + oqn[j][1] * L.gqn[j][1] + oqn[j][2] * L.gqn[j][2];
       }
       const R f = sel[j] ? fn : fs;
       F = F + dvm[j] * (f - q0);
@@ -521,6 +584,7 @@ struct TrAdj {
   R *mu;             // adjoint of the stage output (overwritten by the implicit pass)
   R *lam0;           // accumulates the adjoint of scp0
   R *nu, *lpart, *lgx, *lgy;
+  R *lgq[3];         // order 3: adjoints of the quadratic terms
   R *lq;             // adjoint of the stage input
   const R *vmasc, *wmasc, *rho_old, *ffimp;
   const RP *rho;
@@ -655,7 +719,7 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
   }
 }
 
-template <class R, class RP>
+template <class R, class RP, bool Q3>
 __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
@@ -664,8 +728,9 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
   const bool act = k <= mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
+  constexpr int NQ = Q3 ? 3 : 1;
   int nb[7];
-  R vmj[7], dxs[7], dys[7];
+  R vmj[7], dxs[7], dys[7], oqs[7][NQ];
   bool iw1[7], up[7];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
@@ -673,6 +738,10 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
     const int iv = m.w_face[c * NSLOT + j];
     const R dv = m.w_dirv[c * NSLOT + j];
     dxs[j] = m.w_dxs[c * NSLOT + j]; dys[j] = m.w_dys[c * NSLOT + j];
+    if constexpr (Q3) {
+#pragma unroll
+      for (int t = 0; t < 3; ++t) oqs[j][t] = m.w_os[0][t][c * NSLOT + j];
+    }
     const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
     vmj[j] = vm * dv * dv;                       // zero on padded slots
     iw1[j] = dv < R(0);
@@ -696,7 +765,9 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
     if (a.limited(n)) continue;
     const long off = n * a.tstride;
     const R nuc = L.nuc;
-    R P = R(0), Gx = R(0), Gy = R(0);
+    R P = R(0), Gx = R(0), Gy = R(0), Gq[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) Gq[t] = R(0);
     for_slots(np, [&](int j) {
       const R nun = L.nun[j];
       const R phi = iw1[j] ? vmj[j] * (nun - nuc) : vmj[j] * (nuc - nun);
@@ -704,11 +775,141 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
       P = P + ph;
       Gx = Gx + dxs[j] * ph;
       Gy = Gy + dys[j] * ph;
+      if constexpr (Q3) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) Gq[t] = Gq[t] + oqs[j][t] * ph;
+      }
     });
     if (!body) { P = R(0); Gx = R(0); Gy = R(0); }
     if (act) {
       a.lpart[off + ic] = L.lp + P;
       if (a.order > 1) { a.lgx[off + ic] = Gx; a.lgy[off + ic] = Gy; }
+      if constexpr (Q3) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) a.lgq[t][off + ic] = body ? Gq[t] : R(0);
+      }
+    }
+  }
+}
+
+// k_tr_adj0 and k_tr_adj1 in one pass: the face adjoints need nu of the neighbours,
+// which is rebuilt here from their mu (nu = mu dtr volti / rho, the same expression as
+// in k_tr_adj0) instead of being written and read back.  Per tracer it reads mu of the
+// column and its neighbours and lam0, and writes lam0, lpart, lgx, lgy.
+template <class R, class RP, bool Q3>
+__global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const int np = m.w_npoly[c];
+  constexpr int NQ = Q3 ? 3 : 1;
+  int nb[7];
+  R vmj[7], dxs[7], dys[7], vtn[7], rinvn[7], oqs[7][NQ];
+  bool iw1[7], up[7];
+  R csum = R(0);
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    nb[j] = m.w_nbr[c * NSLOT + j];
+    const int iv = m.w_face[c * NSLOT + j];
+    const R dv = m.w_dirv[c * NSLOT + j];
+    dxs[j] = m.w_dxs[c * NSLOT + j]; dys[j] = m.w_dys[c * NSLOT + j];
+    if constexpr (Q3) {
+#pragma unroll
+      for (int t = 0; t < 3; ++t) oqs[j][t] = m.w_os[0][t][c * NSLOT + j];
+    }
+    const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
+    if (j < 6 || np > 6) csum = csum + dv * vm;
+    vmj[j] = vm * dv * dv;
+    iw1[j] = dv < R(0);
+    up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));
+    const long in = (long)nb[j] * ldk + l;
+    vtn[j] = ld(m.volti, in, act);
+    const R rfn = act ? (R)a.rho[in] : R(1);
+    rinvn[j] = R(1) / (body ? rfn : R(1));
+  }
+  const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
+  const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
+  const R vt = ld(m.volti, ic, act);
+  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R rinv = R(1) / (body ? rf : R(1));
+  const R cc = csum + fexpl * (wm_m1 - wm);
+  const R a11 = m.aw11[l], a12 = m.aw12[l];
+  const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
+  const bool neg = wm < R(0);
+  const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
+  // mu of the stage output with level 1 folded into level 2 (the explicit copy)
+  auto fold = [&](R v) {
+    const R v1 = __shfl(v, 0, WAVE);
+    if (k == 1) v = R(0);
+    if (k == 2) v = v + v1;
+    return v;
+  };
+  // the next tracer's loads are issued before the current one is computed
+  struct Ld { R mu, l0, mun[7]; };
+  auto load = [&](int n, Ld &L) {
+    const long off = n * a.tstride;
+    L.mu = ld(a.mu + off, ic, act);
+    L.l0 = ld(a.lam0 + off, ic, act);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) L.mun[j] = (j < 6 || np > 6) ? ld(a.mu + off, (long)nb[j] * ldk + l, act) : R(0);
+  };
+  Ld nx;
+  if (ti.n0 < ti.n1) load(ti.n0, nx);
+  for (int n = ti.n0; n < ti.n1; ++n) {
+    const Ld L = nx;
+    if (n + 1 < ti.n1) load(n + 1, nx);
+    if (a.limited(n)) continue;
+    const long off = n * a.tstride;
+    const R mu = fold(L.mu);
+    R l0 = L.l0;
+    const R nu = body ? mu * a.dtr * vt * rinv : R(0);
+    if (body) l0 = l0 + mu - nu * cc;
+    // vertical transpose (as k_tr_adj0)
+    const R nu_p1 = shdn(nu, 1);
+    const R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
+    const R lupw_m1 = shup(lupw, 1), lupw_p1 = shdn(lupw, 1);
+    R lqv = R(0), lds = R(0);
+    if (body) {
+      if (inner && !neg) lqv = lqv + lupw;
+      if (k - 1 >= 2 && neg_m1) lqv = lqv + lupw_m1;
+      if (k - 1 >= 2 && neg_m1) lds = lds + lupw_m1 * a11;
+      if (inner && neg) lds = lds + lupw * a21_p1;
+      if (inner && !neg) lds = lds + lupw * a12;
+      if (k + 1 <= mza - 1 && !neg_p1) lds = lds + lupw_p1 * a22_p1;
+    }
+    if (!inner) lds = R(0);
+    const R lds_m1 = shup(lds, 1);
+    if (body) lqv = lqv + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
+    // face adjoints to the upwind side (as k_tr_adj1)
+    R P = R(0), Gx = R(0), Gy = R(0), Gq[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) Gq[t] = R(0);
+    for_slots(np, [&](int j) {
+      const R mun = fold(L.mun[j]);
+      const R nun = body ? mun * a.dtr * vtn[j] * rinvn[j] : R(0);
+      const R phi = iw1[j] ? vmj[j] * (nun - nu) : vmj[j] * (nu - nun);
+      const R ph = up[j] ? phi : R(0);
+      P = P + ph;
+      Gx = Gx + dxs[j] * ph;
+      Gy = Gy + dys[j] * ph;
+      if constexpr (Q3) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) Gq[t] = Gq[t] + oqs[j][t] * ph;
+      }
+    });
+    if (!body) { P = R(0); Gx = R(0); Gy = R(0); }
+    if (act) {
+      a.lam0[off + ic] = l0;
+      a.lpart[off + ic] = lqv + P;
+      if (a.order > 1) { a.lgx[off + ic] = Gx; a.lgy[off + ic] = Gy; }
+      if constexpr (Q3) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) a.lgq[t][off + ic] = body ? Gq[t] : R(0);
+      }
     }
   }
 }
@@ -768,6 +969,48 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     }
     if (!body) lq = a.limited(n) ? lev1 : R(0);
     if (a.last) lq = lq + L.l0;                        // stage 1 read q = scp0
+    if (act) a.lq[off + ic] = lq;
+  }
+}
+
+// k_tr_adj2 at order 3: the transpose of grad_t2d_quadratic, with each tracer's
+// coefficient set (unbounded xy_hu for a monotone-limited final stage, bounded xy_h
+// otherwise).  Padded slots carry no coefficients (their back slot is not a real one).
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_tr_adj2q(DMesh<R> m, TrAdj<R, RP> a) {
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza, body = act && k >= 2;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const int np = m.w_npoly[c];
+  int nb[7], bk[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) { nb[j] = m.w_nbr[c * NSLOT + j]; bk[j] = m.w_back[c * NSLOT + j]; }
+  for (int n = ti.n0; n < ti.n1; ++n) {
+    const long off = n * a.tstride;
+    const int set = (a.limited(n) && a.tmode[n] == 1) ? 1 : 0;
+    const R *lg[5] = {a.lgx + off, a.lgy + off, a.lgq[0] + off, a.lgq[1] + off, a.lgq[2] + off};
+    R lq = ld(a.lpart + off, ic, act);
+    const R lev1 = lq;                                  // limited stage: mu(1) passes through
+    R gs[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) gs[t] = ld(lg[t], ic, act);
+    for_slots(np, [&](int j) {
+      const long in = (long)nb[j] * ldk + l;
+      R gin = R(0), gout = R(0);
+#pragma unroll
+      for (int t = 0; t < 5; ++t) {
+        const R bq = (j < np) ? m.w_q[set][t][nb[j] * NSLOT + bk[j]] : R(0);
+        gin = gin + bq * ld(lg[t], in, act);
+        gout = gout + m.w_q[set][t][c * NSLOT + j] * gs[t];
+      }
+      lq = lq + gin;
+      lq = lq - gout;
+    });
+    if (!body) lq = a.limited(n) ? lev1 : R(0);
+    if (a.last) lq = lq + ld(a.lam0 + off, ic, act);   // stage 1 read q = scp0
     if (act) a.lq[off + ic] = lq;
   }
 }

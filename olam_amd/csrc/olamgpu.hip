@@ -380,6 +380,7 @@ struct Impl : ImplBase {
   DevBuf b_dirv, b_gxc, b_gyc, b_dxs, b_dys, b_dxn, b_dyn, b_ecx, b_ecy, b_ecz, b_xxyy, b_wn[3];
   DevBuf b_vn[3], b_vfact, b_xxyym, b_lev[11], b_lev8[4];
   DevBuf b_arw, b_volti, b_volwi, b_volt, b_arv, b_volvi;
+  DevBuf b_q[2][5], b_os[2][3], b_on[2][3];   // order-3 reconstruction (DMesh::w_q, w_os, w_on)
   // column patches of the LDS-staged tracer kernels (forward and adjoint)
   struct PatchSet {
     Patches h;
@@ -389,12 +390,14 @@ struct Impl : ImplBase {
     size_t lds = 0;
   } pf, pa;
   bool use_patch = false;   // OLAMGPU_TRACER=patch selects the LDS-staged tracer kernels
+  bool fuse_adj = true;     // one pass for the column and face parts of the adjoint stage
   // state
   DevBuf thil[2], wmc, wc, vxe, vye, vze, vmc, vc, vmasc, wmasc, alpha, pwfac, pvfac, thilt, vmxet, vmyet, vmzet, vmt;
   DevBuf rho, press, rho_old;
   int cur_thil = 0;
   // prog_wrtv_rk scratch
   DevBuf rho0, rth0, wmc0, vmc0, thilt_s, vmxet_s, vmyet_s, vmzet_s, vmt_s, vmxet_rk, vmyet_rk, vmzet_rk, grads[8];
+  DevBuf gradq[4][3];          // order 3: quadratic terms of thil, vxe, vye, vze
   DevBuf div2d[2], vort2d[2], fdivh, fvorth;
   int div_ih = 0, vort_ih = 0;
   bool do_div = false, do_vort = false;
@@ -402,6 +405,7 @@ struct Impl : ImplBase {
   int ntr;
   long tstride;
   DevBuf trS, trX, trY, trgx, trgy, trsct;
+  DevBuf trgq[3];              // order 3: quadratic reconstruction terms
   bool have_sct = false;
   // final-stage limiters (iscal_monot = 1 monotone, 2 positive definite)
   std::vector<int> pdef;            // scalar_tab(n)%pdef, default .true. (var_tables.f90:58)
@@ -409,7 +413,7 @@ struct Impl : ImplBase {
   bool any_lim = false, any_pre = false, taped_lim = false;
   DevBuf ffimp, cflmax, nimp, anyimp, masspart, massout;
   // adjoint
-  DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy;
+  DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy, lgq[3];
   int cur_adj = 0;
   bool have_fwd = false;
   double cfl_out[4] = {0, 0, 0, 0};
@@ -436,14 +440,21 @@ struct Impl : ImplBase {
     wsz = (size_t)nw * ldk; vsz = (size_t)nv * ldk; msz = (size_t)nm * ldk;
     if (prm.nrk_wrtv < 2 || prm.nrk_wrtv > 3 || prm.nrk_scal < 2 || prm.nrk_scal > 3)
       throw Err(OLAMGPU_EINVAL, "RK levels must be 2 or 3");
-    for (int o : {prm.thil_horiz_adv_order, prm.wind_horiz_adv_order, prm.scal_horiz_adv_order})
-      if (o < 1 || o > 2) throw Err(OLAMGPU_EINVAL, "horizontal advection order must be 1 or 2");
+    for (int o : {prm.thil_horiz_adv_order, prm.wind_horiz_adv_order, prm.scal_horiz_adv_order}) {
+      if (o < 1 || o > 3) throw Err(OLAMGPU_EINVAL, "horizontal advection order must be 1, 2 or 3");
+      if (o == 3 && !(m->xy_h && m->xy_hu && m->xx0_v && m->xy0_v && m->yy0_v && m->xx0_vu && m->xy0_vu &&
+                      m->yy0_vu))
+        throw Err(OLAMGPU_EINVAL, "order 3 needs xy_h, xy_hu and the xx0_v/xx0_vu face offsets");
+    }
     if (prm.iscal_monot < 0 || prm.iscal_monot > 2) throw Err(OLAMGPU_EINVAL, "iscal_monot must be 0, 1 or 2");
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
     build_mesh(m);
     alloc_state();
     if (const char *e = std::getenv("OLAMGPU_TPW")) tpw = std::max(1, std::atoi(e));
-    if (const char *e = std::getenv("OLAMGPU_TRACER")) use_patch = std::strcmp(e, "patch") == 0;
+    if (const char *e = std::getenv("OLAMGPU_TRACER")) {
+      use_patch = std::strcmp(e, "patch") == 0;
+      fuse_adj = std::strcmp(e, "split") != 0;   // OLAMGPU_TRACER=split: separate adjoint passes
+    }
     build_patch_tables();
   }
 
@@ -511,6 +522,36 @@ struct Impl : ImplBase {
         dxn[i] = m->v_dxps[(size_t)(v + 1) * 2 + 1 - side]; dyn[i] = m->v_dyps[(size_t)(v + 1) * 2 + 1 - side];
       }
       wn[0][c] = m->wnx[o + 1]; wn[1][c] = m->wny[o + 1]; wn[2][c] = m->wnz[o + 1];
+    }
+    // order-3 terms: xy_h(t, n, iw) at [(iw - 1) * 35 + (n - 1) * 5 + t - 1]; face offsets
+    // xx0_v(side, iv) at [(iv - 1) * 2 + side - 1]
+    std::vector<R> q3[2][5], os3[2][3], on3[2][3];
+    const bool have_q3 = m->xy_h && m->xy_hu && m->xx0_v && m->xy0_v && m->yy0_v && m->xx0_vu && m->xy0_vu &&
+                         m->yy0_vu;
+    for (int b = 0; b < 2; ++b) {
+      for (auto &v : q3[b]) v.assign(nw * NSLOT, R(0));
+      for (auto &v : os3[b]) v.assign(nw * NSLOT, R(0));
+      for (auto &v : on3[b]) v.assign(nw * NSLOT, R(0));
+    }
+    if (have_q3) {
+      const float *xy[2] = {m->xy_h, m->xy_hu};
+      const float *off[2][3] = {{m->xx0_v, m->xy0_v, m->yy0_v}, {m->xx0_vu, m->xy0_vu, m->yy0_vu}};
+      for (int c = 0; c < nw; ++c) {
+        const int o = t.perm_w[c];
+        const size_t fo = (size_t)(o + 1) * 7;
+        for (int j = 0; j < t.w_npoly[c]; ++j) {
+          const size_t i = (size_t)c * NSLOT + j;
+          const int v = in.iv(j, o);
+          const int side = (m->w_dirv[fo + j] < 0.f) ? 0 : 1;
+          for (int b = 0; b < 2; ++b) {
+            for (int q = 0; q < 5; ++q) q3[b][q][i] = xy[b][(size_t)(o + 1) * 35 + (size_t)j * 5 + q];
+            for (int q = 0; q < 3; ++q) {
+              os3[b][q][i] = off[b][q][(size_t)(v + 1) * 2 + side];
+              on3[b][q][i] = off[b][q][(size_t)(v + 1) * 2 + 1 - side];
+            }
+          }
+        }
+      }
     }
     // coefficient round trip (bitwise against the FP32 inputs)
     for (int c = 0; c < nw; ++c) {
@@ -597,6 +638,10 @@ struct Impl : ImplBase {
     for (int i = 0; i < 11; ++i) upv(b_lev[i], lev[i]);
     for (int i = 0; i < 4; ++i) upv(b_lev8[i], lev8[i]);
     upv(b_arw, arw); upv(b_volti, volti); upv(b_volwi, volwi); upv(b_volt, volt); upv(b_arv, arv); upv(b_volvi, volvi);
+    for (int b = 0; b < 2; ++b) {
+      for (int q = 0; q < 5; ++q) upv(b_q[b][q], q3[b][q]);
+      for (int q = 0; q < 3; ++q) { upv(b_os[b][q], os3[b][q]); upv(b_on[b][q], on3[b][q]); }
+    }
     // device copies of the integer tables must equal the host tables
     verify_dev(b_nbr, t.w_nbr); verify_dev(b_face, t.w_face); verify_dev(b_viw, t.v_iw); verify_dev(b_miv, t.m_iv);
 
@@ -605,6 +650,10 @@ struct Impl : ImplBase {
     dm.w_dirv = dp<R>(b_dirv); dm.w_gxc = dp<R>(b_gxc); dm.w_gyc = dp<R>(b_gyc);
     dm.w_dxs = dp<R>(b_dxs); dm.w_dys = dp<R>(b_dys); dm.w_dxn = dp<R>(b_dxn); dm.w_dyn = dp<R>(b_dyn);
     dm.w_ecx = dp<R>(b_ecx); dm.w_ecy = dp<R>(b_ecy); dm.w_ecz = dp<R>(b_ecz); dm.w_xxyy = dp<R>(b_xxyy);
+    for (int b = 0; b < 2; ++b) {
+      for (int q = 0; q < 5; ++q) dm.w_q[b][q] = dp<R>(b_q[b][q]);
+      for (int q = 0; q < 3; ++q) { dm.w_os[b][q] = dp<R>(b_os[b][q]); dm.w_on[b][q] = dp<R>(b_on[b][q]); }
+    }
     dm.wnx = dp<R>(b_wn[0]); dm.wny = dp<R>(b_wn[1]); dm.wnz = dp<R>(b_wn[2]);
     dm.v_iw = dp<int>(b_viw); dm.v_im = dp<int>(b_vim);
     dm.vnx = dp<R>(b_vn[0]); dm.vny = dp<R>(b_vn[1]); dm.vnz = dp<R>(b_vn[2]);
@@ -679,10 +728,15 @@ struct Impl : ImplBase {
     for (DevBuf *b : {&vmc, &vc, &vmasc, &pvfac, &vmt, &vmc0, &vmt_s}) { b->alloc(vsz * sizeof(R)); zero(*b); }
     for (DevBuf *b : {&rho, &press, &rho0, &rth0}) { b->alloc(wsz * sizeof(RP)); zero(*b); }
     for (auto &g : grads) { g.alloc(wsz * sizeof(R)); zero(g); }
+    if (prm.thil_horiz_adv_order == 3 || prm.wind_horiz_adv_order == 3)
+      for (auto &gv : gradq)
+        for (auto &g : gv) { g.alloc(wsz * sizeof(R)); zero(g); }
     ntr = prm.num_scalar;
     tstride = (long)wsz;
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
     for (DevBuf *b : {&trS, &trX, &trY, &trgx, &trgy}) { b->alloc(tb); zero(*b); }
+    if (prm.scal_horiz_adv_order == 3)
+      for (DevBuf &b : trgq) { b.alloc(tb); zero(b); }
     pdef.assign(std::max(ntr, 1), 1);
     if (prm.iscal_monot > 0) {
       for (DevBuf *b : {&lim_s1, &lim_s2}) { b->alloc(tb); zero(*b); }
@@ -703,6 +757,8 @@ struct Impl : ImplBase {
     if (adjA.p) return;
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
     for (DevBuf *b : {&adjA, &adjB, &lam0, &nu, &lpart, &lgx, &lgy}) { b->alloc(tb); zero(*b); }
+    if (prm.scal_horiz_adv_order == 3)
+      for (DevBuf &b : lgq) { b.alloc(tb); zero(b); }
     HIPCHK(hipStreamSynchronize(s));
   }
 
@@ -844,15 +900,27 @@ struct Impl : ImplBase {
       b.vmt_s = dp<R>(vmt_s);
       b.vmxet_rk = dp<R>(vmxet_rk); b.vmyet_rk = dp<R>(vmyet_rk); b.vmzet_rk = dp<R>(vmzet_rk);
       for (int i = 0; i < 8; ++i) b.g[i] = dp<R>(grads[i]);
+      for (int q = 0; q < 4; ++q)
+        for (int t = 0; t < 3; ++t) b.gq[q][t] = dp<R>(gradq[q][t]);
       b.thil_order = prm.thil_horiz_adv_order; b.wind_order = prm.wind_horiz_adv_order;
       b.istage = istage; b.nrk = nrk;
       b.dt8 = frac * prm.dtsm;
       b.fcor = (R)prm.fcoriol;
-      int dth = prm.thil_horiz_adv_order > 1, dwn = prm.wind_horiz_adv_order > 1;
+      int dth = prm.thil_horiz_adv_order == 2, dwn = prm.wind_horiz_adv_order == 2;
       if (dth || dwn)
         LAUNCH("k_dyn_grad", (k_dyn_grad<R>), dim3(gw), dim3(256), dm, b.thil_in, b.vxe, b.vye, b.vze,
                            dp<R>(grads[0]), dp<R>(grads[1]), dp<R>(grads[2]), dp<R>(grads[3]), dp<R>(grads[4]),
                            dp<R>(grads[5]), dp<R>(grads[6]), dp<R>(grads[7]), dth, dwn);
+      const int qth = prm.thil_horiz_adv_order == 3, qwn = prm.wind_horiz_adv_order == 3;
+      if (qth || qwn) {
+        DynGrad3<R> o;
+        for (int q = 0; q < 4; ++q) {
+          o.g[q][0] = dp<R>(grads[2 * q]); o.g[q][1] = dp<R>(grads[2 * q + 1]);
+          for (int t = 0; t < 3; ++t) o.g[q][2 + t] = dp<R>(gradq[q][t]);
+        }
+        LAUNCH("k_dyn_grad3", (k_dyn_grad3<R>), dim3(gw), dim3(256), dm, b.thil_in, b.vxe, b.vye, b.vze, o, qth,
+               qwn);
+      }
       LAUNCH("k_dyn_w", (k_dyn_w<R, RP>), dim3(gw), dim3(256), dm, b);
       cur_thil = 1 - cur_thil;
       LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
@@ -942,6 +1010,7 @@ struct Impl : ImplBase {
       TrStage<R, RP> a;
       a.q_in = dp<R>(*in); a.q0 = dp<R>(trS); a.sct = have_sct ? dp<R>(trsct) : nullptr;
       a.gx = dp<R>(trgx); a.gy = dp<R>(trgy); a.q_out = dp<R>(*out);
+      for (int t = 0; t < 3; ++t) a.gq[t] = dp<R>(trgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
       a.rho = dp<RP>(rho);
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.tstride = tstride;
@@ -956,23 +1025,17 @@ struct Impl : ImplBase {
         // limited final stage: gradients, regular update of the other tracers, then the
         // two limiter passes
         const TrLim<R> L = lim_args(prm.adjoint_tape > 0);
-        if (a.order > 1)
-          LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
-                 ntr, tstride, tpw);
-        LAUNCH("k_tr_update", (k_tr_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        tracer_update(a);
         LAUNCH("k_tr_lim_scales", (k_tr_lim_scales<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
         LAUNCH("k_tr_lim_update", (k_tr_lim_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
         taped_lim = prm.adjoint_tape > 0;
-      } else if (use_patch) {
+      } else if (use_patch && a.order < 3) {
         const dim3 g(pf.h.np), b(512);
         if (pf.cpw == 4) LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 4, rows_per_wave(4)>), g, b, pf.lds, dm, a, pf.tab);
         else if (pf.cpw == 2) LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 2, rows_per_wave(2)>), g, b, pf.lds, dm, a, pf.tab);
         else LAUNCH_SH("k_tr_patch", (k_tr_patch<R, RP, 1, rows_per_wave(1)>), g, b, pf.lds, dm, a, pf.tab);
       } else {
-        if (a.order > 1)
-          LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
-                 ntr, tstride, tpw);
-        LAUNCH("k_tr_update", (k_tr_update<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        tracer_update(a);
       }
       if (any_imp) LAUNCH("k_tr_implicit", (k_tr_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
       in = out;
@@ -982,6 +1045,22 @@ struct Impl : ImplBase {
     std::swap(trS.bytes, in->bytes);
     HIPCHK(hipGetLastError());
     have_fwd = true;
+  }
+
+  // reconstruction coefficients of the stage input, then the update of the unlimited
+  // tracers (grad_t2d or grad_t2d_quadratic, scalar_v_column + scalar_w_column)
+  void tracer_update(const TrStage<R, RP> &a) {
+    if (a.order == 2)
+      LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
+             ntr, tstride, tpw);
+    if (a.order == 3) {
+      const Q5<R> g{{dp<R>(trgx), dp<R>(trgy), dp<R>(trgq[0]), dp<R>(trgq[1]), dp<R>(trgq[2])}};
+      LAUNCH("k_tr_grad3", (k_tr_grad3<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, g, ntr, tstride, tpw, a.tmode,
+             a.lim);
+      LAUNCH("k_tr_update", (k_tr_update<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
+    } else {
+      LAUNCH("k_tr_update", (k_tr_update<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
+    }
   }
 
   // -------------------------------------------------------------- adjoint
@@ -1003,6 +1082,7 @@ struct Impl : ImplBase {
       DevBuf &lqB = cur_adj ? adjA : adjB;
       a.mu = dp<R>(muB); a.lam0 = dp<R>(lam0); a.nu = dp<R>(nu); a.lpart = dp<R>(lpart);
       a.lgx = dp<R>(lgx); a.lgy = dp<R>(lgy); a.lq = dp<R>(lqB);
+      for (int t = 0; t < 3; ++t) a.lgq[t] = dp<R>(lgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
       a.rho = dp<RP>(rho);
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.last = istage == 1;
@@ -1013,24 +1093,40 @@ struct Impl : ImplBase {
       if (any_imp) LAUNCH("k_tr_adj_implicit", (k_tr_adj_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
       if (a.lim) {
         const TrLim<R> L = lim_args(true);
-        LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-        LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        adjoint_faces(a);
         LAUNCH("k_tr_lim_adj0", (k_tr_lim_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
         LAUNCH("k_tr_lim_adj1", (k_tr_lim_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a, L);
-        LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-      } else if (use_patch) {
+        adjoint_coefs(a);
+      } else if (use_patch && a.order < 3) {
         const dim3 g(pa.h.np), b(512);
         if (pa.cpw == 4) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 4>), g, b, pa.lds, dm, a, pa.tab);
         else if (pa.cpw == 2) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 2>), g, b, pa.lds, dm, a, pa.tab);
         else LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 1>), g, b, pa.lds, dm, a, pa.tab);
       } else {
-        LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-        LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-        LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+        adjoint_faces(a);
+        adjoint_coefs(a);
       }
       cur_adj = 1 - cur_adj;
     }
     HIPCHK(hipGetLastError());
+  }
+  // column and face parts of an adjoint stage for the unlimited tracers: fused in one
+  // pass (default) or as k_tr_adj0 + k_tr_adj1 (OLAMGPU_TRACER=split)
+  void adjoint_faces(const TrAdj<R, RP> &a) {
+    const bool q3 = a.order == 3;
+    if (fuse_adj) {
+      if (q3) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
+    } else {
+      LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (q3) LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else LAUNCH("k_tr_adj1", (k_tr_adj1<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
+    }
+  }
+  // transpose of the reconstruction coefficients into the stage input's adjoint
+  void adjoint_coefs(const TrAdj<R, RP> &a) {
+    if (a.order == 3) LAUNCH("k_tr_adj2q", (k_tr_adj2q<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+    else LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
   }
 
   // -------------------------------------------------------------- diagnostics

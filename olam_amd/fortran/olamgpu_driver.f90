@@ -189,6 +189,14 @@ contains
     case ('volt'); m%volt = p
     case ('arv'); m%arv = p
     case ('volvi'); m%volvi = p
+    case ('xy_h'); m%xy_h = p
+    case ('xy_hu'); m%xy_hu = p
+    case ('xx0_v'); m%xx0_v = p
+    case ('xy0_v'); m%xy0_v = p
+    case ('yy0_v'); m%yy0_v = p
+    case ('xx0_vu'); m%xx0_vu = p
+    case ('xy0_vu'); m%xy0_vu = p
+    case ('yy0_vu'); m%yy0_vu = p
     case default
        print '(a,a)', 'unknown mesh array ', nm
        error stop 2

@@ -40,6 +40,10 @@ module olamgpu_iface
      type(c_ptr) :: a_w = c_null_ptr, a_wu = c_null_ptr
      type(c_ptr) :: arw = c_null_ptr, volti = c_null_ptr, volwi = c_null_ptr, volt = c_null_ptr
      type(c_ptr) :: arv = c_null_ptr, volvi = c_null_ptr
+     ! order-3 reconstruction (mem_adv.f90:11-17): xy_h, xy_hu (5,7,mwa); face offsets (2,mva)
+     type(c_ptr) :: xy_h = c_null_ptr, xy_hu = c_null_ptr
+     type(c_ptr) :: xx0_v = c_null_ptr, xy0_v = c_null_ptr, yy0_v = c_null_ptr
+     type(c_ptr) :: xx0_vu = c_null_ptr, xy0_vu = c_null_ptr, yy0_vu = c_null_ptr
   end type olamgpu_mesh_t
 
   type, bind(C), public :: olamgpu_params_t

@@ -533,6 +533,137 @@ def _horizontal_metrics(mesh: Mesh, xyz_unit: np.ndarray):
     # mem_adv.f90:121-147 Laplacian weights
     g["xx_yy"] = np.where(act, (1.0 / dnv[ivj]) * dnu[ivj] * arwi[:, None], 0.0)
     g["xx_yy_m"] = dnv[mesh.m_iv] * (1.0 / dnu[mesh.m_iv]) / arm0[:, None]
+    # M points of each W fan in the W's polar-stereographic plane (FP32 as hex_quad's xm)
+    dmn = xm[mesh.w_im] - xw[:, None, :]
+    xmn, ymn = _ps(dmn, clat, slat, clon, slon)
+    g.update(_quadratic_fit(mesh, np.where(act, xmn, 0.0).astype(np.float32),
+                            np.where(act, ymn, 0.0).astype(np.float32), arw0))
+
+
+def cell_second_moments(xm, ym, npoly):
+    """Integrals of x^2, xy, y^2 over each W cell (hex_quad, outils/quadrature.f90:10-81).
+
+    The cell is the fan of triangles (W, M(j-1), M(j)) in the W's polar-stereographic
+    plane, with the W point at the origin.  On a triangle with one vertex at the origin
+    and the others at p, q the integrals of the quadratic monomials are exact:
+    |det(p, q)|/12 * (px^2 + px qx + qx^2) for x^2, likewise for y^2, and
+    |det(p, q)|/24 * (2 px py + px qy + qx py + 2 qx qy) for xy.  The reference
+    evaluates the same integrals with a 2-point conical-product Gauss rule, which is
+    exact for these integrands.  Arguments are (nw, 7) arrays; FP64 result (nw, 3).
+    """
+    xm = xm.astype(np.float64)
+    ym = ym.astype(np.float64)
+    nw = xm.shape[0]
+    out = np.zeros((nw, 3))
+    J = np.arange(7)
+    for j in range(7):
+        on = j < npoly
+        jp = np.where(on, (j - 1) % np.maximum(npoly, 1), 0)
+        px, py = xm[np.arange(nw), jp], ym[np.arange(nw), jp]
+        qx, qy = xm[:, j], ym[:, j]
+        ad = np.abs(px * qy - py * qx)
+        out[:, 0] += np.where(on, ad / 12.0 * (px * px + px * qx + qx * qx), 0.0)
+        out[:, 1] += np.where(on, ad / 24.0 * (2 * px * py + px * qy + qx * py + 2 * qx * qy), 0.0)
+        out[:, 2] += np.where(on, ad / 12.0 * (py * py + py * qy + qy * qy), 0.0)
+    del J
+    return out
+
+
+def _lsq5(X):
+    """(X X^T)^-1 X for a batch of 5 x 7 design matrices X (FP32, padded columns 0),
+    with the arithmetic of mem_adv.f90:190-312: normal matrix, LU without pivoting
+    (ludcmp :396-446: unit lower L, upper U, diagonal stored as 1/U(i,i)), then
+    forward substitution with L and back substitution with U applied to the rows of X,
+    each row scaled by 1/U(i,i) first and the off-diagonal factors rescaled to match.
+    """
+    f = np.float32
+    X = X.astype(f)
+    nb = X.shape[0]
+    A = np.einsum("bin,bjn->bij", X, X).astype(f)
+    # Doolittle LU, column by column (ludcmp)
+    for j in range(5):
+        for i in range(j):
+            s = A[:, i, j].copy()
+            for k in range(i):
+                s = s - A[:, i, k] * A[:, k, j]
+            A[:, i, j] = s
+        for i in range(j, 5):
+            s = A[:, i, j].copy()
+            for k in range(j):
+                s = s - A[:, i, k] * A[:, k, j]
+            A[:, i, j] = s
+        if j != 4:
+            dum = f(1.0) / A[:, j, j]
+            for i in range(j + 1, 5):
+                A[:, i, j] = A[:, i, j] * dum
+    for i in range(5):
+        A[:, i, i] = f(1.0) / A[:, i, i]
+    h = np.transpose(A, (0, 2, 1)).copy()   # h[i, j]: i < j L(j, i); i > j U(j, i); 1/U(i, i)
+    Y = X.copy()
+    for i in range(4, -1, -1):
+        Y[:, i] = Y[:, i] * h[:, i, i][:, None]
+    # rescale column j = 5..1 of h (0-based 4..0)
+    for j in range(4, -1, -1):
+        d = h[:, j, j]
+        for i in range(j + 1, 5):
+            h[:, i, j] = h[:, i, j] * d
+        for i in range(j):
+            h[:, i, j] = h[:, i, j] * d
+        for i in range(j + 1, 5):
+            h[:, j, i] = h[:, j, i] / d
+    # forward substitution: row i -= sum_{k < i} h(k, i) row k
+    for i in range(1, 5):
+        for k in range(i):
+            Y[:, i] = Y[:, i] - h[:, k, i][:, None] * Y[:, k]
+    # back substitution: row i -= sum_{k > i} h(k, i) row k, k from 5 down
+    for i in range(3, -1, -1):
+        for k in range(4, i, -1):
+            Y[:, i] = Y[:, i] - h[:, k, i][:, None] * Y[:, k]
+    return Y
+
+
+def _quadratic_fit(mesh: Mesh, xmn, ymn, arw0):
+    """Horizontal quadratic reconstruction coefficients of mem_adv.f90:161-354.
+
+    For each W cell, the neighbours sit at (xw, yw) = 2 (dxps, dyps) of the shared V
+    face in the cell's plane.  ``xy_hu`` (unbounded) fits s - s0 = c1 x + c2 y + c3 x^2
+    + c4 xy + c5 y^2 by least squares over the neighbours; ``xy_h`` (bounded) fits the
+    same with the quadratic terms taken relative to their cell means (x^2 - xx0, ...),
+    so that the polynomial's mean over the cell is s0.  The face offsets ``xx0_v``
+    (bounded) and ``xx0_vu`` (unbounded) are the quadratic terms at the V point of each
+    side.  Layout: xy_h (nw, 7, 5) [cell, neighbour, term]; *_v (nv, 2) [face, side].
+    """
+    g = mesh.geo
+    f = np.float32
+    nw = mesh.nw
+    npoly = mesh.npoly
+    act = np.arange(7)[None, :] < npoly[:, None]
+    dx32 = g["dxps"].astype(f)
+    dy32 = g["dyps"].astype(f)
+    side = np.where(mesh.dirv < 0, 0, 1)
+    ivj = mesh.w_iv
+    xw = np.where(act, dx32[ivj, side] * f(2.0), f(0.0)).astype(f)
+    yw = np.where(act, dy32[ivj, side] * f(2.0), f(0.0)).astype(f)
+    mom = cell_second_moments(xmn, ymn, npoly)
+    arw0i = (1.0 / arw0).astype(f)
+    xx0 = mom[:, 0].astype(f) * arw0i
+    xy0 = mom[:, 1].astype(f) * arw0i
+    yy0 = mom[:, 2].astype(f) * arw0i
+    m = act.astype(f)
+    Xb = np.stack([xw, yw, (xw * xw - xx0[:, None]) * m, (xw * yw - xy0[:, None]) * m,
+                   (yw * yw - yy0[:, None]) * m], axis=1).astype(f)
+    Xu = np.stack([xw, yw, xw * xw, xw * yw, yw * yw], axis=1).astype(f)
+    xy_h = np.transpose(_lsq5(Xb), (0, 2, 1)) * m[..., None]
+    xy_hu = np.transpose(_lsq5(Xu), (0, 2, 1)) * m[..., None]
+    iw = mesh.v_iw
+    out = dict(xy_h=xy_h.astype(f), xy_hu=xy_hu.astype(f), xx0=xx0, xy0=xy0, yy0=yy0)
+    out["xx0_vu"] = (dx32 * dx32).astype(f)
+    out["xy0_vu"] = (dx32 * dy32).astype(f)
+    out["yy0_vu"] = (dy32 * dy32).astype(f)
+    out["xx0_v"] = (out["xx0_vu"] - xx0[iw]).astype(f)
+    out["xy0_v"] = (out["xy0_vu"] - xy0[iw]).astype(f)
+    out["yy0_v"] = (out["yy0_vu"] - yy0[iw]).astype(f)
+    return out
 
 
 # ----------------------------------------------------------------------------

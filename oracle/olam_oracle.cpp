@@ -10,14 +10,14 @@
 // rho/press and the prog_wrt_begs temporaries the reference keeps in real(r8),
 // RT = type of the vertical tridiagonal solve.  Modes: F32 <float,float,double>,
 // MIXED <float,double,double>, F64 <double,double,double>, and REF
-// <float,double,float> which is the reference's own type map (SURVEY appendix A of
-// the build notes; prog_wrtv_rk.f90:557-580, tridiag_lib.f90:16-21).
+// <float,double,float> which is the reference's own type map (prog_wrtv_rk.f90:557-580,
+// tridiag_lib.f90:16-21).
 //
 // Parity pins (which test checks what) are listed in the header of
 // tests/test_oracle.py.  Arrays use the reference's Fortran layout: (mza, n)
 // column-major, 1-based, dummy index 1.
 //
-// Restricted to flat terrain (lpw = lpv = lpm = 2), horizontal orders 1 and 2,
+// Restricted to flat terrain (lpw = lpv = lpm = 2), horizontal orders 1, 2 and 3,
 // vertical order 3 (iorderv = 3), no Rayleigh friction / nudging / eddy diffusion
 // (their switches are off in every configuration exercised here).
 
@@ -98,6 +98,7 @@ struct Base {
   virtual int set_pdef(int, int) = 0;
   virtual void set_frozen(int) = 0;
   virtual int tape_scales(int, int, double *) = 0;
+  virtual int recon(const float *, int, int, int, double *, double *) = 0;
   std::string err;
 };
 
@@ -114,6 +115,8 @@ struct Model : Base {
   std::vector<int> npoly;
   F2<int> w_iv, w_iw, w_im, v_iw, v_im, m_iv, m_iw, m_im;
   F2<R> dirv, ecx, ecy, ecz, gxc, gyc, xx_yy, dxps, dyps, xx_yy_m;
+  // order-3 reconstruction (mem_adv.f90:11-17): xy_h / xy_hu (5, 7, mwa) as (35, mwa)
+  F2<R> xyh, xyhu, xx0v, xy0v, yy0v, xx0vu, xy0vu, yy0vu;
   // ---- metrics ----
   std::vector<R> wnx, wny, wnz, wnxo2, wnyo2, wnzo2, arw0, vnx, vny, vnz, dnu, dniu, dnv, dniv, arm0;
   std::vector<R> dzim, dzit, zfacit, gdz_wgtm, gdz_wgtp;
@@ -157,6 +160,11 @@ struct Model : Base {
     load2(gxc, m->gxps_coef, 7, mwa); load2(gyc, m->gyps_coef, 7, mwa); load2(xx_yy, m->xx_yy, 7, mwa);
     load2(v_iw, m->v_iw, 2, mva); load2(v_im, m->v_im, 2, mva);
     load2(dxps, m->v_dxps, 2, mva); load2(dyps, m->v_dyps, 2, mva);
+    if (m->xy_h && m->xy_hu && m->xx0_v && m->xy0_v && m->yy0_v && m->xx0_vu && m->xy0_vu && m->yy0_vu) {
+      load2(xyh, m->xy_h, 35, mwa); load2(xyhu, m->xy_hu, 35, mwa);
+      load2(xx0v, m->xx0_v, 2, mva); load2(xy0v, m->xy0_v, 2, mva); load2(yy0v, m->yy0_v, 2, mva);
+      load2(xx0vu, m->xx0_vu, 2, mva); load2(xy0vu, m->xy0_vu, 2, mva); load2(yy0vu, m->yy0_vu, 2, mva);
+    }
     load2(m_iv, m->m_iv, 3, mma); load2(m_iw, m->m_iw, 3, mma); load2(m_im, m->m_im, 3, mma);
     load2(xx_yy_m, m->xx_yy_m, 3, mma);
     load1(wnx, m->wnx, mwa); load1(wny, m->wny, mwa); load1(wnz, m->wnz, mwa); load1(arw0, m->arw0, mwa);
@@ -207,6 +215,26 @@ struct Model : Base {
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
   }
   void set_frozen(int v) override { frozen_lim = v != 0; }
+  // test hook: reconstruction coefficients of a field (5, mwa, mza) and its face values
+  // on both sides of every face (2, mva, mza), for one order and coefficient/offset set
+  int recon(const float *field, int iorder, int bounded_coef, int bounded_off, double *coefs,
+            double *faces) override {
+    if (iorder < 1 || iorder > 3 || (iorder == 3 && !xyh.n)) { err = "bad order"; return OLAMGPU_EINVAL; }
+    F2<R> s, g[5];
+    s.alloc(mza, mwa);
+    for (auto &x : g) x.alloc(mza, mwa);
+    for (size_t i = 0; i < s.v.size(); ++i) s.v[i] = (R)field[i];
+    recon_coefs(s, g, iorder, bounded_coef != 0);
+    for (int t = 0; t < 5; ++t)
+      for (size_t i = 0; i < g[t].v.size(); ++i) coefs[(size_t)t * g[t].v.size() + i] = (double)g[t].v[i];
+    const size_t fv = (size_t)mza * mva;
+    for (int side = 1; side <= 2; ++side)
+      for (int iv = 2; iv <= mva; ++iv)
+        for (int k = 2; k <= mza; ++k)
+          faces[(size_t)(side - 1) * fv + (size_t)(iv - 1) * mza + (k - 1)] =
+              (double)face_value(s, g, iorder, side, iv, k, bounded_off != 0);
+    return 0;
+  }
   // test hook: the taped limiter scales of tracer n (which 0: faces (mza, mva), 1: W levels (mza, mwa))
   int tape_scales(int n, int which, double *out) override {
     if (n < 1 || n > (int)tape_sv.size() || !tape_lim[n - 1]) { err = "no taped scales"; return OLAMGPU_ESTATE; }
@@ -298,6 +326,83 @@ struct Model : Base {
       }
     }
   }
+  // grad_t2d_quadratic, grad_lib.f90:278-341: the five least-squares terms (x, y, xx,
+  // xy, yy) from the neighbour differences, bounded (xy_h) or unbounded (xy_hu)
+  void grad_t2d_quadratic(int iw, const F2<R> &s, F2<R> *g, bool bounded) const {
+    const F2<R> &c = bounded ? xyh : xyhu;
+    for (int t = 0; t < 5; ++t) for (int k = 1; k <= mza; ++k) g[t](k, iw) = 0;
+    for (int n = 1; n <= npoly[iw]; ++n) {
+      int iwn = w_iw(n, iw);
+      for (int k = 2; k <= mza; ++k) {
+        R sc = s(k, iwn) - s(k, iw);
+        for (int t = 0; t < 5; ++t) g[t](k, iw) = g[t](k, iw) + sc * c(t + 1 + 5 * (n - 1), iw);
+      }
+    }
+  }
+  // reconstruction coefficients at every W: order 2 grad_t2d (g[0], g[1]), order 3
+  // grad_t2d_quadratic (g[0..4])
+  void recon_coefs(const F2<R> &s, F2<R> *g, int iorder, bool bounded) const {
+    if (iorder == 2) {
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, s, g[0].col(iw) - 1, g[1].col(iw) - 1);
+    } else if (iorder == 3) {
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) grad_t2d_quadratic(iw, s, g, bounded);
+    }
+  }
+  // the reconstruction of cell v_iw(side, iv) at face iv (scalar_v_column forms,
+  // e.g. scalar_transport_rk.F90:490-525): order 3 adds the quadratic terms with the
+  // bounded (xx0_v) or unbounded (xx0_vu) face offsets
+  R face_value(const F2<R> &s, const F2<R> *g, int iorder, int side, int iv, int k, bool bounded) const {
+    const int iw = v_iw(side, iv);
+    if (iorder == 1) return s(k, iw);
+    R v = s(k, iw) + dxps(side, iv) * g[0](k, iw) + dyps(side, iv) * g[1](k, iw);
+    if (iorder == 3) {
+      const F2<R> &ox = bounded ? xx0v : xx0vu, &oxy = bounded ? xy0v : xy0vu, &oy = bounded ? yy0v : yy0vu;
+      v = v + ox(side, iv) * g[2](k, iw) + oxy(side, iv) * g[3](k, iw) + oy(side, iv) * g[4](k, iw);
+    }
+    return v;
+  }
+  // transposes: h, the adjoint of a face value, into the adjoints of the coefficients
+  void face_value_adj(F2<R> *lg, int iorder, int side, int iv, int k, R h, bool bounded) const {
+    const int iw = v_iw(side, iv);
+    if (iorder == 1) return;
+    lg[0](k, iw) += dxps(side, iv) * h;
+    lg[1](k, iw) += dyps(side, iv) * h;
+    if (iorder == 3) {
+      const F2<R> &ox = bounded ? xx0v : xx0vu, &oxy = bounded ? xy0v : xy0vu, &oy = bounded ? yy0v : yy0vu;
+      lg[2](k, iw) += ox(side, iv) * h;
+      lg[3](k, iw) += oxy(side, iv) * h;
+      lg[4](k, iw) += oy(side, iv) * h;
+    }
+  }
+  // coefficient t of neighbour slot n of cell iw
+  R recon_c(int iorder, bool bounded, int t, int n, int iw) const {
+    if (iorder == 2) return t == 0 ? gxc(n, iw) : gyc(n, iw);
+    return (bounded ? xyh : xyhu)(t + 1 + 5 * (n - 1), iw);
+  }
+  // lq += transpose of recon_coefs applied to lg: g_t(c) = sum_n c_t(n, c) (s(n) - s(c))
+  void recon_coefs_adj(const F2<R> *lg, F2<R> &lq, int iorder, bool bounded) const {
+    if (iorder == 1) return;
+    const int nt = iorder == 2 ? 2 : 5;
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      for (int n = 1; n <= npoly[iw]; ++n) {
+        int iwn = w_iw(n, iw);
+        int back = 0;
+        for (int m = 1; m <= npoly[iwn]; ++m) if (w_iw(m, iwn) == iw) back = m;
+        for (int k = 2; k <= mza; ++k) {
+          R in = R(0), out = R(0);
+          for (int t = 0; t < nt; ++t) {
+            in = in + recon_c(iorder, bounded, t, back, iwn) * lg[t](k, iwn);
+            out = out + recon_c(iorder, bounded, t, n, iw) * lg[t](k, iw);
+          }
+          lq(k, iw) += in;
+          lq(k, iw) -= out;
+        }
+      }
+    }
+  }
   // grad_z_quadratic bounded, itopbc = ibotbc = 0, grad_lib.f90:166-229
   void grad_z_quadratic(const R *s, R *sb, R *st) const {
     std::vector<R> ds(mza + 2, R(0));
@@ -331,13 +436,11 @@ struct Model : Base {
   }
 
   // ---------------------------------------------------------------- fluxes_wrtv
-  // prog_wrtv_rk.f90:950-1129 (iorderh 1 or 2, iorderv = 3)
-  void fluxes_wrtv(const F2<R> &s, F2<R> &upw, F2<R> &upv, int iorderh,
-                   F2<R> &gx, F2<R> &gy) {
-    if (iorderh > 1) {
-#pragma omp parallel for
-      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, s, gx.col(iw) - 1, gy.col(iw) - 1);
-    }
+  // prog_wrtv_rk.f90:950-1129 (iorderv = 3).  Order 3 fits the unbounded quadratic
+  // (grad_t2d_quadratic bounded=.false., :983-984) and evaluates it at the faces with
+  // the bounded offsets xx0_v (:1057-1071), as the reference does.
+  void fluxes_wrtv(const F2<R> &s, F2<R> &upw, F2<R> &upv, int iorderh, F2<R> *g) {
+    recon_coefs(s, g, iorderh, false);
 #pragma omp parallel
     {
       std::vector<R> sb(mza + 2), st(mza + 2);
@@ -356,13 +459,10 @@ struct Model : Base {
 #pragma omp parallel for
     for (int iv = 2; iv <= mva; ++iv) {  // scalar_v_column :1022-1081
       int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      (void)iw1; (void)iw2;
       for (int k = 2; k <= mza; ++k) {
-        R s1, s2;
-        if (iorderh == 1) { s1 = s(k, iw1); s2 = s(k, iw2); }
-        else {
-          s1 = s(k, iw1) + dxps(1, iv) * gx(k, iw1) + dyps(1, iv) * gy(k, iw1);
-          s2 = s(k, iw2) + dxps(2, iv) * gx(k, iw2) + dyps(2, iv) * gy(k, iw2);
-        }
+        const R s1 = face_value(s, g, iorderh, 1, iv, k, true);
+        const R s2 = face_value(s, g, iorderh, 2, iv, k, true);
         upv(k, iv) = s1;
         if (vmc(k, iv) < R(0)) upv(k, iv) = s2;
       }
@@ -479,13 +579,13 @@ struct Model : Base {
   F2<R> thil_upv, vxe_upv, vye_upv, vze_upv, thil_upw, vxe_upw, vye_upw, vze_upw;
   F2<R> vmxet_rk, vmyet_rk, vmzet_rk, thilt_rk, wmc0, vmc0, thilt_s, vmxet_s, vmyet_s, vmzet_s, vmt_s, vmca;
   F2<RP> rho0, rth0;
-  F2<R> gxs, gys;
+  F2<R> gs[5];
 
   void alloc_wrtv_scratch() {
     if (thil_upv.n) return;
     for (auto *a : {&thil_upv, &vxe_upv, &vye_upv, &vze_upv, &vmc0, &vmt_s, &vmca}) a->alloc(mza, mva);
     for (auto *a : {&thil_upw, &vxe_upw, &vye_upw, &vze_upw, &vmxet_rk, &vmyet_rk, &vmzet_rk, &thilt_rk,
-                    &wmc0, &thilt_s, &vmxet_s, &vmyet_s, &vmzet_s, &gxs, &gys})
+                    &wmc0, &thilt_s, &vmxet_s, &vmyet_s, &vmzet_s, &gs[0], &gs[1], &gs[2], &gs[3], &gs[4]})
       a->alloc(mza, mwa);
     rho0.alloc(mza, mwa); rth0.alloc(mza, mwa);
   }
@@ -532,10 +632,10 @@ struct Model : Base {
         if (istage == prm.nrk_wrtv)
           for (int k = 2; k <= mza; ++k) vmasc(k, iv) = vmasc(k, iv) + vmca(k, iv);
       }
-      fluxes_wrtv(thil, thil_upw, thil_upv, prm.thil_horiz_adv_order, gxs, gys);
-      fluxes_wrtv(vxe, vxe_upw, vxe_upv, prm.wind_horiz_adv_order, gxs, gys);
-      fluxes_wrtv(vye, vye_upw, vye_upv, prm.wind_horiz_adv_order, gxs, gys);
-      fluxes_wrtv(vze, vze_upw, vze_upv, prm.wind_horiz_adv_order, gxs, gys);
+      fluxes_wrtv(thil, thil_upw, thil_upv, prm.thil_horiz_adv_order, gs);
+      fluxes_wrtv(vxe, vxe_upw, vxe_upv, prm.wind_horiz_adv_order, gs);
+      fluxes_wrtv(vye, vye_upw, vye_upv, prm.wind_horiz_adv_order, gs);
+      fluxes_wrtv(vze, vze_upw, vze_upv, prm.wind_horiz_adv_order, gs);
 #pragma omp parallel for
       for (int iw = 2; iw <= mwa; ++iw) prog_wrt_begs(iw, istage, dt8);
 #pragma omp parallel for
@@ -690,7 +790,7 @@ struct Model : Base {
   }
 
   // ---------------------------------------------------------------- scalar transport
-  F2<R> scp0, gxt, gyt, upv;
+  F2<R> scp0, gt[5], upv;
   std::vector<R> ffimp;   // ff_implic(n, iw) at (iw-1)*nrk + n-1
   F2<R> rhos[3];
 
@@ -735,20 +835,12 @@ struct Model : Base {
   // from its own old values; rhosf is the final-stage density (:328-337).
   void update_scalar(F2<R> &q, const F2<R> &q0, const F2<R> &st, const F2<R> &rhosf, R dtr,
                      int iorder, int istage, int nrk) {
-    if (iorder > 1) {
-#pragma omp parallel for
-      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, q, gxt.col(iw) - 1, gyt.col(iw) - 1);
-    }
+    recon_coefs(q, gt, iorder, true);   // bounded quadratic (:429-430)
 #pragma omp parallel for
     for (int iv = 2; iv <= mva; ++iv) {  // scalar_v_column :474-533
-      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
       for (int k = 2; k <= mza; ++k) {
-        R s1, s2;
-        if (iorder == 1) { s1 = q(k, iw1); s2 = q(k, iw2); }
-        else {
-          s1 = q(k, iw1) + dxps(1, iv) * gxt(k, iw1) + dyps(1, iv) * gyt(k, iw1);
-          s2 = q(k, iw2) + dxps(2, iv) * gxt(k, iw2) + dyps(2, iv) * gyt(k, iw2);
-        }
+        const R s1 = face_value(q, gt, iorder, 1, iv, k, true);
+        const R s2 = face_value(q, gt, iorder, 2, iv, k, true);
         upv(k, iv) = s1;
         if (vmasc(k, iv) < R(0)) upv(k, iv) = s2;
       }
@@ -844,7 +936,10 @@ struct Model : Base {
 
   // scalar_transport_rk, scalar_transport_rk.F90:30-385 (no sgs mixing, no limiter)
   void scalar_transport_rk(const float *rho_old_host) override {
-    if (!scp0.n) { scp0.alloc(mza, mwa); gxt.alloc(mza, mwa); gyt.alloc(mza, mwa); upv.alloc(mza, mva); }
+    if (!scp0.n) {
+      scp0.alloc(mza, mwa); upv.alloc(mza, mva);
+      for (auto &g : gt) g.alloc(mza, mwa);
+    }
     F2<R> rho_o;
     const F2<R> *ro = &rho_old;
     if (rho_old_host) { load2(rho_o, rho_old_host, mza, mwa); ro = &rho_o; }
@@ -929,10 +1024,10 @@ struct Model : Base {
     if (!tape_sv[n].n) { tape_sv[n].alloc(mza, mva); tape_sw[n].alloc(mza, mwa); }
     tape_lim[n] = (char)mode;
     const bool frozen = frozen_lim;   // the linear map with the last forward's scales
-    if (iorder > 1) {
-#pragma omp parallel for
-      for (int iw = 2; iw <= mwa; ++iw) grad_t2d(iw, q, gxt.col(iw) - 1, gyt.col(iw) - 1);
-    }
+    // order 3: unbounded quadratic with xx0_vu for the monotone form (:665, :793-807),
+    // bounded with xx0_v for the positive-definite form (:1130, :1252-1266)
+    const bool bnd = mode == 2;
+    recon_coefs(q, gt, iorder, bnd);
     // scalar_vflux_monot (:748-827) / scalar_vflux_pd (:1227-1290)
 #pragma omp parallel for
     for (int iv = 2; iv <= mva; ++iv) {
@@ -941,8 +1036,8 @@ struct Model : Base {
         R s1, s2;
         if (iorder == 1) { s1 = q(k, iw1); s2 = (mode == 2) ? q(k, iw1) : q(k, iw2); }
         else {
-          s1 = q(k, iw1) + dxps(1, iv) * gxt(k, iw1) + dyps(1, iv) * gyt(k, iw1);
-          s2 = q(k, iw2) + dxps(2, iv) * gxt(k, iw2) + dyps(2, iv) * gyt(k, iw2);
+          s1 = face_value(q, gt, iorder, 1, iv, k, bnd);
+          s2 = face_value(q, gt, iorder, 2, iv, k, bnd);
         }
         R up = q0(k, iw1);
         if (vmasc(k, iv) < R(0)) up = q0(k, iw2);
@@ -1115,9 +1210,10 @@ struct Model : Base {
                              int iorder, int istage, int nrk, const R *dtlr) {
     const F2<R> &vm = tape_vmasc, &wm = tape_wmasc;
     const int mode = tape_lim[n];
-    if (!a_nu.n) { a_nu.alloc(mza, mwa); a_lqpart.alloc(mza, mwa); a_lgx.alloc(mza, mwa); a_lgy.alloc(mza, mwa); }
-    F2<R> &nu = a_nu, &lgx = a_lgx, &lgy = a_lgy;
-    for (auto *a : {&nu, &lgx, &lgy}) std::fill(a->v.begin(), a->v.end(), R(0));
+    const bool bnd = mode == 2;     // the forward's coefficient and offset sets
+    alloc_adj_scratch();
+    F2<R> &nu = a_nu, *lg = a_lg;
+    for (auto *a : {&nu, &lg[0], &lg[1], &lg[2], &lg[3], &lg[4]}) std::fill(a->v.begin(), a->v.end(), R(0));
     for (int iw = 2; iw <= mwa; ++iw) {
       const int kb = 2;
       R mu[130];
@@ -1206,23 +1302,19 @@ struct Model : Base {
         if (iorder == 1 && mode == 2) side = 1;       // scalar_vflux_pd order-1 form
         const int iwh = side == 1 ? iw1 : iw2;
         lq(k, iwh) += h;
-        if (iorder > 1) { lgx(k, iwh) += dxps(side, iv) * h; lgy(k, iwh) += dyps(side, iv) * h; }
+        face_value_adj(lg, iorder, side, iv, k, h, bnd);
       }
     }
-    if (iorder > 1) {
-      for (int iw = 2; iw <= mwa; ++iw)
-        for (int m = 1; m <= npoly[iw]; ++m) {
-          int iwn = w_iw(m, iw);
-          for (int k = 2; k <= mza; ++k) {
-            lq(k, iwn) += gxc(m, iw) * lgx(k, iw) + gyc(m, iw) * lgy(k, iw);
-            lq(k, iw) -= gxc(m, iw) * lgx(k, iw) + gyc(m, iw) * lgy(k, iw);
-          }
-        }
-    }
+    recon_coefs_adj(lg, lq, iorder, bnd);
   }
 
   // ---------------------------------------------------------------- adjoint
-  F2<R> a_nu, a_lqpart, a_lgx, a_lgy;
+  F2<R> a_nu, a_lqpart, a_lg[5];
+  void alloc_adj_scratch() {
+    if (a_nu.n) return;
+    a_nu.alloc(mza, mwa); a_lqpart.alloc(mza, mwa);
+    for (auto &g : a_lg) g.alloc(mza, mwa);
+  }
   // Transpose of update_scalar (+ implicit column) for fixed winds.  Derivation:
   // q_new = q0 + D (sct + volti F), F linear in (q, q0) with upwind choices frozen;
   // see the formulae in the comments below.  lq = adjoint of the stage input q,
@@ -1230,11 +1322,14 @@ struct Model : Base {
   void update_scalar_adj(const F2<R> &mu_in, F2<R> &lq, F2<R> &lq0, const F2<R> &rhosf, R dtr,
                          int iorder, int istage, int nrk, const R *dtlr) {
     const F2<R> &vm = tape_vmasc, &wm = tape_wmasc;
-    if (!a_nu.n) { a_nu.alloc(mza, mwa); a_lqpart.alloc(mza, mwa); a_lgx.alloc(mza, mwa); a_lgy.alloc(mza, mwa); }
-    F2<R> &nu = a_nu, &lqpart = a_lqpart, &lgx = a_lgx, &lgy = a_lgy;
+    alloc_adj_scratch();
+    F2<R> &nu = a_nu, &lqpart = a_lqpart, *lg = a_lg;
 #pragma omp parallel for
     for (int iw = 1; iw <= mwa; ++iw)
-      for (int k = 1; k <= mza; ++k) { nu(k, iw) = 0; lqpart(k, iw) = 0; lgx(k, iw) = 0; lgy(k, iw) = 0; }
+      for (int k = 1; k <= mza; ++k) {
+        nu(k, iw) = 0; lqpart(k, iw) = 0;
+        for (int t = 0; t < 5; ++t) lg[t](k, iw) = 0;
+      }
     // (1) implicit columns: mu -> lambda of the explicit result; (2) nu = D volti mu_e
 #pragma omp parallel for
     for (int iw = 2; iw <= mwa; ++iw) {
@@ -1332,31 +1427,16 @@ struct Model : Base {
           bool up2 = vm(k, iv) < R(0);
           if ((side == 2) == up2) {
             lqpart(k, iw) += phi;
-            if (iorder > 1) {
-              lgx(k, iw) += dxps(side, iv) * phi;
-              lgy(k, iw) += dyps(side, iv) * phi;
-            }
+            face_value_adj(lg, iorder, side, iv, k, phi, true);
           }
         }
       }
     }
-    // (4) gradient transpose: gx(c) = sum_n gxc(n,c) (q(n) - q(c))
+    // (4) transpose of the reconstruction coefficients (bounded for update_scalar)
 #pragma omp parallel for
-    for (int iw = 2; iw <= mwa; ++iw) {
+    for (int iw = 2; iw <= mwa; ++iw)
       for (int k = 1; k <= mza; ++k) lq(k, iw) = lqpart(k, iw);
-      if (iorder > 1) {
-        for (int n = 1; n <= npoly[iw]; ++n) {
-          int iwn = w_iw(n, iw);
-          // slot of iw in iwn's list
-          int back = 0;
-          for (int m = 1; m <= npoly[iwn]; ++m) if (w_iw(m, iwn) == iw) back = m;
-          for (int k = 2; k <= mza; ++k) {
-            lq(k, iw) += gxc(back, iwn) * lgx(k, iwn) + gyc(back, iwn) * lgy(k, iwn);
-            lq(k, iw) -= gxc(n, iw) * lgx(k, iw) + gyc(n, iw) * lgy(k, iw);
-          }
-        }
-      }
-    }
+    recon_coefs_adj(lg, lq, iorder, true);
   }
 
   void scalar_transport_rk_adj() override {
@@ -1411,6 +1491,9 @@ struct oracle_ctx { Base *m; std::string err; };
 int oracle_create(const olamgpu_mesh *mesh, const olamgpu_params *p, int32_t ref_types, void **out) {
   auto *c = new oracle_ctx;
   if (mesh->mza > 128) { delete c; return OLAMGPU_EINVAL; }
+  const bool q3 = p->thil_horiz_adv_order == 3 || p->wind_horiz_adv_order == 3 || p->scal_horiz_adv_order == 3;
+  if (q3 && !(mesh->xy_h && mesh->xy_hu && mesh->xx0_v && mesh->xy0_v && mesh->yy0_v && mesh->xx0_vu &&
+              mesh->xy0_vu && mesh->yy0_vu)) { delete c; return OLAMGPU_EINVAL; }
   if (ref_types) c->m = new Model<Prec<float, double, float>>(mesh, p);
   else if (p->precision == OLAMGPU_PREC_F64) c->m = new Model<Prec<double, double, double>>(mesh, p);
   else if (p->precision == OLAMGPU_PREC_MIXED) c->m = new Model<Prec<float, double, double>>(mesh, p);
@@ -1438,6 +1521,10 @@ int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(o
 int oracle_set_scalar_pdef(void *h, int32_t n, int32_t pdef) { return ((oracle_ctx *)h)->m->set_pdef(n, pdef); }
 int oracle_set_frozen_limiter(void *h, int32_t on) { ((oracle_ctx *)h)->m->set_frozen(on); return 0; }
 int oracle_tape_scales(void *h, int32_t n, int32_t which, double *out) { return ((oracle_ctx *)h)->m->tape_scales(n, which, out); }
+int oracle_recon(void *h, const float *field, int32_t order, int32_t bounded_coef, int32_t bounded_off, double *coefs,
+                 double *faces) {
+  return ((oracle_ctx *)h)->m->recon(field, order, bounded_coef, bounded_off, coefs, faces);
+}
 int oracle_set_threads(int32_t n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
 
 // tridiffo (tridiag_lib.f90:10-36) on arrays indexed 1..n, for the unit test of the

@@ -52,6 +52,7 @@ def lib():
         L.oracle_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
         L.oracle_set_frozen_limiter.argtypes = [vp, C.c_int32]
         L.oracle_tape_scales.argtypes = [vp, C.c_int32, C.c_int32, vp]
+        L.oracle_recon.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
         _lib = L
     return _lib
 
@@ -143,6 +144,19 @@ class Oracle:
         out = np.zeros((rows, self.grid.mza))
         assert lib().oracle_tape_scales(self.h, int(n), int(which), _p(out)) == 0
         return out
+
+    def recon(self, field, order, bounded_coef=True, bounded_off=True):
+        """Test hook: reconstruction coefficients (5, mwa, mza) of a (mwa, mza) field and
+        its face values (2, mva, mza) on the iw1 / iw2 sides of every face."""
+        g = self.grid
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        coefs = np.zeros((5, g.mwa, g.mza))
+        faces = np.zeros((2, g.mva, g.mza))
+        rc = lib().oracle_recon(self.h, _p(f), int(order), int(bool(bounded_coef)), int(bool(bounded_off)),
+                                _p(coefs), _p(faces))
+        if rc != 0:
+            raise ValueError(f"oracle_recon failed ({rc})")
+        return coefs, faces
 
     def set_frozen_limiter(self, on):
         """Test hook: later forward steps apply the limiter scales of the last step."""

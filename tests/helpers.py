@@ -65,9 +65,9 @@ class PermutedGrid:
 
     W_ROWS = ["w_npoly", "w_iv", "w_iw", "w_im", "w_dirv", "w_ecvec_vx", "w_ecvec_vy", "w_ecvec_vz",
               "gxps_coef", "gyps_coef", "xx_yy", "wnx", "wny", "wnz", "arw0", "lpw",
-              "arw", "volti", "volwi", "volt"]
+              "arw", "volti", "volwi", "volt", "xy_h", "xy_hu"]
     V_ROWS = ["v_iw", "v_im", "v_dxps", "v_dyps", "vnx", "vny", "vnz", "dnu", "dniu", "dnv", "dniv",
-              "lpv", "arv", "volvi"]
+              "lpv", "arv", "volvi", "xx0_v", "xy0_v", "yy0_v", "xx0_vu", "xy0_vu", "yy0_vu"]
     M_ROWS = ["m_iv", "m_iw", "m_im", "xx_yy_m", "arm0", "lpm"]
 
     def __init__(self, grid, seed=0):

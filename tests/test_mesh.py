@@ -150,3 +150,96 @@ def test_fortran_tables_roundtrip():
     back = A["w_iw"][1:] - 2
     for j in range(5):
         assert np.array_equal(back[:, j], m.w_iw[:, j])
+
+
+# ------------------------------------------------------------------ order-3 setup
+def _neighbour_ps(m):
+    """Neighbour positions (xw, yw) = 2 (dxps, dyps) of the shared face, FP32 then FP64."""
+    g = m.geo
+    act = np.arange(7)[None, :] < m.npoly[:, None]
+    side = np.where(m.dirv < 0, 0, 1)
+    f = np.float32
+    xw = np.where(act, g["dxps"].astype(f)[m.w_iv, side] * f(2), 0).astype(np.float64)
+    yw = np.where(act, g["dyps"].astype(f)[m.w_iv, side] * f(2), 0).astype(np.float64)
+    return act, xw, yw
+
+
+def test_cell_moments_match_polygon_boundary_integrals():
+    """hex_quad's integrals of x^2, xy, y^2 over the cell against Green's theorem on
+    the M polygon (a boundary sum independent of the fan decomposition)."""
+    m = M.build_mesh(6)
+    g = m.geo
+    xw = np.stack([g["xew"], g["yew"], g["zew"]], 1)
+    xm = np.stack([g["xem"], g["yem"], g["zem"]], 1)
+    sl, cl = np.sin(np.radians(g["glatw"])), np.cos(np.radians(g["glatw"]))
+    so, co = np.sin(np.radians(g["glonw"])), np.cos(np.radians(g["glonw"]))
+    C = lambda a: np.repeat(a[:, None], 7, axis=1)
+    px, py = M._ps(xm[m.w_im] - xw[:, None, :], C(cl), C(sl), C(co), C(so))
+    act = np.arange(7)[None, :] < m.npoly[:, None]
+    px, py = np.where(act, px, 0).astype(np.float32), np.where(act, py, 0).astype(np.float32)
+    mom = M.cell_second_moments(px, py, m.npoly)
+    for c in range(0, m.nw, 7):
+        n = m.npoly[c]
+        x, y = px[c, :n].astype(np.float64), py[c, :n].astype(np.float64)
+        x1, y1 = np.roll(x, -1), np.roll(y, -1)
+        cr = x * y1 - x1 * y
+        sgn = np.sign(cr.sum())
+        ixx = sgn * np.sum(cr * (x * x + x * x1 + x1 * x1)) / 12
+        iyy = sgn * np.sum(cr * (y * y + y * y1 + y1 * y1)) / 12
+        ixy = sgn * np.sum(cr * (x * y1 + 2 * x * y + 2 * x1 * y1 + x1 * y)) / 24
+        ref = np.array([ixx, ixy, iyy])
+        assert np.allclose(mom[c], ref, rtol=1e-12, atol=1e-12 * abs(ixx))
+    # the fan's area is the cell area arw0 up to the projection's distortion
+    area = np.zeros(m.nw)
+    for j in range(7):
+        on = j < m.npoly
+        jp = np.where(on, (j - 1) % m.npoly, 0)
+        ax, ay = px[np.arange(m.nw), jp].astype(float), py[np.arange(m.nw), jp].astype(float)
+        area += np.where(on, 0.5 * np.abs(ax * py[:, j] - ay * px[:, j]), 0)
+    assert np.abs(area / g["arw0"] - 1).max() < 1e-2          # ~0.5 % at 1000 km cells
+
+
+def test_quadratic_lsq_coefficients_match_fp64_solve():
+    """xy_hu / xy_h (the FP32 no-pivot LU of mem_adv.f90:190-312) against a pivoted FP64
+    solve of the same normal equations."""
+    m = M.build_mesh(12)
+    g = m.geo
+    act, xw, yw = _neighbour_ps(m)
+    xx0, xy0, yy0 = (g[k].astype(np.float64)[:, None] for k in ("xx0", "xy0", "yy0"))
+    for key, X in (("xy_hu", np.stack([xw, yw, xw * xw, xw * yw, yw * yw], 1)),
+                   ("xy_h", np.stack([xw, yw, (xw * xw - xx0) * act, (xw * yw - xy0) * act,
+                                      (yw * yw - yy0) * act], 1))):
+        ref = np.linalg.solve(np.einsum("bin,bjn->bij", X, X), X)
+        got = np.transpose(g[key], (0, 2, 1)).astype(np.float64)
+        err = np.abs(got - ref).max(axis=2) / np.abs(ref).max(axis=2)
+        print(key, "max rel err", err.max())
+        assert err.max() < 5e-6
+        assert np.all(g[key][~act] == 0)
+
+
+def test_quadratic_fit_reproduces_quadratics():
+    """Neighbour values of a quadratic s - s0 = c1 x + c2 y + c3 x^2 + c4 xy + c5 y^2
+    give back (c1..c5) through xy_hu; with the quadratic terms taken about their cell
+    means (x^2 - xx0, ...) they give it back through xy_h.  Face offsets are the
+    quadratic terms at each side's V point (mem_adv.f90:322-354)."""
+    m = M.build_mesh(10)
+    g = m.geo
+    act, xw, yw = _neighbour_ps(m)
+    rng = np.random.default_rng(3)
+    cc = rng.standard_normal(5) * np.array([1e-5, 1e-5, 1e-10, 1e-10, 1e-10])
+    xx0, xy0, yy0 = (g[k].astype(np.float64)[:, None] for k in ("xx0", "xy0", "yy0"))
+    d_u = cc[0] * xw + cc[1] * yw + cc[2] * xw * xw + cc[3] * xw * yw + cc[4] * yw * yw
+    d_b = (cc[0] * xw + cc[1] * yw + cc[2] * (xw * xw - xx0) + cc[3] * (xw * yw - xy0)
+           + cc[4] * (yw * yw - yy0))
+    for key, d in (("xy_hu", d_u), ("xy_h", d_b)):
+        rec = np.einsum("bnt,bn->bt", g[key].astype(np.float64), d * act)
+        err = np.abs(rec / cc - 1).max()
+        print(key, "reproduction rel err", err)
+        assert err < 2e-5
+    f = np.float32
+    dx, dy = g["dxps"].astype(f), g["dyps"].astype(f)
+    assert np.array_equal(g["xx0_vu"], dx * dx) and np.array_equal(g["xy0_vu"], dx * dy)
+    assert np.array_equal(g["xx0_v"], dx * dx - g["xx0"][m.v_iw])
+    assert np.array_equal(g["yy0_v"], dy * dy - g["yy0"][m.v_iw])
+    # a wrong side (iw1 <-> iw2) would differ
+    assert not np.array_equal(g["xx0_v"], dx * dx - g["xx0"][m.v_iw[:, ::-1]])
