@@ -47,14 +47,20 @@ def kernel_alg_bytes(name, ntr, prec):
         "k_tr_update": 6 + 5 * ntr,           # rho, volti, wmasc, vmasc; q, q0, gx, gy -> q
         "k_tr_adj0": 6 + 5 * ntr,             # vmasc, wmasc, volti, rho; mu, lam0 -> lam0, nu, lpart
         "k_tr_adj1": 3 + 5 * ntr,             # vmasc; nu, lpart -> lpart, lgx, lgy
+        "k_tr_adj01": 6 + 6 * ntr,            # vmasc, wmasc, volti, rho; mu, lam0 -> lam0, lpart, lgx, lgy
         "k_tr_adj2": 4 * ntr,                 # lpart, lgx, lgy -> lq
-        "k_dyn_w": 33,                        # P1 of the dynamics stage
+        # thil, vxe, vye, vze, their 8 gradients, 4 short-step tendencies, rho, volt (FP64),
+        # wmc, volti, alpha_press, rho0, rth0, pwfac, volwi, wmc0, wc, arw; vmc, arv on 3
+        # faces per cell -> press, rho, thil, vm[xyz]et_rk, wmc, wc
+        "k_dyn_w": 43,
         "k_dyn_v": 23,                        # P2
         "k_dyn_vel": 7,                       # P3
     }.get(name)
     if per is None:
         return None
     extra = {"k_dyn_w": 20, "k_dyn_v": 8}.get(name, 0) if prec == "mixed" else 0
+    if prec == "f64" and name == "k_dyn_w":
+        extra = -8                           # volt is one FP64 word, counted as two FP32 words
     return per * w + extra
 
 

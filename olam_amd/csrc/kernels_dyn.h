@@ -277,32 +277,44 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   RP hflx = RP(0);
   R tt = ld(b.thilt_s, ic, act), tx = ld(b.vmxet_s, ic, act), ty = ld(b.vmyet_s, ic, act),
     tz = ld(b.vmzet_s, ic, act);
+  // face mass fluxes first: a neighbour's values and gradients are gathered only if some
+  // level takes that face's value from the neighbour (upwind side)
+  R vmca_j[7];
+  bool sel_j[7];
+  unsigned need = 0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const long ie = (long)m.w_face[c * NSLOT + j] * ldk + l;
+    const R dv = m.w_dirv[c * NSLOT + j];
+    const R vmcv = (j < 6 || np > 6) ? ld(b.vmc, ie, act) : R(0);
+    vmca_j[j] = (j < np) ? vmcv * ld(m.arv, ie, act) : R(0);
+    sel_j[j] = (dv < R(0)) ? (vmcv < R(0)) : !(vmcv < R(0));
+    need |= (__ballot(sel_j[j] && act) != 0 ? 1u : 0u) << j;
+  }
   for_slots(np, [&](int j) {
-    const bool pad = j >= np;
-    const int iv = m.w_face[c * NSLOT + j];
     const int nbr = m.w_nbr[c * NSLOT + j];
     const R dv = m.w_dirv[c * NSLOT + j];
     const R dxs = m.w_dxs[c * NSLOT + j], dys = m.w_dys[c * NSLOT + j];
     const R dxn = m.w_dxn[c * NSLOT + j], dyn = m.w_dyn[c * NSLOT + j];
-    const long ie = (long)iv * ldk + l, in = (long)nbr * ldk + l;
-    const R vmcv = ld(b.vmc, ie, act);
-    const R vmca = pad ? R(0) : vmcv * ld(m.arv, ie, act);
-    const bool sel = (dv < R(0)) ? (vmcv < R(0)) : !(vmcv < R(0));
+    const long in = (long)nbr * ldk + l;
+    const R vmca = vmca_j[j];
+    const bool sel = sel_j[j];
+    const bool nact = act && ((need >> j) & 1u);
     R f[4];
     for (int q = 0; q < 4; ++q) {
       const int ord = q == 0 ? b.thil_order : b.wind_order;
       const R *src = q == 0 ? b.thil_in : (q == 1 ? b.vxe : (q == 2 ? b.vye : b.vze));
-      const R qn = ld(src, in, act);
+      const R qn = ld(src, in, nact);
       R fs = qc[q], fn = qn;
       if (ord > 1) {
         fs = qc[q] + dxs * gxs[q] + dys * gys[q];
-        fn = qn + dxn * ld(b.g[2 * q], in, act) + dyn * ld(b.g[2 * q + 1], in, act);
+        fn = qn + dxn * ld(b.g[2 * q], in, nact) + dyn * ld(b.g[2 * q + 1], in, nact);
       }
       if (ord == 3) {   // the unbounded fit at the bounded offsets xx0_v (:1057-1071)
         const long e = (long)c * NSLOT + j;
         fs = fs + m.w_os[0][0][e] * gqs[q][0] + m.w_os[0][1][e] * gqs[q][1] + m.w_os[0][2][e] * gqs[q][2];
-        fn = fn + m.w_on[0][0][e] * ld(b.gq[q][0], in, act) + m.w_on[0][1][e] * ld(b.gq[q][1], in, act) +
-             m.w_on[0][2][e] * ld(b.gq[q][2], in, act);
+        fn = fn + m.w_on[0][0][e] * ld(b.gq[q][0], in, nact) + m.w_on[0][1][e] * ld(b.gq[q][1], in, nact) +
+             m.w_on[0][2][e] * ld(b.gq[q][2], in, nact);
       }
       f[q] = sel ? fn : fs;
     }

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
   const R rf = act ? (R)a.rho[ic] : R(1);
   const R rinv = R(1) / (body ? rf : R(1));
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
-  // a neighbour's gradient is read only if some level takes that face from it
+  // a neighbour is read only if some level takes that face's value from it
   unsigned need = 0;
 #pragma unroll
   for (int j = 0; j < 7; ++j) need |= (__ballot(sel[j] && act) != 0 ? 1u : 0u) << j;
@@ -179,10 +179,10 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
 #pragma unroll
         for (int t = 0; t < 3; ++t) L.gqn[j][t] = R(0);
       }
-      if (j < 6 || np > 6) {
+      if ((j < 6 || np > 6) && ((need >> j) & 1u)) {
         const long in = (long)nb[j] * ldk + l;
         L.qn[j] = ld(qi, in, act);
-        if (a.order > 1 && ((need >> j) & 1u)) {
+        if (a.order > 1) {
           L.gxn[j] = ld(a.gx + off, in, act);
           L.gyn[j] = ld(a.gy + off, in, act);
           if constexpr (Q3) {
@@ -810,6 +810,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   int nb[7];
   R vmj[7], dxs[7], dys[7], vtn[7], rinvn[7], oqs[7][NQ];
   bool iw1[7], up[7];
+  unsigned need = 0;
   R csum = R(0);
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
@@ -826,9 +827,12 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
     vmj[j] = vm * dv * dv;
     iw1[j] = dv < R(0);
     up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));
+    // a neighbour is read only if this column is upwind of that face at some level
+    need |= (__ballot(up[j] && act) != 0 ? 1u : 0u) << j;
+    const bool nact = act && ((need >> j) & 1u);
     const long in = (long)nb[j] * ldk + l;
-    vtn[j] = ld(m.volti, in, act);
-    const R rfn = act ? (R)a.rho[in] : R(1);
+    vtn[j] = ld(m.volti, in, nact);
+    const R rfn = nact ? (R)a.rho[in] : R(1);
     rinvn[j] = R(1) / (body ? rfn : R(1));
   }
   const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
@@ -855,7 +859,8 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
     L.mu = ld(a.mu + off, ic, act);
     L.l0 = ld(a.lam0 + off, ic, act);
 #pragma unroll
-    for (int j = 0; j < 7; ++j) L.mun[j] = (j < 6 || np > 6) ? ld(a.mu + off, (long)nb[j] * ldk + l, act) : R(0);
+    for (int j = 0; j < 7; ++j)
+      L.mun[j] = ((j < 6 || np > 6) && ((need >> j) & 1u)) ? ld(a.mu + off, (long)nb[j] * ldk + l, act) : R(0);
   };
   Ld nx;
   if (ti.n0 < ti.n1) load(ti.n0, nx);
