@@ -111,6 +111,8 @@ typedef struct olamgpu_params {
   int32_t nacoust;                   /* acoustic steps per long step */
   int32_t adjoint_tape;              /* >0: record forward tracer tapes for the adjoint */
   double  fcoriol;                   /* 2 Omega on the global grid */
+  int32_t ithil_monot;               /* >0: monotone thil fluxes on the final acoustic RK stage */
+  int32_t iwind_monot;               /* >0: monotone wind fluxes on the final acoustic RK stage */
 } olamgpu_params;
 
 /* Prognostic and diagnostic fields of mem_basic/mem_tend (mem_basic.f90:8-34,

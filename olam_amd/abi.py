@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("vort_damp_level", C.c_int32), ("divh_damp_fact", C.c_float),
         ("vort_damp_fact", C.c_float), ("dtsm", C.c_double), ("dtlm", C.c_double),
         ("nacoust", C.c_int32), ("adjoint_tape", C.c_int32), ("fcoriol", C.c_double),
+        ("ithil_monot", C.c_int32), ("iwind_monot", C.c_int32),
     ]
 
 
@@ -77,7 +78,7 @@ class StateDesc(C.Structure):
 
 def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, iscal_monot=0,
                 divh_damp_level=3, vort_damp_level=3, divh_damp_fact=0.1, vort_damp_fact=0.075,
-                dtsm=60.0, dtlm=None, nacoust=1, adjoint_tape=1):
+                dtsm=60.0, dtlm=None, nacoust=1, adjoint_tape=1, ithil_monot=0, iwind_monot=0):
     """Namelist defaults of oname_coms.f90:190-209, with RK3 for scalars."""
     if isinstance(precision, str):
         precision = PREC_NAMES[precision]
@@ -85,7 +86,7 @@ def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, 
         dtlm = dtsm * nacoust
     return Params(precision, num_scalar, nrk_wrtv, nrk_scal, order, order, order, iscal_monot,
                   divh_damp_level, vort_damp_level, divh_damp_fact, vort_damp_fact, dtsm, dtlm,
-                  nacoust, adjoint_tape, 2.0 * M.OMEGA)
+                  nacoust, adjoint_tape, 2.0 * M.OMEGA, ithil_monot, iwind_monot)
 
 
 def _ptr(a, ctype):

@@ -30,6 +30,10 @@ struct DynBuf {
   // terms [thil, vxe, vye, vze] x [xx, xy, yy]
   const R *g[8];
   const R *gq[4][3];
+  // fluxes_wrtv_monot on the final stage for the fields with mono[q] != 0 (thil: q = 0,
+  // winds: 1..3): low-order values scpt and the scales of scalar_monot_limiter
+  const R *scpt[4], *ms1[4], *ms2[4];
+  int mono[4];
   int thil_order, wind_order, istage, nrk;
   double dt8;
   R fcor;
@@ -193,7 +197,8 @@ struct DynGrad3 { R *g[4][5]; };
 
 template <class R>
 __global__ __launch_bounds__(256) void k_dyn_grad3(DMesh<R> m, const R *thil, const R *vxe, const R *vye,
-                                                   const R *vze, DynGrad3<R> out, int do_thil, int do_wind) {
+                                                   const R *vze, DynGrad3<R> out, int do_thil, int do_wind,
+                                                   int bounded_mask) {
   NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
@@ -205,11 +210,14 @@ __global__ __launch_bounds__(256) void k_dyn_grad3(DMesh<R> m, const R *thil, co
   for (int q = 0; q < 4; ++q) {
     if ((q == 0 && !do_thil) || (q > 0 && !do_wind)) continue;
     const R s0 = ld(f[q], ic, act);
+    // unbounded fit (xy_hu) except for a monotone-flux field on the final stage, whose
+    // fluxes_wrtv_monot fits the bounded quadratic (:1191-1192)
+    const int set = ((bounded_mask >> q) & 1) ? 0 : 1;
     R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
     for_slots(np, [&](int j) {
       const R sc = ld(f[q], (long)m.w_nbr[c * NSLOT + j] * ldk + l, act) - s0;
 #pragma unroll
-      for (int t = 0; t < 5; ++t) acc[t] = acc[t] + sc * m.w_q[1][t][c * NSLOT + j];
+      for (int t = 0; t < 5; ++t) acc[t] = acc[t] + sc * m.w_q[set][t][c * NSLOT + j];
     });
 #pragma unroll
     for (int t = 0; t < 5; ++t) {
@@ -237,6 +245,31 @@ __device__ __forceinline__ R vert_upw(R q, R wsign, int k, int mza, R a11, R a21
   return u;
 }
 
+// fluxes_wrtv_monot's vertical face values (scalar_wflux_monot :1380-1445 and the
+// renormalisation :1235-1246): scpt upwinded by wmc plus the correction from the
+// unbounded constant-gradient quadratic, scaled by the column's in/out factors.
+template <class R>
+__device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scpt, const R *s1, const R *s2,
+                                            long ic, bool act, R wmcc, R arw, int k, int mza) {
+  NOCONTRACT
+  const bool inner = k >= 2 && k <= mza - 1;
+  R sb, st;
+  lim_vrec(m, q, k, mza, 1, sb, st);
+  const R sc = ld(scpt, ic, act), sc_p1 = shdn(sc, 1);
+  const R low = (wmcc < R(0)) ? sc_p1 : sc;
+  const R sb_p1 = shdn(sb, 1);
+  const R hiw = inner ? ((wmcc < R(0)) ? sb_p1 : st) - low : R(0);
+  const R sfw = inner ? arw * wmcc * hiw : R(0);
+  const R s1c = ld(s1, ic, act), s2c = ld(s2, ic, act);
+  const R s1p = shdn(s1c, 1), s2p = shdn(s2c, 1);
+  const R scale = (sfw < R(0)) ? fmin(s1c, s2p) : fmin(s1p, s2c);
+  const R qp1 = shdn(q, 1);
+  R u = low + hiw * scale;
+  if (k == 1) u = qp1;
+  if (k == mza) u = q;
+  return u;
+}
+
 // ---------------------------------------------------------------- prog_wrt_begs
 // prog_wrtv_rk.f90:494-884 (global domain, no nudging, b20 = 1) fused with the face
 // values of fluxes_wrtv's scalar_v_column (:1022-1081) and scalar_w_column.
@@ -245,7 +278,7 @@ __device__ __forceinline__ R vert_upw(R q, R wsign, int k, int mza, R a11, R a21
 __device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
 __device__ __forceinline__ double pow_r(double x, double y) { return pow(x, y); }
 
-template <class R, class RP>
+template <class R, class RP, bool MONO>
 __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
   const int c = wave_item();
@@ -317,6 +350,31 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
              m.w_on[0][2][e] * ld(b.gq[q][2], in, nact);
       }
       f[q] = sel ? fn : fs;
+      if constexpr (MONO) {
+        if (b.mono[q]) {
+          // fluxes_wrtv_monot: scpt of the upwind cell plus the renormalised correction
+          // (:1326-1374, :1251-1266); at order 3 the bounded fit with xx0_vu offsets
+          R hs = fs, hn = fn;
+          if (ord == 3) {
+            const long e = (long)c * NSLOT + j;
+            hs = qc[q] + dxs * gxs[q] + dys * gys[q];
+            hs = hs + m.w_os[1][0][e] * gqs[q][0] + m.w_os[1][1][e] * gqs[q][1] + m.w_os[1][2][e] * gqs[q][2];
+            hn = qn + dxn * ld(b.g[2 * q], in, nact) + dyn * ld(b.g[2 * q + 1], in, nact);
+            hn = hn + m.w_on[1][0][e] * ld(b.gq[q][0], in, nact) + m.w_on[1][1][e] * ld(b.gq[q][1], in, nact) +
+                 m.w_on[1][2][e] * ld(b.gq[q][2], in, nact);
+          }
+          const R low = sel ? ld(b.scpt[q], in, act) : ld(b.scpt[q], ic, act);
+          const R hiv = (sel ? hn : hs) - low;
+          const R sfv = vmca * hiv;
+          const bool iw1side = dv < R(0);
+          const R s1c = ld(b.ms1[q], ic, act), s2c = ld(b.ms2[q], ic, act);
+          const R s1n = ld(b.ms1[q], in, act), s2n = ld(b.ms2[q], in, act);
+          const R s1_1 = iw1side ? s1c : s1n, s1_2 = iw1side ? s1n : s1c;
+          const R s2_1 = iw1side ? s2c : s2n, s2_2 = iw1side ? s2n : s2c;
+          const R scale = (sfv < R(0)) ? fmin(s1_1, s2_2) : fmin(s1_2, s2_1);
+          f[q] = low + hiv * scale;
+        }
+      }
     }
     if (dv > R(0)) {
       hflx = hflx + (RP)vmca;
@@ -337,10 +395,13 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   // vertical upwinded values (own column, wmc sign)
   const R wmcc = ld(b.wmc, ic, act);
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
-  const R tupw = vert_upw(thc, wmcc, k, mza, a11, a21, a12, a22);
-  const R xupw = vert_upw(vxc, wmcc, k, mza, a11, a21, a12, a22);
-  const R yupw = vert_upw(vyc, wmcc, k, mza, a11, a21, a12, a22);
-  const R zupw = vert_upw(vzc, wmcc, k, mza, a11, a21, a12, a22);
+  R upws[4];
+  for (int q = 0; q < 4; ++q) {
+    upws[q] = vert_upw(qc[q], wmcc, k, mza, a11, a21, a12, a22);
+    if constexpr (MONO)
+      if (b.mono[q]) upws[q] = vert_upw_monot(m, qc[q], b.scpt[q], b.ms1[q], b.ms2[q], ic, act, wmcc, ld(m.arw, ic, act), k, mza);
+  }
+  const R tupw = upws[0], xupw = upws[1], yupw = upws[2], zupw = upws[3];
   // explicit density and theta tendencies (:687-723)
   const R vti = ld(m.volti, ic, act);
   const R alp = ld(b.alpha_press, ic, act);
@@ -424,6 +485,130 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   if (k == 1) { wc_new = wc_new_p1; thil_new = thil_new_p1; }
   if (act && k <= mza - 1) b.wc[ic] = wc_new;
   if (act) b.thil_out[ic] = thil_new;
+}
+
+// ---------------------------------------------------------------- fluxes_wrtv_monot
+// prog_wrtv_rk.f90:1133-1554 on the final stage, for the fields with mono[q] != 0.
+// The RK-start copy of the field advanced by its long-step tendency (:1174-1181).
+template <class R, class RP>
+struct DynMono {
+  const R *q0[4], *st[4];
+  R *scpt[4], *s1[4], *s2[4];
+  int on[4];
+};
+
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_monot_pre(DMesh<R> m, DynMono<R, RP> mo, const RP *rho0, R dts) {
+  NOCONTRACT
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  if (k < 2 || k > m.mza) return;
+  const long ic = (long)c * m.ldk + l;
+  const R vti = m.volti[ic], r0 = (R)rho0[ic];
+  for (int q = 0; q < 4; ++q)
+    if (mo.on[q]) mo.scpt[q][ic] = mo.q0[q][ic] + dts * vti * mo.st[q][ic] / r0;
+}
+
+// scalar_vflux_monot / scalar_wflux_monot / scalar_monot_limiter (:1295-1550): the
+// in/out renormalisation scales of each column and level.  Face values come from the
+// upwind side's reconstruction (bounded fit, xx0_vu offsets at order 3), the vertical
+// ones from the unbounded constant-gradient quadratic; rho is the density before this
+// stage's column update.
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, RP> b, DynMono<R, RP> mo) {
+  NOCONTRACT
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  const int np = m.w_npoly[c];
+  const R eps0 = LimConst<R>::eps0, eps1 = LimConst<R>::eps1;
+  const R onep = R(1) + eps1;
+  const R dtr = (R)b.dt8, dtrp = onep * dtr;
+  R vmca_j[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const long ie = (long)m.w_face[c * NSLOT + j] * ldk + l;
+    vmca_j[j] = (j < np) ? ld(b.vmc, ie, act) * ld(m.arv, ie, act) : R(0);
+  }
+  const R wmcc = ld(b.wmc, ic, act), arw = ld(m.arw, ic, act);
+  const R wmca = inner ? wmcc * arw : R(0);
+  const R wmca_m1 = shup(wmca, 1);
+  const R vti = ld(m.volti, ic, act);
+  const R rhos = act ? (R)b.rho[ic] : R(1);
+  for (int q = 0; q < 4; ++q) {
+    if (!mo.on[q]) continue;
+    const int ord = q == 0 ? b.thil_order : b.wind_order;
+    const R *src = q == 0 ? b.thil_in : (q == 1 ? b.vxe : (q == 2 ? b.vye : b.vze));
+    const R qc = ld(src, ic, act);
+    const R sc = ld(mo.scpt[q], ic, act);
+    R gx = R(0), gy = R(0), gq[3] = {R(0), R(0), R(0)};
+    if (ord > 1) { gx = ld(b.g[2 * q], ic, act); gy = ld(b.g[2 * q + 1], ic, act); }
+    if (ord == 3)
+      for (int t = 0; t < 3; ++t) gq[t] = ld(b.gq[q][t], ic, act);
+    // vertical low-order and high-order values and fluxes (scalar_wflux_monot)
+    R sb, st;
+    lim_vrec(m, qc, k, mza, 1, sb, st);
+    const R sc_p1 = shdn(sc, 1), sc_m1 = shup(sc, 1), sb_p1 = shdn(sb, 1), qp1 = shdn(qc, 1);
+    R upwl = (wmcc < R(0)) ? sc_p1 : sc;
+    if (k == 1) upwl = qp1;
+    if (k == mza) upwl = qc;
+    const R hiw = inner ? ((wmcc < R(0)) ? sb_p1 : st) - upwl : R(0);
+    const R sfw = inner ? arw * wmcc * hiw : R(0);
+    const R sfw_m1 = shup(sfw, 1), upwl_m1 = shup(upwl, 1);
+    // upwind-biased extrema (vertical)
+    R smax = sc, smin = sc;
+    if (k >= 3 && wmca_m1 > R(0)) { smax = fmax(smax, sc_m1); smin = fmin(smin, sc_m1); }
+    if (inner && wmca < R(0)) { smax = fmax(smax, sc_p1); smin = fmin(smin, sc_p1); }
+    // horizontal faces
+    R fl = R(0), fin = R(0), fout = R(0);
+    for_slots(np, [&](int j) {
+      const int nbr = m.w_nbr[c * NSLOT + j];
+      const long in = (long)nbr * ldk + l;
+      const R d = m.w_dirv[c * NSLOT + j];
+      const R vm = vmca_j[j];
+      const bool iw1side = d < R(0);
+      const bool up_c = iw1side ? !(vm < R(0)) : (vm < R(0));
+      const R scn = ld(mo.scpt[q], in, act);
+      const R low = up_c ? sc : scn;
+      R high;
+      const long e = (long)c * NSLOT + j;
+      if (up_c) {
+        high = qc;
+        if (ord > 1) high = qc + m.w_dxs[e] * gx + m.w_dys[e] * gy;
+        if (ord == 3) high = high + m.w_os[1][0][e] * gq[0] + m.w_os[1][1][e] * gq[1] + m.w_os[1][2][e] * gq[2];
+      } else {
+        const R qn = ld(src, in, act);
+        high = qn;
+        if (ord > 1) high = qn + m.w_dxn[e] * ld(b.g[2 * q], in, act) + m.w_dyn[e] * ld(b.g[2 * q + 1], in, act);
+        if (ord == 3)
+          high = high + m.w_on[1][0][e] * ld(b.gq[q][0], in, act) + m.w_on[1][1][e] * ld(b.gq[q][1], in, act) +
+                 m.w_on[1][2][e] * ld(b.gq[q][2], in, act);
+      }
+      const R sfv = vm * (high - low);
+      fl = fl + d * vm * (low - sc);
+      fin = fin + fmax(d * sfv, R(0));
+      fout = fout + fmin(d * sfv, R(0));
+      if (d * vm > R(0)) { smax = fmax(smax, scn); smin = fmin(smin, scn); }
+    });
+    // vertical flux divergence and the scales (scalar_monot_limiter :1531-1548)
+    fl = fl + wmca_m1 * (upwl_m1 - sc) - wmca * (upwl - sc);
+    fin = fin + fmax(sfw_m1, R(0)) - fmin(sfw, R(0));
+    fout = fout + fmin(sfw_m1, R(0)) - fmax(sfw, R(0));
+    const R rlow = sc * rhos + dtr * vti * fl;
+    const R flux_in = fmax(dtrp * vti * fin, eps0);
+    const R flux_out = fmin(dtrp * vti * fout, -eps0);
+    const R epsr = eps1 * fabs(rlow);
+    const R sc_in = fmax(rhos * smax - (rlow + epsr) - eps0, R(0));
+    const R sc_out = fmin(rhos * smin - (rlow - epsr) + eps0, R(0));
+    if (body) {
+      mo.s1[q][ic] = fmin(R(1), sc_in / fmax(flux_in, R(0.01) * sc_in));
+      mo.s2[q][ic] = fmin(R(1), sc_out / fmin(flux_out, R(0.01) * sc_out));
+    }
+  }
 }
 
 // ---------------------------------------------------------------- prog_v_begs

@@ -25,36 +25,6 @@ struct TrLim {
   long vstride;
 };
 
-template <class R>
-struct LimConst {
-  static constexpr R eps0 = R(1.e-32), eps1 = R(3.e-07);
-};
-
-// grad_z_quadratic (grad_lib.f90:166-229) of the column held one level per lane:
-// monotone form (mode 1) unbounded (a_wu) with constant-gradient ends, PD form
-// (mode 2) bounded (a_w) with zero-gradient ends.
-template <class R>
-__device__ __forceinline__ void lim_vrec(const DMesh<R> &m, R qc, int k, int mza, int mode, R &sb, R &st) {
-  NOCONTRACT
-  const int l = k - 1;
-  const R qp1 = shdn(qc, 1);
-  const bool inner = k >= 2 && k <= mza - 1;
-  R ds = inner ? qp1 - qc : R(0);
-  const R ds_p1 = shdn(ds, 1), ds_m1 = shup(ds, 1);
-  if (mode == 1) {
-    if (k == 1) ds = ds_p1;
-    if (k == mza) ds = ds_m1;
-  }
-  const R dsm1 = shup(ds, 1);
-  if (mode == 1) {
-    sb = qc + ds * m.awu11[l] + dsm1 * m.awu21[l];
-    st = qc + ds * m.awu12[l] + dsm1 * m.awu22[l];
-  } else {
-    sb = qc + ds * m.aw11[l] + dsm1 * m.aw21[l];
-    st = qc + ds * m.aw12[l] + dsm1 * m.aw22[l];
-  }
-}
-
 // Column context shared by the two forward passes.
 template <class R>
 struct LimCol {

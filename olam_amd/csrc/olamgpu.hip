@@ -398,6 +398,9 @@ struct Impl : ImplBase {
   // prog_wrtv_rk scratch
   DevBuf rho0, rth0, wmc0, vmc0, thilt_s, vmxet_s, vmyet_s, vmzet_s, vmt_s, vmxet_rk, vmyet_rk, vmzet_rk, grads[8];
   DevBuf gradq[4][3];          // order 3: quadratic terms of thil, vxe, vye, vze
+  // fluxes_wrtv_monot (ithil_monot / iwind_monot): RK-start copies, scpt, scales
+  DevBuf mq0[4], mscpt[4], ms1[4], ms2[4];
+  int mono_on[4] = {0, 0, 0, 0};
   DevBuf div2d[2], vort2d[2], fdivh, fvorth;
   int div_ih = 0, vort_ih = 0;
   bool do_div = false, do_vort = false;
@@ -731,6 +734,11 @@ struct Impl : ImplBase {
     if (prm.thil_horiz_adv_order == 3 || prm.wind_horiz_adv_order == 3)
       for (auto &gv : gradq)
         for (auto &g : gv) { g.alloc(wsz * sizeof(R)); zero(g); }
+    mono_on[0] = prm.ithil_monot > 0;
+    mono_on[1] = mono_on[2] = mono_on[3] = prm.iwind_monot > 0;
+    for (int q = 0; q < 4; ++q)
+      if (mono_on[q])
+        for (DevBuf *b : {&mq0[q], &mscpt[q], &ms1[q], &ms2[q]}) { b->alloc(wsz * sizeof(R)); zero(*b); }
     ntr = prm.num_scalar;
     tstride = (long)wsz;
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
@@ -860,6 +868,12 @@ struct Impl : ImplBase {
     LAUNCH("k_dyn_setup_w", (k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
                        dp<R>(wmc), dp<R>(thilt), dp<R>(vmxet), dp<R>(vmyet), dp<R>(vmzet), dp<RP>(rth0),
                        dp<RP>(rho0), dp<R>(wmc0), dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s));
+    // RK-start copies of the monotone-flux fields (prog_wrtv_rk.f90:188-199)
+    {
+      DevBuf *src[4] = {&cur_thil_buf(), &vxe, &vye, &vze};
+      for (int q = 0; q < 4; ++q)
+        if (mono_on[q]) HIPCHK(hipMemcpyAsync(mq0[q].p, src[q]->p, wsz * sizeof(R), hipMemcpyDeviceToDevice, s));
+    }
     // short-step damping: divergence at W and vorticity at M, then their Laplacians
     if (do_div) {
       LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]));
@@ -911,6 +925,15 @@ struct Impl : ImplBase {
         LAUNCH("k_dyn_grad", (k_dyn_grad<R>), dim3(gw), dim3(256), dm, b.thil_in, b.vxe, b.vye, b.vze,
                            dp<R>(grads[0]), dp<R>(grads[1]), dp<R>(grads[2]), dp<R>(grads[3]), dp<R>(grads[4]),
                            dp<R>(grads[5]), dp<R>(grads[6]), dp<R>(grads[7]), dth, dwn);
+      // the final stage runs fluxes_wrtv_monot for the monotone fields (:370-399)
+      const bool last = istage == nrk;
+      int any_mono = 0, bmask = 0;
+      for (int q = 0; q < 4; ++q) {
+        b.mono[q] = last && mono_on[q];
+        any_mono |= b.mono[q];
+        bmask |= b.mono[q] << q;
+        b.scpt[q] = dp<R>(mscpt[q]); b.ms1[q] = dp<R>(ms1[q]); b.ms2[q] = dp<R>(ms2[q]);
+      }
       const int qth = prm.thil_horiz_adv_order == 3, qwn = prm.wind_horiz_adv_order == 3;
       if (qth || qwn) {
         DynGrad3<R> o;
@@ -919,9 +942,21 @@ struct Impl : ImplBase {
           for (int t = 0; t < 3; ++t) o.g[q][2 + t] = dp<R>(gradq[q][t]);
         }
         LAUNCH("k_dyn_grad3", (k_dyn_grad3<R>), dim3(gw), dim3(256), dm, b.thil_in, b.vxe, b.vye, b.vze, o, qth,
-               qwn);
+               qwn, bmask);
       }
-      LAUNCH("k_dyn_w", (k_dyn_w<R, RP>), dim3(gw), dim3(256), dm, b);
+      if (any_mono) {
+        DynMono<R, RP> mo;
+        const R *sts[4] = {dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s)};
+        for (int q = 0; q < 4; ++q) {
+          mo.q0[q] = dp<R>(mq0[q]); mo.st[q] = sts[q]; mo.scpt[q] = dp<R>(mscpt[q]);
+          mo.s1[q] = dp<R>(ms1[q]); mo.s2[q] = dp<R>(ms2[q]); mo.on[q] = b.mono[q];
+        }
+        LAUNCH("k_dyn_monot_pre", (k_dyn_monot_pre<R, RP>), dim3(gw), dim3(256), dm, mo, dp<RP>(rho0), (R)b.dt8);
+        LAUNCH("k_dyn_monot_scales", (k_dyn_monot_scales<R, RP>), dim3(gw), dim3(256), dm, b, mo);
+        LAUNCH("k_dyn_w", (k_dyn_w<R, RP, true>), dim3(gw), dim3(256), dm, b);
+      } else {
+        LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false>), dim3(gw), dim3(256), dm, b);
+      }
       cur_thil = 1 - cur_thil;
       LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
       LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);

@@ -7,6 +7,7 @@
 !   <dir>/params.txt  one line: precision num_scalar nrk_wrtv nrk_scal thil_order
 !                     wind_order scal_order iscal_monot divh_level vort_level
 !                     divh_fact vort_fact dtsm dtlm nacoust adjoint_tape fcoriol nsteps
+!                     ithil_monot iwind_monot
 !   <dir>/fields.txt  one line per array: group name bytes, group = mesh | state | scalar
 !                     (mesh arrays also carry mza mwa mva mma in params.txt's mesh line)
 !   <dir>/<name>.bin  raw bytes of each array in Fortran layout
@@ -43,7 +44,8 @@ program olamgpu_fdriver
   read(u, *) prm%precision, prm%num_scalar, prm%nrk_wrtv, prm%nrk_scal, &
        prm%thil_horiz_adv_order, prm%wind_horiz_adv_order, prm%scal_horiz_adv_order, &
        prm%iscal_monot, prm%divh_damp_level, prm%vort_damp_level, prm%divh_damp_fact, &
-       prm%vort_damp_fact, prm%dtsm, prm%dtlm, prm%nacoust, prm%adjoint_tape, prm%fcoriol, nsteps
+       prm%vort_damp_fact, prm%dtsm, prm%dtlm, prm%nacoust, prm%adjoint_tape, prm%fcoriol, nsteps, &
+       prm%ithil_monot, prm%iwind_monot
   read(u, *) sizes
   close(u)
   mesh%mza = sizes(1); mesh%mwa = sizes(2); mesh%mva = sizes(3); mesh%mma = sizes(4)

@@ -59,6 +59,7 @@ module olamgpu_iface
      integer(c_int32_t) :: nacoust = 1
      integer(c_int32_t) :: adjoint_tape = 0
      real(c_double)     :: fcoriol = 0.0d0
+     integer(c_int32_t) :: ithil_monot = 0, iwind_monot = 0
   end type olamgpu_params_t
 
   type, bind(C), public :: olamgpu_state_t

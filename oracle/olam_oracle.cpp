@@ -99,6 +99,7 @@ struct Base {
   virtual void set_frozen(int) = 0;
   virtual int tape_scales(int, int, double *) = 0;
   virtual int recon(const float *, int, int, int, double *, double *) = 0;
+  virtual double rhothil_sum() = 0;
   std::string err;
 };
 
@@ -215,6 +216,13 @@ struct Model : Base {
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
   }
   void set_frozen(int v) override { frozen_lim = v != 0; }
+  // test hook: sum of rho * thil * volt over the prognostic cells at full precision
+  double rhothil_sum() override {
+    double t = 0;
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int k = 2; k <= mza; ++k) t += (double)rho(k, iw) * (double)thil(k, iw) * volt(k, iw);
+    return t;
+  }
   // test hook: reconstruction coefficients of a field (5, mwa, mza) and its face values
   // on both sides of every face (2, mva, mza), for one order and coefficient/offset set
   int recon(const float *field, int iorder, int bounded_coef, int bounded_off, double *coefs,
@@ -469,6 +477,119 @@ struct Model : Base {
     }
   }
 
+  // fluxes_wrtv_monot, prog_wrtv_rk.f90:1133-1554 (iorderv = 3, centered_monot =
+  // .false., :26-27): low-order upwind face values from the RK-start field advanced by
+  // its long-step tendency (scpt, :1174-1181), high-order corrections from the stage
+  // field (bounded horizontal fit with the unbounded face offsets xx0_vu, :1186-1196 and
+  // :1341-1354; unbounded constant-gradient vertical quadratic, :1418-1420), the
+  // in/out renormalisation scales of scalar_monot_limiter (:1452-1550) and the
+  // renormalised face values (:1235-1266).  vmca is the stage's vmc * arv, rho the
+  // density before prog_wrt_begs of this stage.
+  void fluxes_wrtv_monot(const F2<R> &s, const F2<R> &s0, const F2<R> &st, F2<R> &upw, F2<R> &upv, R dtr,
+                         int iorderh, F2<R> *g) {
+    const R eps0 = R(1.e-32), eps1 = R(3.e-07), onep = R(1) + eps1;
+    F2<R> &scpt = m_scpt, &hiv = m_hiv, &sfv = m_sfv, &hiw = m_hiw, &sfw = m_sfw, &s1 = m_s1, &s2 = m_s2;
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int k = 2; k <= mza; ++k) scpt(k, iw) = s0(k, iw) + dtr * volti(k, iw) * st(k, iw) / (R)rho0(k, iw);
+    recon_coefs(s, g, iorderh, true);
+    // scalar_wflux_monot (:1380-1445)
+#pragma omp parallel
+    {
+      R sb[130], stp[130];
+#pragma omp for
+      for (int iw = 2; iw <= mwa; ++iw) {
+        const int kb = 2;
+        upw(kb - 1, iw) = s(kb, iw);
+        upw(mza, iw) = s(mza, iw);
+        for (int k = kb; k <= mza - 1; ++k) {
+          upw(k, iw) = scpt(k, iw);
+          if (wmc(k, iw) < R(0)) upw(k, iw) = scpt(k + 1, iw);
+        }
+        grad_z_q(&s(1, iw) - 1, sb, stp, 1, 1, false);
+        sfw(kb - 1, iw) = 0;
+        sfw(mza, iw) = 0;
+        for (int k = kb; k <= mza - 1; ++k) {
+          R h = stp[k];
+          if (wmc(k, iw) < R(0)) h = sb[k + 1];
+          hiw(k, iw) = h - upw(k, iw);
+          sfw(k, iw) = arw(k, iw) * wmc(k, iw) * hiw(k, iw);
+        }
+      }
+    }
+    // scalar_vflux_monot (:1295-1377)
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      const int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        upv(k, iv) = scpt(k, iw1);
+        if (vmca(k, iv) < R(0)) upv(k, iv) = scpt(k, iw2);
+        const R h1 = face_value(s, g, iorderh, 1, iv, k, false);
+        const R h2 = face_value(s, g, iorderh, 2, iv, k, false);
+        R h = h1;
+        if (vmca(k, iv) < R(0)) h = h2;
+        hiv(k, iv) = h - upv(k, iv);
+        sfv(k, iv) = vmca(k, iv) * hiv(k, iv);
+      }
+    }
+    // scalar_monot_limiter (:1450-1550)
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      const int kb = 2;
+      const R dtrp = onep * dtr;
+      R wmca[130], smax[130], smin[130], rhos[130], fl[130], fin[130], fout[130];
+      wmca[kb - 1] = 0;
+      for (int k = kb; k <= mza - 1; ++k) wmca[k] = wmc(k, iw) * arw(k, iw);
+      wmca[mza] = 0;
+      for (int k = kb; k <= mza; ++k) { smax[k] = scpt(k, iw); smin[k] = scpt(k, iw); rhos[k] = (R)rho(k, iw); }
+      for (int k = kb + 1; k <= mza; ++k)
+        if (wmca[k - 1] > R(0)) { smax[k] = std::max(smax[k], scpt(k - 1, iw)); smin[k] = std::min(smin[k], scpt(k - 1, iw)); }
+      for (int k = kb; k <= mza - 1; ++k)
+        if (wmca[k] < R(0)) { smax[k] = std::max(smax[k], scpt(k + 1, iw)); smin[k] = std::min(smin[k], scpt(k + 1, iw)); }
+      for (int k = 1; k <= mza; ++k) { fl[k] = 0; fin[k] = 0; fout[k] = 0; }
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        const int iv = w_iv(jv, iw), iwn = w_iw(jv, iw);
+        const R d = dirv(jv, iw);
+        for (int k = 2; k <= mza; ++k) {
+          fl[k] = fl[k] + d * vmca(k, iv) * (upv(k, iv) - scpt(k, iw));
+          fin[k] = fin[k] + std::max(d * sfv(k, iv), R(0));
+          fout[k] = fout[k] + std::min(d * sfv(k, iv), R(0));
+          if (d * vmca(k, iv) > R(0)) { smax[k] = std::max(smax[k], scpt(k, iwn)); smin[k] = std::min(smin[k], scpt(k, iwn)); }
+        }
+      }
+      for (int k = kb; k <= mza; ++k) {
+        fl[k] = fl[k] + wmca[k - 1] * (upw(k - 1, iw) - scpt(k, iw)) - wmca[k] * (upw(k, iw) - scpt(k, iw));
+        fin[k] = fin[k] + std::max(sfw(k - 1, iw), R(0)) - std::min(sfw(k, iw), R(0));
+        fout[k] = fout[k] + std::min(sfw(k - 1, iw), R(0)) - std::max(sfw(k, iw), R(0));
+        const R rlow = scpt(k, iw) * rhos[k] + dtr * volti(k, iw) * fl[k];
+        const R flux_in = std::max(dtrp * volti(k, iw) * fin[k], eps0);
+        const R flux_out = std::min(dtrp * volti(k, iw) * fout[k], -eps0);
+        const R epsr = eps1 * std::fabs(rlow);
+        const R sc_in = std::max(rhos[k] * smax[k] - (rlow + epsr) - eps0, R(0));
+        const R sc_out = std::min(rhos[k] * smin[k] - (rlow - epsr) + eps0, R(0));
+        s1(k, iw) = std::min(R(1), sc_in / std::max(flux_in, R(0.01) * sc_in));
+        s2(k, iw) = std::min(R(1), sc_out / std::min(flux_out, R(0.01) * sc_out));
+      }
+    }
+    // renormalised vertical (:1235-1246) and horizontal (:1251-1266) face values
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int k = 2; k <= mza - 1; ++k) {
+        R sc = std::min(s1(k + 1, iw), s2(k, iw));
+        if (sfw(k, iw) < R(0)) sc = std::min(s1(k, iw), s2(k + 1, iw));
+        upw(k, iw) = upw(k, iw) + hiw(k, iw) * sc;
+      }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      const int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        R sc = std::min(s1(k, iw2), s2(k, iw1));
+        if (sfv(k, iv) < R(0)) sc = std::min(s1(k, iw1), s2(k, iw2));
+        upv(k, iv) = upv(k, iv) + hiv(k, iv) * sc;
+      }
+    }
+  }
+
   // ---------------------------------------------------------------- damping
   // divh_damp, vort_div_damp.f90:1-230 (dorayfdiv = .false.)
   void divh_damp(F2<R> &vmtx, double dtm) {
@@ -580,6 +701,9 @@ struct Model : Base {
   F2<R> vmxet_rk, vmyet_rk, vmzet_rk, thilt_rk, wmc0, vmc0, thilt_s, vmxet_s, vmyet_s, vmzet_s, vmt_s, vmca;
   F2<RP> rho0, rth0;
   F2<R> gs[5];
+  // RK start values of the monotone-flux fields (prog_wrtv_rk.f90:117-120,188-199)
+  F2<R> thl0, vxe0, vye0, vze0;
+  F2<R> m_scpt, m_hiv, m_sfv, m_hiw, m_sfw, m_s1, m_s2;
 
   void alloc_wrtv_scratch() {
     if (thil_upv.n) return;
@@ -588,6 +712,10 @@ struct Model : Base {
                     &wmc0, &thilt_s, &vmxet_s, &vmyet_s, &vmzet_s, &gs[0], &gs[1], &gs[2], &gs[3], &gs[4]})
       a->alloc(mza, mwa);
     rho0.alloc(mza, mwa); rth0.alloc(mza, mwa);
+    if (prm.ithil_monot > 0 || prm.iwind_monot > 0) {
+      for (auto *a : {&thl0, &vxe0, &vye0, &vze0, &m_scpt, &m_hiw, &m_sfw, &m_s1, &m_s2}) a->alloc(mza, mwa);
+      m_hiv.alloc(mza, mva); m_sfv.alloc(mza, mva);
+    }
   }
 
   void prog_wrtv_rk() override {
@@ -607,6 +735,10 @@ struct Model : Base {
         vmyet_s(k, iw) = vmyet(k, iw) * v4;
         vmzet_s(k, iw) = vmzet(k, iw) * v4;
       }
+      if (prm.ithil_monot > 0)
+        for (int k = 2; k <= mza; ++k) thl0(k, iw) = thil(k, iw);
+      if (prm.iwind_monot > 0)
+        for (int k = 2; k <= mza; ++k) { vxe0(k, iw) = vxe(k, iw); vye0(k, iw) = vye(k, iw); vze0(k, iw) = vze(k, iw); }
     }
 #pragma omp parallel for
     for (int iv = 2; iv <= mva; ++iv)
@@ -632,10 +764,20 @@ struct Model : Base {
         if (istage == prm.nrk_wrtv)
           for (int k = 2; k <= mza; ++k) vmasc(k, iv) = vmasc(k, iv) + vmca(k, iv);
       }
-      fluxes_wrtv(thil, thil_upw, thil_upv, prm.thil_horiz_adv_order, gs);
-      fluxes_wrtv(vxe, vxe_upw, vxe_upv, prm.wind_horiz_adv_order, gs);
-      fluxes_wrtv(vye, vye_upw, vye_upv, prm.wind_horiz_adv_order, gs);
-      fluxes_wrtv(vze, vze_upw, vze_upv, prm.wind_horiz_adv_order, gs);
+      const bool last = istage == prm.nrk_wrtv;
+      if (last && prm.ithil_monot > 0)   // :370-378
+        fluxes_wrtv_monot(thil, thl0, thilt_s, thil_upw, thil_upv, dts, prm.thil_horiz_adv_order, gs);
+      else
+        fluxes_wrtv(thil, thil_upw, thil_upv, prm.thil_horiz_adv_order, gs);
+      if (last && prm.iwind_monot > 0) {  // :380-399
+        fluxes_wrtv_monot(vxe, vxe0, vmxet_s, vxe_upw, vxe_upv, dts, prm.wind_horiz_adv_order, gs);
+        fluxes_wrtv_monot(vye, vye0, vmyet_s, vye_upw, vye_upv, dts, prm.wind_horiz_adv_order, gs);
+        fluxes_wrtv_monot(vze, vze0, vmzet_s, vze_upw, vze_upv, dts, prm.wind_horiz_adv_order, gs);
+      } else {
+        fluxes_wrtv(vxe, vxe_upw, vxe_upv, prm.wind_horiz_adv_order, gs);
+        fluxes_wrtv(vye, vye_upw, vye_upv, prm.wind_horiz_adv_order, gs);
+        fluxes_wrtv(vze, vze_upw, vze_upv, prm.wind_horiz_adv_order, gs);
+      }
 #pragma omp parallel for
       for (int iw = 2; iw <= mwa; ++iw) prog_wrt_begs(iw, istage, dt8);
 #pragma omp parallel for
@@ -1521,6 +1663,7 @@ int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(o
 int oracle_set_scalar_pdef(void *h, int32_t n, int32_t pdef) { return ((oracle_ctx *)h)->m->set_pdef(n, pdef); }
 int oracle_set_frozen_limiter(void *h, int32_t on) { ((oracle_ctx *)h)->m->set_frozen(on); return 0; }
 int oracle_tape_scales(void *h, int32_t n, int32_t which, double *out) { return ((oracle_ctx *)h)->m->tape_scales(n, which, out); }
+int oracle_rhothil_sum(void *h, double *out) { *out = ((oracle_ctx *)h)->m->rhothil_sum(); return 0; }
 int oracle_recon(void *h, const float *field, int32_t order, int32_t bounded_coef, int32_t bounded_off, double *coefs,
                  double *faces) {
   return ((oracle_ctx *)h)->m->recon(field, order, bounded_coef, bounded_off, coefs, faces);

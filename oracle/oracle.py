@@ -53,6 +53,7 @@ def lib():
         L.oracle_set_frozen_limiter.argtypes = [vp, C.c_int32]
         L.oracle_tape_scales.argtypes = [vp, C.c_int32, C.c_int32, vp]
         L.oracle_recon.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
+        L.oracle_rhothil_sum.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -157,6 +158,12 @@ class Oracle:
         if rc != 0:
             raise ValueError(f"oracle_recon failed ({rc})")
         return coefs, faces
+
+    def rhothil_sum(self):
+        """Test hook: sum of rho * thil * volt over prognostic cells, full precision."""
+        out = C.c_double()
+        lib().oracle_rhothil_sum(self.h, C.byref(out))
+        return out.value
 
     def set_frozen_limiter(self, on):
         """Test hook: later forward steps apply the limiter scales of the last step."""

@@ -134,7 +134,8 @@ def write_fortran_case(path, grid, params, state, scalars, nsteps):
             p.precision, p.num_scalar, p.nrk_wrtv, p.nrk_scal, p.thil_horiz_adv_order,
             p.wind_horiz_adv_order, p.scal_horiz_adv_order, p.iscal_monot, p.divh_damp_level,
             p.vort_damp_level, repr(float(p.divh_damp_fact)), repr(float(p.vort_damp_fact)),
-            repr(p.dtsm), repr(p.dtlm), p.nacoust, p.adjoint_tape, repr(p.fcoriol), nsteps)) + "\n")
+            repr(p.dtsm), repr(p.dtlm), p.nacoust, p.adjoint_tape, repr(p.fcoriol), nsteps,
+            p.ithil_monot, p.iwind_monot)) + "\n")
         f.write(f"{grid.mza} {grid.mwa} {grid.mva} {grid.mma}\n")
     lines = []
 
