@@ -109,8 +109,25 @@ __device__ __forceinline__ double pcr_solve(double a, double b, double c, double
   return d / b;
 }
 
+// Column loads.  ld predicates the load per lane (lanes outside the column's levels
+// read nothing); its result may stay in flight past later loads, which the pipelined
+// tracer kernels rely on.  ldv loads for every lane and selects the default after the
+// load; it is safe because the column stride is at least one wave, so every lane
+// addresses an element of its own column.
 template <class T>
 __device__ __forceinline__ T ld(const T *p, long i, bool ok) { return ok ? p[i] : T(0); }
+// Streaming accesses: data a wave reads or writes once and no other wave gathers
+// (own-column inputs, outputs).  The non-temporal hint keeps them from displacing the
+// gathered neighbour columns in the caches.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T *p, long i, bool ok) { return ok ? __builtin_nontemporal_load(p + i) : T(0); }
+template <class T>
+__device__ __forceinline__ void st_nt(T *p, long i, T v) { __builtin_nontemporal_store(v, p + i); }
+template <class T>
+__device__ __forceinline__ T ldv(const T *p, long i, bool ok, T dflt) {
+  const T v = p[i];
+  return ok ? v : dflt;
+}
 
 // Neighbour slots: slots 0..5 are always processed, so every gather of a column is
 // issued without a branch in between (pentagons pad slot 5 with the column itself and

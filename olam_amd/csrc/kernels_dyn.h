@@ -308,20 +308,24 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R qc[4] = {thc, vxc, vyc, vzc};
   // horizontal advective fluxes (:632-661)
   RP hflx = RP(0);
-  R tt = ld(b.thilt_s, ic, act), tx = ld(b.vmxet_s, ic, act), ty = ld(b.vmyet_s, ic, act),
-    tz = ld(b.vmzet_s, ic, act);
+  R tt = ld_nt(b.thilt_s, ic, act), tx = ld_nt(b.vmxet_s, ic, act), ty = ld_nt(b.vmyet_s, ic, act),
+    tz = ld_nt(b.vmzet_s, ic, act);
   // face mass fluxes first: a neighbour's values and gradients are gathered only if some
   // level takes that face's value from the neighbour (upwind side)
-  R vmca_j[7];
+  R vmca_j[7], vmc_j[7], arv_j[7];
   bool sel_j[7];
   unsigned need = 0;
 #pragma unroll
-  for (int j = 0; j < 7; ++j) {
+  for (int j = 0; j < 7; ++j) {       // all face loads first, then the ballots
     const long ie = (long)m.w_face[c * NSLOT + j] * ldk + l;
+    vmc_j[j] = (j < 6 || np > 6) ? ldv(b.vmc, ie, act, R(0)) : R(0);
+    arv_j[j] = (j < 6 || np > 6) ? ldv(m.arv, ie, act, R(0)) : R(0);
+  }
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
     const R dv = m.w_dirv[c * NSLOT + j];
-    const R vmcv = (j < 6 || np > 6) ? ld(b.vmc, ie, act) : R(0);
-    vmca_j[j] = (j < np) ? vmcv * ld(m.arv, ie, act) : R(0);
-    sel_j[j] = (dv < R(0)) ? (vmcv < R(0)) : !(vmcv < R(0));
+    vmca_j[j] = (j < np) ? vmc_j[j] * arv_j[j] : R(0);
+    sel_j[j] = (dv < R(0)) ? (vmc_j[j] < R(0)) : !(vmc_j[j] < R(0));
     need |= (__ballot(sel_j[j] && act) != 0 ? 1u : 0u) << j;
   }
   for_slots(np, [&](int j) {
@@ -404,7 +408,7 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R tupw = upws[0], xupw = upws[1], yupw = upws[2], zupw = upws[3];
   // explicit density and theta tendencies (:687-723)
   const R vti = ld(m.volti, ic, act);
-  const R alp = ld(b.alpha_press, ic, act);
+  const R alp = ld_nt(b.alpha_press, ic, act);
   const RP rhothil = rho_c * (RP)thc;
   const RP b10 = (RP)dt8 * (RP)vti;
   const RP b5 = body ? (RP)(alp * pow_r((R)rhothil, rocv)) : RP(0);
@@ -413,17 +417,17 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   RP delex_rt = b10 * (RP)tt;
   if (b.istage > 1) {
     delex_rho = delex_rho + ld(b.rho0, ic, act) - rho_c;
-    delex_rt = delex_rt + ld(b.rth0, ic, act) - rhothil;
+    delex_rt = delex_rt + ld_nt(b.rth0, ic, act) - rhothil;
   }
   const RP press_ex = b5 * (rhothil + pc2 * delex_rt);
   const RP rho_ex = rho_c + fr * delex_rho;
   // explicit W momentum (:727-745), W levels ka..mza-1
   const RP pex_p1 = shdn(press_ex, 1), rex_p1 = shdn(rho_ex, 1);
   const R tx_p1 = shdn(tx, 1), ty_p1 = shdn(ty, 1), tz_p1 = shdn(tz, 1);
-  const R pwf = ld(b.pwfac, ic, act), vwi = ld(m.volwi, ic, act);
+  const R pwf = ld_nt(b.pwfac, ic, act), vwi = ld(m.volwi, ic, act);
   R delex_wm = R(0);
   if (inner)
-    delex_wm = ld(b.wmc0, ic, act) +
+    delex_wm = ld_nt(b.wmc0, ic, act) +
                dts * ((R)(press_ex - pex_p1) * pwf -
                       (R)((RP)m.gdz_wgtm8[l] * rho_ex + (RP)m.gdz_wgtp8[l] * rex_p1) +
                       (m.wnx[c] * (tx + tx_p1) + m.wny[c] * (ty + ty_p1) + m.wnz[c] * (tz + tz_p1)) * vwi);
@@ -466,25 +470,25 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   if (body) {
     rho_new = ld(b.rho0, ic, act) + (RP)dt8 * (RP)vti * (hflx + (RP)wma_m1 - (RP)wma);
     delex_rt = delex_rt + b10 * (vft_m1 - vflux_thil);
-    b.press[ic] = b5 * (rhothil + pc3 * delex_rt);
+    st_nt(b.press, ic, b5 * (rhothil + pc3 * delex_rt));
     thil_new = (R)((rhothil + delex_rt) / rho_new);
     b.rho[ic] = rho_new;
-    b.vmxet_rk[ic] = tx + vfx_m1 - vfx;
-    b.vmyet_rk[ic] = ty + vfy_m1 - vfy;
-    b.vmzet_rk[ic] = tz + vfz_m1 - vfz;
+    st_nt(b.vmxet_rk, ic, tx + vfx_m1 - vfx);
+    st_nt(b.vmyet_rk, ic, ty + vfy_m1 - vfy);
+    st_nt(b.vmzet_rk, ic, tz + vfz_m1 - vfz);
   }
   // W-level updates (:869-882)
   const RP rho_new_p1 = shdn(rho_new, 1);
   R wc_new = wcc;
   if (inner) {
-    b.wmc[ic] = wmff;
+    st_nt(b.wmc, ic, wmff);
     wc_new = wmff / (R)((RP)m.zwgt_bot8[l] * rho_new + (RP)m.zwgt_top8[l] * rho_new_p1);
   }
   const R wc_new_p1 = shdn(wc_new, 1);
   const R thil_new_p1 = shdn(thil_new, 1);
   if (k == 1) { wc_new = wc_new_p1; thil_new = thil_new_p1; }
-  if (act && k <= mza - 1) b.wc[ic] = wc_new;
-  if (act) b.thil_out[ic] = thil_new;
+  if (act && k <= mza - 1) st_nt(b.wc, ic, wc_new);
+  if (act) st_nt(b.thil_out, ic, thil_new);
 }
 
 // ---------------------------------------------------------------- fluxes_wrtv_monot

@@ -53,7 +53,7 @@ __device__ __forceinline__ void lim_col(const DMesh<R> &m, const TrStage<R, RP> 
   L.fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
   L.fexpl = R(1) - L.fimpl;
   L.vt = ld(m.volti, ic, act);
-  L.rs = act ? (R)a.rho[ic] : R(1);
+  L.rs = (R)ldv(a.rho, ic, act, RP(1));
   if (!body) L.rs = R(1);
 }
 
@@ -214,8 +214,8 @@ __global__ __launch_bounds__(256) void k_tr_lim_update(DMesh<R> m, TrStage<R, RP
     if (a.order > 1) { gxc = ld(a.gx + off, ic, act); gyc = ld(a.gy + off, ic, act); }
     if (a.order == 3)
       for (int t = 0; t < 3; ++t) gqc[t] = ld(a.gq[t] + off, ic, act);
-    const R s1c = body ? T.s1[off + ic] : R(1);
-    const R s2c = (body && mode == 1) ? T.s2[off + ic] : R(1);
+    const R s1c = ldv(T.s1, off + ic, body, R(1));
+    const R s2c = (mode == 1) ? ldv(T.s2, off + ic, body, R(1)) : R(1);
     // horizontal: renormalised face values (:704-716, :1183-1193)
     R fd = R(0);
     for_slots(L.np, [&](int j) {
@@ -223,8 +223,8 @@ __global__ __launch_bounds__(256) void k_tr_lim_update(DMesh<R> m, TrStage<R, RP
       lim_face(m, a, L, j, off, l, act, mode, qc, q0c, gxc, gyc, gqc, low, hiv, sf, q0n);
       const long in = (long)L.nb[j] * ldk + l;
       const bool iw1side = L.dv[j] < R(0);
-      const R s1n = body ? T.s1[off + in] : R(1);
-      const R s2n = (body && mode == 1) ? T.s2[off + in] : R(1);
+      const R s1n = ldv(T.s1, off + in, body, R(1));
+      const R s2n = (mode == 1) ? ldv(T.s2, off + in, body, R(1)) : R(1);
       const R s1_1 = iw1side ? s1c : s1n, s1_2 = iw1side ? s1n : s1c;
       const R s2_1 = iw1side ? s2c : s2n, s2_2 = iw1side ? s2n : s2c;
       R scale;
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
   const R ff = a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R fexp = (ff > R(0.001)) ? R(1) - ff : R(1);
   const R vt = ld(m.volti, ic, act);
-  const R rf = body ? (R)a.rho[ic] : R(1);
+  const R rf = (R)ldv(a.rho, ic, body, RP(1));
   const bool neg = wm < R(0);
   for (int n = ti.n0; n < ti.n1; ++n) {
     const int mode = T.tmode[n];
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_tr_lim_pre(DMesh<R> m, R *q0, const R *
   const int l = lane_id(), k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const long ic = (long)c * m.ldk + l;
-  const R ro = body ? rho_old[ic] : R(1);
+  const R ro = ldv(rho_old, ic, body, R(1));
   for (int n = 0; n < ntr; ++n) {
     if (!pre[n]) continue;
     const long off = n * tstride;

@@ -56,8 +56,8 @@ __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict
     });
     if (!body) { ax = R(0); ay = R(0); }
     if (act) {
-      gx[n * tstride + c * ldk + l] = ax;
-      gy[n * tstride + c * ldk + l] = ay;
+      st_nt(gx, n * tstride + c * ldk + l, ax);
+      st_nt(gy, n * tstride + c * ldk + l, ay);
     }
   }
 }
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
   const R wm_m1 = shup(wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
-  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R rinv = R(1) / (body ? rf : R(1));
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
   // a neighbour is read only if some level takes that face's value from it
@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
     const long off = n * a.tstride;
     const R *qi = a.q_in + off;
     L.qc = ld(qi, ic, act);
-    L.q0 = ld(a.q0 + off, ic, act);
-    L.st = a.sct ? ld(a.sct + off, ic, act) : R(0);
+    L.q0 = ld_nt(a.q0 + off, ic, act);
+    L.st = a.sct ? ld_nt(a.sct + off, ic, act) : R(0);
     L.gxc = R(0); L.gyc = R(0);
     if (a.order > 1) { L.gxc = ld(a.gx + off, ic, act); L.gyc = ld(a.gy + off, ic, act); }
     if constexpr (Q3) {
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
     R qn_new = q0 + a.dtr * (L.st + vt * F) * rinv;
     const R lev1 = shdn(qn_new, 1);             // level 1 copies level 2
     if (k == 1) qn_new = lev1;
-    if (act) a.q_out[off + ic] = qn_new;
+    if (act) st_nt(a.q_out, off + ic, qn_new);
   }
 }
 
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(512) void k_tr_patch(DMesh<R> m, TrStage<R, RP> a, 
       wmr[t] = ld(a.wmasc, ic, act);
       fexr[t] = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
       vtr[t] = ld(m.volti, ic, act);
-      const R rf = act ? (R)a.rho[ic] : R(1);
+      const R rf = (R)ldv(a.rho, ic, act, RP(1));
       rinvr[t] = R(1) / (body ? rf : R(1));
     }
   }
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256) void k_tr_implicit(DMesh<R> m, TrStage<R, RP> 
   const R wm = ld(a.wmasc, ic, act);
   const R wm_m1 = shup(wm, 1);
   const R vt = ld(m.volti, ic, act);
-  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R ro = ld(a.rho_old, ic, act);
   const R rs = stage_rho(ro, rf, a.istage, a.nrk);
   const R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP
   const long ldk = m.ldk, ic = c * ldk + l;
   const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
   const R vt = ld(m.volti, ic, act);
-  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R ro = ld(a.rho_old, ic, act);
   const R rs = stage_rho(ro, rf, a.istage, a.nrk);
   const R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
@@ -669,7 +669,7 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
   const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
-  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R rinv = R(1) / (body ? rf : R(1));
   const R cc = csum + fexpl * (wm_m1 - wm);
   const R a11 = m.aw11[l], a12 = m.aw12[l];
@@ -827,18 +827,22 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
     vmj[j] = vm * dv * dv;
     iw1[j] = dv < R(0);
     up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));
-    // a neighbour is read only if this column is upwind of that face at some level
-    need |= (__ballot(up[j] && act) != 0 ? 1u : 0u) << j;
-    const bool nact = act && ((need >> j) & 1u);
+  }
+  // a neighbour is read only if this column is upwind of that face at some level
+#pragma unroll
+  for (int j = 0; j < 7; ++j) need |= (__ballot(up[j] && act) != 0 ? 1u : 0u) << j;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const bool take = (need >> j) & 1u;      // wave-uniform
     const long in = (long)nb[j] * ldk + l;
-    vtn[j] = ld(m.volti, in, nact);
-    const R rfn = nact ? (R)a.rho[in] : R(1);
+    vtn[j] = take ? ld(m.volti, in, act) : R(0);
+    const R rfn = take ? (R)ldv(a.rho, in, act, RP(1)) : R(1);
     rinvn[j] = R(1) / (body ? rfn : R(1));
   }
   const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
-  const R rf = act ? (R)a.rho[ic] : R(1);
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R rinv = R(1) / (body ? rf : R(1));
   const R cc = csum + fexpl * (wm_m1 - wm);
   const R a11 = m.aw11[l], a12 = m.aw12[l];
@@ -857,7 +861,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
     L.mu = ld(a.mu + off, ic, act);
-    L.l0 = ld(a.lam0 + off, ic, act);
+    L.l0 = ld_nt(a.lam0 + off, ic, act);
 #pragma unroll
     for (int j = 0; j < 7; ++j)
       L.mun[j] = ((j < 6 || np > 6) && ((need >> j) & 1u)) ? ld(a.mu + off, (long)nb[j] * ldk + l, act) : R(0);
@@ -908,9 +912,9 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
     });
     if (!body) { P = R(0); Gx = R(0); Gy = R(0); }
     if (act) {
-      a.lam0[off + ic] = l0;
-      a.lpart[off + ic] = lqv + P;
-      if (a.order > 1) { a.lgx[off + ic] = Gx; a.lgy[off + ic] = Gy; }
+      st_nt(a.lam0, off + ic, l0);
+      st_nt(a.lpart, off + ic, lqv + P);
+      if (a.order > 1) { st_nt(a.lgx, off + ic, Gx); st_nt(a.lgy, off + ic, Gy); }
       if constexpr (Q3) {
 #pragma unroll
         for (int t = 0; t < 3; ++t) a.lgq[t][off + ic] = body ? Gq[t] : R(0);
@@ -942,8 +946,8 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
   struct Ld { R lp, l0, gxs, gys, gxn[7], gyn[7]; };
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
-    L.lp = ld(a.lpart + off, ic, act);
-    L.l0 = a.last ? ld(a.lam0 + off, ic, act) : R(0);
+    L.lp = ld_nt(a.lpart + off, ic, act);
+    L.l0 = a.last ? ld_nt(a.lam0 + off, ic, act) : R(0);
     L.gxs = R(0); L.gys = R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j) { L.gxn[j] = R(0); L.gyn[j] = R(0); }
@@ -974,7 +978,7 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     }
     if (!body) lq = a.limited(n) ? lev1 : R(0);
     if (a.last) lq = lq + L.l0;                        // stage 1 read q = scp0
-    if (act) a.lq[off + ic] = lq;
+    if (act) st_nt(a.lq, off + ic, lq);
   }
 }
 
@@ -1094,7 +1098,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
       if (k == 1) mu = R(0);
       if (k == 2) mu = mu + mu1;
       const R vt = ld(m.volti, ix, act);
-      const R rf = act ? (R)a.rho[ix] : R(1);
+      const R rf = (R)ldv(a.rho, ix, act, RP(1));
       const R rinv = R(1) / (body ? rf : R(1));
       snu[i * WAVE + l] = body ? mu * a.dtr * vt * rinv : R(0);
     }

@@ -160,7 +160,9 @@ struct Topo {
   std::vector<int> w_npoly, w_nbr, w_face, w_back, v_iw, v_im, m_iv, m_im;
 };
 
-static int ld_k(int mza) { return ((mza + 15) / 16) * 16; }
+// Column stride: one wave.  Lane l of a column's wave always addresses element l of
+// that column, which the column loads rely on (see ld in common.h).
+static int ld_k(int mza) { return mza <= WAVE ? WAVE : ((mza + 15) / 16) * 16; }
 
 // Fortran accessors on the input descriptor (1-based, dummy index 1)
 struct In {
