@@ -52,14 +52,15 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, 
   const int l = lane_id(), k = l + 1;
   if (k < 2 || k > m.mza) return;
   const long ic = (long)c * m.ldk + l;
-  const R v4 = (R)m.volt[ic];
-  rth0[ic] = (RP)thil[ic] * rho[ic];
-  rho0[ic] = rho[ic];
-  wmc0[ic] = wmc[ic];
-  thilt_s[ic] = thilt[ic] * v4;
-  vmxet_s[ic] = vmxet[ic] * v4;
-  vmyet_s[ic] = vmyet[ic] * v4;
-  vmzet_s[ic] = vmzet[ic] * v4;
+  const R v4 = (R)__builtin_nontemporal_load(m.volt + ic);
+  const RP r = rho[ic];
+  st_nt(rth0, ic, (RP)thil[ic] * r);
+  st_nt(rho0, ic, r);
+  st_nt(wmc0, ic, __builtin_nontemporal_load(wmc + ic));
+  st_nt(thilt_s, ic, __builtin_nontemporal_load(thilt + ic) * v4);
+  st_nt(vmxet_s, ic, __builtin_nontemporal_load(vmxet + ic) * v4);
+  st_nt(vmyet_s, ic, __builtin_nontemporal_load(vmyet + ic) * v4);
+  st_nt(vmzet_s, ic, __builtin_nontemporal_load(vmzet + ic) * v4);
 }
 
 // ---------------------------------------------------------------- damping
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, con
       ay = ay + m.w_gyc[c * NSLOT + j] * d;
     });
     if (!body) { ax = R(0); ay = R(0); }
-    if (act) { gx[q][ic] = ax; gy[q][ic] = ay; }
+    if (act) { st_nt(gx[q], ic, ax); st_nt(gy[q], ic, ay); }
   }
 }
 
@@ -629,13 +630,13 @@ __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int 
   const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
   const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
   const R dts = (R)b.dt8;
-  const R pgf = (R)(b.press[i1] - b.press[i2]) * b.pvfac[ie];
-  const R v = b.vmc0[ie] + dts * (b.vmt_s[ie] + pgf +
+  const R pgf = (R)(b.press[i1] - b.press[i2]) * __builtin_nontemporal_load(b.pvfac + ie);
+  const R v = __builtin_nontemporal_load(b.vmc0 + ie) + dts * (__builtin_nontemporal_load(b.vmt_s + ie) + pgf +
                                   (m.vnx[iv] * (b.vmxet_rk[i1] + b.vmxet_rk[i2]) +
                                    m.vny[iv] * (b.vmyet_rk[i1] + b.vmyet_rk[i2]) +
                                    m.vnz[iv] * (b.vmzet_rk[i1] + b.vmzet_rk[i2])) * m.volvi[ie]);
-  b.vmc[ie] = v;
-  b.vc[ie] = R(2) * v / (R)(b.rho[i1] + b.rho[i2]);
+  st_nt(b.vmc, ie, v);
+  st_nt(b.vc, ie, R(2) * v / (R)(b.rho[i1] + b.rho[i2]));
   if (acc_vmasc) b.vmasc[ie] = b.vmasc[ie] + v * m.arv[ie];
 }
 
@@ -661,6 +662,6 @@ __global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
   });
   const R x2 = shdn(x, 1), y2 = shdn(y, 1), z2 = shdn(z, 1);
   if (k == 1) { x = x2; y = y2; z = z2; }
-  if (act) { b.vxe[ic] = x; b.vye[ic] = y; b.vze[ic] = z; }
+  if (act) { st_nt(b.vxe, ic, x); st_nt(b.vye, ic, y); st_nt(b.vze, ic, z); }
   (void)body;
 }
