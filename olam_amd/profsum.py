@@ -26,10 +26,11 @@ def short(name):
 
 
 def trace_summary(db):
+    """(kernel, calls, total ms, average ms) from the dispatch records (ns timestamps)."""
     con = sqlite3.connect(db)
-    rows = con.execute("select name, total_calls, total_duration, average from top_kernels").fetchall()
+    rows = con.execute("select name, count(*), sum(end - start) from kernels group by name").fetchall()
     agg = defaultdict(lambda: [0, 0.0])
-    for name, calls, tot, avg in rows:
+    for name, calls, tot in rows:
         k = short(name)
         agg[k][0] += calls
         agg[k][1] += tot
