@@ -81,10 +81,49 @@ __device__ __forceinline__ TrItem tr_item(int nw, int ntr, int tpw) {
 
 // Cross-lane moves.  Call them with every lane of the wave active (never inside a
 // per-lane branch or a per-lane ?: operand): a lane that does not take part supplies 0.
+// One-lane shifts go through the DPP wave_shl:1 / wave_shr:1 operand modifiers (one
+// VALU move per 32-bit word, no LDS round trip); a lane without a source keeps its own
+// value, as __shfl_down / __shfl_up do.  Longer shifts use __shfl_down / __shfl_up.
+template <int CTRL, class T>
+__device__ __forceinline__ T dpp_shift1(T v) {
+  static_assert(sizeof(T) % 4 == 0 || sizeof(T) < 4, "32-bit words");
+  if constexpr (sizeof(T) < 4) {
+    const int w = (int)v;
+    return (T)__builtin_amdgcn_update_dpp(w, w, CTRL, 0xF, 0xF, false);
+  } else {
+    constexpr int NW = sizeof(T) / 4;
+    int w[NW];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_update_dpp(w[i], w[i], CTRL, 0xF, 0xF, false);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+  }
+}
+// The value of lane 0 (level 1 of the column) in every lane: one scalar read per
+// 32-bit word.  Every lane must be active, so lane 0 is the first active lane.
 template <class T>
-__device__ __forceinline__ T shdn(T v, int d) { return __shfl_down(v, d, WAVE); }
+__device__ __forceinline__ T lane0(T v) {
+  constexpr int NW = sizeof(T) / 4;
+  int w[NW];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < NW; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
 template <class T>
-__device__ __forceinline__ T shup(T v, int d) { return __shfl_up(v, d, WAVE); }
+__device__ __forceinline__ T shdn(T v, int d) {
+  if (d == 1) return dpp_shift1<0x130>(v);     // wave_shl:1, lane l reads lane l + 1
+  return __shfl_down(v, d, WAVE);
+}
+template <class T>
+__device__ __forceinline__ T shup(T v, int d) {
+  if (d == 1) return dpp_shift1<0x138>(v);     // wave_shr:1, lane l reads lane l - 1
+  return __shfl_up(v, d, WAVE);
+}
 
 // Parallel cyclic reduction of a tridiagonal system held one row per lane:
 // a x(l-1) + b x(l) + c x(l+1) = d.  Rows outside the system must be identity rows

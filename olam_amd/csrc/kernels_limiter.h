@@ -294,9 +294,10 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     if (mode == 0) continue;
     const long off = n * a.tstride;
     R mu = ld(a.mu + off, ic, act);
-    const R mu1 = __shfl(mu, 0, WAVE);
+    const R mu1 = lane0(mu);
     if (k == 1) mu = R(0);
-    R l0 = ld(a.lam0 + off, ic, act);
+    // on the first adjoint stage lam0 holds only what the implicit pass wrote
+    R l0 = (!a.first || ff > R(0.001)) ? ld(a.lam0 + off, ic, act) : R(0);
     const R nu = body ? mu * a.dtr * vt / rf : R(0);
     if (body) l0 = l0 + mu - nu * (gsum + fexp * (wm_m1 - wm));
     // every lane takes part in a shuffle: a lane that skipped it would hand 0 to its

@@ -589,6 +589,7 @@ struct TrAdj {
   const R *vmasc, *wmasc, *rho_old, *ffimp;
   const RP *rho;
   int ntr, nrk, istage, order, last, tpw;
+  int first;         // the first adjoint stage (istage = nrk): lam0 starts at zero
   long tstride;
   R dtr;
   const int *tmode;  // as in TrStage
@@ -626,9 +627,9 @@ __global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP
     const long off = n * a.tstride;
     const bool limn = a.limited(n);
     R mu = ld(a.mu + off, ic, act);
-    const R mu1 = __shfl(mu, 0, WAVE);
+    const R mu1 = lane0(mu);
     if (k == 1) mu = R(0);
-    R l0 = ld(a.lam0 + off, ic, act);
+    R l0 = a.first ? R(0) : ld(a.lam0 + off, ic, act);
     const double le = pcr_solve(tb1, tb2, tb3, body ? (double)mu : 0.0);
     R lef = body ? (R)le : R(0);
     l0 = l0 + mu - lef;
@@ -679,16 +680,18 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
   // the next tracer's loads are issued before the current one is computed (all of a
   // tracer's loads are in the prefetch, so no later load waits on them)
   R nmu = R(0), nl0 = R(0);
-  if (ti.n0 < ti.n1) { nmu = ld(a.mu + ti.n0 * a.tstride, ic, act); nl0 = ld(a.lam0 + ti.n0 * a.tstride, ic, act); }
+  // on the first adjoint stage lam0 holds only what the implicit pass of this column wrote
+  const bool rd0 = !a.first || a.ffimp[(long)(a.istage - 1) * m.nw + c] > R(0.001);
+  if (ti.n0 < ti.n1) { nmu = ld(a.mu + ti.n0 * a.tstride, ic, act); nl0 = ld(a.lam0 + ti.n0 * a.tstride, ic, act && rd0); }
   for (int n = ti.n0; n < ti.n1; ++n) {
     R mu = nmu, l0 = nl0;
     if (n + 1 < ti.n1) {
       nmu = ld(a.mu + (n + 1) * a.tstride, ic, act);
-      nl0 = ld(a.lam0 + (n + 1) * a.tstride, ic, act);
+      nl0 = ld(a.lam0 + (n + 1) * a.tstride, ic, act && rd0);
     }
     if (a.limited(n)) continue;
     const long off = n * a.tstride;
-    const R mu1 = __shfl(mu, 0, WAVE);
+    const R mu1 = lane0(mu);
     if (k == 1) mu = R(0);
     if (k == 2) mu = mu + mu1;                 // level 1 copied level 2 (explicit)
     const R nu = body ? mu * a.dtr * vt * rinv : R(0);
@@ -841,6 +844,8 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   }
   const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
+  // on the first adjoint stage lam0 holds only what the implicit pass of this column wrote
+  const bool rd0 = !a.first || a.ffimp[(long)(a.istage - 1) * m.nw + c] > R(0.001);
   const R vt = ld(m.volti, ic, act);
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R rinv = R(1) / (body ? rf : R(1));
@@ -851,7 +856,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
   // mu of the stage output with level 1 folded into level 2 (the explicit copy)
   auto fold = [&](R v) {
-    const R v1 = __shfl(v, 0, WAVE);
+    const R v1 = lane0(v);
     if (k == 1) v = R(0);
     if (k == 2) v = v + v1;
     return v;
@@ -861,7 +866,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
     L.mu = ld(a.mu + off, ic, act);
-    L.l0 = ld_nt(a.lam0 + off, ic, act);
+    L.l0 = rd0 ? ld_nt(a.lam0 + off, ic, act) : R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j)
       L.mun[j] = ((j < 6 || np > 6) && ((need >> j) & 1u)) ? ld(a.mu + off, (long)nb[j] * ldk + l, act) : R(0);
@@ -1094,7 +1099,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
     for (int i = w; i < n_tot; i += nwv) {
       const long ix = (long)cells[i] * ldk + l;
       R mu = ld(a.mu + off, ix, act);
-      const R mu1 = __shfl(mu, 0, WAVE);
+      const R mu1 = lane0(mu);
       if (k == 1) mu = R(0);
       if (k == 2) mu = mu + mu1;
       const R vt = ld(m.volti, ix, act);
@@ -1114,10 +1119,12 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
       const long ic = (long)c * ldk + l;
       const PCell<R> &me = sc[i];
       R mu = ld(a.mu + off, ic, act);
-      const R mu1 = __shfl(mu, 0, WAVE);
+      const R mu1 = lane0(mu);
       if (k == 1) mu = R(0);
       if (k == 2) mu = mu + mu1;
-      R l0 = ld(a.lam0 + off, ic, act);
+      // on the first adjoint stage lam0 holds only what the implicit pass wrote
+      const bool rd0 = !a.first || a.ffimp[(long)(a.istage - 1) * m.nw + c] > R(0.001);
+      R l0 = rd0 ? ld(a.lam0 + off, ic, act) : R(0);
       const R nu = snu[i * WAVE + l];
       if (body) l0 = l0 + mu - nu * cc[t];
       const R wm = wmr[t], fexpl = fexr[t];

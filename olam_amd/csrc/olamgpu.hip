@@ -421,6 +421,9 @@ struct Impl : ImplBase {
   DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy, lgq[3];
   int cur_adj = 0;
   bool have_fwd = false;
+  // ffimp (and whether any column is implicit) is current for the present vmasc, wmasc
+  // and rho_old: set by check_cfls, cleared by every call that can change those fields
+  bool cfl_fresh = false, cfl_any = false;
   double cfl_out[4] = {0, 0, 0, 0};
 
   template <class T> T *dp(DevBuf &b) { return b.as<T>(); }
@@ -812,6 +815,7 @@ struct Impl : ImplBase {
   DevBuf &cur_thil_buf() { return thil[cur_thil]; }
 
   void upload_state(const olamgpu_state *h) override {
+    cfl_fresh = false;
     auto W = [&](DevBuf &b, const float *p) { if (p) h2d<R>(b, p, t.perm_w); };
     auto V = [&](DevBuf &b, const float *p) { if (p) h2d<R>(b, p, t.perm_v); };
     W(cur_thil_buf(), h->thil); W(wmc, h->wmc); W(wc, h->wc); W(vxe, h->vxe); W(vye, h->vye); W(vze, h->vze);
@@ -856,6 +860,7 @@ struct Impl : ImplBase {
 
   // -------------------------------------------------------------- tend0 etc.
   void tend0() override {
+    cfl_fresh = false;
     HIPCHK(hipMemsetAsync(vmasc.p, 0, vmasc.bytes, s));
     HIPCHK(hipMemsetAsync(wmasc.p, 0, wmasc.bytes, s));
     if (have_sct) HIPCHK(hipMemsetAsync(trsct.p, 0, trsct.bytes, s));
@@ -866,6 +871,7 @@ struct Impl : ImplBase {
 
   // -------------------------------------------------------------- prog_wrtv_rk
   void prog_wrtv_rk() override {
+    cfl_fresh = false;
     const unsigned gw = nblk(nw), gv = nblk(nv), gm = nblk(nm);
     LAUNCH("k_dyn_setup_w", (k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
                        dp<R>(wmc), dp<R>(thilt), dp<R>(vmxet), dp<R>(vmyet), dp<R>(vmzet), dp<RP>(rth0),
@@ -1036,6 +1042,8 @@ struct Impl : ImplBase {
     R dtlr[3];
     stage_dts(dtlr, nrk);
     const bool any_imp = run_cfl(false);
+    cfl_fresh = true;
+    cfl_any = any_imp;
     if (ntr == 0) { have_fwd = true; return; }
     lim_setup();
     taped_lim = false;
@@ -1111,8 +1119,11 @@ struct Impl : ImplBase {
     const int nrk = prm.nrk_scal;
     R dtlr[3];
     stage_dts(dtlr, nrk);
-    const bool any_imp = run_cfl(false);
-    HIPCHK(hipMemsetAsync(lam0.p, 0, lam0.bytes, s));
+    // the forward step's implicit fractions are reused when nothing changed the winds
+    const bool any_imp = cfl_fresh ? cfl_any : run_cfl(false);
+    cfl_fresh = true;
+    cfl_any = any_imp;
+    // lam0 is not cleared: the first adjoint stage (a.first) starts it from zero
     for (int istage = nrk; istage >= 1; --istage) {
       TrAdj<R, RP> a;
       DevBuf &muB = cur_adj_buf();
@@ -1123,6 +1134,7 @@ struct Impl : ImplBase {
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
       a.rho = dp<RP>(rho);
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.last = istage == 1;
+      a.first = istage == nrk;
       a.tstride = tstride; a.dtr = dtlr[istage - 1];
       a.tpw = tpw;
       a.tmode = lim_mode.as<int>();
@@ -1213,6 +1225,7 @@ struct Impl : ImplBase {
     return tot;
   }
   void checkpoint(int kind, void *buf, int restore) override {
+    if (restore) cfl_fresh = false;
     size_t off = 0;
     for (DevBuf *b : ck_list(kind)) {
       char *d = (char *)buf + off;
@@ -1263,6 +1276,7 @@ void Impl<R, RP>::copy_rho_old() {
 
 template <class R, class RP>
 void Impl<R, RP>::timeavg_momsc() {
+  cfl_fresh = false;
   if (prm.nacoust <= 1) return;
   R f = (R)prm.nacoust;
   LAUNCH("k_scale", (k_scale<R>), dim3((unsigned)((vsz + 255) / 256)), dim3(256), dp<R>(vmasc), (long)vsz, f,
@@ -1274,6 +1288,7 @@ void Impl<R, RP>::timeavg_momsc() {
 
 template <class R, class RP>
 void Impl<R, RP>::copy_field(int id, int index, void *buf, int dir, int *eb) {
+  if (dir) cfl_fresh = false;
   DevBuf *b = nullptr;
   size_t off = 0;
   bool isrp = false, isv = false;
