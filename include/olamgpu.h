@@ -89,6 +89,11 @@ typedef struct olamgpu_mesh {
   const float  *xy_h, *xy_hu;                /* (5, 7, mwa) bounded / unbounded LSQ   */
   const float  *xx0_v, *xy0_v, *yy0_v;       /* (2, mva) face offsets, bounded        */
   const float  *xx0_vu, *xy0_vu, *yy0_vu;    /* (2, mva) face offsets, unbounded      */
+  /* Rayleigh friction near the model top (mem_rayf.f90:41-173).  Read only when one of
+   * the rayf*_fact parameters exceeds 1e-7; may be NULL otherwise. */
+  const float  *zm, *zt;                     /* (mza) heights of M and T levels, mem_grid */
+  const float  *th01d, *dn01d;               /* (mza) reference sounding, misc_coms.f90:112-113;
+                                                read when rayf_fact > 0 and initial == 1 */
 } olamgpu_mesh;
 
 /* Namelist options and time steps the path reads (oname_coms.f90:190-238,
@@ -113,6 +118,17 @@ typedef struct olamgpu_params {
   double  fcoriol;                   /* 2 Omega on the global grid */
   int32_t ithil_monot;               /* >0: monotone thil fluxes on the final acoustic RK stage */
   int32_t iwind_monot;               /* >0: monotone wind fluxes on the final acoustic RK stage */
+  /* Rayleigh friction (oname_coms.f90:224-238, mem_rayf.f90:41-173): each kind is on
+   * when its fact exceeds 1e-7 and some level lies above its zmin.  rayf_* damps thil
+   * and vmc toward the initial state and needs initial == 1 (horizontally homogeneous
+   * initialization); rayfw_* damps W through the implicit solve (prog_wrtv_rk.f90:46-52);
+   * rayfdiv_* adds a second-order divergence damping profile (vort_div_damp.f90:37-121);
+   * rayfmix_* mixes vxe/vye/vze vertically (rayf_mix_top_vxe, mem_rayf.f90:177-220). */
+  int32_t initial;                   /* misc_coms initial: 1 = horizontally homogeneous */
+  float   rayf_zmin, rayf_fact, rayf_expon;
+  float   rayfw_zmin, rayfw_fact, rayfw_expon;
+  float   rayfdiv_zmin, rayfdiv_fact, rayfdiv_expon;
+  float   rayfmix_zmin, rayfmix_fact, rayfmix_expon;
 } olamgpu_params;
 
 /* Prognostic and diagnostic fields of mem_basic/mem_tend (mem_basic.f90:8-34,
@@ -157,8 +173,12 @@ int olamgpu_set_scalar_pdef(olamgpu_ctx *ctx, int32_t n, int32_t pdef);
 
 /* ---- the hot path ---------------------------------------------------------- */
 
-/* init_wrtv_rk (prog_wrtv_rk.f90:36-54): stage weights and the Rayleigh factor b20
- * (1 when Rayleigh friction is off). */
+/* rayf_init (mem_rayf.f90:158-171) and init_wrtv_rk (prog_wrtv_rk.f90:36-54), called
+ * where olam_run.f90:555,566 call them: after the initial state has been uploaded and
+ * before the first prog_wrtv_rk.  With rayf_fact > 0 and initial == 1 it keeps the
+ * current vc as the wind vc03d that the friction relaxes toward; prog_wrtv_rk then
+ * fails with OLAMGPU_EINVAL until it has been called.  The friction profiles and the
+ * W factor b20 are built by olamgpu_create from the params. */
 int olamgpu_init_wrtv_rk(olamgpu_ctx *ctx);
 
 /* tend0 (timestep.f90:430-496): zero vmasc/wmasc and the tracer tendencies, save
@@ -172,6 +192,13 @@ int olamgpu_prog_wrtv_rk(olamgpu_ctx *ctx);
 
 /* timeavg_momsc (timestep.f90:563-612): vmasc, wmasc /= nacoust when nacoust > 1. */
 int olamgpu_timeavg_momsc(olamgpu_ctx *ctx);
+
+/* comp_alpha_press (timestep.f90:500-559), for a moist atmosphere (OLAM calls it only
+ * when miclevel > 0): from the host's vapor and total water mixing ratios rr_v, rr_w
+ * and potential temperature theta ((mza, mwa) FP32 each, Fortran layout) and the
+ * context's thil, sets alpha_press and pwfac at W and pvfac at V on the device, the
+ * factors prog_wrtv_rk reads (olamgpu_upload_state can set them instead). */
+int olamgpu_comp_alpha_press(olamgpu_ctx *ctx, const float *rr_v, const float *rr_w, const float *theta);
 
 /* scalar_transport_rk(rho_old) (scalar_transport_rk.F90:30-385): one long step of
  * all registered tracers.  rho_old = NULL uses the copy saved by olamgpu_tend0;

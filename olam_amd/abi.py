@@ -44,6 +44,7 @@ class MeshDesc(C.Structure):
         ("xy_h", _fp), ("xy_hu", _fp),
         ("xx0_v", _fp), ("xy0_v", _fp), ("yy0_v", _fp),
         ("xx0_vu", _fp), ("xy0_vu", _fp), ("yy0_vu", _fp),
+        ("zm", _fp), ("zt", _fp), ("th01d", _fp), ("dn01d", _fp),
     ]
 
 
@@ -57,6 +58,11 @@ class Params(C.Structure):
         ("vort_damp_fact", C.c_float), ("dtsm", C.c_double), ("dtlm", C.c_double),
         ("nacoust", C.c_int32), ("adjoint_tape", C.c_int32), ("fcoriol", C.c_double),
         ("ithil_monot", C.c_int32), ("iwind_monot", C.c_int32),
+        ("initial", C.c_int32),
+        ("rayf_zmin", C.c_float), ("rayf_fact", C.c_float), ("rayf_expon", C.c_float),
+        ("rayfw_zmin", C.c_float), ("rayfw_fact", C.c_float), ("rayfw_expon", C.c_float),
+        ("rayfdiv_zmin", C.c_float), ("rayfdiv_fact", C.c_float), ("rayfdiv_expon", C.c_float),
+        ("rayfmix_zmin", C.c_float), ("rayfmix_fact", C.c_float), ("rayfmix_expon", C.c_float),
     ]
 
 
@@ -76,17 +82,28 @@ class StateDesc(C.Structure):
     _fields_ = [(name, _dp if dt == np.float64 else _fp) for name, _, dt in STATE_FIELDS]
 
 
+RAYF_KINDS = ("rayf", "rayfw", "rayfdiv", "rayfmix")
+
+
 def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, iscal_monot=0,
                 divh_damp_level=3, vort_damp_level=3, divh_damp_fact=0.1, vort_damp_fact=0.075,
-                dtsm=60.0, dtlm=None, nacoust=1, adjoint_tape=1, ithil_monot=0, iwind_monot=0):
-    """Namelist defaults of oname_coms.f90:190-209, with RK3 for scalars."""
+                dtsm=60.0, dtlm=None, nacoust=1, adjoint_tape=1, ithil_monot=0, iwind_monot=0,
+                initial=1, rayf=None):
+    """Namelist defaults of oname_coms.f90:190-238, with RK3 for scalars.
+
+    ``rayf`` maps a Rayleigh friction kind ("rayf", "rayfw", "rayfdiv", "rayfmix") to
+    (zmin, fact, expon); kinds left out keep the namelist default (30 km, 0, 1: off).
+    """
     if isinstance(precision, str):
         precision = PREC_NAMES[precision]
     if dtlm is None:
         dtlm = dtsm * nacoust
+    rf = []
+    for kind in RAYF_KINDS:
+        rf.extend((rayf or {}).get(kind, (30000.0, 0.0, 1.0)))
     return Params(precision, num_scalar, nrk_wrtv, nrk_scal, order, order, order, iscal_monot,
                   divh_damp_level, vort_damp_level, divh_damp_fact, vort_damp_fact, dtsm, dtlm,
-                  nacoust, adjoint_tape, 2.0 * M.OMEGA, ithil_monot, iwind_monot)
+                  nacoust, adjoint_tape, 2.0 * M.OMEGA, ithil_monot, iwind_monot, initial, *rf)
 
 
 def _ptr(a, ctype):
@@ -179,16 +196,29 @@ class FortranGrid:
             keep[name] = per(g[name].reshape(mesh.nw, 35)).reshape(self.mwa, 7, 5)
         for name in ("xx0_v", "xy0_v", "yy0_v", "xx0_vu", "xy0_vu", "yy0_vu"):
             keep[name] = per(g[name])
+        keep["zm"] = lev(vg["zm"])
+        keep["zt"] = lev(vg["zt"])
+        keep["th01d"] = keep["dn01d"] = None     # set_sounding()
         self.arrays = keep
         d = MeshDesc()
         d.mza, d.mwa, d.mva, d.mma = mza, self.mwa, self.mva, self.mma
         for fname, ftype in MeshDesc._fields_[4:]:
             a = keep[fname]
-            ct = ftype._type_
-            setattr(d, fname, _ptr(a, ct))
+            if a is not None:
+                setattr(d, fname, _ptr(a, ftype._type_))
         self.desc = d
         self.pwfac = col(np.repeat(vg["dzim"][:, None], mesh.nw, axis=1))
         self.pvfac = col(cm["pvfac"])
+
+    def set_sounding(self, th01d, dn01d):
+        """The 1-D reference sounding (misc_coms.f90:112-113: potential temperature and
+        density per level k = 1..mza) that Rayleigh friction relaxes thil toward."""
+        for name, a in (("th01d", th01d), ("dn01d", dn01d)):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            if a.shape != (self.mza,):
+                raise ValueError(f"{name} must have shape ({self.mza},)")
+            self.arrays[name] = a
+            setattr(self.desc, name, _ptr(a, C.c_float))
 
     # helpers to move fields between (n, mza) real-point arrays and Fortran layout
     def to_f(self, a, kind="w", dtype=np.float32):

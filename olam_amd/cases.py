@@ -226,3 +226,13 @@ def jet_state(grid: FortranGrid, seed: int = 3, u_jet: float = 35.0, bump: float
     A["vye"][...] = grid.to_f(vye[1:].T.astype(np.float32))
     A["vze"][...] = grid.to_f(vze[1:].T.astype(np.float32))
     return st
+
+
+def reference_sounding(grid: FortranGrid, st: FortranState):
+    """A 1-D reference sounding (th01d, dn01d per level k = 1..mza, the arrays of
+    misc_coms.f90:112-113) from a state: the horizontal means of thil and rho, with
+    level 1 copied from level 2 as ohhi.f90:641 does."""
+    th = st.arrays["thil"][1:].astype(np.float64).mean(axis=0)
+    dn = st.arrays["rho"][1:].mean(axis=0)
+    th[0], dn[0] = th[1], dn[1]
+    return th.astype(np.float32), dn.astype(np.float32)

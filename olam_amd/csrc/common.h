@@ -33,7 +33,7 @@ struct DMesh {
   // of neighbour j, and the quadratic face offsets (xx, xy, yy) of this cell's side
   // (w_os) and of the neighbour's side (w_on), bounded ([0], xx0_v) / unbounded ([1])
   const R *w_q[2][5], *w_os[2][3], *w_on[2][3];
-  const R *wnx, *wny, *wnz;
+  const R *wnx, *wny, *wnz, *arw0;
   // V records (nv x 2)
   const int *v_iw, *v_im;
   const R *vnx, *vny, *vnz;
@@ -44,9 +44,16 @@ struct DMesh {
   const R *dzim, *gdz_wgtm, *gdz_wgtp, *aw11, *aw21, *aw12, *aw22;
   const R *awu11, *awu21, *awu12, *awu22;      // unbounded vertical quadratic (a_wu)
   const double *zwgt_top8, *zwgt_bot8, *gdz_wgtm8, *gdz_wgtp8;
+  const R *zfacit;                             // per level
+  const R *dniv;                               // per V face
   // control volumes (n x ldk)
   const R *arw, *volti, *volwi, *arv, *volvi;
   const double *volt;
+  // Rayleigh friction near the model top (mem_rayf.f90), per level (ldk): rayf_cof is
+  // null unless thil/vmc friction is on, rayf_cofmix unless the vertical wind mixing is
+  // on, b20 (the W factor of the implicit solve) unless W friction is on
+  const R *rayf_cof, *rayf_cofmix, *th01d, *dn01d, *b20;
+  int krayf_bot, krayfmix_bot;
 };
 
 // Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous range of

@@ -40,27 +40,95 @@ struct DynBuf {
 };
 
 // ---------------------------------------------------------------- setup loop
-// prog_wrtv_rk.f90:165-200: RK start values and volume-weighted tendencies.
+// prog_wrtv_rk.f90:165-200: RK start values and volume-weighted tendencies, then the
+// Rayleigh terms of :219-233: vertical mixing of vxe/vye/vze near the top
+// (rayf_mix_top_vxe, mem_rayf.f90:177-220) and the relaxation of thil toward th01d.
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, const RP *rho, const R *wmc,
                                                      const R *thilt, const R *vmxet, const R *vmyet,
                                                      const R *vmzet, RP *rth0, RP *rho0, R *wmc0, R *thilt_s,
-                                                     R *vmxet_s, R *vmyet_s, R *vmzet_s) {
+                                                     R *vmxet_s, R *vmyet_s, R *vmzet_s, const R *vxe,
+                                                     const R *vye, const R *vze) {
   NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
   const int l = lane_id(), k = l + 1;
-  if (k < 2 || k > m.mza) return;
   const long ic = (long)c * m.ldk + l;
+  // rayf_mix_top_vxe: vflux(k) = arw0 * rayf_cofmix(k) * real(rho(k+1) + rho(k)) *
+  // (v(k) - v(k+1)) on ka..mza-1, zero at ka-1 and mza; the tendency of level k gains
+  // vflux(k-1) - vflux(k) on ka..mza.  Every lane takes part in the shifts.
+  R fk[3] = {R(0), R(0), R(0)}, fkm1[3] = {R(0), R(0), R(0)};
+  if (m.rayf_cofmix) {
+    const bool fl = k >= m.krayfmix_bot && k <= m.mza - 1;
+    const RP r0 = rho[ic], r1 = shdn(r0, 1);
+    const R fact = m.arw0[c] * m.rayf_cofmix[l] * (R)(r1 + r0);
+    const R *v[3] = {vxe, vye, vze};
+    for (int q = 0; q < 3; ++q) {
+      const R a = v[q][ic], a1 = shdn(a, 1);
+      fk[q] = fl ? fact * (a - a1) : R(0);
+      fkm1[q] = shup(fk[q], 1);
+    }
+  }
+  if (k < 2 || k > m.mza) return;
   const R v4 = (R)__builtin_nontemporal_load(m.volt + ic);
   const RP r = rho[ic];
-  st_nt(rth0, ic, (RP)thil[ic] * r);
+  const R th = thil[ic];
+  st_nt(rth0, ic, (RP)th * r);
   st_nt(rho0, ic, r);
   st_nt(wmc0, ic, __builtin_nontemporal_load(wmc + ic));
-  st_nt(thilt_s, ic, __builtin_nontemporal_load(thilt + ic) * v4);
-  st_nt(vmxet_s, ic, __builtin_nontemporal_load(vmxet + ic) * v4);
-  st_nt(vmyet_s, ic, __builtin_nontemporal_load(vmyet + ic) * v4);
-  st_nt(vmzet_s, ic, __builtin_nontemporal_load(vmzet + ic) * v4);
+  R tt = __builtin_nontemporal_load(thilt + ic) * v4;
+  if (m.rayf_cof && k >= m.krayf_bot)
+    tt = tt + m.rayf_cof[l] * m.dn01d[l] * (m.th01d[l] - th) * v4;
+  st_nt(thilt_s, ic, tt);
+  R vm[3] = {__builtin_nontemporal_load(vmxet + ic) * v4, __builtin_nontemporal_load(vmyet + ic) * v4,
+             __builtin_nontemporal_load(vmzet + ic) * v4};
+  if (m.rayf_cofmix && k >= m.krayfmix_bot)
+    for (int q = 0; q < 3; ++q) vm[q] = vm[q] + fkm1[q] - fk[q];
+  st_nt(vmxet_s, ic, vm[0]);
+  st_nt(vmyet_s, ic, vm[1]);
+  st_nt(vmzet_s, ic, vm[2]);
+}
+
+// R-precision power evaluated through an FP64 pow and rounded once, so the FP32
+// result does not depend on which math library computes it (the oracle does the same).
+__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ double pow_r(double x, double y) { return pow(x, y); }
+
+// ---------------------------------------------------------------- moist factors
+// comp_alpha_press, timestep.f90:515-536 (W): alpha_press = pc1 * ((rdry + rvap * rr_v)
+// * theta / thil)**cpocv on lpw..mza and pwfac = dzim * (1 - rw + rw**2) with rw the
+// W-level interpolation of rr_w on lpw..mza-1.
+template <class R>
+__global__ __launch_bounds__(256) void k_alpha_w(DMesh<R> m, const R *rr_v, const R *rr_w, const R *theta,
+                                                 const R *thil, R *alpha, R *pwfac, R pc1) {
+  NOCONTRACT
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const long ic = (long)c * m.ldk + l;
+  const R w0 = rr_w[ic], w1 = shdn(w0, 1);    // rr_w(k), rr_w(k+1); every lane shifts
+  if (k < 2 || k > m.mza) return;
+  const R rdry = R(287), rvap = R(461), cpocv = (R)(1004.0 / 717.0);
+  alpha[ic] = pc1 * pow_r((rdry + rvap * rr_v[ic]) * theta[ic] / thil[ic], cpocv);
+  if (k <= m.mza - 1) {
+    const R rw = (R)m.zwgt_top8[l] * w1 + (R)m.zwgt_bot8[l] * w0;
+    pwfac[ic] = m.dzim[l] * (R(1) - rw + rw * rw);
+  }
+}
+
+// comp_alpha_press, timestep.f90:540-553 (V): pvfac = dniv * zfacit * (1 - rw + rw**2)
+// with rw the mean rr_w of the face's two cells, on lpv..mza
+template <class R>
+__global__ __launch_bounds__(256) void k_alpha_v(DMesh<R> m, const R *rr_w, R *pvfac) {
+  NOCONTRACT
+  const int iv = wave_item();
+  if (iv >= m.nv) return;
+  const int l = lane_id(), k = l + 1;
+  if (k < 2 || k > m.mza) return;
+  const long ldk = m.ldk;
+  const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
+  const R rw = R(0.5) * (rr_w[(long)iw1 * ldk + l] + rr_w[(long)iw2 * ldk + l]);
+  pvfac[(long)iv * ldk + l] = m.dniv[iv] * m.zfacit[l] * (R(1) - rw + rw * rw);
 }
 
 // ---------------------------------------------------------------- damping
@@ -68,7 +136,7 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, 
 // reference and never reset, so the divergence accumulates onto what the previous
 // call left there (the last Laplacian for level 3); reproduced as is.
 template <class R>
-__global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, R *div) {
+__global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, const R *div_in, R *div_out) {
   NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
@@ -76,12 +144,12 @@ __global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, R *div) {
   if (k < 2 || k > m.mza) return;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
-  R d = div[ic];
+  R d = div_in[ic];
   for_slots(np, [&](int j) {
     const long iv = (long)m.w_face[c * NSLOT + j] * ldk + l;
     d = d - m.arv[iv] * m.w_dirv[c * NSLOT + j] * vmc[iv];
   });
-  div[ic] = d * m.volti[ic];
+  div_out[ic] = d * m.volti[ic];
 }
 
 // laplacian2d, grad_lib.f90:494-524
@@ -135,12 +203,14 @@ __global__ __launch_bounds__(256) void k_lap_m(DMesh<R> m, const R *src, R *dst)
   dst[ic] = d;
 }
 
-// V setup (prog_wrtv_rk.f90:256-262) with the damping tendencies added to vmt_short
-// (vort_div_damp.f90:216-228 and :424-441).
+// V setup (prog_wrtv_rk.f90:256-262), then the Rayleigh friction of vmc toward vc03d
+// (:268-284, global domain) and the damping tendencies (vort_div_damp.f90:169-185,
+// 216-228 and 424-441), added to vmt_short in the reference's order.
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, const R *vmt, R *vmc0, R *vmt_s,
-                                                     const R *div, const R *fdivh, const R *vort,
-                                                     const R *fvorth, const RP *rho) {
+                                                     const R *divl, const R *fdivl, int kdivl, const R *divh,
+                                                     const R *fdivh, const R *vort, const R *fvorth,
+                                                     const RP *rho, const R *vc, const R *vc03d) {
   NOCONTRACT
   const int iv = wave_item();
   if (iv >= m.nv) return;
@@ -151,11 +221,14 @@ __global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, c
   R t = vmt[ie];
   if (k >= 2) {
     const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
-    if (div) t = t + fdivh[ie] * (div[(long)iw2 * ldk + l] - div[(long)iw1 * ldk + l]);
+    const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
+    if (m.rayf_cof && k >= m.krayf_bot)
+      t = (R)((RP)t + (RP)(m.rayf_cof[l] * R(0.5)) * (rho[i1] + rho[i2]) * (RP)(vc03d[ie] - vc[ie]));
+    if (divl && k >= kdivl) t = t + fdivl[ie] * (divl[i2] - divl[i1]);
+    if (divh) t = t + fdivh[ie] * (divh[i2] - divh[i1]);
     if (vort) {
       const int im1 = m.v_im[iv * 2], im2 = m.v_im[iv * 2 + 1];
-      t = t + fvorth[iv] * (vort[(long)im2 * ldk + l] - vort[(long)im1 * ldk + l]) *
-                  (R)(rho[(long)iw1 * ldk + l] + rho[(long)iw2 * ldk + l]);
+      t = t + fvorth[iv] * (vort[(long)im2 * ldk + l] - vort[(long)im1 * ldk + l]) * (R)(rho[i1] + rho[i2]);
     }
   }
   vmt_s[ie] = t;
@@ -272,12 +345,8 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
 }
 
 // ---------------------------------------------------------------- prog_wrt_begs
-// prog_wrtv_rk.f90:494-884 (global domain, no nudging, b20 = 1) fused with the face
+// prog_wrtv_rk.f90:494-884 (global domain, no nudging) fused with the face
 // values of fluxes_wrtv's scalar_v_column (:1022-1081) and scalar_w_column.
-// R-precision power evaluated through an FP64 pow and rounded once, so the FP32
-// result does not depend on which math library computes it (the oracle does the same).
-__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
-__device__ __forceinline__ double pow_r(double x, double y) { return pow(x, y); }
 
 template <class R, class RP, bool MONO>
 __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
@@ -432,7 +501,9 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
                dts * ((R)(press_ex - pex_p1) * pwf -
                       (R)((RP)m.gdz_wgtm8[l] * rho_ex + (RP)m.gdz_wgtp8[l] * rex_p1) +
                       (m.wnx[c] * (tx + tx_p1) + m.wny[c] * (ty + ty_p1) + m.wnz[c] * (tz + tz_p1)) * vwi);
-  // implicit W matrix (:749-769); b20 = 1 without Rayleigh friction on W
+  // implicit W matrix (:749-769); b20 = 1 + dtsm * rayf_cofw (:46-52), 1 without
+  // Rayleigh friction on W
+  const R b20 = m.b20 ? m.b20[l] : R(1);
   const R c8 = dts * (R)pc2;
   const R c9 = -dts * dts * (R)fr;
   const R wcc = ld(b.wc, ic, act);
@@ -451,7 +522,7 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     const R b25 = c9 * vti * m.gdz_wgtm[l];
     const R b26 = c9 * vti_p1 * m.gdz_wgtp[l];
     ma = (double)(-arw_m1 * (b21 + b25 + b23 * tupw_m1));
-    mb = (double)(R(1) + arw * (b25 - b26 + (b23 + b24) * tupw));
+    mb = (double)(b20 + arw * (b25 - b26 + (b23 + b24) * tupw));
     mc = (double)(arw_p1 * (b22 + b26 - b24 * tupw_p1));
     md = (double)delex_wm;
   }

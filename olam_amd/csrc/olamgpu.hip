@@ -353,9 +353,11 @@ struct ImplBase {
   virtual void download_scalar(int, float *) = 0;
   virtual void upload_adj(int, const float *) = 0;
   virtual void download_adj(int, float *) = 0;
+  virtual void init_wrtv_rk() = 0;
   virtual void tend0() = 0;
   virtual void prog_wrtv_rk() = 0;
   virtual void timeavg_momsc() = 0;
+  virtual void comp_alpha_press(const float *, const float *, const float *) = 0;
   virtual void scalar_transport_rk(const float *) = 0;
   virtual void scalar_transport_rk_adj() = 0;
   virtual void mass_sums(double *) = 0;
@@ -366,7 +368,73 @@ struct ImplBase {
   virtual void set_pdef(int, int) = 0;
 };
 
+// x**n for an integer n >= 0 in the type of x, one rounding per factor (Fortran's
+// real**integer, e.g. vort_div_damp.f90:80,318)
 template <class T> static inline T ipow(T x, int n) { T r = T(1); for (int i = 0; i < n; ++i) r *= x; return r; }
+
+// FP32 x**y of a real exponent: FP64 pow rounded to FP32
+static inline float powr(float x, float y) { return (float)std::pow((double)x, (double)y); }
+
+// Rayleigh friction profiles of rayf_init (mem_rayf.f90:41-156), the W factor b20 of
+// init_wrtv_rk (prog_wrtv_rk.f90:46-52) and the divergence profile zf of divh_damp
+// (vort_div_damp.f90:60-66), in the reference's FP32.  Arrays are per level [k-1].
+struct Rayf {
+  bool dorayf = false, dorayfw = false, dorayfdiv = false, dorayfmix = false;
+  int krayf_bot, krayfw_bot, krayfdiv_bot, krayfmix_bot;
+  std::vector<float> cof, cofw, cofmix, zf, b20;
+};
+static Rayf rayf_profiles(const olamgpu_params &p, int mza, const float *zm, const float *zt) {
+  Rayf r;
+  r.krayf_bot = r.krayfw_bot = r.krayfdiv_bot = r.krayfmix_bot = mza + 1;
+  r.cof.assign(mza, 0.f); r.cofw.assign(mza, 0.f); r.cofmix.assign(mza, 0.f); r.zf.assign(mza, 0.f);
+  r.b20.assign(mza, 1.f);
+  const bool any = p.rayf_fact > 1.e-7f || p.rayfw_fact > 1.e-7f || p.rayfdiv_fact > 1.e-7f ||
+                   p.rayfmix_fact > 1.e-7f;
+  if (!any) return r;
+  if (!zm || !zt) throw Err(OLAMGPU_EINVAL, "Rayleigh friction needs the zm and zt level heights");
+  auto Z = [&](const float *a, int k) { return a[k - 1]; };
+  const float dti = 1.0f / (float)p.dtsm;
+  const float ztop = Z(zm, mza);
+  if (p.rayf_fact > 1.e-7f && p.initial == 1) {
+    for (int k = 2; k <= mza; ++k) if (Z(zt, k) > p.rayf_zmin) { r.krayf_bot = k; break; }
+    if (r.krayf_bot <= mza) {
+      r.dorayf = true;
+      const float distimi = p.rayf_fact * dti;
+      for (int k = r.krayf_bot; k <= mza; ++k)
+        r.cof[k - 1] = distimi * powr((Z(zt, k) - p.rayf_zmin) / (ztop - p.rayf_zmin), p.rayf_expon);
+    }
+  }
+  if (p.rayfw_fact > 1.e-7f) {
+    for (int k = 2; k <= mza - 1; ++k) if (Z(zm, k) > p.rayfw_zmin) { r.krayfw_bot = k; break; }
+    if (r.krayfw_bot < mza) {
+      r.dorayfw = true;
+      const float distimi = p.rayfw_fact * dti;
+      for (int k = r.krayfw_bot; k <= mza - 1; ++k) {
+        r.cofw[k - 1] = distimi * powr((Z(zm, k) - p.rayfw_zmin) / (ztop - p.rayfw_zmin), p.rayfw_expon);
+        r.b20[k - 1] = 1.0f + (float)p.dtsm * r.cofw[k - 1];
+      }
+    }
+  }
+  if (p.rayfdiv_fact > 1.e-7f) {
+    for (int k = 2; k <= mza; ++k) if (Z(zt, k) > p.rayfdiv_zmin) { r.krayfdiv_bot = k; break; }
+    if (r.krayfdiv_bot <= mza) {
+      r.dorayfdiv = true;
+      for (int k = r.krayfdiv_bot; k <= mza; ++k)
+        r.zf[k - 1] = powr((Z(zt, k) - p.rayfdiv_zmin) / (ztop - p.rayfdiv_zmin), p.rayfdiv_expon);
+    }
+  }
+  if (p.rayfmix_fact > 1.e-7f) {
+    for (int k = 2; k <= mza - 1; ++k) if (Z(zm, k) > p.rayfmix_zmin) { r.krayfmix_bot = k; break; }
+    if (r.krayfmix_bot < mza) {
+      r.dorayfmix = true;
+      // the profile is normalised by zm(mza) - rayfw_zmin, as in mem_rayf.f90:150
+      for (int k = r.krayfmix_bot; k <= mza - 1; ++k)
+        r.cofmix[k - 1] = 0.5f * p.rayfmix_fact *
+                          powr((Z(zm, k) - p.rayfmix_zmin) / (ztop - p.rayfw_zmin), p.rayfmix_expon);
+    }
+  }
+  return r;
+}
 
 template <class R, class RP>
 struct Impl : ImplBase {
@@ -379,7 +447,7 @@ struct Impl : ImplBase {
   DMesh<R> dm;
   // mesh storage
   DevBuf b_npoly, b_nbr, b_face, b_back, b_viw, b_vim, b_miv, b_mim, b_invw, b_invv;
-  DevBuf b_dirv, b_gxc, b_gyc, b_dxs, b_dys, b_dxn, b_dyn, b_ecx, b_ecy, b_ecz, b_xxyy, b_wn[3];
+  DevBuf b_dirv, b_gxc, b_gyc, b_dxs, b_dys, b_dxn, b_dyn, b_ecx, b_ecy, b_ecz, b_xxyy, b_wn[3], b_arw0, b_zfacit, b_dniv;
   DevBuf b_vn[3], b_vfact, b_xxyym, b_lev[11], b_lev8[4];
   DevBuf b_arw, b_volti, b_volwi, b_volt, b_arv, b_volvi;
   DevBuf b_q[2][5], b_os[2][3], b_on[2][3];   // order-3 reconstruction (DMesh::w_q, w_os, w_on)
@@ -403,9 +471,14 @@ struct Impl : ImplBase {
   // fluxes_wrtv_monot (ithil_monot / iwind_monot): RK-start copies, scpt, scales
   DevBuf mq0[4], mscpt[4], ms1[4], ms2[4];
   int mono_on[4] = {0, 0, 0, 0};
-  DevBuf div2d[2], vort2d[2], fdivh, fvorth;
-  int div_ih = 0, vort_ih = 0;
-  bool do_div = false, do_vort = false;
+  DevBuf div2d[3], vort2d[2], fdivl, fdivh, fvorth;
+  int div_ih = 0, vort_ih = 0, div_l = 0, kdivl = 2;
+  bool do_div = false, do_vort = false, do_divl = false, do_divh = false;
+  // Rayleigh friction: profiles, the wind vc03d the friction relaxes toward, and
+  // whether init_wrtv_rk has taken vc03d
+  Rayf rf;
+  DevBuf b_rayf, vc03d;
+  bool rayf_ready = false;
   // tracers
   int ntr;
   long tstride;
@@ -514,7 +587,7 @@ struct Impl : ImplBase {
     const bool f64 = sizeof(R) == 8;
     std::vector<R> dirv(nw * NSLOT, 0), gxc(nw * NSLOT, 0), gyc(nw * NSLOT, 0), dxs(nw * NSLOT, 0),
         dys(nw * NSLOT, 0), dxn(nw * NSLOT, 0), dyn(nw * NSLOT, 0), ecx(nw * NSLOT, 0), ecy(nw * NSLOT, 0),
-        ecz(nw * NSLOT, 0), xxyy(nw * NSLOT, 0), wn[3] = {std::vector<R>(nw), std::vector<R>(nw), std::vector<R>(nw)};
+        ecz(nw * NSLOT, 0), xxyy(nw * NSLOT, 0), wn[3] = {std::vector<R>(nw), std::vector<R>(nw), std::vector<R>(nw)}, aw0(nw);
     for (int c = 0; c < nw; ++c) {
       int o = t.perm_w[c];
       size_t fo = (size_t)(o + 1) * 7;
@@ -530,6 +603,7 @@ struct Impl : ImplBase {
         dxn[i] = m->v_dxps[(size_t)(v + 1) * 2 + 1 - side]; dyn[i] = m->v_dyps[(size_t)(v + 1) * 2 + 1 - side];
       }
       wn[0][c] = m->wnx[o + 1]; wn[1][c] = m->wny[o + 1]; wn[2][c] = m->wnz[o + 1];
+      aw0[c] = m->arw0[o + 1];
     }
     // order-3 terms: xy_h(t, n, iw) at [(iw - 1) * 35 + (n - 1) * 5 + t - 1]; face offsets
     // xx0_v(side, iv) at [(iv - 1) * 2 + side - 1]
@@ -642,6 +716,14 @@ struct Impl : ImplBase {
     upv(b_dirv, dirv); upv(b_gxc, gxc); upv(b_gyc, gyc); upv(b_dxs, dxs); upv(b_dys, dys);
     upv(b_dxn, dxn); upv(b_dyn, dyn); upv(b_ecx, ecx); upv(b_ecy, ecy); upv(b_ecz, ecz); upv(b_xxyy, xxyy);
     for (int i = 0; i < 3; ++i) { upv(b_wn[i], wn[i]); upv(b_vn[i], vn[i]); }
+    upv(b_arw0, aw0);
+    {
+      std::vector<R> zf(ldk, R(0)), dn(nv, R(0));
+      for (int k = 1; k <= mza; ++k) zf[k - 1] = (R)m->zfacit[k - 1];
+      for (int e = 0; e < nv; ++e) dn[e] = (R)m->dniv[t.perm_v[e] + 1];
+      upv(b_zfacit, zf);
+      upv(b_dniv, dn);
+    }
     upv(b_vfact, vfact); upv(b_xxyym, xxyym);
     for (int i = 0; i < 11; ++i) upv(b_lev[i], lev[i]);
     for (int i = 0; i < 4; ++i) upv(b_lev8[i], lev8[i]);
@@ -662,7 +744,8 @@ struct Impl : ImplBase {
       for (int q = 0; q < 5; ++q) dm.w_q[b][q] = dp<R>(b_q[b][q]);
       for (int q = 0; q < 3; ++q) { dm.w_os[b][q] = dp<R>(b_os[b][q]); dm.w_on[b][q] = dp<R>(b_on[b][q]); }
     }
-    dm.wnx = dp<R>(b_wn[0]); dm.wny = dp<R>(b_wn[1]); dm.wnz = dp<R>(b_wn[2]);
+    dm.wnx = dp<R>(b_wn[0]); dm.wny = dp<R>(b_wn[1]); dm.wnz = dp<R>(b_wn[2]); dm.arw0 = dp<R>(b_arw0);
+    dm.zfacit = dp<R>(b_zfacit); dm.dniv = dp<R>(b_dniv);
     dm.v_iw = dp<int>(b_viw); dm.v_im = dp<int>(b_vim);
     dm.vnx = dp<R>(b_vn[0]); dm.vny = dp<R>(b_vn[1]); dm.vnz = dp<R>(b_vn[2]);
     dm.m_iv = dp<int>(b_miv); dm.m_im = dp<int>(b_mim); dm.m_vfact = dp<R>(b_vfact); dm.m_xxyym = dp<R>(b_xxyym);
@@ -674,34 +757,71 @@ struct Impl : ImplBase {
     dm.arw = dp<R>(b_arw); dm.volti = dp<R>(b_volti); dm.volwi = dp<R>(b_volwi); dm.arv = dp<R>(b_arv);
     dm.volvi = dp<R>(b_volvi); dm.volt = dp<double>(b_volt);
 
-    // damping coefficients (first-call SAVEd factors of vort_div_damp.f90:39-119, 264-302)
+    // Rayleigh friction profiles and b20 (mem_rayf.f90:41-156, prog_wrtv_rk.f90:46-52)
+    rf = rayf_profiles(prm, mza, m->zm, m->zt);
+    if (rf.dorayf && !(m->th01d && m->dn01d))
+      throw Err(OLAMGPU_EINVAL, "rayf_fact > 0 with initial == 1 needs the th01d/dn01d reference sounding");
+    {
+      std::vector<R> lv((size_t)5 * ldk, R(0));
+      for (int k = 1; k <= mza; ++k) {
+        lv[0 * ldk + k - 1] = (R)rf.cof[k - 1];
+        lv[1 * ldk + k - 1] = (R)rf.cofmix[k - 1];
+        lv[2 * ldk + k - 1] = rf.dorayf ? (R)m->th01d[k - 1] : R(0);
+        lv[3 * ldk + k - 1] = rf.dorayf ? (R)m->dn01d[k - 1] : R(0);
+        lv[4 * ldk + k - 1] = (R)rf.b20[k - 1];
+      }
+      upv(b_rayf, lv);
+      R *base = dp<R>(b_rayf);
+      dm.rayf_cof = rf.dorayf ? base : nullptr;
+      dm.rayf_cofmix = rf.dorayfmix ? base + ldk : nullptr;
+      dm.th01d = base + 2 * ldk;
+      dm.dn01d = base + 3 * ldk;
+      dm.b20 = rf.dorayfw ? base + 4 * ldk : nullptr;
+      dm.krayf_bot = rf.krayf_bot;
+      dm.krayfmix_bot = rf.krayfmix_bot;
+    }
+
+    // damping coefficients (first-call SAVEd factors of vort_div_damp.f90:39-119, 271-328)
     const double dtm = prm.dtsm;
     const int dl = prm.divh_damp_level, vl = prm.vort_damp_level;
     do_div = prm.divh_damp_fact > 1.e-7f && dl > 0;
     do_vort = prm.vort_damp_fact >= 1.e-7f && vl > 0;
-    if (do_div) {
-      std::vector<R> f(vsz, R(0));
-      const double facth = -std::pow(-(double)prm.divh_damp_fact, dl) / dtm;
-      const R fact2 = (R)(prm.divh_damp_fact / (R)dtm);
+    // second-order (fdivl: level 1 and/or the Rayleigh profile) and higher-order (fdivh)
+    // divergence damping factors, vort_div_damp.f90:68-119
+    do_divl = rf.dorayfdiv || (do_div && dl == 1);
+    do_divh = do_div && dl > 1;
+    kdivl = (do_div && dl == 1) ? 2 : rf.krayfdiv_bot;
+    if (do_divl || do_divh) {
+      std::vector<R> fl(do_divl ? vsz : 0, R(0)), fh(do_divh ? vsz : 0, R(0));
+      const double facth = do_divh ? -(double)ipow((R)(-prm.divh_damp_fact), dl) / dtm : 0.0;
+      const R fact2 = (do_div && dl == 1) ? (R)(prm.divh_damp_fact / (R)dtm) : R(0);
+      R factr = R(0);
+      if (rf.dorayfdiv && do_div) factr = (R)(std::max(prm.divh_damp_fact, prm.rayfdiv_fact) / (R)dtm);
+      else if (rf.dorayfdiv) factr = (R)(prm.rayfdiv_fact / (R)dtm);
       for (int e = 0; e < nv; ++e) {
         int o = t.perm_v[e] + 1;
         int w1o = in.viw(0, o - 1) + 1, w2o = in.viw(1, o - 1) + 1;
-        double a0 = std::min(m->arw0[w1o], m->arw0[w2o]);
+        const double factb = do_divh ? facth * (double)ipow((R)std::min(m->arw0[w1o], m->arw0[w2o]), dl - 1) : 0.0;
         for (int k = 2; k <= mza; ++k) {
-          double vmin = std::min(m->volt[(size_t)w1o * mza + k - 1], m->volt[(size_t)w2o * mza + k - 1]);
-          if (dl > 1) {
-            double factb = facth * std::pow(a0, dl - 1);
-            f[(size_t)e * ldk + k - 1] = (R)(factb * m->dniv[o] * m->zfacit[k - 1] * m->dzit[k - 1] * vmin);
-          } else {
-            f[(size_t)e * ldk + k - 1] =
-                (R)((double)(fact2 * (R)m->dniv[o] * (R)m->zfacit[k - 1] * (R)m->dzit[k - 1]) * vmin);
-          }
+          const double vmin = std::min(m->volt[(size_t)w1o * mza + k - 1], m->volt[(size_t)w2o * mza + k - 1]);
+          const R zf = (R)rf.zf[k - 1];
+          if (do_divl && k >= kdivl)
+            fl[(size_t)e * ldk + k - 1] = (R)((double)((fact2 * (R(1) - zf) + factr * zf) * (R)m->dniv[o] *
+                                                       (R)m->zfacit[k - 1] * (R)m->dzit[k - 1]) * vmin);
+          if (do_divh)
+            fh[(size_t)e * ldk + k - 1] = (R)(factb * m->dniv[o] * m->zfacit[k - 1] * m->dzit[k - 1] * vmin *
+                                              (double)(R(1) - zf));
         }
       }
-      upv(fdivh, f);
-      div2d[0].alloc(wsz * sizeof(R)); div2d[1].alloc(wsz * sizeof(R));
-      HIPCHK(hipMemsetAsync(div2d[0].p, 0, div2d[0].bytes, s));
-      HIPCHK(hipMemsetAsync(div2d[1].p, 0, div2d[1].bytes, s));
+      if (do_divl) upv(fdivl, fl);
+      if (do_divh) upv(fdivh, fh);
+      // the divergence and its Laplacians; a third buffer keeps the divergence for
+      // fdivl when two Laplacians follow it (level 3 with the Rayleigh profile)
+      const int nb = (do_divl && dl == 3 && do_div) ? 3 : 2;
+      for (int i = 0; i < nb; ++i) {
+        div2d[i].alloc(wsz * sizeof(R));
+        HIPCHK(hipMemsetAsync(div2d[i].p, 0, div2d[i].bytes, s));
+      }
     }
     if (do_vort) {
       std::vector<R> f(nv, R(0));
@@ -709,7 +829,7 @@ struct Impl : ImplBase {
         int o = t.perm_v[e] + 1;
         int m1 = in.vim(0, o - 1) + 1, m2 = in.vim(1, o - 1) + 1;
         R a = std::min((R)m->arm0[m1], (R)m->arm0[m2]);
-        f[e] = R(0.5) * (R)m->dniu[o] / (R)dtm * (R)std::pow((double)(-(R)prm.vort_damp_fact * a), vl);
+        f[e] = R(0.5) * (R)m->dniu[o] / (R)dtm * ipow(-(R)prm.vort_damp_fact * a, vl);
       }
       upv(fvorth, f);
       vort2d[0].alloc(msz * sizeof(R)); vort2d[1].alloc(msz * sizeof(R));
@@ -870,26 +990,67 @@ struct Impl : ImplBase {
   void timeavg_momsc() override;
 
   // -------------------------------------------------------------- prog_wrtv_rk
+  // comp_alpha_press (timestep.f90:500-559) from host moisture fields
+  DevBuf moist[3];
+  void comp_alpha_press(const float *rr_v, const float *rr_w, const float *theta) override {
+    if (!rr_v || !rr_w || !theta) throw Err(OLAMGPU_EINVAL, "comp_alpha_press needs rr_v, rr_w and theta");
+    const float *h[3] = {rr_v, rr_w, theta};
+    for (int i = 0; i < 3; ++i) {
+      if (!moist[i].p) moist[i].alloc(wsz * sizeof(R));
+      h2d<R>(moist[i], h[i], t.perm_w);
+    }
+    // pc1 = p00i**rocv (consts_coms.f90:76)
+    const R pc1 = sizeof(R) == 4 ? (R)powr(1.0f / 1.0e5f, 287.0f / 717.0f) : (R)std::pow(1.0 / 1.0e5, 287.0 / 717.0);
+    LAUNCH("k_alpha_w", (k_alpha_w<R>), dim3(nblk(nw)), dim3(256), dm, dp<R>(moist[0]), dp<R>(moist[1]),
+           dp<R>(moist[2]), dp<R>(cur_thil_buf()), dp<R>(alpha), dp<R>(pwfac), pc1);
+    LAUNCH("k_alpha_v", (k_alpha_v<R>), dim3(nblk(nv)), dim3(256), dm, dp<R>(moist[1]), dp<R>(pvfac));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
+  // rayf_init's copy of the initial wind (mem_rayf.f90:158-171); the profiles and b20
+  // were built with the mesh
+  void init_wrtv_rk() override {
+    if (rf.dorayf) {
+      if (!vc03d.p) vc03d.alloc(vsz * sizeof(R));
+      HIPCHK(hipMemcpyAsync(vc03d.p, vc.p, vsz * sizeof(R), hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    rayf_ready = true;
+  }
+
   void prog_wrtv_rk() override {
+    if (rf.dorayf && !rayf_ready)
+      throw Err(OLAMGPU_EINVAL, "Rayleigh friction on thil/vmc: call olamgpu_init_wrtv_rk after the initial state upload");
     cfl_fresh = false;
     const unsigned gw = nblk(nw), gv = nblk(nv), gm = nblk(nm);
     LAUNCH("k_dyn_setup_w", (k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
                        dp<R>(wmc), dp<R>(thilt), dp<R>(vmxet), dp<R>(vmyet), dp<R>(vmzet), dp<RP>(rth0),
-                       dp<RP>(rho0), dp<R>(wmc0), dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s));
+                       dp<RP>(rho0), dp<R>(wmc0), dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s),
+                       dp<R>(vxe), dp<R>(vye), dp<R>(vze));
     // RK-start copies of the monotone-flux fields (prog_wrtv_rk.f90:188-199)
     {
       DevBuf *src[4] = {&cur_thil_buf(), &vxe, &vye, &vze};
       for (int q = 0; q < 4; ++q)
         if (mono_on[q]) HIPCHK(hipMemcpyAsync(mq0[q].p, src[q]->p, wsz * sizeof(R), hipMemcpyDeviceToDevice, s));
     }
-    // short-step damping: divergence at W and vorticity at M, then their Laplacians
-    if (do_div) {
-      LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]));
-      int ih = 0;
-      for (int nn = 1; nn <= prm.divh_damp_level - 1; ++nn) {
-        int il = (nn - 1) % 2;
-        ih = nn % 2;
-        LAUNCH("k_lap_w", (k_lap_w<R>), dim3(gw), dim3(256), dm, dp<R>(div2d[il]), dp<R>(div2d[ih]));
+    // short-step damping: divergence at W and vorticity at M, then their Laplacians.
+    // div2d[0] plays the reference's SAVEd div2d(:,:,1), onto which each call adds the
+    // new divergence; with three buffers the divergence itself goes to div2d[2] so that
+    // fdivl still sees it after the two Laplacians have overwritten div2d[0].
+    if (do_divl || do_divh) {
+      const int dst = div2d[2].p ? 2 : 0;
+      LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]), dp<R>(div2d[dst]));
+      div_l = dst;
+      int ih = dst;
+      if (do_divh) {
+        // Laplacians ping-pong between the two other buffers and end in div2d[0] for
+        // level 3 (vort_div_damp.f90:196-212 with il/ih = 1,2 then 2,1)
+        const int order[3] = {dst, 1, 0};
+        for (int nn = 1; nn <= prm.divh_damp_level - 1; ++nn) {
+          const int il = order[nn - 1], io = order[nn];
+          LAUNCH("k_lap_w", (k_lap_w<R>), dim3(gw), dim3(256), dm, dp<R>(div2d[il]), dp<R>(div2d[io]));
+          ih = io;
+        }
       }
       div_ih = ih;
     }
@@ -904,8 +1065,10 @@ struct Impl : ImplBase {
       vort_ih = ih;
     }
     LAUNCH("k_dyn_setup_v", (k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
-                       dp<R>(vmt_s), do_div ? dp<R>(div2d[div_ih]) : (R *)nullptr, dp<R>(fdivh),
-                       do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr, dp<R>(fvorth), dp<RP>(rho));
+                       dp<R>(vmt_s), do_divl ? dp<R>(div2d[div_l]) : (R *)nullptr, dp<R>(fdivl), kdivl,
+                       do_divh ? dp<R>(div2d[div_ih]) : (R *)nullptr, dp<R>(fdivh),
+                       do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc),
+                       dp<R>(vc03d));
     const int nrk = prm.nrk_wrtv;
     for (int istage = 1; istage <= nrk; ++istage) {
       double frac = (nrk <= 2) ? (istage == 1 ? 0.5 : 1.0)
@@ -1419,7 +1582,10 @@ int olamgpu_upload_scalar(olamgpu_ctx *c, int32_t n, const float *p, const float
 int olamgpu_download_scalar(olamgpu_ctx *c, int32_t n, float *p) {
   return guarded(c, [&] { if (!p) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->download_scalar(n, p); });
 }
-int olamgpu_init_wrtv_rk(olamgpu_ctx *c) { return guarded(c, [&] {}); }
+int olamgpu_comp_alpha_press(olamgpu_ctx *c, const float *rr_v, const float *rr_w, const float *theta) {
+  return guarded(c, [&] { c->impl->comp_alpha_press(rr_v, rr_w, theta); });
+}
+int olamgpu_init_wrtv_rk(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->init_wrtv_rk(); }); }
 int olamgpu_tend0(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->tend0(); }); }
 int olamgpu_prog_wrtv_rk(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->prog_wrtv_rk(); }); }
 int olamgpu_timeavg_momsc(olamgpu_ctx *c) { return guarded(c, [&] { c->impl->timeavg_momsc(); }); }

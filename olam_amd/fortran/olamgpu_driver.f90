@@ -7,7 +7,8 @@
 !   <dir>/params.txt  one line: precision num_scalar nrk_wrtv nrk_scal thil_order
 !                     wind_order scal_order iscal_monot divh_level vort_level
 !                     divh_fact vort_fact dtsm dtlm nacoust adjoint_tape fcoriol nsteps
-!                     ithil_monot iwind_monot
+!                     ithil_monot iwind_monot initial, then (zmin fact expon) of
+!                     rayf, rayfw, rayfdiv, rayfmix
 !   <dir>/fields.txt  one line per array: group name bytes, group = mesh | state | scalar
 !                     (mesh arrays also carry mza mwa mva mma in params.txt's mesh line)
 !   <dir>/<name>.bin  raw bytes of each array in Fortran layout
@@ -45,7 +46,10 @@ program olamgpu_fdriver
        prm%thil_horiz_adv_order, prm%wind_horiz_adv_order, prm%scal_horiz_adv_order, &
        prm%iscal_monot, prm%divh_damp_level, prm%vort_damp_level, prm%divh_damp_fact, &
        prm%vort_damp_fact, prm%dtsm, prm%dtlm, prm%nacoust, prm%adjoint_tape, prm%fcoriol, nsteps, &
-       prm%ithil_monot, prm%iwind_monot
+       prm%ithil_monot, prm%iwind_monot, prm%initial, &
+       prm%rayf_zmin, prm%rayf_fact, prm%rayf_expon, prm%rayfw_zmin, prm%rayfw_fact, prm%rayfw_expon, &
+       prm%rayfdiv_zmin, prm%rayfdiv_fact, prm%rayfdiv_expon, &
+       prm%rayfmix_zmin, prm%rayfmix_fact, prm%rayfmix_expon
   read(u, *) sizes
   close(u)
   mesh%mza = sizes(1); mesh%mwa = sizes(2); mesh%mva = sizes(3); mesh%mma = sizes(4)
@@ -71,8 +75,10 @@ program olamgpu_fdriver
   close(u)
 
   call check(olamgpu_create(mesh, prm, 0_c_int32_t, ctx), 'create')
-  call check(olamgpu_init_wrtv_rk(ctx), 'init_wrtv_rk')
+  ! the initial state first: init_wrtv_rk keeps its vc for Rayleigh friction, as
+  ! rayf_init does after initialization (olam_run.f90:555,566)
   call check(olamgpu_upload_state(ctx, st), 'upload_state')
+  call check(olamgpu_init_wrtv_rk(ctx), 'init_wrtv_rk')
   n = 0
   do i = 1, nbuf
      if (trim(bufs(i)%group) == 'scalar') then
@@ -199,6 +205,10 @@ contains
     case ('xx0_vu'); m%xx0_vu = p
     case ('xy0_vu'); m%xy0_vu = p
     case ('yy0_vu'); m%yy0_vu = p
+    case ('zm'); m%zm = p
+    case ('zt'); m%zt = p
+    case ('th01d'); m%th01d = p
+    case ('dn01d'); m%dn01d = p
     case default
        print '(a,a)', 'unknown mesh array ', nm
        error stop 2

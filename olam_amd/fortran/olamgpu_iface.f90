@@ -44,6 +44,9 @@ module olamgpu_iface
      type(c_ptr) :: xy_h = c_null_ptr, xy_hu = c_null_ptr
      type(c_ptr) :: xx0_v = c_null_ptr, xy0_v = c_null_ptr, yy0_v = c_null_ptr
      type(c_ptr) :: xx0_vu = c_null_ptr, xy0_vu = c_null_ptr, yy0_vu = c_null_ptr
+     ! Rayleigh friction (mem_rayf.f90:41-173): zm, zt (mza) of mem_grid; the reference
+     ! sounding th01d, dn01d (mza) of misc_coms
+     type(c_ptr) :: zm = c_null_ptr, zt = c_null_ptr, th01d = c_null_ptr, dn01d = c_null_ptr
   end type olamgpu_mesh_t
 
   type, bind(C), public :: olamgpu_params_t
@@ -60,6 +63,12 @@ module olamgpu_iface
      integer(c_int32_t) :: adjoint_tape = 0
      real(c_double)     :: fcoriol = 0.0d0
      integer(c_int32_t) :: ithil_monot = 0, iwind_monot = 0
+     ! Rayleigh friction namelist values (oname_coms.f90:224-238) and misc_coms initial
+     integer(c_int32_t) :: initial = 1
+     real(c_float)      :: rayf_zmin = 30000.0, rayf_fact = 0.0, rayf_expon = 1.0
+     real(c_float)      :: rayfw_zmin = 30000.0, rayfw_fact = 0.0, rayfw_expon = 1.0
+     real(c_float)      :: rayfdiv_zmin = 30000.0, rayfdiv_fact = 0.0, rayfdiv_expon = 1.0
+     real(c_float)      :: rayfmix_zmin = 30000.0, rayfmix_fact = 0.0, rayfmix_expon = 1.0
   end type olamgpu_params_t
 
   type, bind(C), public :: olamgpu_state_t
@@ -76,6 +85,7 @@ module olamgpu_iface
   public :: olamgpu_create, olamgpu_destroy, olamgpu_last_error, olamgpu_upload_state, &
        olamgpu_download_state, olamgpu_upload_scalar, olamgpu_download_scalar, &
        olamgpu_init_wrtv_rk, olamgpu_tend0, olamgpu_prog_wrtv_rk, olamgpu_timeavg_momsc, &
+       olamgpu_comp_alpha_press, &
        olamgpu_scalar_transport_rk, olamgpu_scalar_transport_rk_adj, &
        olamgpu_upload_scalar_adj, olamgpu_download_scalar_adj, olamgpu_mass_sums, &
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
@@ -154,6 +164,13 @@ module olamgpu_iface
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function olamgpu_timeavg_momsc
+
+     ! comp_alpha_press (timestep.f90:500-559): c_loc of real(4) (mza, mwa) rr_v, rr_w, theta
+     integer(c_int) function olamgpu_comp_alpha_press(ctx, rr_v, rr_w, theta) &
+          bind(C, name="olamgpu_comp_alpha_press")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, rr_v, rr_w, theta
+     end function olamgpu_comp_alpha_press
 
      ! rho_old: c_loc of a real(4) (mza, mwa) array, or c_null_ptr for the copy saved by tend0
      integer(c_int) function olamgpu_scalar_transport_rk(ctx, rho_old) &

@@ -39,6 +39,7 @@ def lib():
                      "olamgpu_scalar_transport_rk_adj"):
             getattr(L, name).argtypes = [vp]
         L.olamgpu_upload_state.argtypes = [vp, vp]
+        L.olamgpu_comp_alpha_press.argtypes = [vp, vp, vp, vp]
         L.olamgpu_download_state.argtypes = [vp, vp]
         L.olamgpu_upload_scalar.argtypes = [vp, C.c_int32, vp, vp]
         L.olamgpu_download_scalar.argtypes = [vp, C.c_int32, vp]
@@ -65,7 +66,8 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_scalar_transport_rk_adj", "olamgpu_upload_scalar_adj", "olamgpu_download_scalar_adj",
            "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
-           "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef"]
+           "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef",
+           "olamgpu_comp_alpha_press"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
@@ -148,6 +150,14 @@ class Context:
 
     def timeavg_momsc(self):
         self._chk(lib().olamgpu_timeavg_momsc(self.h), "timeavg_momsc")
+
+    def comp_alpha_press(self, rr_v, rr_w, theta):
+        """Moist alpha_press, pwfac, pvfac from (mwa, mza) FP32 rr_v, rr_w, theta."""
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (rr_v, rr_w, theta)]
+        for x in a:
+            if x.shape != (self.grid.mwa, self.grid.mza):
+                raise ValueError("rr_v, rr_w and theta must be (mwa, mza)")
+        self._chk(lib().olamgpu_comp_alpha_press(self.h, *(_p(x) for x in a)), "comp_alpha_press")
 
     def scalar_transport_rk(self, rho_old=None):
         r = None if rho_old is None else np.ascontiguousarray(rho_old, dtype=np.float32)

@@ -18,8 +18,8 @@
 // column-major, 1-based, dummy index 1.
 //
 // Restricted to flat terrain (lpw = lpv = lpm = 2), horizontal orders 1, 2 and 3,
-// vertical order 3 (iorderv = 3), no Rayleigh friction / nudging / eddy diffusion
-// (their switches are off in every configuration exercised here).
+// vertical order 3 (iorderv = 3), the global domain (mdomain /= 4), no nudging or
+// eddy diffusion (their switches are off in every configuration exercised here).
 
 #include <cmath>
 #include <cstdint>
@@ -64,6 +64,8 @@ struct Prec { using R = R_; using RP = RP_; using RT = RT_; };
 // prog_wrtv_rk.f90:691, evaluated without a math-library dependent last bit)
 inline float pow_r(float x, float y) { return (float)std::pow((double)x, (double)y); }
 inline double pow_r(double x, double y) { return std::pow(x, y); }
+// Fortran real**integer: repeated products in the type of x
+template <class T> T powi(T x, int n) { T r = T(1); for (int i = 0; i < n; ++i) r = r * x; return r; }
 
 // tridiffo, tridiag_lib.f90:10-36: Thomas algorithm on rows ka..kz (arrays indexed by
 // row), cim1 = sub-diagonal, ci = diagonal, cip1 = super-diagonal.
@@ -88,6 +90,8 @@ struct Base {
   virtual int download_scalar(int, float *) = 0;
   virtual int upload_adj(int, const float *) = 0;
   virtual int download_adj(int, float *) = 0;
+  virtual void init_wrtv_rk() = 0;
+  virtual void comp_alpha_press(const float *, const float *, const float *) = 0;
   virtual void tend0() = 0;
   virtual void prog_wrtv_rk() = 0;
   virtual void timeavg_momsc() = 0;
@@ -100,6 +104,8 @@ struct Base {
   virtual int tape_scales(int, int, double *) = 0;
   virtual int recon(const float *, int, int, int, double *, double *) = 0;
   virtual double rhothil_sum() = 0;
+  virtual void rayf_profiles(float *, int32_t *) = 0;
+  virtual int setup_tendencies(double *) = 0;
   std::string err;
 };
 
@@ -136,9 +142,14 @@ struct Model : Base {
   std::vector<F2<R>> tape_sv, tape_sw;      // per V face / per W level
   std::vector<char> tape_lim;               // 0 none, 1 monotone, 2 positive definite
   bool frozen_lim = false;                  // test hook: replay the taped scales
+  // ---- Rayleigh friction (mem_rayf.f90:7-37, prog_wrtv_rk.f90:7), FP32 as there ----
+  bool dorayf = false, dorayfw = false, dorayfdiv = false, dorayfmix = false;
+  int krayf_bot = 0, krayfw_bot = 0, krayfdiv_bot = 0, krayfmix_bot = 0;
+  std::vector<float> zm, zt, th01d, dn01d, rayf_cof, rayf_cofw, rayf_cofmix, b20;
+  F2<R> vc03d;
   // ---- SAVEd damping state (vort_div_damp.f90:23-33,255-262) ----
   bool div_first = true, vort_first = true;
-  F2<R> div2d[2], vort2d[2], fdivh;
+  F2<R> div2d[2], vort2d[2], fdivh, fdivl;
   std::vector<R> fvorth, fvortl;
   F2<R> vfact;
   // ---- last tracer step (for the adjoint) ----
@@ -214,6 +225,117 @@ struct Model : Base {
     for (int n = 0; n < ns; ++n) { scp[n].alloc(mza, mwa); sct[n].alloc(mza, mwa); lam[n].alloc(mza, mwa); }
     pdef.assign(ns, 1);
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
+    if (m->zm && m->zt) {   // oracle_create checks that Rayleigh friction has them
+      load1(zm, m->zm, mza);
+      load1(zt, m->zt, mza);
+      if (m->th01d) load1(th01d, m->th01d, mza);
+      if (m->dn01d) load1(dn01d, m->dn01d, mza);
+      rayf_init();
+    } else {
+      b20.assign(mza + 1, 1.f);
+    }
+  }
+
+  // rayf_init, mem_rayf.f90:41-156: friction profiles from the namelist values, and
+  // init_wrtv_rk's b20 = 1 + dtsm * rayf_cofw (prog_wrtv_rk.f90:46-52)
+  void rayf_init() {
+    const float dti = 1.0f / (float)prm.dtsm;
+    rayf_cof.assign(mza + 1, 0.f); rayf_cofw.assign(mza + 1, 0.f); rayf_cofmix.assign(mza + 1, 0.f);
+    b20.assign(mza + 1, 1.f);
+    krayf_bot = krayfw_bot = krayfdiv_bot = krayfmix_bot = mza + 1;
+    if (prm.rayf_fact > 1.e-7f && prm.initial == 1) {
+      for (int k = 2; k <= mza; ++k)
+        if (zt[k] > prm.rayf_zmin) { krayf_bot = k; break; }
+      if (krayf_bot <= mza) {
+        dorayf = true;
+        float distimi = prm.rayf_fact * dti;
+        for (int k = krayf_bot; k <= mza; ++k)
+          rayf_cof[k] = distimi * pow_r((zt[k] - prm.rayf_zmin) / (zm[mza] - prm.rayf_zmin), prm.rayf_expon);
+      }
+    }
+    if (prm.rayfw_fact > 1.e-7f) {
+      for (int k = 2; k <= mza - 1; ++k)
+        if (zm[k] > prm.rayfw_zmin) { krayfw_bot = k; break; }
+      if (krayfw_bot < mza) {
+        dorayfw = true;
+        float distimi = prm.rayfw_fact * dti;
+        for (int k = krayfw_bot; k <= mza - 1; ++k)
+          rayf_cofw[k] = distimi * pow_r((zm[k] - prm.rayfw_zmin) / (zm[mza] - prm.rayfw_zmin), prm.rayfw_expon);
+      }
+    }
+    if (prm.rayfdiv_fact > 1.e-7f) {
+      for (int k = 2; k <= mza; ++k)
+        if (zt[k] > prm.rayfdiv_zmin) { krayfdiv_bot = k; break; }
+      if (krayfdiv_bot <= mza) dorayfdiv = true;
+    }
+    if (prm.rayfmix_fact > 1.e-7f) {
+      for (int k = 2; k <= mza - 1; ++k)
+        if (zm[k] > prm.rayfmix_zmin) { krayfmix_bot = k; break; }
+      if (krayfmix_bot < mza) {
+        dorayfmix = true;
+        for (int k = krayfmix_bot; k <= mza - 1; ++k)   // normalised by zm(mza) - rayfw_zmin
+          rayf_cofmix[k] = 0.5f * prm.rayfmix_fact *
+                           pow_r((zm[k] - prm.rayfmix_zmin) / (zm[mza] - prm.rayfw_zmin), prm.rayfmix_expon);
+      }
+    }
+    if (dorayfw)
+      for (int k = krayfw_bot; k <= mza - 1; ++k) b20[k] = 1.0f + (float)prm.dtsm * rayf_cofw[k];
+  }
+
+  // comp_alpha_press, timestep.f90:500-559 (called when miclevel > 0).  rr_v, rr_w and
+  // theta are host (mza, mwa) FP32 arrays; zwgt_top/zwgt_bot are the FP32 roundings of
+  // zwgt_top8/zwgt_bot8 (mem_grid.f90:432-433).
+  void comp_alpha_press(const float *rr_v_h, const float *rr_w_h, const float *theta_h) override {
+    F2<R> rr_v, rr_w, theta;
+    load2(rr_v, rr_v_h, mza, mwa); load2(rr_w, rr_w_h, mza, mwa); load2(theta, theta_h, mza, mwa);
+    const R pc1 = sizeof(R) == 4 ? (R)pow_r(1.0f / 1.0e5f, 287.0f / 717.0f) : (R)std::pow(1.0 / 1.0e5, 287.0 / 717.0);
+    const R rdry = R(287), rvap = R(461), cpocv = (R)(1004.0 / 717.0);
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      for (int k = 2; k <= mza; ++k)
+        alpha_press(k, iw) = pc1 * pow_r((rdry + rvap * rr_v(k, iw)) * theta(k, iw) / thil(k, iw), cpocv);
+      for (int k = 2; k <= mza - 1; ++k) {
+        R rw = (R)zwgt_top8[k] * rr_w(k + 1, iw) + (R)zwgt_bot8[k] * rr_w(k, iw);
+        pwfac(k, iw) = dzim[k] * (R(1) - rw + rw * rw);
+      }
+    }
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      for (int k = 2; k <= mza; ++k) {
+        R rw = R(0.5) * (rr_w(k, iw1) + rr_w(k, iw2));
+        pvfac(k, iv) = dniv[iv] * zfacit[k] * (R(1) - rw + rw * rw);
+      }
+    }
+  }
+
+  // rayf_init's copy of the initial wind (mem_rayf.f90:158-171), taken where
+  // olam_run.f90:555 calls it: after the initial state exists
+  void init_wrtv_rk() override {
+    if (!dorayf) return;
+    vc03d.alloc(mza, mva);
+    for (int iv = 2; iv <= mva; ++iv)
+      for (int k = 2; k <= mza; ++k) vc03d(k, iv) = vc(k, iv);
+  }
+
+  // rayf_mix_top_vxe, mem_rayf.f90:177-220 (vmxet.. are the volume-weighted short-step
+  // tendencies of column iw)
+  void rayf_mix_top_vxe(int iw, R *vmxet_c, R *vmyet_c, R *vmzet_c) const {
+    R vxflux[130], vyflux[130], vzflux[130];
+    int ka = std::max(krayfmix_bot, 2);
+    vxflux[mza] = vyflux[mza] = vzflux[mza] = 0;
+    vxflux[ka - 1] = vyflux[ka - 1] = vzflux[ka - 1] = 0;
+    for (int k = ka; k <= mza - 1; ++k) {
+      R fact = arw0[iw] * (R)rayf_cofmix[k] * (R)(rho(k + 1, iw) + rho(k, iw));
+      vxflux[k] = fact * (vxe(k, iw) - vxe(k + 1, iw));
+      vyflux[k] = fact * (vye(k, iw) - vye(k + 1, iw));
+      vzflux[k] = fact * (vze(k, iw) - vze(k + 1, iw));
+    }
+    for (int k = ka; k <= mza; ++k) {
+      vmxet_c[k] = vmxet_c[k] + vxflux[k - 1] - vxflux[k];
+      vmyet_c[k] = vmyet_c[k] + vyflux[k - 1] - vyflux[k];
+      vmzet_c[k] = vmzet_c[k] + vzflux[k - 1] - vzflux[k];
+    }
   }
   void set_frozen(int v) override { frozen_lim = v != 0; }
   // test hook: sum of rho * thil * volt over the prognostic cells at full precision
@@ -222,6 +344,25 @@ struct Model : Base {
     for (int iw = 2; iw <= mwa; ++iw)
       for (int k = 2; k <= mza; ++k) t += (double)rho(k, iw) * (double)thil(k, iw) * volt(k, iw);
     return t;
+  }
+  // test hook: the Rayleigh friction profiles per level k = 1..mza (rayf_cof, rayf_cofw,
+  // rayf_cofmix, b20) and krayf_bot, krayfw_bot, krayfdiv_bot, krayfmix_bot
+  void rayf_profiles(float *out, int32_t *kb) override {
+    const std::vector<float> *a[4] = {&rayf_cof, &rayf_cofw, &rayf_cofmix, &b20};
+    for (int i = 0; i < 4; ++i)
+      for (int k = 1; k <= mza; ++k) out[(size_t)i * mza + k - 1] = (*a[i]).empty() ? (i == 3 ? 1.f : 0.f) : (*a[i])[k];
+    kb[0] = krayf_bot; kb[1] = krayfw_bot; kb[2] = krayfdiv_bot; kb[3] = krayfmix_bot;
+  }
+  // test hook: run the setup part of prog_wrtv_rk (which advances the SAVEd divergence
+  // of the damping) and return thilt_short, vmxet_short, vmyet_short, vmzet_short
+  // (mza, mwa each) and vmt_short (mza, mva) in Fortran layout
+  int setup_tendencies(double *out) override {
+    if (dorayf && !vc03d.n) return OLAMGPU_EINVAL;
+    setup_short();
+    size_t o = 0;
+    for (const F2<R> *a : {&thilt_s, &vmxet_s, &vmyet_s, &vmzet_s, &vmt_s})
+      for (size_t i = 0; i < a->v.size(); ++i) out[o++] = (double)a->v[i];
+    return 0;
   }
   // test hook: reconstruction coefficients of a field (5, mwa, mza) and its face values
   // on both sides of every face (2, mva, mza), for one order and coefficient/offset set
@@ -591,53 +732,72 @@ struct Model : Base {
   }
 
   // ---------------------------------------------------------------- damping
-  // divh_damp, vort_div_damp.f90:1-230 (dorayfdiv = .false.)
+  // divh_damp, vort_div_damp.f90:1-230
   void divh_damp(F2<R> &vmtx, double dtm) {
-    int lev = prm.divh_damp_level;
-    bool dodiv = (prm.divh_damp_fact > 1.e-7f) && lev > 0;
-    if (!dodiv) return;
+    const int lev = prm.divh_damp_level;
+    const bool dodiv = (prm.divh_damp_fact > 1.e-7f) && lev > 0;
+    if (!dorayfdiv && !dodiv) return;
+    const bool dol = dorayfdiv || (dodiv && lev == 1), doh = dodiv && lev > 1;
     if (div_first) {
       div_first = false;
       div2d[0].alloc(mza, mwa); div2d[1].alloc(mza, mwa);
-      fdivh.alloc(mza, mva);
-      if (lev > 1) {
-        double facth = -std::pow(-(double)prm.divh_damp_fact, lev) / dtm;
-        for (int iv = 2; iv <= mva; ++iv) {
-          int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
-          double factb = facth * std::pow((double)std::min(arw0[iw1], arw0[iw2]), lev - 1);
-          for (int k = 2; k <= mza; ++k)
-            fdivh(k, iv) = (R)(factb * (double)(dniv[iv] * zfacit[k] * dzit[k]) *
+      // Rayleigh friction divergence damping profile, :60-66
+      std::vector<R> zf(mza + 1, R(0));
+      if (dorayfdiv)
+        for (int k = krayfdiv_bot; k <= mza; ++k)
+          zf[k] = (R)pow_r((zt[k] - prm.rayfdiv_zmin) / (zm[mza] - prm.rayfdiv_zmin), prm.rayfdiv_expon);
+      if (dol) fdivl.alloc(mza, mva);
+      if (doh) fdivh.alloc(mza, mva);
+      const double facth = doh ? -(double)powi(-(R)prm.divh_damp_fact, lev) / dtm : 0.0;
+      const R fact2 = (dodiv && lev == 1) ? (R)(prm.divh_damp_fact / (R)dtm) : R(0);
+      R factr = R(0);
+      if (dorayfdiv && dodiv) factr = (R)(std::max(prm.divh_damp_fact, prm.rayfdiv_fact) / (R)dtm);
+      else if (dorayfdiv) factr = (R)(prm.rayfdiv_fact / (R)dtm);
+      for (int iv = 2; iv <= mva; ++iv) {
+        int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+        if (dol) {
+          int ka = (dodiv && lev == 1) ? 2 : krayfdiv_bot;
+          for (int k = ka; k <= mza; ++k)
+            fdivl(k, iv) = (R)((double)((fact2 * (R(1) - zf[k]) + factr * zf[k]) * dniv[iv] * zfacit[k] * dzit[k]) *
                                std::min(volt(k, iw1), volt(k, iw2)));
         }
-      } else {
-        // level 1: fdivl (vort_div_damp.f90:97-103) with zf = 0
-        R fact2 = (R)(prm.divh_damp_fact / (R)dtm);
-        for (int iv = 2; iv <= mva; ++iv) {
-          int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+        if (doh) {
+          double factb = facth * (double)powi(std::min(arw0[iw1], arw0[iw2]), lev - 1);
           for (int k = 2; k <= mza; ++k)
-            fdivh(k, iv) = (R)((double)(fact2 * dniv[iv] * zfacit[k] * dzit[k]) *
-                               std::min(volt(k, iw1), volt(k, iw2)));
+            fdivh(k, iv) = (R)(factb * (double)dniv[iv] * (double)zfacit[k] * (double)dzit[k] *
+                               std::min(volt(k, iw1), volt(k, iw2)) * (double)(R(1) - zf[k]));
         }
       }
     }
     // horizontal divergence (accumulates onto the saved div2d(:,:,1), :140-156)
+    const int kd = dodiv ? 2 : std::max(2, krayfdiv_bot);
 #pragma omp parallel for
     for (int iw = 2; iw <= mwa; ++iw) {
       for (int j = 1; j <= npoly[iw]; ++j) {
         int iv = w_iv(j, iw);
-        for (int k = 2; k <= mza; ++k)
+        for (int k = kd; k <= mza; ++k)
           div2d[0](k, iw) = div2d[0](k, iw) - arv(k, iv) * dirv(j, iw) * vmc(k, iv);
       }
-      for (int k = 2; k <= mza; ++k) div2d[0](k, iw) = div2d[0](k, iw) * volti(k, iw);
+      for (int k = kd; k <= mza; ++k) div2d[0](k, iw) = div2d[0](k, iw) * volti(k, iw);
     }
-    int ih = 0;
-    if (lev > 1) {
-      for (int nn = 1; nn <= lev - 1; ++nn) {
-        int il = (nn - 1) % 2;
-        ih = nn % 2;
+    // second-order (and Rayleigh) damping, :168-185
+    if (dol) {
+      const int ka = (dodiv && lev == 1) ? 2 : krayfdiv_bot;
 #pragma omp parallel for
-        for (int iw = 2; iw <= mwa; ++iw) laplacian2d(iw, div2d[il], div2d[ih].col(iw) - 1);
+      for (int iv = 2; iv <= mva; ++iv) {
+        int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+        for (int k = ka; k <= mza; ++k)
+          vmtx(k, iv) = vmtx(k, iv) + fdivl(k, iv) * (div2d[0](k, iw2) - div2d[0](k, iw1));
       }
+    }
+    if (!doh) return;
+    // Laplacians of the divergence, :189-207, and the higher-order tendency, :211-226
+    int ih = 0;
+    for (int nn = 1; nn <= lev - 1; ++nn) {
+      int il = (nn - 1) % 2;
+      ih = nn % 2;
+#pragma omp parallel for
+      for (int iw = 2; iw <= mwa; ++iw) laplacian2d(iw, div2d[il], div2d[ih].col(iw) - 1);
     }
 #pragma omp parallel for
     for (int iv = 2; iv <= mva; ++iv) {
@@ -664,9 +824,9 @@ struct Model : Base {
       for (int iv = 2; iv <= mva; ++iv) {
         int im1 = v_im(1, iv), im2 = v_im(2, iv);
         R a = std::min(arm0[im1], arm0[im2]);
-        fvorth[iv] = R(0.5) * dniu[iv] / (R)dtm * (R)std::pow((double)(-prm.vort_damp_fact * a), lev);
+        fvorth[iv] = R(0.5) * dniu[iv] / (R)dtm * powi(-(R)prm.vort_damp_fact * a, lev);
         if (lev > 2)
-          fvortl[iv] = R(0.5) * dniu[iv] / (R)dtm * (R)std::pow((double)(-prm.vort_damp_fact * a), lev - 1);
+          fvortl[iv] = R(0.5) * dniu[iv] / (R)dtm * powi(-(R)prm.vort_damp_fact * a, lev - 1);
       }
     }
 #pragma omp parallel for
@@ -718,10 +878,13 @@ struct Model : Base {
     }
   }
 
-  void prog_wrtv_rk() override {
+  // the part of prog_wrtv_rk before the RK stages (:165-308): RK start values, the
+  // volume-weighted short-step tendencies with the Rayleigh terms, vmc0/vmt_short and
+  // the damping tendencies
+  void setup_short() {
     alloc_wrtv_scratch();
     const double dtsm = prm.dtsm;
-    // setup loop, prog_wrtv_rk.f90:165-253 (no eddy diffusion / Rayleigh)
+    // setup loop, prog_wrtv_rk.f90:165-253 (no eddy diffusion)
 #pragma omp parallel for
     for (int iw = 2; iw <= mwa; ++iw) {
       thil_upw(mza, iw) = 0; thil_upw(1, iw) = 0;
@@ -735,6 +898,13 @@ struct Model : Base {
         vmyet_s(k, iw) = vmyet(k, iw) * v4;
         vmzet_s(k, iw) = vmzet(k, iw) * v4;
       }
+      // Rayleigh friction on the velocity gradient at the top and on thil, :219-233
+      if (dorayfmix) rayf_mix_top_vxe(iw, vmxet_s.col(iw) - 1, vmyet_s.col(iw) - 1, vmzet_s.col(iw) - 1);
+      if (dorayf && prm.initial == 1)
+        for (int k = krayf_bot; k <= mza; ++k) {
+          R v4 = (R)volt(k, iw);
+          thilt_s(k, iw) = thilt_s(k, iw) + (R)rayf_cof[k] * (R)dn01d[k] * ((R)th01d[k] - thil(k, iw)) * v4;
+        }
       if (prm.ithil_monot > 0)
         for (int k = 2; k <= mza; ++k) thl0(k, iw) = thil(k, iw);
       if (prm.iwind_monot > 0)
@@ -743,10 +913,25 @@ struct Model : Base {
 #pragma omp parallel for
     for (int iv = 2; iv <= mva; ++iv)
       for (int k = 1; k <= mza; ++k) { vmc0(k, iv) = vmc(k, iv); vmt_s(k, iv) = vmt(k, iv); }
+    // Rayleigh friction on vmc toward vc03d, :268-284 (mdomain /= 4)
+    if (dorayf) {
+#pragma omp parallel for
+      for (int iv = 2; iv <= mva; ++iv) {
+        int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+        for (int k = krayf_bot; k <= mza; ++k)
+          vmt_s(k, iv) = (R)((RP)vmt_s(k, iv) + (RP)((R)rayf_cof[k] * R(0.5)) * (rho(k, iw1) + rho(k, iw2)) *
+                                                    (RP)(vc03d(k, iv) - vc(k, iv)));
+      }
+    }
     // short-step damping, :307-308
     divh_damp(vmt_s, dtsm);
     vort_damp(vmt_s, dtsm);
+  }
 
+  void prog_wrtv_rk() override {
+    if (dorayf && !vc03d.n) { err = "Rayleigh friction on thil/vmc needs init_wrtv_rk"; return; }
+    setup_short();
+    const double dtsm = prm.dtsm;
     const double onethird = 1.0 / 3.0;
     for (int istage = 1; istage <= prm.nrk_wrtv; ++istage) {
       double dt8 = (prm.nrk_wrtv <= 2) ? (istage == 1 ? 0.5 : 1.0) * dtsm
@@ -787,7 +972,7 @@ struct Model : Base {
     }
   }
 
-  // prog_wrt_begs, prog_wrtv_rk.f90:494-884 (global domain, no nudging, b20 = 1)
+  // prog_wrt_begs, prog_wrtv_rk.f90:494-884 (global domain, no nudging)
   void prog_wrt_begs(int iw, int istage, double dt8) {
     const RP fr = RP(0.70), fp2 = RP(0.70), fp3 = RP(0.70);
     const R cpocv = (R)(1004.0 / 717.0), rocv = (R)(287.0 / 717.0);
@@ -866,7 +1051,7 @@ struct Model : Base {
       R b25 = c9 * volti(k, iw) * gdz_wgtm[k];
       R b26 = c9 * volti(k + 1, iw) * gdz_wgtp[k];
       b31[k] = (RT)(-arw(k - 1, iw) * (b21 + b25 + b23 * thil_upw(k - 1, iw)));
-      b32[k] = (RT)(R(1) + arw(k, iw) * (b25 - b26 + (b23 + b24) * thil_upw(k, iw)));
+      b32[k] = (RT)((R)b20[k] + arw(k, iw) * (b25 - b26 + (b23 + b24) * thil_upw(k, iw)));
       b33[k] = (RT)(arw(k + 1, iw) * (b22 + b26 - b24 * thil_upw(k + 1, iw)));
       rhs[k] = (RT)delex_wm[k];
     }
@@ -1636,6 +1821,10 @@ int oracle_create(const olamgpu_mesh *mesh, const olamgpu_params *p, int32_t ref
   const bool q3 = p->thil_horiz_adv_order == 3 || p->wind_horiz_adv_order == 3 || p->scal_horiz_adv_order == 3;
   if (q3 && !(mesh->xy_h && mesh->xy_hu && mesh->xx0_v && mesh->xy0_v && mesh->yy0_v && mesh->xx0_vu &&
               mesh->xy0_vu && mesh->yy0_vu)) { delete c; return OLAMGPU_EINVAL; }
+  const bool rayf = p->rayf_fact > 1.e-7f || p->rayfw_fact > 1.e-7f || p->rayfdiv_fact > 1.e-7f ||
+                    p->rayfmix_fact > 1.e-7f;
+  if (rayf && !(mesh->zm && mesh->zt)) { delete c; return OLAMGPU_EINVAL; }
+  if (p->rayf_fact > 1.e-7f && p->initial == 1 && !(mesh->th01d && mesh->dn01d)) { delete c; return OLAMGPU_EINVAL; }
   if (ref_types) c->m = new Model<Prec<float, double, float>>(mesh, p);
   else if (p->precision == OLAMGPU_PREC_F64) c->m = new Model<Prec<double, double, double>>(mesh, p);
   else if (p->precision == OLAMGPU_PREC_MIXED) c->m = new Model<Prec<float, double, double>>(mesh, p);
@@ -1651,7 +1840,19 @@ int oracle_download_scalar(void *h, int32_t n, float *p) { return ((oracle_ctx *
 int oracle_upload_scalar_adj(void *h, int32_t n, const float *p) { return ((oracle_ctx *)h)->m->upload_adj(n, p); }
 int oracle_download_scalar_adj(void *h, int32_t n, float *p) { return ((oracle_ctx *)h)->m->download_adj(n, p); }
 int oracle_tend0(void *h) { ((oracle_ctx *)h)->m->tend0(); return 0; }
-int oracle_prog_wrtv_rk(void *h) { ((oracle_ctx *)h)->m->prog_wrtv_rk(); return 0; }
+int oracle_rayf_profiles(void *h, float *out, int32_t *kb) { ((oracle_ctx *)h)->m->rayf_profiles(out, kb); return 0; }
+int oracle_setup_tendencies(void *h, double *out) { return ((oracle_ctx *)h)->m->setup_tendencies(out); }
+int oracle_comp_alpha_press(void *h, const float *rr_v, const float *rr_w, const float *theta) {
+  ((oracle_ctx *)h)->m->comp_alpha_press(rr_v, rr_w, theta);
+  return 0;
+}
+int oracle_init_wrtv_rk(void *h) { ((oracle_ctx *)h)->m->init_wrtv_rk(); return 0; }
+int oracle_prog_wrtv_rk(void *h) {
+  Base *m = ((oracle_ctx *)h)->m;
+  m->err.clear();
+  m->prog_wrtv_rk();
+  return m->err.empty() ? 0 : OLAMGPU_EINVAL;
+}
 int oracle_timeavg_momsc(void *h) { ((oracle_ctx *)h)->m->timeavg_momsc(); return 0; }
 int oracle_scalar_transport_rk(void *h, const float *rho_old) { ((oracle_ctx *)h)->m->scalar_transport_rk(rho_old); return 0; }
 int oracle_scalar_transport_rk_adj(void *h) {

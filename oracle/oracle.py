@@ -36,7 +36,7 @@ def lib():
         vp = C.c_void_p
         L.oracle_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]
         for name in ("oracle_destroy", "oracle_tend0", "oracle_prog_wrtv_rk", "oracle_timeavg_momsc",
-                     "oracle_scalar_transport_rk_adj"):
+                     "oracle_scalar_transport_rk_adj", "oracle_init_wrtv_rk"):
             getattr(L, name).argtypes = [vp]
         L.oracle_upload_state.argtypes = [vp, vp]
         L.oracle_download_state.argtypes = [vp, vp]
@@ -54,6 +54,9 @@ def lib():
         L.oracle_tape_scales.argtypes = [vp, C.c_int32, C.c_int32, vp]
         L.oracle_recon.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
         L.oracle_rhothil_sum.argtypes = [vp, vp]
+        L.oracle_rayf_profiles.argtypes = [vp, vp, vp]
+        L.oracle_comp_alpha_press.argtypes = [vp, vp, vp, vp]
+        L.oracle_setup_tendencies.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -108,11 +111,45 @@ class Oracle:
         lib().oracle_download_scalar_adj(self.h, n, _p(out))
         return out
 
+    def init_wrtv_rk(self):
+        lib().oracle_init_wrtv_rk(self.h)
+
+    def comp_alpha_press(self, rr_v, rr_w, theta):
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (rr_v, rr_w, theta)]
+        lib().oracle_comp_alpha_press(self.h, *(_p(x) for x in a))
+
+    def rayf_profiles(self):
+        """Rayleigh friction profiles per level k = 1..mza: dict of rayf_cof, rayf_cofw,
+        rayf_cofmix, b20 (float32 arrays) and the four k*_bot levels."""
+        mza = self.grid.mza
+        out = np.zeros(4 * mza, np.float32)
+        kb = np.zeros(4, np.int32)
+        lib().oracle_rayf_profiles(self.h, _p(out), _p(kb))
+        d = {n: out[i * mza:(i + 1) * mza].copy() for i, n in enumerate(("rayf_cof", "rayf_cofw", "rayf_cofmix", "b20"))}
+        d.update(zip(("krayf_bot", "krayfw_bot", "krayfdiv_bot", "krayfmix_bot"), (int(x) for x in kb)))
+        return d
+
+    def setup_tendencies(self):
+        """Runs the setup part of prog_wrtv_rk; returns thilt_short, vmxet_short,
+        vmyet_short, vmzet_short ((mwa, mza) each) and vmt_short ((mva, mza)), float64."""
+        g = self.grid
+        nw, nv = g.mwa * g.mza, g.mva * g.mza
+        out = np.zeros(4 * nw + nv)
+        rc = lib().oracle_setup_tendencies(self.h, _p(out))
+        if rc:
+            raise RuntimeError(f"oracle setup_tendencies failed ({rc})")
+        names = ("thilt_s", "vmxet_s", "vmyet_s", "vmzet_s")
+        d = {n: out[i * nw:(i + 1) * nw].reshape(g.mwa, g.mza) for i, n in enumerate(names)}
+        d["vmt_s"] = out[4 * nw:].reshape(g.mva, g.mza)
+        return d
+
     def tend0(self):
         lib().oracle_tend0(self.h)
 
     def prog_wrtv_rk(self):
-        lib().oracle_prog_wrtv_rk(self.h)
+        rc = lib().oracle_prog_wrtv_rk(self.h)
+        if rc:
+            raise RuntimeError(f"oracle prog_wrtv_rk failed ({rc})")
 
     def timeavg_momsc(self):
         lib().oracle_timeavg_momsc(self.h)

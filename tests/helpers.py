@@ -77,7 +77,7 @@ class PermutedGrid:
         nw, nv, nm = grid.mwa - 1, grid.mva - 1, grid.mma - 1
         self.pw, self.pv, self.pm = rng.permutation(nw), rng.permutation(nv), rng.permutation(nm)
         src = grid.arrays
-        out = {k: v.copy() for k, v in src.items()}
+        out = {k: (None if v is None else v.copy()) for k, v in src.items()}
 
         def rows(a, p):
             b = a.copy()
@@ -107,7 +107,8 @@ class PermutedGrid:
         d = abi.MeshDesc()
         d.mza, d.mwa, d.mva, d.mma = self.mza, self.mwa, self.mva, self.mma
         for fname, ftype in abi.MeshDesc._fields_[4:]:
-            setattr(d, fname, out[fname].ctypes.data_as(abi.C.POINTER(ftype._type_)))
+            if out[fname] is not None:
+                setattr(d, fname, out[fname].ctypes.data_as(abi.C.POINTER(ftype._type_)))
         self.desc = d
 
     def fwd(self, a, kind="w"):
@@ -135,7 +136,9 @@ def write_fortran_case(path, grid, params, state, scalars, nsteps):
             p.wind_horiz_adv_order, p.scal_horiz_adv_order, p.iscal_monot, p.divh_damp_level,
             p.vort_damp_level, repr(float(p.divh_damp_fact)), repr(float(p.vort_damp_fact)),
             repr(p.dtsm), repr(p.dtlm), p.nacoust, p.adjoint_tape, repr(p.fcoriol), nsteps,
-            p.ithil_monot, p.iwind_monot)) + "\n")
+            p.ithil_monot, p.iwind_monot, p.initial,
+            *(repr(float(getattr(p, f"{kind}_{x}"))) for kind in abi.RAYF_KINDS
+              for x in ("zmin", "fact", "expon")))) + "\n")
         f.write(f"{grid.mza} {grid.mwa} {grid.mva} {grid.mma}\n")
     lines = []
 
@@ -145,7 +148,8 @@ def write_fortran_case(path, grid, params, state, scalars, nsteps):
         lines.append(f"{group} {name} {a.nbytes}")
 
     for name, a in grid.arrays.items():
-        put("mesh", name, a)
+        if a is not None:
+            put("mesh", name, a)
     for name, a in state.arrays.items():
         put("state", name, a)
     for n, q in enumerate(scalars):

@@ -1,0 +1,54 @@
+"""GPU parity of comp_alpha_press (timestep.f90:500-559) against the oracle through the
+C-ABI: the moist factors alpha_press, pwfac, pvfac themselves, then acoustic steps
+that read them.  Tolerances as in test_gpu_parity.py.
+"""
+
+import numpy as np
+import pytest
+
+from olam_amd import abi, cases, mesh as M
+from olam_amd.gpu import Context
+from helpers import DYN_FIELDS, dyn_fields, rel_linf
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_moist_factors_and_step_parity(prec):
+    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(20))
+    st = cases.jet_state(grid)
+    rng = np.random.default_rng(12)
+    shape = (grid.mwa, grid.mza)
+    rr_v = rng.uniform(0.0, 0.02, shape).astype(np.float32)
+    rr_w = (rr_v + rng.uniform(0.0, 0.005, shape)).astype(np.float32)
+    theta = (st.arrays["thil"] * (1.0 + 0.01 * rng.uniform(0.0, 1.0, shape))).astype(np.float32)
+    prm = abi.make_params(prec, num_scalar=1, dtsm=60.0)
+    print(f"moist factors {prec}", flush=True)
+    fac, dyn = {}, {}
+    for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
+        be.upload_state(st)
+        be.init_wrtv_rk()
+        be.comp_alpha_press(rr_v, rr_w, theta)
+        fac[name] = dyn_fields(be, grid)
+        be.tend0()
+        for _ in range(2):
+            be.prog_wrtv_rk()
+        dyn[name] = dyn_fields(be, grid)
+        be.close()
+    for f in ("alpha_press", "pwfac", "pvfac"):
+        e = rel_linf(fac["gpu"][f], fac["oracle"][f])
+        print(f"  {f:11s} rel Linf {e:.3e}")
+        # one FP32 ulp: the factors come back through the FP32 state download
+        assert e < 1.2e-7, (f, e)
+    # the moist factors change the step
+    assert not np.array_equal(fac["gpu"]["alpha_press"], st.arrays["alpha_press"])
+    tol_big = 1e-11 if prec == "f64" else 1e-5
+    tol_small = 5e-9 if prec == "f64" else 1e-4
+    for f in DYN_FIELDS:
+        e = rel_linf(dyn["gpu"][f], dyn["oracle"][f])
+        print(f"  {f:11s} rel Linf {e:.3e}")
+        tol = tol_small if f in ("wmc", "wc", "wmasc") else tol_big
+        if prec == "f64" and f == "wc":
+            tol = 1e-7
+        assert e < tol, (f, e)
