@@ -94,6 +94,8 @@ typedef struct olamgpu_mesh {
   const float  *zm, *zt;                     /* (mza) heights of M and T levels, mem_grid */
   const float  *th01d, *dn01d;               /* (mza) reference sounding, misc_coms.f90:112-113;
                                                 read when rayf_fact > 0 and initial == 1 */
+  const float  *dzit_bot;                    /* (mza) 1 / (zt(k) - zm(k-1)), mem_grid.f90:374-378;
+                                                read by olamgpu_set_turbulence (surface drag) */
 } olamgpu_mesh;
 
 /* Namelist options and time steps the path reads (oname_coms.f90:190-238,
@@ -129,6 +131,8 @@ typedef struct olamgpu_params {
   float   rayfw_zmin, rayfw_fact, rayfw_expon;
   float   rayfdiv_zmin, rayfdiv_fact, rayfdiv_expon;
   float   rayfmix_zmin, rayfmix_fact, rayfmix_expon;
+  float   akmin;                     /* akmin(1), oname_coms.f90:266: minimum horizontal eddy
+                                        diffusivity ratio of comp_horiz_k */
 } olamgpu_params;
 
 /* Prognostic and diagnostic fields of mem_basic/mem_tend (mem_basic.f90:8-34,
@@ -199,6 +203,19 @@ int olamgpu_timeavg_momsc(olamgpu_ctx *ctx);
  * context's thil, sets alpha_press and pwfac at W and pvfac at V on the device, the
  * factors prog_wrtv_rk reads (olamgpu_upload_state can set them instead). */
 int olamgpu_comp_alpha_press(olamgpu_ctx *ctx, const float *rr_v, const float *rr_w, const float *theta);
+
+/* The turbulence fields of the host's PBL scheme (mem_turb: eddy diffusivities vkh,
+ * vkm (mza, mwa) at W levels, the top levels khtop, kmtop (mwa) of heat and momentum
+ * diffusion, and akm_sfc (mwa), the surface drag coefficient of the one surface level
+ * of flat terrain), followed by comp_horiz_k (pbl_driver.f90:136-238) on the device.
+ * Each later prog_wrtv_rk then applies, in its setup loop, the implicit vertical eddy
+ * diffusion of solve_eddy_diff_heat and solve_eddy_diff_vxe (pbl_driver.f90:519-742)
+ * and the horizontal diffusion of prog_wrtv_rk.f90:235-251.  OLAM's pbl_driver sets
+ * these once per long step; call this where it does.  Before the first call the
+ * diffusion is off.  khtop and kmtop must lie in [0, mza - 1]; the non-local PBL term
+ * of idiffk = 1 is not part of this entry. */
+int olamgpu_set_turbulence(olamgpu_ctx *ctx, const float *vkh, const float *vkm, const int32_t *khtop,
+                           const int32_t *kmtop, const float *akm_sfc);
 
 /* scalar_transport_rk(rho_old) (scalar_transport_rk.F90:30-385): one long step of
  * all registered tracers.  rho_old = NULL uses the copy saved by olamgpu_tend0;

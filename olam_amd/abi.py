@@ -44,7 +44,7 @@ class MeshDesc(C.Structure):
         ("xy_h", _fp), ("xy_hu", _fp),
         ("xx0_v", _fp), ("xy0_v", _fp), ("yy0_v", _fp),
         ("xx0_vu", _fp), ("xy0_vu", _fp), ("yy0_vu", _fp),
-        ("zm", _fp), ("zt", _fp), ("th01d", _fp), ("dn01d", _fp),
+        ("zm", _fp), ("zt", _fp), ("th01d", _fp), ("dn01d", _fp), ("dzit_bot", _fp),
     ]
 
 
@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("rayfw_zmin", C.c_float), ("rayfw_fact", C.c_float), ("rayfw_expon", C.c_float),
         ("rayfdiv_zmin", C.c_float), ("rayfdiv_fact", C.c_float), ("rayfdiv_expon", C.c_float),
         ("rayfmix_zmin", C.c_float), ("rayfmix_fact", C.c_float), ("rayfmix_expon", C.c_float),
+        ("akmin", C.c_float),
     ]
 
 
@@ -88,7 +89,7 @@ RAYF_KINDS = ("rayf", "rayfw", "rayfdiv", "rayfmix")
 def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, iscal_monot=0,
                 divh_damp_level=3, vort_damp_level=3, divh_damp_fact=0.1, vort_damp_fact=0.075,
                 dtsm=60.0, dtlm=None, nacoust=1, adjoint_tape=1, ithil_monot=0, iwind_monot=0,
-                initial=1, rayf=None):
+                initial=1, rayf=None, akmin=0.0):
     """Namelist defaults of oname_coms.f90:190-238, with RK3 for scalars.
 
     ``rayf`` maps a Rayleigh friction kind ("rayf", "rayfw", "rayfdiv", "rayfmix") to
@@ -103,7 +104,7 @@ def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, 
         rf.extend((rayf or {}).get(kind, (30000.0, 0.0, 1.0)))
     return Params(precision, num_scalar, nrk_wrtv, nrk_scal, order, order, order, iscal_monot,
                   divh_damp_level, vort_damp_level, divh_damp_fact, vort_damp_fact, dtsm, dtlm,
-                  nacoust, adjoint_tape, 2.0 * M.OMEGA, ithil_monot, iwind_monot, initial, *rf)
+                  nacoust, adjoint_tape, 2.0 * M.OMEGA, ithil_monot, iwind_monot, initial, *rf, akmin)
 
 
 def _ptr(a, ctype):
@@ -198,6 +199,7 @@ class FortranGrid:
             keep[name] = per(g[name])
         keep["zm"] = lev(vg["zm"])
         keep["zt"] = lev(vg["zt"])
+        keep["dzit_bot"] = lev(1.0 / np.where(vg["dzt_bot"] != 0, vg["dzt_bot"], 1.0))
         keep["th01d"] = keep["dn01d"] = None     # set_sounding()
         self.arrays = keep
         d = MeshDesc()

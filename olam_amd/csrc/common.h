@@ -155,6 +155,33 @@ __device__ __forceinline__ double pcr_solve(double a, double b, double c, double
   return d / b;
 }
 
+// pcr_solve for three right-hand sides of the same matrix (the eliminations of each d are
+// those of pcr_solve, so each solution equals pcr_solve's for that d).
+__device__ __forceinline__ void pcr_solve3(double a, double b, double c, double d0, double d1, double d2,
+                                           double &x0, double &x1, double &x2) {
+  const int l = lane_id();
+#pragma unroll
+  for (int s = 1; s < WAVE; s <<= 1) {
+    double rb = 1.0 / b;
+    double am = shup(a, s), rbm = shup(rb, s), cm = shup(c, s);
+    double ap = shdn(a, s), rbp = shdn(rb, s), cp = shdn(c, s);
+    double d0m = shup(d0, s), d1m = shup(d1, s), d2m = shup(d2, s);
+    double d0p = shdn(d0, s), d1p = shdn(d1, s), d2p = shdn(d2, s);
+    if (l < s) { am = 0.0; rbm = 1.0; cm = 0.0; d0m = d1m = d2m = 0.0; }
+    if (l + s >= WAVE) { ap = 0.0; rbp = 1.0; cp = 0.0; d0p = d1p = d2p = 0.0; }
+    double alpha = -a * rbm;
+    double gamma = -c * rbp;
+    double na = alpha * am;
+    double nc = gamma * cp;
+    double nb = b + alpha * cm + gamma * ap;
+    d0 = d0 + alpha * d0m + gamma * d0p;
+    d1 = d1 + alpha * d1m + gamma * d1p;
+    d2 = d2 + alpha * d2m + gamma * d2p;
+    a = na; b = nb; c = nc;
+  }
+  x0 = d0 / b; x1 = d1 / b; x2 = d2 / b;
+}
+
 // Column loads.  ld predicates the load per lane (lanes outside the column's levels
 // read nothing); its result may stay in flight past later loads, which the pipelined
 // tracer kernels rely on.  ldv loads for every lane and selects the default after the

@@ -39,60 +39,216 @@ struct DynBuf {
   R fcor;
 };
 
+// R-precision power evaluated through an FP64 pow and rounded once, so the FP32
+// result does not depend on which math library computes it (the oracle does the same).
+__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ double pow_r(double x, double y) { return pow(x, y); }
+
+// ---------------------------------------------------------------- turbulence
+// The turbulence fields of the PBL scheme (mem_turb) and the per-face horizontal
+// diffusion coefficients of comp_horiz_k; on = 0 until the host has set them.
+template <class R>
+struct DTurb {
+  const R *vkh, *vkm;            // eddy diffusivities at W levels (nw x ldk)
+  const R *akm_sfc;              // surface drag coefficient per column
+  const R *dzit_bot;             // per level
+  const int *khtop, *kmtop;      // top levels of heat / momentum diffusion per column
+  R *akhodx, *akmodx;            // arv * K / dx per face and level (nv x ldk)
+  int *khtopv, *kmtopv;          // per face
+  int on;
+};
+
 // ---------------------------------------------------------------- setup loop
-// prog_wrtv_rk.f90:165-200: RK start values and volume-weighted tendencies, then the
-// Rayleigh terms of :219-233: vertical mixing of vxe/vye/vze near the top
-// (rayf_mix_top_vxe, mem_rayf.f90:177-220) and the relaxation of thil toward th01d.
+// prog_wrtv_rk.f90:165-251: RK start values and volume-weighted tendencies, then in the
+// reference's order the implicit vertical eddy diffusion of heat and momentum
+// (solve_eddy_diff_heat / _vxe, pbl_driver.f90:519-742, flat terrain: one surface
+// level), the Rayleigh terms (vertical mixing of vxe/vye/vze near the top,
+// rayf_mix_top_vxe mem_rayf.f90:177-220, and the relaxation of thil toward th01d), and
+// the horizontal diffusion across the faces.  The column solves are FP64 PCR within
+// the wave (the reference's tridiffo / tridv are FP32 Thomas).
 template <class R, class RP>
-__global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, const R *thil, const RP *rho, const R *wmc,
-                                                     const R *thilt, const R *vmxet, const R *vmyet,
-                                                     const R *vmzet, RP *rth0, RP *rho0, R *wmc0, R *thilt_s,
-                                                     R *vmxet_s, R *vmyet_s, R *vmzet_s, const R *vxe,
-                                                     const R *vye, const R *vze) {
+__global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R dts, const R *thil,
+                                                     const RP *rho, const R *wmc, const R *thilt,
+                                                     const R *vmxet, const R *vmyet, const R *vmzet, RP *rth0,
+                                                     RP *rho0, R *wmc0, R *thilt_s, R *vmxet_s, R *vmyet_s,
+                                                     R *vmzet_s, const R *vxe, const R *vye, const R *vze) {
   NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
-  const long ic = (long)c * m.ldk + l;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k >= 2 && k <= mza;
+  const long ldk = m.ldk, ic = (long)c * ldk + l;
+  // own-column values; lanes outside 2..mza read the padding of this column
+  const R *v[3] = {vxe, vye, vze};
+  const RP r = rho[ic];
+  const R th = thil[ic];
+  const R vv[3] = {v[0][ic], v[1][ic], v[2][ic]};
+  const R v4 = (R)__builtin_nontemporal_load(m.volt + ic);
+  R tt = __builtin_nontemporal_load(thilt + ic) * v4;
+  R vm[3] = {__builtin_nontemporal_load(vmxet + ic) * v4, __builtin_nontemporal_load(vmyet + ic) * v4,
+             __builtin_nontemporal_load(vmzet + ic) * v4};
+  if (act) {
+    st_nt(rth0, ic, (RP)th * r);
+    st_nt(rho0, ic, r);
+    st_nt(wmc0, ic, __builtin_nontemporal_load(wmc + ic));
+  }
+  if (tb.on) {
+    const R forw_imp = R(1.1);
+    const R vti = m.volti[ic], arw = m.arw[ic], dzim = m.dzim[l];
+    // solve_eddy_diff_heat, pbl_driver.f90:519-605: levels ka..kmax = khtop + 1, fluxes at
+    // W levels ka..khtop from the implicit system vctr5 x(k-1) + vctr6 x(k) + vctr7 x(k+1)
+    const int khtop = tb.khtop[c];
+    if (khtop >= 2) {
+      const int kmax = khtop + 1;
+      const R dtom = dts * vti / (R)r;
+      const R dtomass = forw_imp * dtom;
+      const R varp = th + dtom * tt;
+      const R varp1 = shdn(varp, 1), dtomass1 = shdn(dtomass, 1);
+      const bool row = k >= 2 && k <= khtop;
+      double a = 0.0, b = 1.0, cc = 0.0, d = 0.0;
+      if (row) {
+        const R akodz = arw * tb.vkh[ic] * dzim;
+        const R rhs = akodz * (varp - varp1);
+        const R v5 = -akodz * dtomass;
+        const R v7 = -akodz * dtomass1;
+        const R v6 = R(1) - v5 - v7;
+        a = (k > 2) ? (double)v5 : 0.0;
+        b = (double)v6;
+        cc = (k < khtop) ? (double)v7 : 0.0;
+        d = (double)rhs;
+      }
+      const double x = pcr_solve(a, b, cc, d);
+      const R flux = row ? (R)x : R(0);
+      const R flux_m1 = shup(flux, 1);
+      if (k >= 2 && k <= kmax) tt = tt + (R)((double)flux_m1 - (double)flux);
+    }
+    // solve_eddy_diff_vxe, pbl_driver.f90:609-742: levels ka..kmax = max(kmtop + 1, ka),
+    // surface drag on level ka
+    {
+      const int kmtop = tb.kmtop[c];
+      const int kmax = max(kmtop + 1, 2);
+      const R akodz = (k >= 2 && k <= kmtop) ? arw * tb.vkm[ic] * dzim : R(0);
+      const R akodz_m1 = shup(akodz, 1);
+      const bool row = k >= 2 && k <= kmax;
+      const R dtom = (R)((RP)(dts * vti) / r);
+      const R dtomass = forw_imp * dtom;
+      R v5 = -dtomass * akodz_m1;
+      const R v7 = -dtomass * akodz;
+      R v6 = R(1) - v5 - v7;
+      R vctr3 = R(0);
+      if (k == 2) {
+        vctr3 = tb.akm_sfc[c] * tb.dzit_bot[l];
+        v6 = v6 + dtomass * vctr3;
+      }
+      double a = 0.0, b = 1.0, cc = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+      if (row) {
+        a = (k > 2) ? (double)v5 : 0.0;
+        b = (double)v6;
+        cc = (k < kmax) ? (double)v7 : 0.0;
+        d0 = (double)vv[0]; d1 = (double)vv[1]; d2 = (double)vv[2];
+      }
+      double x[3];
+      pcr_solve3(a, b, cc, d0, d1, d2, x[0], x[1], x[2]);
+      for (int q = 0; q < 3; ++q) {
+        const R so = row ? (R)x[q] : R(0);
+        const R so1 = shdn(so, 1);
+        const R v2 = (k >= 2 && k <= kmax - 1) ? akodz * (so - so1) : R(0);
+        const R v2m = shup(v2, 1);
+        if (k == 2) vm[q] = vm[q] + (v2m - v2 - vctr3 * so);
+        else if (k >= 3 && k <= kmax) vm[q] = vm[q] + v2m - v2;
+      }
+    }
+  }
   // rayf_mix_top_vxe: vflux(k) = arw0 * rayf_cofmix(k) * real(rho(k+1) + rho(k)) *
   // (v(k) - v(k+1)) on ka..mza-1, zero at ka-1 and mza; the tendency of level k gains
   // vflux(k-1) - vflux(k) on ka..mza.  Every lane takes part in the shifts.
-  R fk[3] = {R(0), R(0), R(0)}, fkm1[3] = {R(0), R(0), R(0)};
   if (m.rayf_cofmix) {
-    const bool fl = k >= m.krayfmix_bot && k <= m.mza - 1;
-    const RP r0 = rho[ic], r1 = shdn(r0, 1);
-    const R fact = m.arw0[c] * m.rayf_cofmix[l] * (R)(r1 + r0);
-    const R *v[3] = {vxe, vye, vze};
+    const bool fl = k >= m.krayfmix_bot && k <= mza - 1;
+    const RP r1 = shdn(r, 1);
+    const R fact = m.arw0[c] * m.rayf_cofmix[l] * (R)(r1 + r);
     for (int q = 0; q < 3; ++q) {
-      const R a = v[q][ic], a1 = shdn(a, 1);
-      fk[q] = fl ? fact * (a - a1) : R(0);
-      fkm1[q] = shup(fk[q], 1);
+      const R a1 = shdn(vv[q], 1);
+      const R fk = fl ? fact * (vv[q] - a1) : R(0);
+      const R fkm1 = shup(fk, 1);
+      if (act && k >= m.krayfmix_bot) vm[q] = vm[q] + fkm1 - fk;
     }
   }
-  if (k < 2 || k > m.mza) return;
-  const R v4 = (R)__builtin_nontemporal_load(m.volt + ic);
-  const RP r = rho[ic];
-  const R th = thil[ic];
-  st_nt(rth0, ic, (RP)th * r);
-  st_nt(rho0, ic, r);
-  st_nt(wmc0, ic, __builtin_nontemporal_load(wmc + ic));
-  R tt = __builtin_nontemporal_load(thilt + ic) * v4;
-  if (m.rayf_cof && k >= m.krayf_bot)
-    tt = tt + m.rayf_cof[l] * m.dn01d[l] * (m.th01d[l] - th) * v4;
+  if (!act) return;
+  if (m.rayf_cof && k >= m.krayf_bot) tt = tt + m.rayf_cof[l] * m.dn01d[l] * (m.th01d[l] - th) * v4;
+  // horizontal diffusion, prog_wrtv_rk.f90:235-251
+  if (tb.on) {
+    const int np = m.w_npoly[c];
+    for_slots(np, [&](int j) {
+      if (j >= np) return;            // a pentagon's padded slot adds nothing in the reference
+      const int iv = m.w_face[c * NSLOT + j];
+      const long in = (long)m.w_nbr[c * NSLOT + j] * ldk + l, ie = (long)iv * ldk + l;
+      if (k <= tb.khtopv[iv]) tt = tt + tb.akhodx[ie] * (thil[in] - th);
+      if (k <= tb.kmtopv[iv]) {
+        const R am = tb.akmodx[ie];
+        for (int q = 0; q < 3; ++q) vm[q] = vm[q] + am * (v[q][in] - vv[q]);
+      }
+    });
+  }
   st_nt(thilt_s, ic, tt);
-  R vm[3] = {__builtin_nontemporal_load(vmxet + ic) * v4, __builtin_nontemporal_load(vmyet + ic) * v4,
-             __builtin_nontemporal_load(vmzet + ic) * v4};
-  if (m.rayf_cofmix && k >= m.krayfmix_bot)
-    for (int q = 0; q < 3; ++q) vm[q] = vm[q] + fkm1[q] - fk[q];
   st_nt(vmxet_s, ic, vm[0]);
   st_nt(vmyet_s, ic, vm[1]);
   st_nt(vmzet_s, ic, vm[2]);
 }
 
-// R-precision power evaluated through an FP64 pow and rounded once, so the FP32
-// result does not depend on which math library computes it (the oracle does the same).
-__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
-__device__ __forceinline__ double pow_r(double x, double y) { return pow(x, y); }
+// comp_horiz_k_column, pbl_driver.f90:154-238 (flat terrain: lpv = 2; one resolution
+// level, so akmin(mrl) is akmin): the face's diffusion tops, the stability bound
+// akstab and arv * K / dx for momentum (akmodx) and heat (akhodx).
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_horiz_k(DMesh<R> m, DTurb<R> tb, const RP *rho, R dtl, R akmin) {
+  NOCONTRACT
+  const int iv = wave_item();
+  if (iv >= m.nv) return;
+  const int l = lane_id(), k = l + 1;
+  const long ldk = m.ldk, ie = (long)iv * ldk + l;
+  const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
+  const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
+  const R fact1 = R(0.95) / (dtl * (R)m.w_npoly[iw1]);
+  const R fact2 = R(0.95) / (dtl * (R)m.w_npoly[iw2]);
+  // every lane loads and shifts (k-1 levels of the W-level diffusivities)
+  const R m1 = tb.vkm[i1], m2 = tb.vkm[i2], h1 = tb.vkh[i1], h2 = tb.vkh[i2];
+  const R m1m = shup(m1, 1), m2m = shup(m2, 1), h1m = shup(h1, 1), h2m = shup(h2, 1);
+  const RP r1 = rho[i1], r2 = rho[i2];
+  const bool amin = akmin > R(1.e-7);
+  int khv, kmv;
+  R hk0 = R(0);
+  if (amin) {
+    const R bkmin = akmin * R(0.075) * pow_r(min(m.arw0[iw1], m.arw0[iw2]), (R)0.66666666f);
+    khv = kmv = m.mza;
+    hk0 = R(0.5) * (R)(r1 + r2) * bkmin;
+  } else {
+    khv = max(tb.khtop[iw1], tb.khtop[iw2]);
+    khv = (khv >= 2) ? khv + 1 : 1;
+    kmv = max(tb.kmtop[iw1], tb.kmtop[iw2]);
+    kmv = (kmv >= 2) ? kmv + 1 : 1;
+  }
+  if (l == 0) { tb.khtopv[iv] = khv; tb.kmtopv[iv] = kmv; }
+  if (k < 2 || k > m.mza) return;
+  R akstab = R(0);
+  if (k <= max(kmv, khv)) {
+    const R stab1 = (R)(r1 * (RP)(R)m.volt[i1] * (RP)fact1);
+    const R stab2 = (R)(r2 * (RP)(R)m.volt[i2] * (RP)fact2);
+    akstab = min(stab1, stab2);
+  }
+  R am = R(0), ah = R(0);
+  if (k <= kmv) {
+    R hkm = R(0.25) * (m1 + m2 + m1m + m2m);
+    if (amin) hkm = max(hkm, hk0);
+    am = min(m.dniv[iv] * m.arv[ie] * hkm, akstab);
+  }
+  if (k <= khv) {
+    R hkh = R(0.25) * (h1 + h2 + h1m + h2m);
+    if (amin) hkh = max(hkh, hk0);
+    ah = min(m.dniv[iv] * m.arv[ie] * hkh, akstab);
+  }
+  tb.akmodx[ie] = am;
+  tb.akhodx[ie] = ah;
+}
+
 
 // ---------------------------------------------------------------- moist factors
 // comp_alpha_press, timestep.f90:515-536 (W): alpha_press = pc1 * ((rdry + rvap * rr_v)

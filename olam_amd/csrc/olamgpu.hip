@@ -358,6 +358,7 @@ struct ImplBase {
   virtual void prog_wrtv_rk() = 0;
   virtual void timeavg_momsc() = 0;
   virtual void comp_alpha_press(const float *, const float *, const float *) = 0;
+  virtual void set_turbulence(const float *, const float *, const int32_t *, const int32_t *, const float *) = 0;
   virtual void scalar_transport_rk(const float *) = 0;
   virtual void scalar_transport_rk_adj() = 0;
   virtual void mass_sums(double *) = 0;
@@ -479,6 +480,9 @@ struct Impl : ImplBase {
   Rayf rf;
   DevBuf b_rayf, vc03d;
   bool rayf_ready = false;
+  // turbulence of the host's PBL scheme and comp_horiz_k's face coefficients
+  DevBuf tb_vkh, tb_vkm, tb_akm, tb_dzitb, tb_khtop, tb_kmtop, tb_akhodx, tb_akmodx, tb_khtopv, tb_kmtopv;
+  DTurb<R> dturb{};
   // tracers
   int ntr;
   long tstride;
@@ -723,6 +727,11 @@ struct Impl : ImplBase {
       for (int e = 0; e < nv; ++e) dn[e] = (R)m->dniv[t.perm_v[e] + 1];
       upv(b_zfacit, zf);
       upv(b_dniv, dn);
+    }
+    if (m->dzit_bot) {
+      std::vector<R> db(ldk, R(0));
+      for (int k = 1; k <= mza; ++k) db[k - 1] = (R)m->dzit_bot[k - 1];
+      upv(tb_dzitb, db);
     }
     upv(b_vfact, vfact); upv(b_xxyym, xxyym);
     for (int i = 0; i < 11; ++i) upv(b_lev[i], lev[i]);
@@ -990,6 +999,40 @@ struct Impl : ImplBase {
   void timeavg_momsc() override;
 
   // -------------------------------------------------------------- prog_wrtv_rk
+  // the PBL scheme's mem_turb fields, then comp_horiz_k (pbl_driver.f90:136-238)
+  void set_turbulence(const float *vkh, const float *vkm, const int32_t *khtop, const int32_t *kmtop,
+                      const float *akm_sfc) override {
+    if (!vkh || !vkm || !khtop || !kmtop || !akm_sfc)
+      throw Err(OLAMGPU_EINVAL, "set_turbulence needs vkh, vkm, khtop, kmtop and akm_sfc");
+    if (!tb_dzitb.p) throw Err(OLAMGPU_EINVAL, "set_turbulence needs dzit_bot in the mesh descriptor");
+    std::vector<int> kh(nw), km(nw);
+    std::vector<R> as(nw);
+    for (int c = 0; c < nw; ++c) {
+      const int o = t.perm_w[c] + 1;
+      kh[c] = khtop[o]; km[c] = kmtop[o]; as[c] = (R)akm_sfc[o];
+      if (kh[c] < 0 || kh[c] > mza - 1 || km[c] < 0 || km[c] > mza - 1)
+        throw Err(OLAMGPU_EINVAL, "khtop and kmtop must lie in [0, mza - 1]");
+    }
+    if (!tb_vkh.p) {
+      tb_vkh.alloc(wsz * sizeof(R)); tb_vkm.alloc(wsz * sizeof(R));
+      tb_akhodx.alloc(vsz * sizeof(R)); tb_akmodx.alloc(vsz * sizeof(R));
+      tb_khtopv.alloc(nv * sizeof(int)); tb_kmtopv.alloc(nv * sizeof(int));
+    }
+    h2d<R>(tb_vkh, vkh, t.perm_w);
+    h2d<R>(tb_vkm, vkm, t.perm_w);
+    upv(tb_khtop, kh); upv(tb_kmtop, km); upv(tb_akm, as);
+    DTurb<R> d{};
+    d.vkh = dp<R>(tb_vkh); d.vkm = dp<R>(tb_vkm); d.akm_sfc = dp<R>(tb_akm); d.dzit_bot = dp<R>(tb_dzitb);
+    d.khtop = dp<int>(tb_khtop); d.kmtop = dp<int>(tb_kmtop);
+    d.akhodx = dp<R>(tb_akhodx); d.akmodx = dp<R>(tb_akmodx);
+    d.khtopv = dp<int>(tb_khtopv); d.kmtopv = dp<int>(tb_kmtopv);
+    d.on = 1;
+    LAUNCH("k_horiz_k", (k_horiz_k<R, RP>), dim3(nblk(nv)), dim3(256), dm, d, dp<RP>(rho), (R)prm.dtlm,
+           (R)prm.akmin);
+    HIPCHK(hipStreamSynchronize(s));
+    dturb = d;
+  }
+
   // comp_alpha_press (timestep.f90:500-559) from host moisture fields
   DevBuf moist[3];
   void comp_alpha_press(const float *rr_v, const float *rr_w, const float *theta) override {
@@ -1023,7 +1066,8 @@ struct Impl : ImplBase {
       throw Err(OLAMGPU_EINVAL, "Rayleigh friction on thil/vmc: call olamgpu_init_wrtv_rk after the initial state upload");
     cfl_fresh = false;
     const unsigned gw = nblk(nw), gv = nblk(nv), gm = nblk(nm);
-    LAUNCH("k_dyn_setup_w", (k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), dm, dp<R>(cur_thil_buf()), dp<RP>(rho),
+    LAUNCH("k_dyn_setup_w", (k_dyn_setup_w<R, RP>), dim3(gw), dim3(256), dm, dturb, (R)prm.dtsm,
+                       dp<R>(cur_thil_buf()), dp<RP>(rho),
                        dp<R>(wmc), dp<R>(thilt), dp<R>(vmxet), dp<R>(vmyet), dp<R>(vmzet), dp<RP>(rth0),
                        dp<RP>(rho0), dp<R>(wmc0), dp<R>(thilt_s), dp<R>(vmxet_s), dp<R>(vmyet_s), dp<R>(vmzet_s),
                        dp<R>(vxe), dp<R>(vye), dp<R>(vze));
@@ -1581,6 +1625,10 @@ int olamgpu_upload_scalar(olamgpu_ctx *c, int32_t n, const float *p, const float
 }
 int olamgpu_download_scalar(olamgpu_ctx *c, int32_t n, float *p) {
   return guarded(c, [&] { if (!p) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->download_scalar(n, p); });
+}
+int olamgpu_set_turbulence(olamgpu_ctx *c, const float *vkh, const float *vkm, const int32_t *khtop,
+                           const int32_t *kmtop, const float *akm_sfc) {
+  return guarded(c, [&] { c->impl->set_turbulence(vkh, vkm, khtop, kmtop, akm_sfc); });
 }
 int olamgpu_comp_alpha_press(olamgpu_ctx *c, const float *rr_v, const float *rr_w, const float *theta) {
   return guarded(c, [&] { c->impl->comp_alpha_press(rr_v, rr_w, theta); });

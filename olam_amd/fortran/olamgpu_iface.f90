@@ -47,6 +47,8 @@ module olamgpu_iface
      ! Rayleigh friction (mem_rayf.f90:41-173): zm, zt (mza) of mem_grid; the reference
      ! sounding th01d, dn01d (mza) of misc_coms
      type(c_ptr) :: zm = c_null_ptr, zt = c_null_ptr, th01d = c_null_ptr, dn01d = c_null_ptr
+     ! surface drag of the eddy diffusion: dzit_bot (mza) of mem_grid
+     type(c_ptr) :: dzit_bot = c_null_ptr
   end type olamgpu_mesh_t
 
   type, bind(C), public :: olamgpu_params_t
@@ -69,6 +71,7 @@ module olamgpu_iface
      real(c_float)      :: rayfw_zmin = 30000.0, rayfw_fact = 0.0, rayfw_expon = 1.0
      real(c_float)      :: rayfdiv_zmin = 30000.0, rayfdiv_fact = 0.0, rayfdiv_expon = 1.0
      real(c_float)      :: rayfmix_zmin = 30000.0, rayfmix_fact = 0.0, rayfmix_expon = 1.0
+     real(c_float)      :: akmin = 0.0     ! nl%akmin(1), comp_horiz_k
   end type olamgpu_params_t
 
   type, bind(C), public :: olamgpu_state_t
@@ -85,7 +88,7 @@ module olamgpu_iface
   public :: olamgpu_create, olamgpu_destroy, olamgpu_last_error, olamgpu_upload_state, &
        olamgpu_download_state, olamgpu_upload_scalar, olamgpu_download_scalar, &
        olamgpu_init_wrtv_rk, olamgpu_tend0, olamgpu_prog_wrtv_rk, olamgpu_timeavg_momsc, &
-       olamgpu_comp_alpha_press, &
+       olamgpu_comp_alpha_press, olamgpu_set_turbulence, &
        olamgpu_scalar_transport_rk, olamgpu_scalar_transport_rk_adj, &
        olamgpu_upload_scalar_adj, olamgpu_download_scalar_adj, olamgpu_mass_sums, &
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
@@ -171,6 +174,14 @@ module olamgpu_iface
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx, rr_v, rr_w, theta
      end function olamgpu_comp_alpha_press
+
+     ! mem_turb fields of the PBL scheme: vkh, vkm real(4) (mza, mwa); khtop, kmtop
+     ! integer(4) (mwa); akm_sfc real(4) (mwa); then comp_horiz_k (pbl_driver.f90:136-238)
+     integer(c_int) function olamgpu_set_turbulence(ctx, vkh, vkm, khtop, kmtop, akm_sfc) &
+          bind(C, name="olamgpu_set_turbulence")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, vkh, vkm, khtop, kmtop, akm_sfc
+     end function olamgpu_set_turbulence
 
      ! rho_old: c_loc of a real(4) (mza, mwa) array, or c_null_ptr for the copy saved by tend0
      integer(c_int) function olamgpu_scalar_transport_rk(ctx, rho_old) &

@@ -40,6 +40,7 @@ def lib():
             getattr(L, name).argtypes = [vp]
         L.olamgpu_upload_state.argtypes = [vp, vp]
         L.olamgpu_comp_alpha_press.argtypes = [vp, vp, vp, vp]
+        L.olamgpu_set_turbulence.argtypes = [vp, vp, vp, vp, vp, vp]
         L.olamgpu_download_state.argtypes = [vp, vp]
         L.olamgpu_upload_scalar.argtypes = [vp, C.c_int32, vp, vp]
         L.olamgpu_download_scalar.argtypes = [vp, C.c_int32, vp]
@@ -67,7 +68,7 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
            "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef",
-           "olamgpu_comp_alpha_press"]
+           "olamgpu_comp_alpha_press", "olamgpu_set_turbulence"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
@@ -150,6 +151,19 @@ class Context:
 
     def timeavg_momsc(self):
         self._chk(lib().olamgpu_timeavg_momsc(self.h), "timeavg_momsc")
+
+    def set_turbulence(self, vkh, vkm, khtop, kmtop, akm_sfc):
+        """mem_turb fields of the PBL scheme: vkh, vkm (mwa, mza) FP32 at W levels, khtop,
+        kmtop (mwa) int32, akm_sfc (mwa) FP32; then comp_horiz_k on the device."""
+        g = self.grid
+        f = [np.ascontiguousarray(x, dtype=np.float32) for x in (vkh, vkm, akm_sfc)]
+        i = [np.ascontiguousarray(x, dtype=np.int32) for x in (khtop, kmtop)]
+        if f[0].shape != (g.mwa, g.mza) or f[1].shape != (g.mwa, g.mza):
+            raise ValueError("vkh and vkm must be (mwa, mza)")
+        if f[2].shape != (g.mwa,) or i[0].shape != (g.mwa,) or i[1].shape != (g.mwa,):
+            raise ValueError("khtop, kmtop and akm_sfc must be (mwa,)")
+        self._chk(lib().olamgpu_set_turbulence(self.h, _p(f[0]), _p(f[1]), _p(i[0]), _p(i[1]), _p(f[2])),
+                  "set_turbulence")
 
     def comp_alpha_press(self, rr_v, rr_w, theta):
         """Moist alpha_press, pwfac, pvfac from (mwa, mza) FP32 rr_v, rr_w, theta."""

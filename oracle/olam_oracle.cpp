@@ -92,6 +92,7 @@ struct Base {
   virtual int download_adj(int, float *) = 0;
   virtual void init_wrtv_rk() = 0;
   virtual void comp_alpha_press(const float *, const float *, const float *) = 0;
+  virtual int set_turbulence(const float *, const float *, const int32_t *, const int32_t *, const float *) = 0;
   virtual void tend0() = 0;
   virtual void prog_wrtv_rk() = 0;
   virtual void timeavg_momsc() = 0;
@@ -146,6 +147,7 @@ struct Model : Base {
   bool dorayf = false, dorayfw = false, dorayfdiv = false, dorayfmix = false;
   int krayf_bot = 0, krayfw_bot = 0, krayfdiv_bot = 0, krayfmix_bot = 0;
   std::vector<float> zm, zt, th01d, dn01d, rayf_cof, rayf_cofw, rayf_cofmix, b20;
+  std::vector<R> dzit_bot;                  // mem_grid.f90:374-378, for the surface drag
   F2<R> vc03d;
   // ---- SAVEd damping state (vort_div_damp.f90:23-33,255-262) ----
   bool div_first = true, vort_first = true;
@@ -225,6 +227,7 @@ struct Model : Base {
     for (int n = 0; n < ns; ++n) { scp[n].alloc(mza, mwa); sct[n].alloc(mza, mwa); lam[n].alloc(mza, mwa); }
     pdef.assign(ns, 1);
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
+    if (m->dzit_bot) load1(dzit_bot, m->dzit_bot, mza);
     if (m->zm && m->zt) {   // oracle_create checks that Rayleigh friction has them
       load1(zm, m->zm, mza);
       load1(zt, m->zt, mza);
@@ -280,6 +283,126 @@ struct Model : Base {
     }
     if (dorayfw)
       for (int k = krayfw_bot; k <= mza - 1; ++k) b20[k] = 1.0f + (float)prm.dtsm * rayf_cofw[k];
+  }
+
+  // ---- turbulence of the host's PBL scheme (mem_turb) and comp_horiz_k's results ----
+  bool turb_on = false;
+  F2<R> vkh, vkm, akhodx, akmodx;
+  std::vector<int> khtop, kmtop, khtopv, kmtopv;
+  std::vector<R> akm_sfc;
+
+  int set_turbulence(const float *vkh_h, const float *vkm_h, const int32_t *kh, const int32_t *km,
+                     const float *as) override {
+    if (dzit_bot.empty() || !vkh_h || !vkm_h || !kh || !km || !as) return OLAMGPU_EINVAL;
+    khtop.assign(mwa + 1, 0); kmtop.assign(mwa + 1, 0); akm_sfc.assign(mwa + 1, R(0));
+    for (int iw = 2; iw <= mwa; ++iw) {
+      khtop[iw] = kh[iw - 1]; kmtop[iw] = km[iw - 1]; akm_sfc[iw] = (R)as[iw - 1];
+      if (khtop[iw] < 0 || khtop[iw] > mza - 1 || kmtop[iw] < 0 || kmtop[iw] > mza - 1) return OLAMGPU_EINVAL;
+    }
+    load2(vkh, vkh_h, mza, mwa); load2(vkm, vkm_h, mza, mwa);
+    comp_horiz_k();
+    turb_on = true;
+    return 0;
+  }
+
+  // comp_horiz_k_column, pbl_driver.f90:154-238 (lpv = 2, akmin(mrl) = akmin)
+  void comp_horiz_k() {
+    akhodx.alloc(mza, mva); akmodx.alloc(mza, mva);
+    khtopv.assign(mva + 1, 0); kmtopv.assign(mva + 1, 0);
+    const R dtl = (R)prm.dtlm, akmin = (R)prm.akmin;
+#pragma omp parallel for
+    for (int iv = 2; iv <= mva; ++iv) {
+      int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
+      R fact1 = R(0.95) / (dtl * (R)npoly[iw1]);
+      R fact2 = R(0.95) / (dtl * (R)npoly[iw2]);
+      R hk0[130], akstab[130];
+      if (akmin > R(1.e-7)) {
+        R bkmin = akmin * R(0.075) * pow_r(std::min(arw0[iw1], arw0[iw2]), (R)0.66666666f);
+        khtopv[iv] = mza;
+        kmtopv[iv] = mza;
+        for (int k = 2; k <= mza; ++k) hk0[k] = R(0.5) * (R)(rho(k, iw1) + rho(k, iw2)) * bkmin;
+      } else {
+        khtopv[iv] = std::max(khtop[iw1], khtop[iw2]);
+        khtopv[iv] = (khtopv[iv] >= 2) ? khtopv[iv] + 1 : 1;
+        kmtopv[iv] = std::max(kmtop[iw1], kmtop[iw2]);
+        kmtopv[iv] = (kmtopv[iv] >= 2) ? kmtopv[iv] + 1 : 1;
+      }
+      for (int k = 2; k <= std::max(kmtopv[iv], khtopv[iv]); ++k) {
+        R stab1 = (R)(rho(k, iw1) * (RP)(R)volt(k, iw1) * (RP)fact1);
+        R stab2 = (R)(rho(k, iw2) * (RP)(R)volt(k, iw2) * (RP)fact2);
+        akstab[k] = std::min(stab1, stab2);
+      }
+      for (int k = 2; k <= kmtopv[iv]; ++k) {
+        R hkm = R(0.25) * (vkm(k, iw1) + vkm(k, iw2) + vkm(k - 1, iw1) + vkm(k - 1, iw2));
+        if (akmin > R(1.e-7)) hkm = std::max(hkm, hk0[k]);
+        akmodx(k, iv) = std::min(dniv[iv] * arv(k, iv) * hkm, akstab[k]);
+      }
+      for (int k = 2; k <= khtopv[iv]; ++k) {
+        R hkh = R(0.25) * (vkh(k, iw1) + vkh(k, iw2) + vkh(k - 1, iw1) + vkh(k - 1, iw2));
+        if (akmin > R(1.e-7)) hkh = std::max(hkh, hk0[k]);
+        akhodx(k, iv) = std::min(dniv[iv] * arv(k, iv) * hkh, akstab[k]);
+      }
+    }
+  }
+
+  // solve_eddy_diff_heat, pbl_driver.f90:519-605 (idiffk /= 1: no non-local term)
+  void solve_eddy_diff_heat(int iw, R *thiltc) const {
+    if (khtop[iw] < 2) return;
+    const int ka = 2, kz = khtop[iw], kmax = kz + 1;
+    const R dts = (R)prm.dtsm, forw_imp = R(1.1);
+    R dtomass[130], varp[130];
+    RT v5[130], v6[130], v7[130], rhs[130], soln[130];
+    for (int k = ka; k <= kmax; ++k) {
+      R dtom = dts * volti(k, iw) / (R)rho(k, iw);
+      dtomass[k] = forw_imp * dtom;
+      varp[k] = thil(k, iw) + dtom * thiltc[k];
+    }
+    for (int k = ka; k <= kz; ++k) {
+      R akodz = arw(k, iw) * vkh(k, iw) * dzim[k];
+      rhs[k] = (RT)(akodz * (varp[k] - varp[k + 1]));
+      R a5 = -akodz * dtomass[k], a7 = -akodz * dtomass[k + 1];
+      v5[k] = (RT)a5; v7[k] = (RT)a7; v6[k] = (RT)(R(1) - a5 - a7);
+    }
+    tridiffo(ka, kz, v5, v6, v7, rhs, soln);
+    double flux[130];
+    flux[ka - 1] = 0.0;
+    flux[kmax] = 0.0;
+    for (int k = ka; k <= kmax - 1; ++k) flux[k] = (double)(R)soln[k];
+    for (int k = ka; k <= kmax; ++k) thiltc[k] = thiltc[k] + (R)(flux[k - 1] - flux[k]);
+  }
+
+  // solve_eddy_diff_vxe, pbl_driver.f90:609-742 (lsw = 1 on flat terrain); tridv is
+  // tridiffo's elimination applied to each right-hand side
+  void solve_eddy_diff_vxe(int iw, R *vmx, R *vmy, R *vmz) const {
+    const int ka = 2, kmax = std::max(kmtop[iw] + 1, ka);
+    const R dts = (R)prm.dtsm, forw_imp = R(1.1);
+    R akodz[130], dtomass[130], a5[130], a6[130], a7[130];
+    for (int k = 0; k <= kmax; ++k) akodz[k] = R(0);
+    for (int k = ka; k <= kmtop[iw]; ++k) akodz[k] = arw(k, iw) * vkm(k, iw) * dzim[k];
+    for (int k = ka; k <= kmax; ++k) {
+      R dtom = (R)((RP)(dts * volti(k, iw)) / rho(k, iw));
+      dtomass[k] = forw_imp * dtom;
+      a5[k] = -dtomass[k] * akodz[k - 1];
+      a7[k] = -dtomass[k] * akodz[k];
+      a6[k] = R(1) - a5[k] - a7[k];
+    }
+    const R vctr3 = akm_sfc[iw] * dzit_bot[ka];
+    a6[ka] = a6[ka] + dtomass[ka] * vctr3;
+    RT v5[130], v6[130], v7[130], rhs[130], soln[130];
+    for (int k = ka; k <= kmax; ++k) { v5[k] = (RT)a5[k]; v6[k] = (RT)a6[k]; v7[k] = (RT)a7[k]; }
+    const F2<R> *vel[3] = {&vxe, &vye, &vze};
+    R *ten[3] = {vmx, vmy, vmz};
+    for (int q = 0; q < 3; ++q) {
+      for (int k = ka; k <= kmax; ++k) rhs[k] = (RT)(*vel[q])(k, iw);
+      tridiffo(ka, kmax, v5, v6, v7, rhs, soln);
+      R so[130], v2[130];
+      for (int k = ka; k <= kmax; ++k) so[k] = (R)soln[k];
+      v2[ka - 1] = R(0);
+      v2[kmax] = R(0);
+      for (int k = ka; k <= kmax - 1; ++k) v2[k] = akodz[k] * (so[k] - so[k + 1]);
+      ten[q][ka] = ten[q][ka] + (v2[ka - 1] - v2[ka] - vctr3 * so[ka]);
+      for (int k = ka + 1; k <= kmax; ++k) ten[q][k] = ten[q][k] + v2[k - 1] - v2[k];
+    }
   }
 
   // comp_alpha_press, timestep.f90:500-559 (called when miclevel > 0).  rr_v, rr_w and
@@ -898,12 +1021,29 @@ struct Model : Base {
         vmyet_s(k, iw) = vmyet(k, iw) * v4;
         vmzet_s(k, iw) = vmzet(k, iw) * v4;
       }
+      // vertical eddy diffusion, :213-217
+      if (turb_on) {
+        solve_eddy_diff_heat(iw, thilt_s.col(iw) - 1);
+        solve_eddy_diff_vxe(iw, vmxet_s.col(iw) - 1, vmyet_s.col(iw) - 1, vmzet_s.col(iw) - 1);
+      }
       // Rayleigh friction on the velocity gradient at the top and on thil, :219-233
       if (dorayfmix) rayf_mix_top_vxe(iw, vmxet_s.col(iw) - 1, vmyet_s.col(iw) - 1, vmzet_s.col(iw) - 1);
       if (dorayf && prm.initial == 1)
         for (int k = krayf_bot; k <= mza; ++k) {
           R v4 = (R)volt(k, iw);
           thilt_s(k, iw) = thilt_s(k, iw) + (R)rayf_cof[k] * (R)dn01d[k] * ((R)th01d[k] - thil(k, iw)) * v4;
+        }
+      // horizontal diffusion, :235-251
+      if (turb_on)
+        for (int jv = 1; jv <= npoly[iw]; ++jv) {
+          int iv = w_iv(jv, iw), iwn = w_iw(jv, iw);
+          for (int k = 2; k <= khtopv[iv]; ++k)
+            thilt_s(k, iw) = thilt_s(k, iw) + akhodx(k, iv) * (thil(k, iwn) - thil(k, iw));
+          for (int k = 2; k <= kmtopv[iv]; ++k) {
+            vmxet_s(k, iw) = vmxet_s(k, iw) + akmodx(k, iv) * (vxe(k, iwn) - vxe(k, iw));
+            vmyet_s(k, iw) = vmyet_s(k, iw) + akmodx(k, iv) * (vye(k, iwn) - vye(k, iw));
+            vmzet_s(k, iw) = vmzet_s(k, iw) + akmodx(k, iv) * (vze(k, iwn) - vze(k, iw));
+          }
         }
       if (prm.ithil_monot > 0)
         for (int k = 2; k <= mza; ++k) thl0(k, iw) = thil(k, iw);
@@ -1842,6 +1982,10 @@ int oracle_download_scalar_adj(void *h, int32_t n, float *p) { return ((oracle_c
 int oracle_tend0(void *h) { ((oracle_ctx *)h)->m->tend0(); return 0; }
 int oracle_rayf_profiles(void *h, float *out, int32_t *kb) { ((oracle_ctx *)h)->m->rayf_profiles(out, kb); return 0; }
 int oracle_setup_tendencies(void *h, double *out) { return ((oracle_ctx *)h)->m->setup_tendencies(out); }
+int oracle_set_turbulence(void *h, const float *vkh, const float *vkm, const int32_t *khtop, const int32_t *kmtop,
+                          const float *akm_sfc) {
+  return ((oracle_ctx *)h)->m->set_turbulence(vkh, vkm, khtop, kmtop, akm_sfc);
+}
 int oracle_comp_alpha_press(void *h, const float *rr_v, const float *rr_w, const float *theta) {
   ((oracle_ctx *)h)->m->comp_alpha_press(rr_v, rr_w, theta);
   return 0;

@@ -56,6 +56,7 @@ def lib():
         L.oracle_rhothil_sum.argtypes = [vp, vp]
         L.oracle_rayf_profiles.argtypes = [vp, vp, vp]
         L.oracle_comp_alpha_press.argtypes = [vp, vp, vp, vp]
+        L.oracle_set_turbulence.argtypes = [vp, vp, vp, vp, vp, vp]
         L.oracle_setup_tendencies.argtypes = [vp, vp]
         _lib = L
     return _lib
@@ -117,6 +118,13 @@ class Oracle:
     def comp_alpha_press(self, rr_v, rr_w, theta):
         a = [np.ascontiguousarray(x, dtype=np.float32) for x in (rr_v, rr_w, theta)]
         lib().oracle_comp_alpha_press(self.h, *(_p(x) for x in a))
+
+    def set_turbulence(self, vkh, vkm, khtop, kmtop, akm_sfc):
+        f = [np.ascontiguousarray(x, dtype=np.float32) for x in (vkh, vkm, akm_sfc)]
+        i = [np.ascontiguousarray(x, dtype=np.int32) for x in (khtop, kmtop)]
+        rc = lib().oracle_set_turbulence(self.h, _p(f[0]), _p(f[1]), _p(i[0]), _p(i[1]), _p(f[2]))
+        if rc:
+            raise RuntimeError(f"oracle set_turbulence failed ({rc})")
 
     def rayf_profiles(self):
         """Rayleigh friction profiles per level k = 1..mza: dict of rayf_cof, rayf_cofw,
