@@ -133,6 +133,13 @@ typedef struct olamgpu_params {
   float   rayfmix_zmin, rayfmix_fact, rayfmix_expon;
   float   akmin;                     /* akmin(1), oname_coms.f90:266: minimum horizontal eddy
                                         diffusivity ratio of comp_horiz_k */
+  /* nudging (oname_coms.f90:134-140): with nudflag > 0 and nud_preserve_total_mass the
+   * nudging mass fluxes vmanud/wmanud join the face and vertical mass fluxes
+   * (prog_wrtv_rk.f90:347-355,782-788; with nudflag == 1 also the vertical nudging
+   * fluxes of :605-621), otherwise the density tendency rhot_nud enters the explicit
+   * density (:703-706).  The fields are set by olamgpu_set_nudging (zero until then). */
+  int32_t nudflag;
+  int32_t nud_preserve_total_mass;
 } olamgpu_params;
 
 /* Prognostic and diagnostic fields of mem_basic/mem_tend (mem_basic.f90:8-34,
@@ -216,6 +223,13 @@ int olamgpu_comp_alpha_press(olamgpu_ctx *ctx, const float *rr_v, const float *r
  * of idiffk = 1 is not part of this entry. */
 int olamgpu_set_turbulence(olamgpu_ctx *ctx, const float *vkh, const float *vkm, const int32_t *khtop,
                            const int32_t *kmtop, const float *akm_sfc);
+
+/* The nudging fields of mem_nudge (oisan/mem_nudge.f90:46,90-91): the mass fluxes
+ * wmanud (mza, mwa) and vmanud (mza, mva) FP32, and the density tendency rhot_nud
+ * (mza, mwa) FP64.  Each may be NULL (left as it was; all start at zero, as the
+ * reference allocates them).  Set them where OLAM's nudging updates them, once per
+ * long step before the acoustic steps. */
+int olamgpu_set_nudging(olamgpu_ctx *ctx, const float *wmanud, const float *vmanud, const double *rhot_nud);
 
 /* scalar_transport_rk(rho_old) (scalar_transport_rk.F90:30-385): one long step of
  * all registered tracers.  rho_old = NULL uses the copy saved by olamgpu_tend0;

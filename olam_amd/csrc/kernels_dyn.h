@@ -34,6 +34,12 @@ struct DynBuf {
   // winds: 1..3): low-order values scpt and the scales of scalar_monot_limiter
   const R *scpt[4], *ms1[4], *ms2[4];
   int mono[4];
+  // nudging (mem_nudge): mass fluxes wmanud/vmanud added to the W and face mass fluxes
+  // (nud_vm), the vertical nudging fluxes of nudflag == 1 (nud_w), the density
+  // tendency rhot_nud (nud_rho)
+  const R *wmanud, *vmanud;
+  const RP *rhot_nud;
+  int nud_vm, nud_w, nud_rho;
   int thil_order, wind_order, istage, nrk;
   double dt8;
   R fcor;
@@ -501,10 +507,10 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
 }
 
 // ---------------------------------------------------------------- prog_wrt_begs
-// prog_wrtv_rk.f90:494-884 (global domain, no nudging) fused with the face
+// prog_wrtv_rk.f90:494-884 (global domain) fused with the face
 // values of fluxes_wrtv's scalar_v_column (:1022-1081) and scalar_w_column.
 
-template <class R, class RP, bool MONO>
+template <class R, class RP, bool MONO, bool NUD>
 __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
   const int c = wave_item();
@@ -536,6 +542,29 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   RP hflx = RP(0);
   R tt = ld_nt(b.thilt_s, ic, act), tx = ld_nt(b.vmxet_s, ic, act), ty = ld_nt(b.vmyet_s, ic, act),
     tz = ld_nt(b.vmzet_s, ic, act);
+  // nudging mass flux wmanud carrying the vertically upwinded values (:605-621,
+  // nudflag == 1 with nud_preserve_total_mass), before the face sums
+  if (NUD && b.nud_w) {
+    const R wn = ldv(b.wmanud, ic, act, R(0)), wn_m1 = shup(wn, 1);
+    const R wmc_n = ld(b.wmc, ic, act);
+    R up[4];
+    for (int q = 0; q < 4; ++q) {
+      up[q] = vert_upw(qc[q], wmc_n, k, mza, m.aw11[l], m.aw21[l], m.aw12[l], m.aw22[l]);
+      if constexpr (MONO)
+        if (b.mono[q]) up[q] = vert_upw_monot(m, qc[q], b.scpt[q], b.ms1[q], b.ms2[q], ic, act, wmc_n, ld(m.arw, ic, act), k, mza);
+    }
+    const RP vt = inner ? (RP)(wn * up[0]) : RP(0);
+    const R vx = inner ? wn * up[1] : R(0), vy = inner ? wn * up[2] : R(0), vz = inner ? wn * up[3] : R(0);
+    const RP vt_m1 = shup(vt, 1);
+    const R vx_m1 = shup(vx, 1), vy_m1 = shup(vy, 1), vz_m1 = shup(vz, 1);
+    if (body) {
+      hflx = (RP)(wn_m1 - wn);
+      tt = (R)((RP)tt + vt_m1 - vt);
+      tx = tx + vx_m1 - vx;
+      ty = ty + vy_m1 - vy;
+      tz = tz + vz_m1 - vz;
+    }
+  }
   // face mass fluxes first: a neighbour's values and gradients are gathered only if some
   // level takes that face's value from the neighbour (upwind side)
   R vmca_j[7], vmc_j[7], arv_j[7];
@@ -551,6 +580,8 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   for (int j = 0; j < 7; ++j) {
     const R dv = m.w_dirv[c * NSLOT + j];
     vmca_j[j] = (j < np) ? vmc_j[j] * arv_j[j] : R(0);
+    if (NUD && b.nud_vm && j < np)   // vmca = vmc * arv + vmanud (:347-355)
+      vmca_j[j] = vmca_j[j] + ldv(b.vmanud, (long)m.w_face[c * NSLOT + j] * ldk + l, act, R(0));
     sel_j[j] = (dv < R(0)) ? (vmc_j[j] < R(0)) : !(vmc_j[j] < R(0));
     need |= (__ballot(sel_j[j] && act) != 0 ? 1u : 0u) << j;
   }
@@ -641,6 +672,8 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R b15 = (R)(b10 * b5);
   RP delex_rho = b10 * hflx;
   RP delex_rt = b10 * (RP)tt;
+  if (NUD && b.nud_rho)   // :703-706, in FP64 as the reference's real(r8) terms
+    delex_rho = (RP)((double)delex_rho + dt8 * (double)ld(b.rhot_nud, ic, act));
   if (b.istage > 1) {
     delex_rho = delex_rho + ld(b.rho0, ic, act) - rho_c;
     delex_rt = delex_rt + ld_nt(b.rth0, ic, act) - rhothil;
@@ -685,7 +718,8 @@ __global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   // tridiffo (tridiag_lib.f90:10-36) replaced by FP64 PCR within the wavefront
   const double wmf = pcr_solve(ma, mb, mc, md);
   const R wmff = (R)wmf;
-  const R wma = inner ? wmff * arw : R(0);
+  R wma = inner ? wmff * arw : R(0);
+  if (NUD && b.nud_vm && inner) wma = wma + b.wmanud[ic];                                 // :782-788
   const RP vflux_thil = (RP)(wma * tupw);
   const R vfx = wma * xupw, vfy = wma * yupw, vfz = wma * zupw;
   if (b.istage == b.nrk && inner) b.wmasc[ic] = b.wmasc[ic] + wma;
@@ -765,6 +799,7 @@ __global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, 
   for (int j = 0; j < 7; ++j) {
     const long ie = (long)m.w_face[c * NSLOT + j] * ldk + l;
     vmca_j[j] = (j < np) ? ld(b.vmc, ie, act) * ld(m.arv, ie, act) : R(0);
+    if (b.nud_vm && j < np) vmca_j[j] = vmca_j[j] + ld(b.vmanud, ie, act);
   }
   const R wmcc = ld(b.wmc, ic, act), arw = ld(m.arw, ic, act);
   const R wmca = inner ? wmcc * arw : R(0);
@@ -864,7 +899,7 @@ __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int 
                                    m.vnz[iv] * (b.vmzet_rk[i1] + b.vmzet_rk[i2])) * m.volvi[ie]);
   st_nt(b.vmc, ie, v);
   st_nt(b.vc, ie, R(2) * v / (R)(b.rho[i1] + b.rho[i2]));
-  if (acc_vmasc) b.vmasc[ie] = b.vmasc[ie] + v * m.arv[ie];
+  if (acc_vmasc) b.vmasc[ie] = b.vmasc[ie] + (b.nud_vm ? v * m.arv[ie] + b.vmanud[ie] : v * m.arv[ie]);
 }
 
 // ---------------------------------------------------------------- vel_t3d_hex

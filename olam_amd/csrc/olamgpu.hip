@@ -359,6 +359,7 @@ struct ImplBase {
   virtual void timeavg_momsc() = 0;
   virtual void comp_alpha_press(const float *, const float *, const float *) = 0;
   virtual void set_turbulence(const float *, const float *, const int32_t *, const int32_t *, const float *) = 0;
+  virtual void set_nudging(const float *, const float *, const double *) = 0;
   virtual void scalar_transport_rk(const float *) = 0;
   virtual void scalar_transport_rk_adj() = 0;
   virtual void mass_sums(double *) = 0;
@@ -483,6 +484,8 @@ struct Impl : ImplBase {
   // turbulence of the host's PBL scheme and comp_horiz_k's face coefficients
   DevBuf tb_vkh, tb_vkm, tb_akm, tb_dzitb, tb_khtop, tb_kmtop, tb_akhodx, tb_akmodx, tb_khtopv, tb_kmtopv;
   DTurb<R> dturb{};
+  // nudging fields of mem_nudge (zero until olamgpu_set_nudging)
+  DevBuf nud_w, nud_v, nud_r;
   // tracers
   int ntr;
   long tstride;
@@ -535,6 +538,7 @@ struct Impl : ImplBase {
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
     build_mesh(m);
     alloc_state();
+    if (prm.nudflag > 0) alloc_nudging();
     if (const char *e = std::getenv("OLAMGPU_TPW")) tpw = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("OLAMGPU_TRACER")) {
       use_patch = std::strcmp(e, "patch") == 0;
@@ -999,6 +1003,19 @@ struct Impl : ImplBase {
   void timeavg_momsc() override;
 
   // -------------------------------------------------------------- prog_wrtv_rk
+  // mem_nudge's wmanud, vmanud, rhot_nud (host Fortran layout; NULL keeps a field)
+  void set_nudging(const float *wm, const float *vm, const double *rt) override {
+    if (!nud_w.p) alloc_nudging();
+    if (wm) h2d<R>(nud_w, wm, t.perm_w);
+    if (vm) h2d<R>(nud_v, vm, t.perm_v);
+    if (rt) h2d<RP>(nud_r, rt, t.perm_w);
+  }
+  void alloc_nudging() {
+    nud_w.alloc(wsz * sizeof(R)); nud_v.alloc(vsz * sizeof(R)); nud_r.alloc(wsz * sizeof(RP));
+    zero(nud_w); zero(nud_v); zero(nud_r);
+    HIPCHK(hipStreamSynchronize(s));
+  }
+
   // the PBL scheme's mem_turb fields, then comp_horiz_k (pbl_driver.f90:136-238)
   void set_turbulence(const float *vkh, const float *vkm, const int32_t *khtop, const int32_t *kmtop,
                       const float *akm_sfc) override {
@@ -1135,6 +1152,11 @@ struct Impl : ImplBase {
       b.istage = istage; b.nrk = nrk;
       b.dt8 = frac * prm.dtsm;
       b.fcor = (R)prm.fcoriol;
+      b.wmanud = dp<R>(nud_w); b.vmanud = dp<R>(nud_v); b.rhot_nud = dp<RP>(nud_r);
+      b.nud_vm = prm.nudflag > 0 && prm.nud_preserve_total_mass;
+      b.nud_w = prm.nudflag == 1 && prm.nud_preserve_total_mass;
+      b.nud_rho = prm.nudflag > 0 && !prm.nud_preserve_total_mass;
+      const bool nud = b.nud_vm || b.nud_w || b.nud_rho;
       int dth = prm.thil_horiz_adv_order == 2, dwn = prm.wind_horiz_adv_order == 2;
       if (dth || dwn)
         LAUNCH("k_dyn_grad", (k_dyn_grad<R>), dim3(gw), dim3(256), dm, b.thil_in, b.vxe, b.vye, b.vze,
@@ -1168,9 +1190,11 @@ struct Impl : ImplBase {
         }
         LAUNCH("k_dyn_monot_pre", (k_dyn_monot_pre<R, RP>), dim3(gw), dim3(256), dm, mo, dp<RP>(rho0), (R)b.dt8);
         LAUNCH("k_dyn_monot_scales", (k_dyn_monot_scales<R, RP>), dim3(gw), dim3(256), dm, b, mo);
-        LAUNCH("k_dyn_w", (k_dyn_w<R, RP, true>), dim3(gw), dim3(256), dm, b);
+        if (nud) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, true, true>), dim3(gw), dim3(256), dm, b);
+        else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, true, false>), dim3(gw), dim3(256), dm, b);
       } else {
-        LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false>), dim3(gw), dim3(256), dm, b);
+        if (nud) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, true>), dim3(gw), dim3(256), dm, b);
+        else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, false>), dim3(gw), dim3(256), dm, b);
       }
       cur_thil = 1 - cur_thil;
       LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
@@ -1625,6 +1649,9 @@ int olamgpu_upload_scalar(olamgpu_ctx *c, int32_t n, const float *p, const float
 }
 int olamgpu_download_scalar(olamgpu_ctx *c, int32_t n, float *p) {
   return guarded(c, [&] { if (!p) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->download_scalar(n, p); });
+}
+int olamgpu_set_nudging(olamgpu_ctx *c, const float *wmanud, const float *vmanud, const double *rhot_nud) {
+  return guarded(c, [&] { c->impl->set_nudging(wmanud, vmanud, rhot_nud); });
 }
 int olamgpu_set_turbulence(olamgpu_ctx *c, const float *vkh, const float *vkm, const int32_t *khtop,
                            const int32_t *kmtop, const float *akm_sfc) {

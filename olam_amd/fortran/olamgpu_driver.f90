@@ -8,7 +8,8 @@
 !                     wind_order scal_order iscal_monot divh_level vort_level
 !                     divh_fact vort_fact dtsm dtlm nacoust adjoint_tape fcoriol nsteps
 !                     ithil_monot iwind_monot initial, then (zmin fact expon) of
-!                     rayf, rayfw, rayfdiv, rayfmix, then akmin
+!                     rayf, rayfw, rayfdiv, rayfmix, then akmin nudflag
+!                     nud_preserve_total_mass
 !   <dir>/fields.txt  one line per array: group name bytes, group = mesh | state | scalar
 !                     (mesh arrays also carry mza mwa mva mma in params.txt's mesh line)
 !   <dir>/<name>.bin  raw bytes of each array in Fortran layout
@@ -49,7 +50,8 @@ program olamgpu_fdriver
        prm%ithil_monot, prm%iwind_monot, prm%initial, &
        prm%rayf_zmin, prm%rayf_fact, prm%rayf_expon, prm%rayfw_zmin, prm%rayfw_fact, prm%rayfw_expon, &
        prm%rayfdiv_zmin, prm%rayfdiv_fact, prm%rayfdiv_expon, &
-       prm%rayfmix_zmin, prm%rayfmix_fact, prm%rayfmix_expon, prm%akmin
+       prm%rayfmix_zmin, prm%rayfmix_fact, prm%rayfmix_expon, prm%akmin, &
+       prm%nudflag, prm%nud_preserve_total_mass
   read(u, *) sizes
   close(u)
   mesh%mza = sizes(1); mesh%mwa = sizes(2); mesh%mva = sizes(3); mesh%mma = sizes(4)

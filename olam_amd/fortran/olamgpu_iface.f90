@@ -72,6 +72,8 @@ module olamgpu_iface
      real(c_float)      :: rayfdiv_zmin = 30000.0, rayfdiv_fact = 0.0, rayfdiv_expon = 1.0
      real(c_float)      :: rayfmix_zmin = 30000.0, rayfmix_fact = 0.0, rayfmix_expon = 1.0
      real(c_float)      :: akmin = 0.0     ! nl%akmin(1), comp_horiz_k
+     ! nudging: nl%nudflag, nl%nud_preserve_total_mass (1 = .true.)
+     integer(c_int32_t) :: nudflag = 0, nud_preserve_total_mass = 1
   end type olamgpu_params_t
 
   type, bind(C), public :: olamgpu_state_t
@@ -88,7 +90,7 @@ module olamgpu_iface
   public :: olamgpu_create, olamgpu_destroy, olamgpu_last_error, olamgpu_upload_state, &
        olamgpu_download_state, olamgpu_upload_scalar, olamgpu_download_scalar, &
        olamgpu_init_wrtv_rk, olamgpu_tend0, olamgpu_prog_wrtv_rk, olamgpu_timeavg_momsc, &
-       olamgpu_comp_alpha_press, olamgpu_set_turbulence, &
+       olamgpu_comp_alpha_press, olamgpu_set_turbulence, olamgpu_set_nudging, &
        olamgpu_scalar_transport_rk, olamgpu_scalar_transport_rk_adj, &
        olamgpu_upload_scalar_adj, olamgpu_download_scalar_adj, olamgpu_mass_sums, &
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
@@ -182,6 +184,14 @@ module olamgpu_iface
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx, vkh, vkm, khtop, kmtop, akm_sfc
      end function olamgpu_set_turbulence
+
+     ! mem_nudge fields: wmanud real(4) (mza, mwa), vmanud real(4) (mza, mva), rhot_nud
+     ! real(8) (mza, mwa); c_null_ptr leaves a field as it was
+     integer(c_int) function olamgpu_set_nudging(ctx, wmanud, vmanud, rhot_nud) &
+          bind(C, name="olamgpu_set_nudging")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, wmanud, vmanud, rhot_nud
+     end function olamgpu_set_nudging
 
      ! rho_old: c_loc of a real(4) (mza, mwa) array, or c_null_ptr for the copy saved by tend0
      integer(c_int) function olamgpu_scalar_transport_rk(ctx, rho_old) &

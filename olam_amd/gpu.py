@@ -41,6 +41,7 @@ def lib():
         L.olamgpu_upload_state.argtypes = [vp, vp]
         L.olamgpu_comp_alpha_press.argtypes = [vp, vp, vp, vp]
         L.olamgpu_set_turbulence.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.olamgpu_set_nudging.argtypes = [vp, vp, vp, vp]
         L.olamgpu_download_state.argtypes = [vp, vp]
         L.olamgpu_upload_scalar.argtypes = [vp, C.c_int32, vp, vp]
         L.olamgpu_download_scalar.argtypes = [vp, C.c_int32, vp]
@@ -68,7 +69,7 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
            "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef",
-           "olamgpu_comp_alpha_press", "olamgpu_set_turbulence"]
+           "olamgpu_comp_alpha_press", "olamgpu_set_turbulence", "olamgpu_set_nudging"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
@@ -151,6 +152,13 @@ class Context:
 
     def timeavg_momsc(self):
         self._chk(lib().olamgpu_timeavg_momsc(self.h), "timeavg_momsc")
+
+    def set_nudging(self, wmanud=None, vmanud=None, rhot_nud=None):
+        """mem_nudge fields: wmanud (mwa, mza), vmanud (mva, mza) FP32, rhot_nud (mwa, mza)
+        FP64; None leaves a field as it was."""
+        a = [None if x is None else np.ascontiguousarray(x, dtype=dt)
+             for x, dt in ((wmanud, np.float32), (vmanud, np.float32), (rhot_nud, np.float64))]
+        self._chk(lib().olamgpu_set_nudging(self.h, *(_p(x) for x in a)), "set_nudging")
 
     def set_turbulence(self, vkh, vkm, khtop, kmtop, akm_sfc):
         """mem_turb fields of the PBL scheme: vkh, vkm (mwa, mza) FP32 at W levels, khtop,

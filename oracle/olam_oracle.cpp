@@ -93,6 +93,7 @@ struct Base {
   virtual void init_wrtv_rk() = 0;
   virtual void comp_alpha_press(const float *, const float *, const float *) = 0;
   virtual int set_turbulence(const float *, const float *, const int32_t *, const int32_t *, const float *) = 0;
+  virtual int set_nudging(const float *, const float *, const double *) = 0;
   virtual void tend0() = 0;
   virtual void prog_wrtv_rk() = 0;
   virtual void timeavg_momsc() = 0;
@@ -228,6 +229,7 @@ struct Model : Base {
     pdef.assign(ns, 1);
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
     if (m->dzit_bot) load1(dzit_bot, m->dzit_bot, mza);
+    if (prm.nudflag > 0) { wmanud.alloc(mza, mwa); vmanud.alloc(mza, mva); rhot_nud.alloc(mza, mwa); }
     if (m->zm && m->zt) {   // oracle_create checks that Rayleigh friction has them
       load1(zm, m->zm, mza);
       load1(zt, m->zt, mza);
@@ -283,6 +285,17 @@ struct Model : Base {
     }
     if (dorayfw)
       for (int k = krayfw_bot; k <= mza - 1; ++k) b20[k] = 1.0f + (float)prm.dtsm * rayf_cofw[k];
+  }
+
+  // ---- nudging fields (oisan/mem_nudge.f90:46,90-91), zero until set ----
+  F2<R> wmanud, vmanud;
+  F2<RP> rhot_nud;
+  int set_nudging(const float *wm, const float *vm, const double *rt) override {
+    if (!wmanud.n) { wmanud.alloc(mza, mwa); vmanud.alloc(mza, mva); rhot_nud.alloc(mza, mwa); }
+    if (wm) load2(wmanud, wm, mza, mwa);
+    if (vm) load2(vmanud, vm, mza, mva);
+    if (rt) load2(rhot_nud, rt, mza, mwa);
+    return 0;
   }
 
   // ---- turbulence of the host's PBL scheme (mem_turb) and comp_horiz_k's results ----
@@ -1085,7 +1098,10 @@ struct Model : Base {
         }
 #pragma omp parallel for
       for (int iv = 2; iv <= mva; ++iv) {
-        for (int k = 2; k <= mza; ++k) vmca(k, iv) = vmc(k, iv) * arv(k, iv);
+        if (prm.nudflag > 0 && prm.nud_preserve_total_mass)  // :347-355
+          for (int k = 2; k <= mza; ++k) vmca(k, iv) = vmc(k, iv) * arv(k, iv) + vmanud(k, iv);
+        else
+          for (int k = 2; k <= mza; ++k) vmca(k, iv) = vmc(k, iv) * arv(k, iv);
         if (istage == prm.nrk_wrtv)
           for (int k = 2; k <= mza; ++k) vmasc(k, iv) = vmasc(k, iv) + vmca(k, iv);
       }
@@ -1129,7 +1145,23 @@ struct Model : Base {
     vflux_vxe[ka - 1] = 0; vflux_vxe[mza] = 0;
     vflux_vye[ka - 1] = 0; vflux_vye[mza] = 0;
     vflux_vze[ka - 1] = 0; vflux_vze[mza] = 0;
-    for (int k = ka; k <= mza; ++k) hflx_rho[k] = 0;
+    if (prm.nudflag == 1 && prm.nud_preserve_total_mass) {  // nudging mass fluxes, :605-621
+      for (int k = ka; k <= mza - 1; ++k) {
+        vflux_thil[k] = (RP)(wmanud(k, iw) * thil_upw(k, iw));
+        vflux_vxe[k] = wmanud(k, iw) * vxe_upw(k, iw);
+        vflux_vye[k] = wmanud(k, iw) * vye_upw(k, iw);
+        vflux_vze[k] = wmanud(k, iw) * vze_upw(k, iw);
+      }
+      for (int k = ka; k <= mza; ++k) {
+        hflx_rho[k] = (RP)(wmanud(k - 1, iw) - wmanud(k, iw));
+        thilt_rk(k, iw) = (R)((RP)thilt_rk(k, iw) + vflux_thil[k - 1] - vflux_thil[k]);
+        vmxet_rk(k, iw) = vmxet_rk(k, iw) + vflux_vxe[k - 1] - vflux_vxe[k];
+        vmyet_rk(k, iw) = vmyet_rk(k, iw) + vflux_vye[k - 1] - vflux_vye[k];
+        vmzet_rk(k, iw) = vmzet_rk(k, iw) + vflux_vze[k - 1] - vflux_vze[k];
+      }
+    } else {
+      for (int k = ka; k <= mza; ++k) hflx_rho[k] = 0;
+    }
     for (int jv = 1; jv <= npoly[iw]; ++jv) {  // :635-661
       int iv = w_iv(jv, iw);
       if (dirv(jv, iw) > R(0)) {
@@ -1163,6 +1195,8 @@ struct Model : Base {
       delex_rho[k] = b10[k] * hflx_rho[k];
       delex_rhothil[k] = b10[k] * (RP)thilt_rk(k, iw);
     }
+    if (prm.nudflag > 0 && !prm.nud_preserve_total_mass)  // :703-706
+      for (int k = ka; k <= mza; ++k) delex_rho[k] = delex_rho[k] + dt8 * rhot_nud(k, iw);
     if (istage > 1)
       for (int k = ka; k <= mza; ++k) {  // :710-715
         delex_rho[k] = delex_rho[k] + rho0(k, iw) - rho(k, iw);
@@ -1197,7 +1231,10 @@ struct Model : Base {
     }
     if (ka <= mza - 1) tridiffo(ka, mza - 1, b31, b32, b33, rhs, wmf);
     wma[ka - 1] = 0; wma[mza] = 0;
-    for (int k = ka; k <= mza - 1; ++k) wma[k] = (R)wmf[k] * arw(k, iw);
+    if (prm.nudflag > 0 && prm.nud_preserve_total_mass)  // :782-788
+      for (int k = ka; k <= mza - 1; ++k) wma[k] = (R)wmf[k] * arw(k, iw) + wmanud(k, iw);
+    else
+      for (int k = ka; k <= mza - 1; ++k) wma[k] = (R)wmf[k] * arw(k, iw);
     for (int k = ka; k <= mza - 1; ++k) {
       vflux_thil[k] = (RP)(wma[k] * thil_upw(k, iw));
       vflux_vxe[k] = wma[k] * vxe_upw(k, iw);
@@ -1982,6 +2019,9 @@ int oracle_download_scalar_adj(void *h, int32_t n, float *p) { return ((oracle_c
 int oracle_tend0(void *h) { ((oracle_ctx *)h)->m->tend0(); return 0; }
 int oracle_rayf_profiles(void *h, float *out, int32_t *kb) { ((oracle_ctx *)h)->m->rayf_profiles(out, kb); return 0; }
 int oracle_setup_tendencies(void *h, double *out) { return ((oracle_ctx *)h)->m->setup_tendencies(out); }
+int oracle_set_nudging(void *h, const float *wmanud, const float *vmanud, const double *rhot_nud) {
+  return ((oracle_ctx *)h)->m->set_nudging(wmanud, vmanud, rhot_nud);
+}
 int oracle_set_turbulence(void *h, const float *vkh, const float *vkm, const int32_t *khtop, const int32_t *kmtop,
                           const float *akm_sfc) {
   return ((oracle_ctx *)h)->m->set_turbulence(vkh, vkm, khtop, kmtop, akm_sfc);

@@ -57,6 +57,7 @@ def lib():
         L.oracle_rayf_profiles.argtypes = [vp, vp, vp]
         L.oracle_comp_alpha_press.argtypes = [vp, vp, vp, vp]
         L.oracle_set_turbulence.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.oracle_set_nudging.argtypes = [vp, vp, vp, vp]
         L.oracle_setup_tendencies.argtypes = [vp, vp]
         _lib = L
     return _lib
@@ -118,6 +119,11 @@ class Oracle:
     def comp_alpha_press(self, rr_v, rr_w, theta):
         a = [np.ascontiguousarray(x, dtype=np.float32) for x in (rr_v, rr_w, theta)]
         lib().oracle_comp_alpha_press(self.h, *(_p(x) for x in a))
+
+    def set_nudging(self, wmanud=None, vmanud=None, rhot_nud=None):
+        a = [None if x is None else np.ascontiguousarray(x, dtype=dt)
+             for x, dt in ((wmanud, np.float32), (vmanud, np.float32), (rhot_nud, np.float64))]
+        lib().oracle_set_nudging(self.h, *(_p(x) for x in a))
 
     def set_turbulence(self, vkh, vkm, khtop, kmtop, akm_sfc):
         f = [np.ascontiguousarray(x, dtype=np.float32) for x in (vkh, vkm, akm_sfc)]

@@ -138,7 +138,8 @@ def write_fortran_case(path, grid, params, state, scalars, nsteps):
             repr(p.dtsm), repr(p.dtlm), p.nacoust, p.adjoint_tape, repr(p.fcoriol), nsteps,
             p.ithil_monot, p.iwind_monot, p.initial,
             *(repr(float(getattr(p, f"{kind}_{x}"))) for kind in abi.RAYF_KINDS
-              for x in ("zmin", "fact", "expon")), repr(float(p.akmin)))) + "\n")
+              for x in ("zmin", "fact", "expon")), repr(float(p.akmin)), p.nudflag,
+            p.nud_preserve_total_mass)) + "\n")
         f.write(f"{grid.mza} {grid.mwa} {grid.mva} {grid.mma}\n")
     lines = []
 

@@ -1,0 +1,48 @@
+"""GPU parity of the nudging terms of prog_wrtv_rk (prog_wrtv_rk.f90:347-355,605-631,
+703-706,782-788) against the oracle through the C-ABI, in the three modes (nudging
+mass fluxes with and without the vertical nudging fluxes of nudflag == 1, and the
+density tendency rhot_nud), including the long-step mass fluxes vmasc/wmasc.
+Tolerances as in test_gpu_turbulence.py.
+"""
+
+import numpy as np
+import pytest
+
+from olam_amd import abi, cases, mesh as M
+from olam_amd.gpu import Context
+from helpers import DYN_FIELDS, dyn_fields, rel_linf
+from oracle import Oracle
+from test_nudging import nudging_fields
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec,nudflag,preserve", [("f64", 1, 1), ("f32", 1, 1), ("f64", 2, 1), ("f64", 1, 0),
+                                                   ("f32", 1, 0)])
+def test_nudging_step_parity(prec, nudflag, preserve):
+    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(20))
+    st = cases.jet_state(grid)
+    rng = np.random.default_rng(31)
+    st.arrays["thil"][1:] += rng.standard_normal(st.arrays["thil"][1:].shape).astype(np.float32)
+    fields = nudging_fields(grid, 7)
+    prm = abi.make_params(prec, num_scalar=1, dtsm=60.0, nudflag=nudflag, nud_preserve_total_mass=preserve)
+    print(f"nudging {prec} nudflag {nudflag} preserve {preserve}", flush=True)
+    out = {}
+    for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
+        be.upload_state(st)
+        be.init_wrtv_rk()
+        be.set_nudging(*fields)
+        be.tend0()
+        for _ in range(3):
+            be.prog_wrtv_rk()
+        out[name] = dyn_fields(be, grid)
+        be.close()
+    tol_big = 1e-11 if prec == "f64" else 1e-5
+    tol_small = 1.2e-7 if prec == "f64" else 1e-4
+    for f in DYN_FIELDS:
+        e = rel_linf(out["gpu"][f], out["oracle"][f])
+        print(f"  {f:6s} rel Linf {e:.3e}")
+        tol = tol_small if f in ("wmc", "wc", "wmasc") else tol_big
+        if prec == "f64" and f == "wc":
+            tol = 1e-7
+        assert e < tol, (f, e)
