@@ -1,0 +1,81 @@
+"""End-to-end GPU parity in the configuration of the reference's own test namelist
+(build_olam_test/OLAMIN:251-259,391-400,404-458,497-519): dtlong 60 s with 3 acoustic
+steps, RK3 acoustic and RK2 scalar steps, second-order advection, sixth-order
+divergence (0.12) and vorticity (0.075) damping, Rayleigh friction on W, on the
+divergence and on the top winds above 27 km, horizontally inhomogeneous initial state
+(initial = 2), a moist atmosphere (comp_alpha_press every long step) and PBL
+turbulence (idiffk = 2: eddy and horizontal diffusion from the PBL scheme's fields).
+Three long steps of tend0, 3 x prog_wrtv_rk, timeavg_momsc and scalar_transport_rk with
+three tracers, then the tracer adjoint of the last step, against the oracle through the
+C-ABI.  FP32 tolerances as in test_gpu_parity.py; FP64 as explained below.
+"""
+
+import numpy as np
+import pytest
+
+from olam_amd import abi, cases, mesh as M
+from olam_amd.gpu import Context
+from helpers import DYN_FIELDS, dyn_fields, rel_linf
+from oracle import Oracle
+from test_full_config import OLAMIN_RAYF
+from test_turbulence import turbulence
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_olamin_configuration_parity(prec):
+    grid = abi.FortranGrid(M.build_mesh(16), M.vertical_grid(30))
+    st = cases.jet_state(grid)
+    rng = np.random.default_rng(41)
+    shape = (grid.mwa, grid.mza)
+    rr_v = rng.uniform(0.0, 0.015, shape).astype(np.float32)
+    rr_w = (rr_v + rng.uniform(0.0, 0.003, shape)).astype(np.float32)
+    theta = (st.arrays["thil"] * (1.0 + 0.005 * rng.uniform(0.0, 1.0, shape))).astype(np.float32)
+    turb = turbulence(grid, st, seed=9, drag=True)
+    tracers = cases.lognormal_tracers(grid, 3)
+    prm = abi.make_params(prec, num_scalar=3, nrk_wrtv=3, nrk_scal=2, order=2, dtsm=20.0, dtlm=60.0,
+                          nacoust=3, divh_damp_level=3, divh_damp_fact=0.12, vort_damp_level=3,
+                          vort_damp_fact=0.075, initial=2, rayf=OLAMIN_RAYF)
+    lam = [rng.standard_normal(shape).astype(np.float32) for _ in range(3)]
+    print(f"OLAMIN configuration {prec}: NXP=16, L=30, 3 long steps", flush=True)
+    out, trc, adj = {}, {}, {}
+    for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
+        be.upload_state(st)
+        be.init_wrtv_rk()
+        for n in range(3):
+            be.upload_scalar(n + 1, tracers[n], None)
+        for _ in range(3):
+            be.set_turbulence(*turb)
+            be.comp_alpha_press(rr_v, rr_w, theta)
+            be.tend0()
+            for _ in range(prm.nacoust):
+                be.prog_wrtv_rk()
+            be.timeavg_momsc()
+            be.scalar_transport_rk()
+        out[name] = dyn_fields(be, grid)
+        trc[name] = [be.download_scalar(n + 1) for n in range(3)]
+        for n in range(3):
+            be.upload_scalar_adj(n + 1, lam[n])
+        be.scalar_transport_rk_adj()
+        adj[name] = [be.download_scalar_adj(n + 1) for n in range(3)]
+        be.close()
+    # FP64 mode: rho and press come back in FP64 (1e-11); every other field, the tracers
+    # and their adjoints come back through FP32 downloads, where FP64 values that differ
+    # in their last bits (FP64 PCR against Thomas in the W and eddy-diffusion solves,
+    # amplified over nine acoustic steps) can round to neighbouring FP32 values: one such
+    # flip at the largest value is 2**-23 relative
+    for f in DYN_FIELDS:
+        e = rel_linf(out["gpu"][f], out["oracle"][f])
+        print(f"  {f:6s} rel Linf {e:.3e}")
+        if prec == "f64":
+            tol = 1e-11 if f in ("rho", "press") else 1.2e-7
+        else:
+            tol = 1e-4 if f in ("wmc", "wc", "wmasc") else 1e-5
+        assert e < tol, (f, e)
+    for n in range(3):
+        e = rel_linf(trc["gpu"][n], trc["oracle"][n])
+        ea = rel_linf(adj["gpu"][n], adj["oracle"][n])
+        print(f"  tracer {n + 1}: rel Linf {e:.3e}, adjoint {ea:.3e}")
+        assert e < (1.2e-7 if prec == "f64" else 1e-5)
+        assert ea < (1.2e-7 if prec == "f64" else 1e-5)
