@@ -510,8 +510,11 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
 // prog_wrtv_rk.f90:494-884 (global domain) fused with the face
 // values of fluxes_wrtv's scalar_v_column (:1022-1081) and scalar_w_column.
 
+// The FP32 default instance is held to 64 VGPRs (8 waves per SIMD; a few spills, 3 %
+// faster at NXP=320, L=60); the other instances would spill far more under that bound.
 template <class R, class RP, bool MONO, bool NUD>
-__global__ __launch_bounds__(256) void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
+__global__ __launch_bounds__(256, (sizeof(R) == 4 && sizeof(RP) == 4 && !MONO && !NUD) ? 8 : 1)
+void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
   const int c = wave_item();
   if (c >= m.nw) return;
