@@ -21,6 +21,12 @@ stage weight) would violate:
                                          dry mass sum rho*volt conserved to 1e-14
   compute_mass_sums (check_nans.F90)     equals a numpy FP64 sum
   relabelling                            relabelled mesh gives bit-identical results
+
+The option branches have their own pin files: order-3 reconstruction (test_order3.py,
+test_mesh.py), monotone dynamics fluxes (test_monot_fluxes.py), Rayleigh friction
+(test_rayleigh.py), eddy and horizontal diffusion (test_turbulence.py), moist pressure
+factors (test_moist_factors.py), nudging (test_nudging.py) and all of them together
+(test_full_config.py).
 """
 
 import numpy as np
