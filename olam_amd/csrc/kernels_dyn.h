@@ -585,8 +585,9 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     vmca_j[j] = (j < np) ? vmc_j[j] * arv_j[j] : R(0);
     if (NUD && b.nud_vm && j < np)   // vmca = vmc * arv + vmanud (:347-355)
       vmca_j[j] = vmca_j[j] + ldv(b.vmanud, (long)m.w_face[c * NSLOT + j] * ldk + l, act, R(0));
-    sel_j[j] = (dv < R(0)) ? (vmc_j[j] < R(0)) : !(vmc_j[j] < R(0));
-    need |= (__ballot(sel_j[j] && act) != 0 ? 1u : 0u) << j;
+    const bool selj = (dv < R(0)) ? (vmc_j[j] < R(0)) : !(vmc_j[j] < R(0));
+    if constexpr (NUD) sel_j[j] = selj;
+    need |= (__ballot(selj && act) != 0 ? 1u : 0u) << j;
   }
   for_slots(np, [&](int j) {
     const int nbr = m.w_nbr[c * NSLOT + j];
@@ -595,7 +596,12 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     const R dxn = m.w_dxn[c * NSLOT + j], dyn = m.w_dyn[c * NSLOT + j];
     const long in = (long)nbr * ldk + l;
     const R vmca = vmca_j[j];
-    const bool sel = sel_j[j];
+    // upwind side from the sign of vmc; without nudging vmca = vmc * arv has that sign on
+    // every level 2..mza (arv > 0 there), so it is taken from vmca instead of being kept
+    // per face (fewer live registers)
+    bool sel;
+    if constexpr (NUD) sel = sel_j[j];
+    else sel = (dv < R(0)) ? (vmca < R(0)) : !(vmca < R(0));
     const bool nact = act && ((need >> j) & 1u);
     R f[4];
     for (int q = 0; q < 4; ++q) {
