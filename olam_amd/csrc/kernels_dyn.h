@@ -512,8 +512,9 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
 
 // The FP32 default instance is held to 64 VGPRs (8 waves per SIMD; a few spills, 3 %
 // faster at NXP=320, L=60); the other instances would spill far more under that bound.
-template <class R, class RP, bool MONO, bool NUD>
-__global__ __launch_bounds__(256, (sizeof(R) == 4 && sizeof(RP) == 4 && !MONO && !NUD) ? 8 : 1)
+// Q3: some field has horizontal order 3 (the quadratic terms are only kept then).
+template <class R, class RP, bool MONO, bool NUD, bool Q3>
+__global__ __launch_bounds__(256, (sizeof(R) == 4 && sizeof(RP) == 4 && !MONO && !NUD && !Q3) ? 8 : 1)
 void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
   const int c = wave_item();
@@ -537,7 +538,7 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   for (int q = 0; q < 4; ++q) {
     const int ord = q == 0 ? b.thil_order : b.wind_order;
     if (ord > 1) { gxs[q] = ld(b.g[2 * q], ic, act); gys[q] = ld(b.g[2 * q + 1], ic, act); }
-    if (ord == 3)
+    if (Q3 && ord == 3)
       for (int t = 0; t < 3; ++t) gqs[q][t] = ld(b.gq[q][t], ic, act);
   }
   const R qc[4] = {thc, vxc, vyc, vzc};
@@ -613,7 +614,7 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
         fs = qc[q] + dxs * gxs[q] + dys * gys[q];
         fn = qn + dxn * ld(b.g[2 * q], in, nact) + dyn * ld(b.g[2 * q + 1], in, nact);
       }
-      if (ord == 3) {   // the unbounded fit at the bounded offsets xx0_v (:1057-1071)
+      if (Q3 && ord == 3) {   // the unbounded fit at the bounded offsets xx0_v (:1057-1071)
         const long e = (long)c * NSLOT + j;
         fs = fs + m.w_os[0][0][e] * gqs[q][0] + m.w_os[0][1][e] * gqs[q][1] + m.w_os[0][2][e] * gqs[q][2];
         fn = fn + m.w_on[0][0][e] * ld(b.gq[q][0], in, nact) + m.w_on[0][1][e] * ld(b.gq[q][1], in, nact) +
@@ -625,7 +626,7 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
           // fluxes_wrtv_monot: scpt of the upwind cell plus the renormalised correction
           // (:1326-1374, :1251-1266); at order 3 the bounded fit with xx0_vu offsets
           R hs = fs, hn = fn;
-          if (ord == 3) {
+          if (Q3 && ord == 3) {
             const long e = (long)c * NSLOT + j;
             hs = qc[q] + dxs * gxs[q] + dys * gys[q];
             hs = hs + m.w_os[1][0][e] * gqs[q][0] + m.w_os[1][1][e] * gqs[q][1] + m.w_os[1][2][e] * gqs[q][2];

@@ -1067,6 +1067,18 @@ struct Impl : ImplBase {
     HIPCHK(hipStreamSynchronize(s));
   }
 
+  // k_dyn_w instance for the stage: monotone fluxes, nudging, order-3 fields
+  template <bool MONO> void launch_dyn_w(const DynBuf<R, RP> &b, bool nud, unsigned gw) {
+    const bool q3 = b.thil_order == 3 || b.wind_order == 3;
+    if (nud) {
+      if (q3) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, true, true>), dim3(gw), dim3(256), dm, b);
+      else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, true, false>), dim3(gw), dim3(256), dm, b);
+    } else {
+      if (q3) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, true>), dim3(gw), dim3(256), dm, b);
+      else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, false>), dim3(gw), dim3(256), dm, b);
+    }
+  }
+
   // rayf_init's copy of the initial wind (mem_rayf.f90:158-171); the profiles and b20
   // were built with the mesh
   void init_wrtv_rk() override {
@@ -1190,11 +1202,9 @@ struct Impl : ImplBase {
         }
         LAUNCH("k_dyn_monot_pre", (k_dyn_monot_pre<R, RP>), dim3(gw), dim3(256), dm, mo, dp<RP>(rho0), (R)b.dt8);
         LAUNCH("k_dyn_monot_scales", (k_dyn_monot_scales<R, RP>), dim3(gw), dim3(256), dm, b, mo);
-        if (nud) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, true, true>), dim3(gw), dim3(256), dm, b);
-        else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, true, false>), dim3(gw), dim3(256), dm, b);
+        launch_dyn_w<true>(b, nud, gw);
       } else {
-        if (nud) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, true>), dim3(gw), dim3(256), dm, b);
-        else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, false>), dim3(gw), dim3(256), dm, b);
+        launch_dyn_w<false>(b, nud, gw);
       }
       cur_thil = 1 - cur_thil;
       LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
