@@ -88,7 +88,10 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
   const R *v[3] = {vxe, vye, vze};
   const RP r = rho[ic];
   const R th = thil[ic];
-  const R vv[3] = {v[0][ic], v[1][ic], v[2][ic]};
+  // the own winds are read only by the diffusion and the Rayleigh top mixing
+  R vv[3] = {R(0), R(0), R(0)};
+  if (tb.on || m.rayf_cofmix)
+    for (int q = 0; q < 3; ++q) vv[q] = v[q][ic];
   const R v4 = (R)__builtin_nontemporal_load(m.volt + ic);
   R tt = __builtin_nontemporal_load(thilt + ic) * v4;
   R vm[3] = {__builtin_nontemporal_load(vmxet + ic) * v4, __builtin_nontemporal_load(vmyet + ic) * v4,
