@@ -25,9 +25,10 @@
  *    the ABI.
  *  - Calls are ordered on the context's HIP stream and return after the stream is
  *    synchronised.  One host thread per context.
- *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere), mza <= 64 and horizontal
- *    advection orders 1 and 2 are supported; create() rejects anything else with
- *    OLAMGPU_EINVAL.
+ *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere) and mza <= 64 are supported;
+ *    horizontal advection orders 1, 2 and 3 are built (order 3 needs the mem_adv
+ *    least-squares arrays in the mesh descriptor).  create() rejects anything else
+ *    with OLAMGPU_EINVAL.
  */
 #ifndef OLAMGPU_H
 #define OLAMGPU_H
@@ -140,6 +141,11 @@ typedef struct olamgpu_params {
    * density (:703-706).  The fields are set by olamgpu_set_nudging (zero until then). */
   int32_t nudflag;
   int32_t nud_preserve_total_mass;
+  /* nud_preserve_mix_ratio (oname_coms.f90:139, default true): with nudflag > 0 the
+   * tracer step's final-stage density is rho - dtlm * rhot_nud (scalar_transport_rk.F90:
+   * 120-133), so the nudging of the density does not change the mixing ratios.  As
+   * oname_check.f90:218 does, it is ignored while nud_preserve_total_mass is on. */
+  int32_t nud_preserve_mix_ratio;
 } olamgpu_params;
 
 /* Prognostic and diagnostic fields of mem_basic/mem_tend (mem_basic.f90:8-34,
@@ -181,6 +187,13 @@ int olamgpu_download_scalar(olamgpu_ctx *ctx, int32_t n, float *var_p);
  * tracers are limited, and with iscal_monot > 0 only pdef tracers get the low-order
  * tendency update of scp0 before the final stage (scalar_transport_rk.F90:297-337). */
 int olamgpu_set_scalar_pdef(olamgpu_ctx *ctx, int32_t n, int32_t pdef);
+/* scalar_tab(n)%do_sgsmix (var_tables.f90:59, default true): once olamgpu_set_turbulence
+ * has supplied the horizontal diffusivities, each such tracer's tendency gets the
+ * horizontal diffusion of scalar_transport_rk.F90:244-274 at the start of every tracer
+ * step (built from the tracer before the step, added to var_t).  The adjoint includes
+ * its transpose; the adjoint of a diffused tracer that also has a final-stage limiter
+ * is not built and fails with OLAMGPU_EINVAL. */
+int olamgpu_set_scalar_sgsmix(olamgpu_ctx *ctx, int32_t n, int32_t do_sgsmix);
 
 /* ---- the hot path ---------------------------------------------------------- */
 

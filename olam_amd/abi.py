@@ -65,6 +65,7 @@ class Params(C.Structure):
         ("rayfmix_zmin", C.c_float), ("rayfmix_fact", C.c_float), ("rayfmix_expon", C.c_float),
         ("akmin", C.c_float),
         ("nudflag", C.c_int32), ("nud_preserve_total_mass", C.c_int32),
+        ("nud_preserve_mix_ratio", C.c_int32),
     ]
 
 
@@ -90,7 +91,8 @@ RAYF_KINDS = ("rayf", "rayfw", "rayfdiv", "rayfmix")
 def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, iscal_monot=0,
                 divh_damp_level=3, vort_damp_level=3, divh_damp_fact=0.1, vort_damp_fact=0.075,
                 dtsm=60.0, dtlm=None, nacoust=1, adjoint_tape=1, ithil_monot=0, iwind_monot=0,
-                initial=1, rayf=None, akmin=0.0, nudflag=0, nud_preserve_total_mass=1):
+                initial=1, rayf=None, akmin=0.0, nudflag=0, nud_preserve_total_mass=1,
+                nud_preserve_mix_ratio=1):
     """Namelist defaults of oname_coms.f90:190-238, with RK3 for scalars.
 
     ``rayf`` maps a Rayleigh friction kind ("rayf", "rayfw", "rayfdiv", "rayfmix") to
@@ -106,7 +108,7 @@ def make_params(precision="f32", num_scalar=1, nrk_wrtv=3, nrk_scal=3, order=2, 
     return Params(precision, num_scalar, nrk_wrtv, nrk_scal, order, order, order, iscal_monot,
                   divh_damp_level, vort_damp_level, divh_damp_fact, vort_damp_fact, dtsm, dtlm,
                   nacoust, adjoint_tape, 2.0 * M.OMEGA, ithil_monot, iwind_monot, initial, *rf, akmin,
-                  nudflag, nud_preserve_total_mass)
+                  nudflag, nud_preserve_total_mass, nud_preserve_mix_ratio)
 
 
 def _ptr(a, ctype):

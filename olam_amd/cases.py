@@ -14,6 +14,10 @@
   u = 35 sin^2(2 phi) F(p) with F = exp(-(ln(p/250 hPa)/0.5)^2 / 2), and a 1 K
   Gaussian theta bump at a seeded position.  The jet is not in thermal-wind
   balance (only the vertical is balanced); it is a benchmark state, not a climate.
+* ``rotating_state``: a steady solution of the equations prog_wrtv_rk discretises
+  (deep, non-hydrostatic, compressible, constant gravity, full Coriolis 2 Omega):
+  an isothermal atmosphere in rigid rotation about the polar axis, v = w' z x r,
+  whose pressure balances gravity, Coriolis and the centripetal acceleration.
 
 Random draws use a counter-based generator keyed by (seed, stream, level, canonical
 point id), so values do not depend on the order in which points are numbered.
@@ -236,3 +240,124 @@ def reference_sounding(grid: FortranGrid, st: FortranState):
     dn = st.arrays["rho"][1:].mean(axis=0)
     th[0], dn[0] = th[1], dn[1]
     return th.astype(np.float32), dn.astype(np.float32)
+
+
+def _face_mean_rho(rh, mesh):
+    iw1, iw2 = mesh.v_iw[:, 0], mesh.v_iw[:, 1]
+    return 0.5 * (rh[:, iw1] + rh[:, iw2])
+
+
+def rotating_state(grid: FortranGrid, u0: float = 20.0, t0: float = 250.0, fcoriol: float = 2.0 * M.OMEGA):
+    """Rigid rotation v = w' (z x r) with w' = u0 / a (u0 m/s eastward at the equator at
+    the ground) in an isothermal atmosphere at t0, for a model run with Coriolis
+    parameter ``fcoriol`` (2 Omega, or 0).
+
+    In the rotating frame the flow's acceleration is the centripetal -w'^2 r_perp, the
+    Coriolis force -fcoriol z x v = fcoriol w' r_perp, and gravity -g r_hat (constant g,
+    as the model's vertical weights gdz_wgt*8 use).  Steady balance needs
+    grad p / rho = -grad Phi with Phi = g (r - a) - C |r_perp|^2 / 2, C = fcoriol w' + w'^2,
+    so an isothermal state p = p_s exp(-Phi / (R t0)) is an exact steady solution: theta
+    and rho are functions of Phi, whose surfaces the flow moves along.
+
+    Discretely the state is built as the model sees it:
+    * theta from the closed form at each level; p at the lowest level from the closed
+      form and above it integrated with the model's own vertical balance of
+      prog_wrt_begs (prog_wrtv_rk.f90:727-745: (p_k - p_k+1) dzim = gdz_wgtm8 rho_k +
+      gdz_wgtp8 rho_k+1 - radial part of the flow's acceleration, mass-weighted over the
+      two levels as wnx (vmxet_k + vmxet_k+1) volwi does); press the model's
+      alpha (rho theta)^(cp/cv).  Each column's profile depends on its latitude only.
+    * The face mass fluxes from a discrete mass streamfunction at the M points,
+      vmc arv = (Psi(m1) - Psi(m2)) dzt zfact, Psi(lat) = -w' a^2 zfact
+      int_0^lat rho(lat') cos(lat') dlat' (rho of the same column integration, on a fine
+      latitude table), so the discrete mass divergence of every cell is zero and rho stays
+      put; vc = 2 vmc / (rho1 + rho2) (prog_v_begs) and vxe, vye, vze from vc by
+      vel_t3d_hex.
+    What remains unbalanced is the horizontal truncation error of the discrete pressure
+    gradient, Coriolis and momentum advection terms, which falls with the cell size.
+    """
+    mesh, vg = grid.mesh, grid.vg
+    g = mesh.geo
+    mza = vg["mza"]
+    nw, nv = mesh.nw, mesh.nv
+    zt, dzm, zfact = vg["zt"], vg["dzm"], vg["zfact"]
+    wr = u0 / M.ERAD
+    C = fcoriol * wr + wr * wr
+    rt = t0 * M.RDRY
+    alpha = M.ALPHA_DRY
+    ps = 1.0e5
+    phi0 = -0.5 * C * M.ERAD ** 2                                         # Phi at the ground, equator
+    volt = grid.cm["volt"][:, 0]                                           # volt ratios are per level
+    zb, ztp = vg["zwgt_bot8"], vg["zwgt_top8"]
+
+    def profiles(cos2):
+        """theta, p, rho (mza+1, n) of columns at the given cos^2(lat)."""
+        n = cos2.shape[0]
+        th = np.zeros((mza + 1, n)); pr = np.zeros((mza + 1, n)); rh = np.zeros((mza + 1, n))
+
+        def phi(r):
+            return M.GRAV * (r - M.ERAD) - 0.5 * C * r * r * cos2
+
+        def rho_of(pp, th_):
+            return (pp / alpha) ** (1.0 / M.CPOCV) / th_
+
+        for k in range(2, mza + 1):
+            th[k] = t0 * (M.P00 / (ps * np.exp(-(phi(M.ERAD + zt[k]) - phi0) / rt))) ** M.ROCP
+        pr[2] = ps * np.exp(-(phi(M.ERAD + zt[2]) - phi0) / rt)
+        rh[2] = rho_of(pr[2], th[2])
+        for k in range(2, mza):
+            vk, vn = volt[k], volt[k + 1]
+            rk, rn_ = M.ERAD + zt[k], M.ERAD + zt[k + 1]
+            pk, rhk = pr[k], rh[k]
+            pn = pk * np.exp(-dzm[k] * M.GRAV / rt)
+            for _ in range(10):
+                rn = rho_of(pn, th[k + 1])
+                acc = C * cos2 * (rhk * vk * rk + rn * vn * rn_) / (vk + vn)
+                f = pk - pn - dzm[k] * (M.GRAV * (zb[k] * rhk + ztp[k] * rn) - acc)
+                dfdp = -1.0 - dzm[k] * (M.GRAV * ztp[k] - C * cos2 * vn * rn_ / (vk + vn)) * rn / (M.CPOCV * pn)
+                pn = pn - f / dfdp
+            pr[k + 1] = pn
+            rh[k + 1] = rho_of(pn, th[k + 1])
+        pr[2:] = alpha * (rh[2:] * th[2:]) ** M.CPOCV
+        return th, pr, rh
+
+    xw = np.stack([g["xew"], g["yew"], g["zew"]], 1)
+    th, pr, rh = profiles((xw[:, 0] ** 2 + xw[:, 1] ** 2) / M.ERAD ** 2)
+    # mass streamfunction on a fine latitude table, evaluated at the M points
+    lat_t = np.linspace(-0.5 * math.pi, 0.5 * math.pi, 20001)
+    _, _, rh_t = profiles(np.cos(lat_t) ** 2)
+    integ = rh_t * np.cos(lat_t)[None, :]
+    cum = np.concatenate([np.zeros((mza + 1, 1)),
+                          np.cumsum(0.5 * (integ[:, 1:] + integ[:, :-1]) * np.diff(lat_t)[None, :], axis=1)], 1)
+    xm = np.stack([g["xem"], g["yem"], g["zem"]], 1)
+    lat_m = np.arcsin(np.clip(xm[:, 2] / np.linalg.norm(xm, axis=1), -1, 1))
+    im1, im2 = mesh.v_im[:, 0], mesh.v_im[:, 1]
+    vc = np.zeros((mza + 1, nv))
+    vmc = np.zeros((mza + 1, nv))
+    rface = _face_mean_rho(rh, mesh)
+    for k in range(2, mza + 1):
+        psi = -wr * M.ERAD ** 2 * zfact[k] * np.interp(lat_m, lat_t, cum[k])
+        vmc_k = (psi[im1] - psi[im2]) / g["dnu"]
+        vc[k] = (vmc_k / rface[k]).astype(np.float32)
+        vmc[k] = vmc_k.astype(np.float32)
+    vcT = vc.T
+    ivs = np.where(np.arange(7)[None, :] < mesh.npoly[:, None], mesh.w_iv, 0)
+    vxe = np.zeros((nw, mza + 1)); vye = np.zeros_like(vxe); vze = np.zeros_like(vxe)
+    for j in range(7):
+        col = vcT[ivs[:, j]]
+        vxe += g["ecvec_vx"][:, j, None] * col
+        vye += g["ecvec_vy"][:, j, None] * col
+        vze += g["ecvec_vz"][:, j, None] * col
+    vxe, vye, vze = vxe.T.copy(), vye.T.copy(), vze.T.copy()
+    for a in (vxe, vye, vze, th):
+        a[1] = a[2]
+    st = FortranState(grid)
+    A = st.arrays
+    A["thil"][...] = grid.to_f(th[1:].T.astype(np.float32))
+    A["rho"][...] = grid.to_f(rh[1:].T, "w", np.float64)
+    A["press"][...] = grid.to_f(pr[1:].T, "w", np.float64)
+    A["vmc"][...] = grid.to_f(vmc[1:].T.astype(np.float32), "v")
+    A["vc"][...] = grid.to_f(vc[1:].T.astype(np.float32), "v")
+    A["vxe"][...] = grid.to_f(vxe[1:].T.astype(np.float32))
+    A["vye"][...] = grid.to_f(vye[1:].T.astype(np.float32))
+    A["vze"][...] = grid.to_f(vze[1:].T.astype(np.float32))
+    return st

@@ -1214,6 +1214,113 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
   }
 }
 
+// ---------------------------------------------------------------- nudged density, sgs mixing
+// rhof of scalar_transport_rk.F90:120-133 when nudging preserves the mixing ratio: the
+// density less the long step's nudging density tendency, formed in FP64.
+template <class RP>
+__global__ void k_tr_rhof(const RP *rho, const RP *rhot_nud, double dtl, RP *out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (RP)((double)rho[i] - dtl * (double)rhot_nud[i]);
+}
+
+// Horizontal diffusion of the tracers into their tendencies (scalar_transport_rk.F90:
+// 244-274): sct += volti sum_faces akhodx (q_nbr - q) on levels 2..khtopv of each face.
+// Only columns with some face diffusing (kwtop >= 2, :160) change; sgs[n] is the
+// tracer's do_sgsmix.  One wave per column, all tracers.
+template <class R>
+__global__ __launch_bounds__(256) void k_tr_sgsmix(DMesh<R> m, const R *__restrict__ q, R *__restrict__ sct,
+                                                   const R *akhodx, const int *khtopv, const int *sgs, int ntr,
+                                                   long tstride) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const int np = m.w_npoly[c];
+  int nb[7], kwtop = 0;
+  R ak[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    nb[j] = m.w_nbr[c * NSLOT + j];
+    const int iv = m.w_face[c * NSLOT + j];
+    const int kt = j < np ? khtopv[iv] : 0;
+    kwtop = max(kwtop, kt);
+    ak[j] = (act && k >= 2 && k <= kt) ? akhodx[(long)iv * ldk + l] : R(0);
+  }
+  if (kwtop < 2) return;
+  const R vt = ld(m.volti, ic, act);
+  const bool upd = act && k >= 2 && k <= kwtop;
+  for (int n = 0; n < ntr; ++n) {
+    if (!sgs[n]) continue;
+    const R *qn = q + n * tstride;
+    const R qc = ld(qn, ic, act);
+    R fd = R(0);
+    for_slots(np, [&](int j) { fd = fd + ak[j] * (ld(qn, (long)nb[j] * ldk + l, act) - qc); });
+    if (upd) sct[n * tstride + ic] = sct[n * tstride + ic] + fd * vt;
+  }
+}
+
+// Adjoint of the tendency of the diffused tracers: per stage, lsct += dtr mu / rhos on
+// the prognostic levels, with mu of level 1 folded into level 2 (the explicit copy; the
+// implicit pass has already moved it there).  The first adjoint stage starts from zero.
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_tr_adj_sct(DMesh<R> m, TrAdj<R, RP> a, R *lsct, const int *sgs) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza, body = act && k >= 2;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
+  const R rinv = R(1) / (body ? rf : R(1));
+  for (int n = 0; n < a.ntr; ++n) {
+    if (!sgs[n]) continue;
+    const long off = n * a.tstride;
+    R mu = ld(a.mu + off, ic, act);
+    const R mu1 = lane0(mu);
+    if (k == 1) mu = R(0);
+    if (k == 2) mu = mu + mu1;
+    const R v = body ? mu * a.dtr * rinv : R(0);
+    const R prev = a.first ? R(0) : ld(lsct + off, ic, act);
+    if (act) lsct[off + ic] = prev + v;
+  }
+}
+
+// lq += D^T lsct for the diffused tracers: the transpose of k_tr_sgsmix,
+// lq(c) += sum_faces akhodx (volti(nbr) lsct(nbr) - volti(c) lsct(c)).
+template <class R>
+__global__ __launch_bounds__(256) void k_tr_sgsmix_adj(DMesh<R> m, const R *__restrict__ lsct, R *__restrict__ lq,
+                                                       const R *akhodx, const int *khtopv, const int *sgs, int ntr,
+                                                       long tstride) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool act = k <= m.mza;
+  const long ldk = m.ldk, ic = c * ldk + l;
+  const int np = m.w_npoly[c];
+  int nb[7];
+  R ak[7], vtn[7];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    nb[j] = m.w_nbr[c * NSLOT + j];
+    const int iv = m.w_face[c * NSLOT + j];
+    const int kt = j < np ? khtopv[iv] : 0;
+    any = any || kt >= 2;
+    ak[j] = (act && k >= 2 && k <= kt) ? akhodx[(long)iv * ldk + l] : R(0);
+    vtn[j] = ld(m.volti, (long)nb[j] * ldk + l, act);
+  }
+  if (!any) return;
+  const R vt = ld(m.volti, ic, act);
+  for (int n = 0; n < ntr; ++n) {
+    if (!sgs[n]) continue;
+    const long off = n * tstride;
+    const R lc = ld(lsct + off, ic, act);
+    R acc = ld(lq + off, ic, act);
+    for_slots(np, [&](int j) { acc = acc + ak[j] * (vtn[j] * ld(lsct + off, (long)nb[j] * ldk + l, act) - vt * lc); });
+    if (act) lq[off + ic] = acc;
+  }
+}
+
 // ---------------------------------------------------------------- mass sums
 // compute_mass_sums (check_nans.F90:224-262): per-block FP64 partial sums of
 // rho*volt and rho*volt*q_n, reduced in a fixed order (deterministic).

@@ -368,6 +368,7 @@ struct ImplBase {
   virtual size_t checkpoint_bytes(int) = 0;
   virtual void checkpoint(int, void *, int) = 0;
   virtual void set_pdef(int, int) = 0;
+  virtual void set_sgsmix(int, int) = 0;
 };
 
 // x**n for an integer n >= 0 in the type of x, one rounding per factor (Fortran's
@@ -494,6 +495,8 @@ struct Impl : ImplBase {
   bool have_sct = false;
   // final-stage limiters (iscal_monot = 1 monotone, 2 positive definite)
   std::vector<int> pdef;            // scalar_tab(n)%pdef, default .true. (var_tables.f90:58)
+  std::vector<int> sgsmix;          // scalar_tab(n)%do_sgsmix, default .true. (var_tables.f90:59)
+  DevBuf sgs_dev, rhof, lsct;       // do_sgsmix per tracer; nudged tracer density; adjoint of sct
   DevBuf lim_mode, lim_pre, lim_s1, lim_s2, lim_tsv, lim_tsw;
   bool any_lim = false, any_pre = false, taped_lim = false;
   DevBuf ffimp, cflmax, nimp, anyimp, masspart, massout;
@@ -884,6 +887,8 @@ struct Impl : ImplBase {
     if (prm.scal_horiz_adv_order == 3)
       for (DevBuf &b : trgq) { b.alloc(tb); zero(b); }
     pdef.assign(std::max(ntr, 1), 1);
+    sgsmix.assign(std::max(ntr, 1), 1);
+    sgs_dev.alloc(std::max(ntr, 1) * sizeof(int));
     if (prm.iscal_monot > 0) {
       for (DevBuf *b : {&lim_s1, &lim_s2}) { b->alloc(tb); zero(*b); }
       lim_mode.alloc(std::max(ntr, 1) * sizeof(int)); lim_pre.alloc(std::max(ntr, 1) * sizeof(int));
@@ -1249,6 +1254,29 @@ struct Impl : ImplBase {
   }
 
   void set_pdef(int n, int v) override { check_n(n); pdef[n - 1] = v != 0; }
+  void set_sgsmix(int n, int v) override { check_n(n); sgsmix[n - 1] = v != 0; }
+
+  // whether the tracer step diffuses some tracer horizontally (scalar_transport_rk.F90:
+  // 244-274: do_sgsmix tracers once the PBL's diffusivities are set); uploads the flags
+  bool sgs_setup() {
+    bool any = false;
+    for (int n = 0; n < ntr; ++n) any = any || sgsmix[n];
+    if (!dturb.on || !any) return false;
+    HIPCHK(hipMemcpyAsync(sgs_dev.p, sgsmix.data(), ntr * sizeof(int), hipMemcpyHostToDevice, s));
+    return true;
+  }
+  // the density the tracer step divides by: rho, or with nudging that preserves mixing
+  // ratios rho - dtlm * rhot_nud (scalar_transport_rk.F90:120-133); oname_check.f90:218
+  // turns nud_preserve_mix_ratio off when nud_preserve_total_mass is on
+  RP *tracer_rho() {
+    if (!(prm.nudflag > 0 && prm.nud_preserve_mix_ratio && !prm.nud_preserve_total_mass && nud_r.p))
+      return dp<RP>(rho);
+    if (!rhof.p) rhof.alloc(wsz * sizeof(RP));
+    const long n = (long)wsz;
+    LAUNCH("k_tr_rhof", (k_tr_rhof<RP>), dim3((unsigned)((n + 255) / 256)), dim3(256), dp<RP>(rho), dp<RP>(nud_r),
+           prm.dtlm, dp<RP>(rhof), n);
+    return dp<RP>(rhof);
+  }
 
   // Per-tracer final-stage treatment (scalar_transport_rk.F90:297-337): mode 1 with
   // iscal_monot = 1, mode 2 with iscal_monot = 2 for pdef tracers, else 0 (regular
@@ -1288,6 +1316,13 @@ struct Impl : ImplBase {
     if (ntr == 0) { have_fwd = true; return; }
     lim_setup();
     taped_lim = false;
+    if (sgs_setup()) {
+      if (!trsct.p) { trsct.alloc(std::max<size_t>(1, ntr) * wsz * sizeof(R)); zero(trsct); }
+      have_sct = true;
+      LAUNCH("k_tr_sgsmix", (k_tr_sgsmix<R>), dim3(nblk(nw)), dim3(256), dm, dp<R>(trS), dp<R>(trsct),
+             dp<R>(tb_akhodx), dp<int>(tb_khtopv), sgs_dev.as<int>(), ntr, tstride);
+    }
+    RP *rho_tr = tracer_rho();
     // RK3: S -> X -> Y -> X;  RK2: S -> X -> Y
     DevBuf *in = &trS, *outs[3] = {&trX, &trY, &trX};
     if (nrk == 2) { outs[0] = &trX; outs[1] = &trY; }
@@ -1298,7 +1333,7 @@ struct Impl : ImplBase {
       a.gx = dp<R>(trgx); a.gy = dp<R>(trgy); a.q_out = dp<R>(*out);
       for (int t = 0; t < 3; ++t) a.gq[t] = dp<R>(trgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
-      a.rho = dp<RP>(rho);
+      a.rho = rho_tr;
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.tstride = tstride;
       a.dtr = dtlr[istage - 1];
       a.tpw = tpw;
@@ -1364,6 +1399,14 @@ struct Impl : ImplBase {
     const bool any_imp = cfl_fresh ? cfl_any : run_cfl(false);
     cfl_fresh = true;
     cfl_any = any_imp;
+    const bool sgs = sgs_setup();
+    if (sgs) {
+      for (int n = 0; n < ntr; ++n)
+        if (sgsmix[n] && any_lim && (prm.iscal_monot == 1 || pdef[n]))
+          throw Err(OLAMGPU_EINVAL, "adjoint of tracer horizontal diffusion with a final-stage limiter is not built");
+      if (!lsct.p) lsct.alloc(std::max<size_t>(1, ntr) * wsz * sizeof(R));
+    }
+    RP *rho_tr = tracer_rho();
     // lam0 is not cleared: the first adjoint stage (a.first) starts it from zero
     for (int istage = nrk; istage >= 1; --istage) {
       TrAdj<R, RP> a;
@@ -1373,7 +1416,7 @@ struct Impl : ImplBase {
       a.lgx = dp<R>(lgx); a.lgy = dp<R>(lgy); a.lq = dp<R>(lqB);
       for (int t = 0; t < 3; ++t) a.lgq[t] = dp<R>(lgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
-      a.rho = dp<RP>(rho);
+      a.rho = rho_tr;
       a.ntr = ntr; a.nrk = nrk; a.istage = istage; a.order = prm.scal_horiz_adv_order; a.last = istage == 1;
       a.first = istage == nrk;
       a.tstride = tstride; a.dtr = dtlr[istage - 1];
@@ -1381,6 +1424,9 @@ struct Impl : ImplBase {
       a.tmode = lim_mode.as<int>();
       a.lim = (istage == nrk && any_lim) ? 1 : 0;
       if (any_imp) LAUNCH("k_tr_adj_implicit", (k_tr_adj_implicit<R, RP>), dim3(nblk(nw)), dim3(256), dm, a);
+      if (sgs)
+        LAUNCH("k_tr_adj_sct", (k_tr_adj_sct<R, RP>), dim3(nblk(nw)), dim3(256), dm, a, dp<R>(lsct),
+               sgs_dev.as<int>());
       if (a.lim) {
         const TrLim<R> L = lim_args(true);
         adjoint_faces(a);
@@ -1398,6 +1444,10 @@ struct Impl : ImplBase {
       }
       cur_adj = 1 - cur_adj;
     }
+    // the diffusion tendency was built from the tracers before the step
+    if (sgs)
+      LAUNCH("k_tr_sgsmix_adj", (k_tr_sgsmix_adj<R>), dim3(nblk(nw)), dim3(256), dm, dp<R>(lsct),
+             dp<R>(cur_adj_buf()), dp<R>(tb_akhodx), dp<int>(tb_khtopv), sgs_dev.as<int>(), ntr, tstride);
     HIPCHK(hipGetLastError());
   }
   // column and face parts of an adjoint stage for the unlimited tracers: fused in one
@@ -1696,6 +1746,9 @@ int olamgpu_copy_field(olamgpu_ctx *c, int32_t field_id, int32_t index, void *de
     if (!dev_buf || (direction != 0 && direction != 1)) throw Err(OLAMGPU_EINVAL, "bad copy arguments");
     c->impl->copy_field(field_id, index, dev_buf, direction, elem_bytes);
   });
+}
+int olamgpu_set_scalar_sgsmix(olamgpu_ctx *c, int32_t n, int32_t v) {
+  return guarded(c, [&] { c->impl->set_sgsmix(n, v); });
 }
 int olamgpu_set_scalar_pdef(olamgpu_ctx *c, int32_t n, int32_t pdef) {
   return guarded(c, [&] { c->impl->set_pdef(n, pdef); });

@@ -51,7 +51,7 @@ program olamgpu_fdriver
        prm%rayf_zmin, prm%rayf_fact, prm%rayf_expon, prm%rayfw_zmin, prm%rayfw_fact, prm%rayfw_expon, &
        prm%rayfdiv_zmin, prm%rayfdiv_fact, prm%rayfdiv_expon, &
        prm%rayfmix_zmin, prm%rayfmix_fact, prm%rayfmix_expon, prm%akmin, &
-       prm%nudflag, prm%nud_preserve_total_mass
+       prm%nudflag, prm%nud_preserve_total_mass, prm%nud_preserve_mix_ratio
   read(u, *) sizes
   close(u)
   mesh%mza = sizes(1); mesh%mwa = sizes(2); mesh%mva = sizes(3); mesh%mma = sizes(4)

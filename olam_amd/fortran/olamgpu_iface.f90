@@ -72,8 +72,9 @@ module olamgpu_iface
      real(c_float)      :: rayfdiv_zmin = 30000.0, rayfdiv_fact = 0.0, rayfdiv_expon = 1.0
      real(c_float)      :: rayfmix_zmin = 30000.0, rayfmix_fact = 0.0, rayfmix_expon = 1.0
      real(c_float)      :: akmin = 0.0     ! nl%akmin(1), comp_horiz_k
-     ! nudging: nl%nudflag, nl%nud_preserve_total_mass (1 = .true.)
+     ! nudging: nl%nudflag, nl%nud_preserve_total_mass, nl%nud_preserve_mix_ratio (1 = .true.)
      integer(c_int32_t) :: nudflag = 0, nud_preserve_total_mass = 1
+     integer(c_int32_t) :: nud_preserve_mix_ratio = 1
   end type olamgpu_params_t
 
   type, bind(C), public :: olamgpu_state_t
@@ -96,6 +97,7 @@ module olamgpu_iface
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
        olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_checkpoint_bytes, &
        olamgpu_checkpoint_save, olamgpu_checkpoint_restore, olamgpu_set_scalar_pdef, &
+       olamgpu_set_scalar_sgsmix, &
        olamgpu_error_message
 
   interface
@@ -149,6 +151,12 @@ module olamgpu_iface
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: n, pdef
      end function olamgpu_set_scalar_pdef
+
+     integer(c_int) function olamgpu_set_scalar_sgsmix(ctx, n, do_sgsmix) bind(C, name="olamgpu_set_scalar_sgsmix")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n, do_sgsmix
+     end function olamgpu_set_scalar_sgsmix
 
      integer(c_int) function olamgpu_init_wrtv_rk(ctx) bind(C, name="olamgpu_init_wrtv_rk")
        import :: c_int, c_ptr

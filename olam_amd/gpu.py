@@ -53,6 +53,7 @@ def lib():
         L.olamgpu_copy_field.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
         L.olamgpu_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
+        L.olamgpu_set_scalar_sgsmix.argtypes = [vp, C.c_int32, C.c_int32]
         L.olamgpu_checkpoint_bytes.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.olamgpu_checkpoint_save.argtypes = [vp, C.c_int32, vp]
         L.olamgpu_checkpoint_restore.argtypes = [vp, C.c_int32, vp]
@@ -69,7 +70,8 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_mass_sums", "olamgpu_cfl_report", "olamgpu_copy_field", "olamgpu_last_call_ms",
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
            "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef",
-           "olamgpu_comp_alpha_press", "olamgpu_set_turbulence", "olamgpu_set_nudging"]
+           "olamgpu_comp_alpha_press", "olamgpu_set_turbulence", "olamgpu_set_nudging",
+           "olamgpu_set_scalar_sgsmix"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
@@ -130,6 +132,9 @@ class Context:
 
     def set_scalar_pdef(self, n, pdef):
         self._chk(lib().olamgpu_set_scalar_pdef(self.h, int(n), int(bool(pdef))), "set_scalar_pdef")
+
+    def set_scalar_sgsmix(self, n, on):
+        self._chk(lib().olamgpu_set_scalar_sgsmix(self.h, int(n), int(bool(on))), "set_scalar_sgsmix")
 
     def upload_scalar_adj(self, n, lam):
         lam = np.ascontiguousarray(lam, dtype=np.float32)

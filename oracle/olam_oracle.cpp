@@ -102,6 +102,7 @@ struct Base {
   virtual void mass_sums(double *) = 0;
   virtual void cfl_report(double *) = 0;
   virtual int set_pdef(int, int) = 0;
+  virtual int set_sgsmix(int, int) = 0;
   virtual void set_frozen(int) = 0;
   virtual int tape_scales(int, int, double *) = 0;
   virtual int recon(const float *, int, int, int, double *, double *) = 0;
@@ -109,6 +110,9 @@ struct Base {
   virtual void rayf_profiles(float *, int32_t *) = 0;
   virtual int setup_tendencies(double *) = 0;
   std::string err;
+  // Test hook: a deliberate mistake planted in one term, so a pin can show that it
+  // detects that mistake (0 = none; the codes are listed at oracle_plant below).
+  int plant = 0;
 };
 
 template <class P>
@@ -140,6 +144,7 @@ struct Model : Base {
   F2<RP> rho, press;
   std::vector<F2<R>> scp, sct, lam;
   std::vector<int> pdef;                    // scalar_tab(n)%pdef (var_tables.f90:58)
+  std::vector<int> sgsmix;                  // scalar_tab(n)%do_sgsmix (var_tables.f90:59)
   // frozen limiter scales of the last forward step's final stage, per tracer
   std::vector<F2<R>> tape_sv, tape_sw;      // per V face / per W level
   std::vector<char> tape_lim;               // 0 none, 1 monotone, 2 positive definite
@@ -227,6 +232,7 @@ struct Model : Base {
     scp.resize(ns); sct.resize(ns); lam.resize(ns);
     for (int n = 0; n < ns; ++n) { scp[n].alloc(mza, mwa); sct[n].alloc(mza, mwa); lam[n].alloc(mza, mwa); }
     pdef.assign(ns, 1);
+    sgsmix.assign(ns, 1);
     tape_sv.resize(ns); tape_sw.resize(ns); tape_lim.assign(ns, 0);
     if (m->dzit_bot) load1(dzit_bot, m->dzit_bot, mza);
     if (prm.nudflag > 0) { wmanud.alloc(mza, mwa); vmanud.alloc(mza, mva); rhot_nud.alloc(mza, mwa); }
@@ -532,6 +538,11 @@ struct Model : Base {
     pdef[n - 1] = v != 0;
     return 0;
   }
+  int set_sgsmix(int n, int v) override {
+    if (n < 1 || n > (int)sgsmix.size()) { err = "bad scalar index"; return OLAMGPU_EINVAL; }
+    sgsmix[n - 1] = v != 0;
+    return 0;
+  }
 
   // ---------------------------------------------------------------- transfers
   template <class T, class S>
@@ -641,7 +652,8 @@ struct Model : Base {
   R face_value(const F2<R> &s, const F2<R> *g, int iorder, int side, int iv, int k, bool bounded) const {
     const int iw = v_iw(side, iv);
     if (iorder == 1) return s(k, iw);
-    R v = s(k, iw) + dxps(side, iv) * g[0](k, iw) + dyps(side, iv) * g[1](k, iw);
+    const int os = plant == 4 ? 3 - side : side;   // plant 4: the other side's offsets
+    R v = s(k, iw) + dxps(os, iv) * g[0](k, iw) + dyps(os, iv) * g[1](k, iw);
     if (iorder == 3) {
       const F2<R> &ox = bounded ? xx0v : xx0vu, &oxy = bounded ? xy0v : xy0vu, &oy = bounded ? yy0v : yy0vu;
       v = v + ox(side, iv) * g[2](k, iw) + oxy(side, iv) * g[3](k, iw) + oy(side, iv) * g[4](k, iw);
@@ -884,7 +896,9 @@ struct Model : Base {
           zf[k] = (R)pow_r((zt[k] - prm.rayfdiv_zmin) / (zm[mza] - prm.rayfdiv_zmin), prm.rayfdiv_expon);
       if (dol) fdivl.alloc(mza, mva);
       if (doh) fdivh.alloc(mza, mva);
-      const double facth = doh ? -(double)powi(-(R)prm.divh_damp_fact, lev) / dtm : 0.0;
+      double facth = doh ? -(double)powi(-(R)prm.divh_damp_fact, lev) / dtm : 0.0;
+      if (plant == 5) facth = -facth;
+      if (plant == 6 && doh) facth = -(double)powi(-(R)prm.divh_damp_fact, lev - 1) / dtm;
       const R fact2 = (dodiv && lev == 1) ? (R)(prm.divh_damp_fact / (R)dtm) : R(0);
       R factr = R(0);
       if (dorayfdiv && dodiv) factr = (R)(std::max(prm.divh_damp_fact, prm.rayfdiv_fact) / (R)dtm);
@@ -961,6 +975,7 @@ struct Model : Base {
         int im1 = v_im(1, iv), im2 = v_im(2, iv);
         R a = std::min(arm0[im1], arm0[im2]);
         fvorth[iv] = R(0.5) * dniu[iv] / (R)dtm * powi(-(R)prm.vort_damp_fact * a, lev);
+        if (plant == 7) fvorth[iv] = -fvorth[iv];
         if (lev > 2)
           fvortl[iv] = R(0.5) * dniu[iv] / (R)dtm * powi(-(R)prm.vort_damp_fact * a, lev - 1);
       }
@@ -1089,6 +1104,7 @@ struct Model : Base {
     for (int istage = 1; istage <= prm.nrk_wrtv; ++istage) {
       double dt8 = (prm.nrk_wrtv <= 2) ? (istage == 1 ? 0.5 : 1.0) * dtsm
                                        : (istage == 1 ? onethird : (istage == 2 ? 0.5 : 1.0)) * dtsm;
+      if (plant == 8 && istage == 1) dt8 = 0.5 * dtsm;   // plant 8: a wrong first-stage weight
       R dts = (R)dt8;
 #pragma omp parallel for
       for (int iw = 2; iw <= mwa; ++iw)
@@ -1133,7 +1149,7 @@ struct Model : Base {
     const RP fr = RP(0.70), fp2 = RP(0.70), fp3 = RP(0.70);
     const R cpocv = (R)(1004.0 / 717.0), rocv = (R)(287.0 / 717.0);
     const RP pc2 = fp2 * (RP)cpocv, pc3 = fp3 * (RP)cpocv;
-    const R fcor = (R)prm.fcoriol;
+    const R fcor = plant == 1 ? -(R)prm.fcoriol : (R)prm.fcoriol;
     R dts = (R)dt8;
     int ka = 2;
     RP hflx_rho[130], delex_rho[130], delex_rhothil[130], rhothil[130], vflux_thil[130];
@@ -1164,9 +1180,10 @@ struct Model : Base {
     }
     for (int jv = 1; jv <= npoly[iw]; ++jv) {  // :635-661
       int iv = w_iv(jv, iw);
-      if (dirv(jv, iw) > R(0)) {
+      if ((dirv(jv, iw) > R(0)) != (plant == 3)) {
+        // plant 3 flips the sign of the face fluxes of the momentum (and thil) only
         for (int k = 2; k <= mza; ++k) {
-          hflx_rho[k] = hflx_rho[k] + (RP)vmca(k, iv);
+          hflx_rho[k] = hflx_rho[k] + (plant == 3 ? -(RP)vmca(k, iv) : (RP)vmca(k, iv));
           thilt_rk(k, iw) = thilt_rk(k, iw) + vmca(k, iv) * thil_upv(k, iv);
           vmxet_rk(k, iw) = vmxet_rk(k, iw) + vmca(k, iv) * vxe_upv(k, iv);
           vmyet_rk(k, iw) = vmyet_rk(k, iw) + vmca(k, iv) * vye_upv(k, iv);
@@ -1174,7 +1191,7 @@ struct Model : Base {
         }
       } else {
         for (int k = 2; k <= mza; ++k) {
-          hflx_rho[k] = hflx_rho[k] - (RP)vmca(k, iv);
+          hflx_rho[k] = hflx_rho[k] - (plant == 3 ? -(RP)vmca(k, iv) : (RP)vmca(k, iv));
           thilt_rk(k, iw) = thilt_rk(k, iw) - vmca(k, iv) * thil_upv(k, iv);
           vmxet_rk(k, iw) = vmxet_rk(k, iw) - vmca(k, iv) * vxe_upv(k, iv);
           vmyet_rk(k, iw) = vmyet_rk(k, iw) - vmca(k, iv) * vye_upv(k, iv);
@@ -1264,7 +1281,7 @@ struct Model : Base {
   void prog_v_begs(int iv, R dts) {
     int iw1 = v_iw(1, iv), iw2 = v_iw(2, iv);
     for (int k = 2; k <= mza; ++k) {
-      R pgf = (R)(press(k, iw1) - press(k, iw2)) * pvfac(k, iv);
+      R pgf = (R)(plant == 2 ? press(k, iw2) - press(k, iw1) : press(k, iw1) - press(k, iw2)) * pvfac(k, iv);
       vmc(k, iv) = vmc0(k, iv) + dts * (vmt_s(k, iv) + pgf +
                                         (vnx[iv] * (vmxet_rk(k, iw1) + vmxet_rk(k, iw2)) +
                                          vny[iv] * (vmyet_rk(k, iw1) + vmyet_rk(k, iw2)) +
@@ -1414,12 +1431,54 @@ struct Model : Base {
     for (int k = kb; k <= mza; ++k) q(k, iw) = q0(k, iw) + del[k];
   }
 
+  // rhof of scalar_transport_rk.F90:120-133: the density minus the long step's nudging
+  // density tendency when nudging preserves the mixing ratio (global grid: every column
+  // is within max_nud_mrl), the density otherwise; FP64 arithmetic rounded to FP32.
+  // oname_check.f90:218 turns nud_preserve_mix_ratio off when nud_preserve_total_mass
+  // is on (rhot_nud is then the potential of the nudging mass fluxes).
+  bool nudged_rhof() const {
+    return prm.nudflag > 0 && prm.nud_preserve_mix_ratio && !prm.nud_preserve_total_mass && rhot_nud.n;
+  }
+  R rhof_at(int k, int iw) const {
+    if (nudged_rhof()) return (R)((double)rho(k, iw) - prm.dtlm * (double)rhot_nud(k, iw));
+    return (R)rho(k, iw);
+  }
+
+  // horizontal diffusion of a tracer into its tendency, scalar_transport_rk.F90:244-274
+  // (kwtop(iw) = max khtopv over the cell's faces, :160; a face contributes up to its
+  // own khtopv)
+  void tracer_sgsmix(const F2<R> &q, F2<R> &st) const {
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw) {
+      R fd[130];
+      int kwtop = 0;
+      for (int jv = 1; jv <= npoly[iw]; ++jv) kwtop = std::max(kwtop, khtopv[w_iv(jv, iw)]);
+      if (kwtop < 2) continue;
+      for (int k = 2; k <= kwtop; ++k) fd[k] = 0;
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw), iwn = w_iw(jv, iw);
+        for (int k = 2; k <= khtopv[iv]; ++k) fd[k] = fd[k] + akhodx(k, iv) * (q(k, iwn) - q(k, iw));
+      }
+      for (int k = 2; k <= kwtop; ++k) st(k, iw) = st(k, iw) + fd[k] * volti(k, iw);
+    }
+  }
+  // transpose of tracer_sgsmix: lq += D^T ls, D q = volti sum_faces akhodx (q_n - q_c)
+  void tracer_sgsmix_adj(const F2<R> &ls, F2<R> &lq) const {
+#pragma omp parallel for
+    for (int iw = 2; iw <= mwa; ++iw)
+      for (int jv = 1; jv <= npoly[iw]; ++jv) {
+        int iv = w_iv(jv, iw), iwn = w_iw(jv, iw);
+        for (int k = 2; k <= khtopv[iv]; ++k)
+          lq(k, iw) = lq(k, iw) + akhodx(k, iv) * (volti(k, iwn) * ls(k, iwn) - volti(k, iw) * ls(k, iw));
+      }
+  }
+
   void stage_densities(const F2<R> &rho_o, int nrk) {
     for (int n = 0; n < nrk; ++n) if (!rhos[n].n) rhos[n].alloc(mza, mwa);
 #pragma omp parallel for
     for (int iw = 2; iw <= mwa; ++iw)
-      for (int k = 2; k <= mza; ++k) {  // scalar_transport_rk.F90:131-150
-        R rf = (R)rho(k, iw);
+      for (int k = 2; k <= mza; ++k) {  // scalar_transport_rk.F90:120-150
+        R rf = rhof_at(k, iw);
         if (nrk == 3) {
           rhos[0](k, iw) = R(2.) / R(3.) * rho_o(k, iw) + R(1.) / R(3.) * rf;
           rhos[1](k, iw) = R(0.5) * (rho_o(k, iw) + rf);
@@ -1436,9 +1495,10 @@ struct Model : Base {
     const R one3 = R(1.) / R(3.);
     if (nrk == 3) { dtlr[0] = dtl * one3; dtlr[1] = dtl * R(0.5); dtlr[2] = dtl * R(1.0); }
     else { dtlr[0] = dtl * R(0.5); dtlr[1] = dtl * R(1.0); }
+    if (plant == 9 && nrk == 3) dtlr[0] = dtl * R(0.5);   // plant 9: a wrong first-stage weight
   }
 
-  // scalar_transport_rk, scalar_transport_rk.F90:30-385 (no sgs mixing, no limiter)
+  // scalar_transport_rk, scalar_transport_rk.F90:30-385 (limiters in update_scalar_lim)
   void scalar_transport_rk(const float *rho_old_host) override {
     if (!scp0.n) {
       scp0.alloc(mza, mwa); upv.alloc(mza, mva);
@@ -1462,6 +1522,7 @@ struct Model : Base {
     for (size_t n = 0; n < scp.size(); ++n) {
       F2<R> &q = scp[n];
       scp0 = q;
+      if (turb_on && sgsmix[n]) tracer_sgsmix(q, sct[n]);
       const int imonot = prm.iscal_monot;
       tape_lim[n] = 0;
       for (int istage = 1; istage <= nrk; ++istage) {
@@ -1814,9 +1875,12 @@ struct Model : Base {
 
   // ---------------------------------------------------------------- adjoint
   F2<R> a_nu, a_lqpart, a_lg[5];
+  F2<R> a_lsct;                 // adjoint of the tracer tendency (with horizontal diffusion)
+  bool a_lsct_on = false;
   void alloc_adj_scratch() {
     if (a_nu.n) return;
     a_nu.alloc(mza, mwa); a_lqpart.alloc(mza, mwa);
+    a_lsct.alloc(mza, mwa);
     for (auto &g : a_lg) g.alloc(mza, mwa);
   }
   // Transpose of update_scalar (+ implicit column) for fixed winds.  Derivation:
@@ -1886,6 +1950,9 @@ struct Model : Base {
         int iv = w_iv(jv, iw);
         for (int k = 2; k <= mza; ++k) gsum[k] = gsum[k] + dirv(jv, iw) * vm(k, iv);
       }
+      // the tendency sct enters as dtr sct / rhos
+      if (a_lsct_on)
+        for (int k = kb; k <= mza; ++k) a_lsct(k, iw) = a_lsct(k, iw) + mu[k] * dtr / rhosf(k, iw);
       for (int k = kb; k <= mza; ++k) {
         R v = mu[k] * dtr * volti(k, iw) / rhosf(k, iw);
         nu(k, iw) = v;
@@ -1951,6 +2018,14 @@ struct Model : Base {
     F2<R> lq, lq0, mu;
     lq.alloc(mza, mwa); lq0.alloc(mza, mwa);
     for (size_t n = 0; n < lam.size(); ++n) {
+      const bool sgs = turb_on && sgsmix[n];
+      if (sgs && tape_lim[n]) {
+        err = "adjoint of tracer horizontal diffusion with a final-stage limiter is not built";
+        throw 1;
+      }
+      alloc_adj_scratch();
+      a_lsct_on = sgs;
+      if (sgs) std::fill(a_lsct.v.begin(), a_lsct.v.end(), R(0));
       mu = lam[n];
       std::fill(lq0.v.begin(), lq0.v.end(), R(0));
       for (int istage = nrk; istage >= 1; --istage) {
@@ -1965,6 +2040,8 @@ struct Model : Base {
       }
       // stage 1 read q = q0: both adjoints land on the initial tracer
       for (size_t i = 0; i < lq0.v.size(); ++i) lam[n].v[i] = lq0.v[i] + mu.v[i];
+      // the diffusion tendency was built from the initial tracer
+      if (sgs) tracer_sgsmix_adj(a_lsct, lam[n]);
     }
   }
 
@@ -2046,6 +2123,7 @@ int oracle_scalar_transport_rk_adj(void *h) {
 int oracle_mass_sums(void *h, double *out) { ((oracle_ctx *)h)->m->mass_sums(out); return 0; }
 int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(out); return 0; }
 int oracle_set_scalar_pdef(void *h, int32_t n, int32_t pdef) { return ((oracle_ctx *)h)->m->set_pdef(n, pdef); }
+int oracle_set_scalar_sgsmix(void *h, int32_t n, int32_t v) { return ((oracle_ctx *)h)->m->set_sgsmix(n, v); }
 int oracle_set_frozen_limiter(void *h, int32_t on) { ((oracle_ctx *)h)->m->set_frozen(on); return 0; }
 int oracle_tape_scales(void *h, int32_t n, int32_t which, double *out) { return ((oracle_ctx *)h)->m->tape_scales(n, which, out); }
 int oracle_rhothil_sum(void *h, double *out) { *out = ((oracle_ctx *)h)->m->rhothil_sum(); return 0; }
@@ -2053,6 +2131,13 @@ int oracle_recon(void *h, const float *field, int32_t order, int32_t bounded_coe
                  double *faces) {
   return ((oracle_ctx *)h)->m->recon(field, order, bounded_coef, bounded_off, coefs, faces);
 }
+// Test hook (pins only): plant a deliberate mistake.  1 Coriolis sign; 2 sign of the
+// pressure difference of prog_v_begs; 3 sign of the face fluxes of prog_wrt_begs; 4
+// order-2 face values with the other side's offsets; 5 sign of the divergence damping;
+// 6 divergence damping coefficient with exponent level-1; 7 sign of the vorticity
+// damping; 8 first RK stage of prog_wrtv_rk with weight 1/2; 9 first RK stage of
+// scalar_transport_rk with weight 1/2.
+int oracle_plant(void *h, int32_t code) { ((oracle_ctx *)h)->m->plant = code; return 0; }
 int oracle_set_threads(int32_t n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
 
 // tridiffo (tridiag_lib.f90:10-36) on arrays indexed 1..n, for the unit test of the

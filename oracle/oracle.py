@@ -50,6 +50,7 @@ def lib():
         L.oracle_set_threads.argtypes = [C.c_int32]
         L.oracle_tridiffo.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
         L.oracle_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
+        L.oracle_set_scalar_sgsmix.argtypes = [vp, C.c_int32, C.c_int32]
         L.oracle_set_frozen_limiter.argtypes = [vp, C.c_int32]
         L.oracle_tape_scales.argtypes = [vp, C.c_int32, C.c_int32, vp]
         L.oracle_recon.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
@@ -59,6 +60,7 @@ def lib():
         L.oracle_set_turbulence.argtypes = [vp, vp, vp, vp, vp, vp]
         L.oracle_set_nudging.argtypes = [vp, vp, vp, vp]
         L.oracle_setup_tendencies.argtypes = [vp, vp]
+        L.oracle_plant.argtypes = [vp, C.c_int32]
         _lib = L
     return _lib
 
@@ -190,6 +192,9 @@ class Oracle:
     def set_scalar_pdef(self, n, pdef):
         assert lib().oracle_set_scalar_pdef(self.h, int(n), int(bool(pdef))) == 0
 
+    def set_scalar_sgsmix(self, n, on):
+        assert lib().oracle_set_scalar_sgsmix(self.h, int(n), int(bool(on))) == 0
+
     def tape_scales(self, n, which):
         """Test hook: taped limiter scales of tracer n, which 0 = faces (mva, mza), 1 = W levels."""
         rows = self.grid.mva if which == 0 else self.grid.mwa
@@ -215,6 +220,11 @@ class Oracle:
         out = C.c_double()
         lib().oracle_rhothil_sum(self.h, C.byref(out))
         return out.value
+
+    def plant(self, code):
+        """Test hook: plant a deliberate mistake in one term (codes at oracle_plant in
+        olam_oracle.cpp), so a pin can show that it catches it."""
+        lib().oracle_plant(self.h, int(code))
 
     def set_frozen_limiter(self, on):
         """Test hook: later forward steps apply the limiter scales of the last step."""

@@ -139,7 +139,7 @@ def write_fortran_case(path, grid, params, state, scalars, nsteps):
             p.ithil_monot, p.iwind_monot, p.initial,
             *(repr(float(getattr(p, f"{kind}_{x}"))) for kind in abi.RAYF_KINDS
               for x in ("zmin", "fact", "expon")), repr(float(p.akmin)), p.nudflag,
-            p.nud_preserve_total_mass)) + "\n")
+            p.nud_preserve_total_mass, p.nud_preserve_mix_ratio)) + "\n")
         f.write(f"{grid.mza} {grid.mwa} {grid.mva} {grid.mma}\n")
     lines = []
 

@@ -48,6 +48,7 @@ program probe
   print '(a,1x,i0)', 'params.rayfmix_expon', off(c_loc(p%rayfmix_expon), c_loc(p))
   print '(a,1x,i0)', 'params.akmin', off(c_loc(p%akmin), c_loc(p))
   print '(a,1x,i0)', 'params.nud_preserve_total_mass', off(c_loc(p%nud_preserve_total_mass), c_loc(p))
+  print '(a,1x,i0)', 'params.nud_preserve_mix_ratio', off(c_loc(p%nud_preserve_mix_ratio), c_loc(p))
   print '(a,1x,i0)', 'mesh.dzit_bot', off(c_loc(m%dzit_bot), c_loc(m))
   print '(a,1x,i0)', 'state.rho', off(c_loc(s%rho), c_loc(s))
   print '(a,1x,i0)', 'state.vmt', off(c_loc(s%vmt), c_loc(s))

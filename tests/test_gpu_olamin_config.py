@@ -4,7 +4,8 @@ steps, RK3 acoustic and RK2 scalar steps, second-order advection, sixth-order
 divergence (0.12) and vorticity (0.075) damping, Rayleigh friction on W, on the
 divergence and on the top winds above 27 km, horizontally inhomogeneous initial state
 (initial = 2), a moist atmosphere (comp_alpha_press every long step) and PBL
-turbulence (idiffk = 2: eddy and horizontal diffusion from the PBL scheme's fields).
+turbulence (idiffk = 2: eddy and horizontal diffusion from the PBL scheme's fields,
+which also diffuse the tracers horizontally, scalar_transport_rk.F90:244-274).
 Three long steps of tend0, 3 x prog_wrtv_rk, timeavg_momsc and scalar_transport_rk with
 three tracers, then the tracer adjoint of the last step, against the oracle through the
 C-ABI.  FP32 tolerances as in test_gpu_parity.py; FP64 as explained below.
