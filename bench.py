@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -38,9 +40,31 @@ def alg_bytes(ntr, prec):
     return dyn, tr, tr
 
 
+def stage_alg_bytes(name, ntr, prec):
+    """Algorithmic bytes per cell-level of the RK stage that one launch of the named
+    kernel carries out, from the per-stage byte model of the design notes: a tracer stage
+    (all tracers, forward or adjoint) is (120 + 48 n) / 3, the dynamics' W-column phase
+    132 (152 with FP64 rho/press), its V phase 92 (100) and its velocity phase 28.  A
+    stage that runs as two kernels credits each with the whole stage's bytes.  This is
+    what the headline roofline uses, whatever the kernel moves besides."""
+    tr = (120 + 48 * ntr) / 3.0
+    mixed = prec == "mixed"
+    per = {
+        "k_tr_grad": tr, "k_tr_update": tr, "k_tr_stage": tr,
+        "k_tr_adj0": tr, "k_tr_adj1": tr, "k_tr_adj01": tr, "k_tr_adj2": tr,
+        "k_dyn_w": 152 if mixed else 132,
+        "k_dyn_v": 100 if mixed else 92,
+        "k_dyn_vel": 28,
+    }.get(name)
+    if per is None:
+        return None
+    return 2 * per if prec == "f64" else per
+
+
 def kernel_alg_bytes(name, ntr, prec):
-    """Algorithmic bytes per cell-level for one launch of the named kernel: every array
-    the kernel must read or write, counted once (V-face fields as 3 faces per cell)."""
+    """Bytes per cell-level of every array one launch of the named kernel must read or
+    write, counted once (V-face fields as 3 faces per cell), including intermediates
+    that the stage model does not count (gradients, partial adjoints)."""
     w = 8 if prec == "f64" else 4
     per = {
         "k_tr_grad": 3 * ntr,                 # q -> gx, gy
@@ -49,6 +73,7 @@ def kernel_alg_bytes(name, ntr, prec):
         "k_tr_adj1": 3 + 5 * ntr,             # vmasc; nu, lpart -> lpart, lgx, lgy
         "k_tr_adj01": 6 + 6 * ntr,            # vmasc, wmasc, volti, rho; mu, lam0 -> lam0, lpart, lgx, lgy
         "k_tr_adj2": 4 * ntr,                 # lpart, lgx, lgy -> lq
+        "k_tr_stage": 6 + 3 * ntr,            # rho, volti, wmasc, vmasc; q, q0 -> q
         # thil, vxe, vye, vze, their 8 gradients, 4 short-step tendencies, rho, volt (FP64),
         # wmc, volti, alpha_press, rho0, rth0, pwfac, volwi, wmc0, wc, arw; vmc, arv on 3
         # faces per cell -> press, rho, thil, vm[xyz]et_rk, wmc, wc
@@ -81,13 +106,25 @@ def committed_traffic(kernel, nxp, nlev, ntr, prec):
     return None, None
 
 
-def cpu_baseline(nxp, nlev, ntr, prec, dt, threads, steps=2):
-    """Oracle (CPU restatement, OpenMP) on a bounded sample of the same workload."""
+def host_cpu():
+    """CPU model and the number of cores this process may run on (its share of the host)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(grid, ntr, prec, dt, threads, max_steps=5, time_box=60.0):
+    """Oracle (CPU restatement, OpenMP) on the metric workload: one warm-up step, then
+    timed steps until max_steps or the time box (at least 2); the median step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from olam_amd import abi, cases, mesh
-    m = mesh.build_mesh(nxp)
-    grid = abi.FortranGrid(m, mesh.vertical_grid(nlev))
+    from olam_amd import abi, cases
     jet = cases.jet_state(grid)
     tr = cases.lognormal_tracers(grid, ntr)
     used = O.set_threads(threads)
@@ -106,15 +143,21 @@ def cpu_baseline(nxp, nlev, ntr, prec, dt, threads, steps=2):
         o.scalar_transport_rk_adj()
 
     step()                                   # warm-up (allocates scratch)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < max_steps and (len(times) < 2 or time.perf_counter() - t_start < time_box):
+        t0 = time.perf_counter()
         step()
-    sec = (time.perf_counter() - t0) / steps
-    cells = m.nw * nlev
-    return {"value": cells / sec, "unit": "cell-updates/s", "cores": used, "kind": "port",
-            "sample": f"oracle (C++ restatement, OpenMP) NXP={nxp} ({m.nw} columns), L={nlev}, {ntr} tracers, "
-                      f"{prec}, 1 warm-up + {steps} timed steps of dynamics+tracer fwd+adjoint; "
-                      f"{sec:.2f} s/step"}
+        times.append(time.perf_counter() - t0)
+    o.close()
+    sec = float(sorted(times)[len(times) // 2])
+    nw, nlev = grid.mwa - 1, grid.mza - 1
+    model, _ = host_cpu()
+    return {"value": nw * nlev / sec, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "cpu_model": model, "seconds_per_step": sec, "steps_timed": len(times),
+            "sample": f"oracle (C++ restatement, OpenMP, -O3 -march=x86-64-v3 -ffp-contract=off) on {nw} columns, "
+                      f"L={nlev}, {ntr} tracers, precision {prec}: one warm-up, then the median of {len(times)} "
+                      f"timed steps of dynamics + tracer forward + tracer adjoint (time box {time_box:g} s)"}
 
 
 def main():
@@ -127,9 +170,18 @@ def main():
     ap.add_argument("--ntr", type=int, default=10)
     ap.add_argument("--dt", type=float, default=30.0)
     ap.add_argument("--prec", default="f32", choices=["f32", "mixed", "f64"])
-    ap.add_argument("--cpu-nxp", type=int, default=160)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-nxp", type=int, default=None, help="CPU baseline mesh (default: the workload's)")
+    ap.add_argument("--cpu-prec", default="mixed", help="CPU baseline precision (default: the reference's)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may run on")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-time-box", type=float, default=60.0)
+    ap.add_argument("--cpu-1thread-nxp", type=int, default=40,
+                    help="mesh of the additional 1-thread CPU figure (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=5,
+                    help="also time this many steps in the reference-precision mode (0: skip)")
+    ap.add_argument("--host-resident", action="store_true",
+                    help="also time long steps whose state and tracers start and end in host memory")
     ap.add_argument("--no-copy", action="store_true", help="skip the achievable-copy-bandwidth probe")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -163,22 +215,29 @@ def main():
     jet = cases.jet_state(grid)
     tr = cases.lognormal_tracers(grid, args.ntr)
     log(f"[bench] synthetic state + {args.ntr} tracers: {time.perf_counter() - t:.1f}s")
-    prm = abi.make_params(args.prec, num_scalar=args.ntr, dtsm=args.dt)
+    cells = m.nw * args.nlev
+
+    def make_ctx(prec):
+        c = Context(grid, abi.make_params(prec, num_scalar=args.ntr, dtsm=args.dt), device=0)
+        c.upload_state(jet)
+        for n in range(args.ntr):
+            c.upload_scalar(n + 1, tr[n])
+            c.upload_scalar_adj(n + 1, tr[n])
+        return c
+
+    def stepper(c):
+        def step():
+            c.tend0()
+            c.prog_wrtv_rk()
+            c.timeavg_momsc()
+            c.scalar_transport_rk()
+            c.scalar_transport_rk_adj()
+        return step
+
     t = time.perf_counter()
-    ctx = Context(grid, prm, device=0)
-    ctx.upload_state(jet)
-    for n in range(args.ntr):
-        ctx.upload_scalar(n + 1, tr[n])
-        ctx.upload_scalar_adj(n + 1, tr[n])
+    ctx = make_ctx(args.prec)
+    step = stepper(ctx)
     log(f"[bench] device setup (renumbering, checks, uploads): {time.perf_counter() - t:.1f}s")
-
-    def step():
-        ctx.tend0()
-        ctx.prog_wrtv_rk()
-        ctx.timeavg_momsc()
-        ctx.scalar_transport_rk()
-        ctx.scalar_transport_rk_adj()
-
     for _ in range(args.warmup):
         step()
     log("[bench] warm-up done")
@@ -189,33 +248,81 @@ def main():
     sec = (time.perf_counter() - t0) / args.steps
     prof = ctx.kernel_profile()
     ctx.set_profiling(False)
-    cells = m.nw * args.nlev
     value = cells / sec
     log(f"[bench] {sec * 1e3:.2f} ms/step, {value:.3e} cell-updates/s")
     kern_ms = {k: v[1] / args.steps for k, v in prof.items()}
     for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
         log(f"[bench]   {k:16s} {n // args.steps:3d} launches/step  {ms / args.steps:8.3f} ms/step  "
             f"{ms / n:8.3f} ms/launch")
+
+    host = None
+    if args.host_resident:
+        # the drop-in as INTEGRATION.md's shim drives it when the host owns the state:
+        # upload the state and tracers, one long step, download them again
+        st_out = abi.FortranState(grid)
+        qh = [np.empty((grid.mwa, grid.mza), np.float32) for _ in range(args.ntr)]
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ctx.upload_state(jet)
+            for n in range(args.ntr):
+                ctx.upload_scalar(n + 1, tr[n])
+            step()
+            ctx.download_state(st_out)
+            for n in range(args.ntr):
+                qh[n][...] = ctx.download_scalar(n + 1)
+            ts.append(time.perf_counter() - t0)
+        hsec = sorted(ts)[1]
+        host = {"ms_per_step": hsec * 1e3, "transfer_ms": (hsec - sec) * 1e3,
+                "what": "upload of the full state and tracers, one long step, download of both (median of 3)"}
+        log(f"[bench] host-resident step {hsec * 1e3:.1f} ms ({(hsec - sec) * 1e3:.1f} ms transfers)")
+    ctx.close()
+
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
     n_launch, tot_ms = prof[dom]
     per_launch_s = tot_ms / n_launch * 1e-3
-    ab = kernel_alg_bytes(dom, args.ntr, args.prec)
+    sb = stage_alg_bytes(dom, args.ntr, args.prec)
+    kb = kernel_alg_bytes(dom, args.ntr, args.prec)
     roof = None
-    if ab is not None:
-        ach = ab * cells / per_launch_s
+    if sb is not None:
+        ach = sb * cells / per_launch_s
         traffic, tsrc = committed_traffic(dom, args.nxp, args.nlev, args.ntr, args.prec)
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": ach / PEAK_HBM, "traffic": traffic,
-                "traffic_source": tsrc,
+                "frac": ach / PEAK_HBM, "traffic": traffic, "traffic_source": tsrc,
                 "frac_of_copy": (ach / (copy_bw * 1e9)) if copy_bw else None,
-                "alg_bytes_per_launch": ab * cells, "avg_launch_ms": per_launch_s * 1e3}
+                "alg_bytes_per_launch": sb * cells, "alg_bytes_basis": "stage model (per-stage bytes)",
+                "avg_launch_ms": per_launch_s * 1e3,
+                "frac_kernel_arrays": (kb * cells / per_launch_s / PEAK_HBM) if kb else None}
     bd, bf, ba = alg_bytes(args.ntr, args.prec)
     step_frac = (bd + bf + ba) * cells / sec / PEAK_HBM
-    cpu = None
+
+    mixed = None
+    if args.mixed_steps > 0 and args.prec == "f32":
+        c2 = make_ctx("mixed")
+        s2 = stepper(c2)
+        s2()
+        t0 = time.perf_counter()
+        for _ in range(args.mixed_steps):
+            s2()
+        msec = (time.perf_counter() - t0) / args.mixed_steps
+        c2.close()
+        mixed = {"ms_per_step": msec * 1e3, "value": cells / msec, "steps": args.mixed_steps,
+                 "dtype": "fp32 + fp64 rho/press (the reference's precision map)"}
+        log(f"[bench] reference-precision mode: {msec * 1e3:.2f} ms/step")
+
+    cpu = cpu1 = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle) ...")
-        cpu = cpu_baseline(args.cpu_nxp, args.nlev, args.ntr, args.prec, args.dt, args.cpu_threads)
-        log(f"[bench] CPU {cpu['value']:.3e} cell-updates/s on {cpu['cores']} threads")
+        threads = args.cpu_threads or host_cpu()[1]
+        cgrid = grid if args.cpu_nxp in (None, args.nxp) else \
+            abi.FortranGrid(mesh.build_mesh(args.cpu_nxp), mesh.vertical_grid(args.nlev))
+        log(f"[bench] CPU baseline (oracle) on {threads} threads ...")
+        cpu = cpu_baseline(cgrid, args.ntr, args.cpu_prec, args.dt, threads, args.cpu_steps, args.cpu_time_box)
+        log(f"[bench] CPU {cpu['value']:.3e} cell-updates/s on {cpu['cores']} threads "
+            f"({cpu['seconds_per_step']:.2f} s/step)")
+        if args.cpu_1thread_nxp:
+            g1 = abi.FortranGrid(mesh.build_mesh(args.cpu_1thread_nxp), mesh.vertical_grid(args.nlev))
+            cpu1 = cpu_baseline(g1, args.ntr, args.cpu_prec, args.dt, 1, 3, 20.0)
+            log(f"[bench] CPU 1 thread {cpu1['value']:.3e} cell-updates/s")
     out = {
         "metric": "cell-updates/s per RK3 step (fwd+adjoint) at NXP=320, L=60; % HBM roofline",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -233,7 +340,10 @@ def main():
                           "copy_GBps": copy_bw,
                           "frac_of_copy": ((bd + bf + ba) * cells / sec / (copy_bw * 1e9)) if copy_bw else None},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kern_ms.items()},
+        "reference_precision": mixed,
+        "host_resident": host,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(out), flush=True)

@@ -266,6 +266,11 @@ int olamgpu_download_scalar_adj(olamgpu_ctx *ctx, int32_t n, float *lambda);
  * out[1+n] = sum rho*volt*q_n for n = 0..num_scalar-1, all FP64 accumulations. */
 int olamgpu_mass_sums(olamgpu_ctx *ctx, double *out);
 
+/* tracer_maxmin (check_nans.F90:164-203): out[2n] and out[2n+1] = the maximum and the
+ * minimum of tracer n+1 (n = 0..num_scalar-1) over levels lpw..mza of every column, FP32
+ * as var_p; NaN values are skipped.  The reference prints them; here they are returned. */
+int olamgpu_tracer_maxmin(olamgpu_ctx *ctx, float *out);
+
 /* check_cfls diagnostics (scalar_transport_rk.F90:1476-1712) of the last tracer
  * step: out[0] = max total CFL, out[1] = max horizontal CFL, out[2] = max |wc|,
  * out[3] = number of implicit columns (any stage). */

@@ -196,6 +196,26 @@ template <class T>
 __device__ __forceinline__ T ld_nt(const T *p, long i, bool ok) { return ok ? __builtin_nontemporal_load(p + i) : T(0); }
 template <class T>
 __device__ __forceinline__ void st_nt(T *p, long i, T v) { __builtin_nontemporal_store(v, p + i); }
+// Two values of one cell and level stored side by side (a gradient pair), so one
+// 8- or 16-byte gather per lane fetches both.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <class R> struct Pair2;
+template <> struct Pair2<float> { using T = f32x2; };
+template <> struct Pair2<double> { using T = f64x2; };
+template <class R> using P2 = typename Pair2<R>::T;
+template <class R>
+__device__ __forceinline__ P2<R> ld2(const P2<R> *p, long i, bool ok) {
+  P2<R> z;
+  z.x = R(0); z.y = R(0);
+  return ok ? p[i] : z;
+}
+template <class R>
+__device__ __forceinline__ P2<R> mk2(R x, R y) {
+  P2<R> v;
+  v.x = x; v.y = y;
+  return v;
+}
 template <class T>
 __device__ __forceinline__ T ldv(const T *p, long i, bool ok, T dflt) {
   const T v = p[i];

@@ -14,6 +14,7 @@
 template <class R, class RP>
 struct TrStage {
   const R *q_in, *q0, *sct, *gx, *gy;
+  const P2<R> *gxy;  // order 2, unlimited stage: the gradient pairs (gx, gy) side by side
   const R *gq[3];    // order 3: the quadratic terms (xx, xy, yy) of the reconstruction
   R *q_out;
   const R *vmasc, *wmasc, *rho_old, *ffimp;
@@ -27,9 +28,10 @@ struct TrStage {
 };
 
 // ---------------------------------------------------------------- grad_t2d
-template <class R>
+template <class R, bool PK>
 __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict__ q, R *__restrict__ gx,
-                                                 R *__restrict__ gy, int ntr, long tstride, int tpw) {
+                                                 R *__restrict__ gy, P2<R> *__restrict__ gxy, int ntr, long tstride,
+                                                 int tpw) {
   const TrItem ti = tr_item(m.nw, ntr, tpw);
   const int c = ti.c;
   if (c >= m.nw) return;
@@ -56,8 +58,12 @@ __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict
     });
     if (!body) { ax = R(0); ay = R(0); }
     if (act) {
-      st_nt(gx, n * tstride + c * ldk + l, ax);
-      st_nt(gy, n * tstride + c * ldk + l, ay);
+      if constexpr (PK) {
+        st_nt(gxy, n * tstride + c * ldk + l, mk2(ax, ay));
+      } else {
+        st_nt(gx, n * tstride + c * ldk + l, ax);
+        st_nt(gy, n * tstride + c * ldk + l, ay);
+      }
     }
   }
 }
@@ -116,7 +122,7 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 // ---------------------------------------------------------------- update_scalar
 // One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
 // (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
-template <class R, class RP, bool Q3>
+template <class R, class RP, bool Q3, bool PK = false>
 __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
@@ -167,7 +173,12 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
     L.q0 = ld_nt(a.q0 + off, ic, act);
     L.st = a.sct ? ld_nt(a.sct + off, ic, act) : R(0);
     L.gxc = R(0); L.gyc = R(0);
-    if (a.order > 1) { L.gxc = ld(a.gx + off, ic, act); L.gyc = ld(a.gy + off, ic, act); }
+    if constexpr (PK) {
+      const P2<R> g = ld2<R>(a.gxy + off, ic, act);
+      L.gxc = g.x; L.gyc = g.y;
+    } else if (a.order > 1) {
+      L.gxc = ld(a.gx + off, ic, act); L.gyc = ld(a.gy + off, ic, act);
+    }
     if constexpr (Q3) {
 #pragma unroll
       for (int t = 0; t < 3; ++t) L.gqc[t] = ld(a.gq[t] + off, ic, act);
@@ -182,7 +193,10 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
       if ((j < 6 || np > 6) && ((need >> j) & 1u)) {
         const long in = (long)nb[j] * ldk + l;
         L.qn[j] = ld(qi, in, act);
-        if (a.order > 1) {
+        if constexpr (PK) {
+          const P2<R> g = ld2<R>(a.gxy + off, in, act);
+          L.gxn[j] = g.x; L.gyn[j] = g.y;
+        } else if (a.order > 1) {
           L.gxn[j] = ld(a.gx + off, in, act);
           L.gyn[j] = ld(a.gy + off, in, act);
           if constexpr (Q3) {
@@ -211,6 +225,123 @@ __global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrSta
       if constexpr (Q3) {   // bounded offsets xx0_v (:515-524)
         fs = fs + oqs[j][0] * L.gqc[0] + oqs[j][1] * L.gqc[1] + oqs[j][2] * L.gqc[2];
         fn = fn + oqn[j][0] * L.gqn[j][0] + oqn[j][1] * L.gqn[j][1] + oqn[j][2] * L.gqn[j][2];
+      }
+      const R f = sel[j] ? fn : fs;
+      F = F + dvm[j] * (f - q0);
+    });
+    // vertical: grad_z_quadratic (bounded, zero-gradient ends) and upwinding
+    const R qp1 = shdn(qc, 1);
+    const R ds = inner ? qp1 - qc : R(0);
+    const R dsm1 = shup(ds, 1);
+    const R scpb = qc + ds * a11 + dsm1 * a21;
+    const R scpt = qc + ds * a12 + dsm1 * a22;
+    const R scpb_p1 = shdn(scpb, 1);
+    R upw = (wm < R(0)) ? scpb_p1 : scpt;
+    if (k == 1) upw = qp1;
+    if (k == mza) upw = qc;
+    const R upw_m1 = shup(upw, 1);
+    F = F + fexpl * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
+    R qn_new = q0 + a.dtr * (L.st + vt * F) * rinv;
+    const R lev1 = shdn(qn_new, 1);             // level 1 copies level 2
+    if (k == 1) qn_new = lev1;
+    if (act) st_nt(a.q_out, off + ic, qn_new);
+  }
+}
+
+// ---------------------------------------------------------------- fused stage
+// One explicit RK stage of all tracers at order 1 or 2 with the reconstruction computed
+// where it is used instead of materialised: grad_t2d (grad_lib.f90:236-269) of the
+// column from its neighbours, and for every face whose value some level takes from the
+// neighbour (a wave ballot), grad_t2d of that neighbour from the neighbour's own
+// stencil (its record read with scalar loads, its neighbours' columns gathered).  The
+// gradients are formed with the expressions of k_tr_grad and the update with those of
+// k_tr_update, so the results match the two-pass arrangement; no gradient array is
+// written or read back.  Per tracer the HBM traffic is the stage input, scp0, sct and
+// the output.
+template <class R, class RP>
+__global__ __launch_bounds__(256, 4) void k_tr_stage(DMesh<R> m, TrStage<R, RP> a) {
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const int c = ti.c;
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza, body = act && k >= 2;
+  const bool inner = k >= 2 && k <= mza - 1;
+  const long ldk = m.ldk;
+  const long ic = c * ldk + l;
+  const int np = m.w_npoly[c];
+  const bool o2 = a.order > 1;
+  int nb[7];
+  R dvm[7];
+  bool sel[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    nb[j] = m.w_nbr[c * NSLOT + j];
+    const int iv = m.w_face[c * NSLOT + j];
+    const R dv = m.w_dirv[c * NSLOT + j];
+    const R vm = (j < 6 || np > 6) ? ld(a.vmasc, (long)iv * ldk + l, act) : R(0);
+    dvm[j] = dv * vm;
+    // neighbour is upwind: this cell is iw1 (dirv < 0) and vm < 0, or iw2 and vm >= 0
+    sel[j] = (dv < R(0)) ? (vm < R(0)) : !(vm < R(0));
+  }
+  const R wm = ld(a.wmasc, ic, act);
+  const R wm_m1 = shup(wm, 1);
+  const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
+  const R vt = ld(m.volti, ic, act);
+  const R rf = (R)ldv(a.rho, ic, act, RP(1));
+  const R rinv = R(1) / (body ? rf : R(1));
+  const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
+  unsigned need = 0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) need |= (__ballot(sel[j] && act) != 0 ? 1u : 0u) << j;
+  if (!o2) need = 0;
+  // the next tracer's column and neighbour values are loaded before the current one is
+  // computed
+  struct Ld { R qc, q0, st, qn[7]; };
+  auto load = [&](int n, Ld &L) {
+    const long off = n * a.tstride;
+    const R *qi = a.q_in + off;
+    L.qc = ld(qi, ic, act);
+    L.q0 = ld_nt(a.q0 + off, ic, act);
+    L.st = a.sct ? ld_nt(a.sct + off, ic, act) : R(0);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) L.qn[j] = (j < 6 || np > 6) ? ld(qi, (long)nb[j] * ldk + l, act) : R(0);
+  };
+  Ld nx;
+  if (ti.n0 < ti.n1) load(ti.n0, nx);
+  for (int n = ti.n0; n < ti.n1; ++n) {
+    const Ld L = nx;
+    if (n + 1 < ti.n1) load(n + 1, nx);
+    if (a.limited(n)) continue;
+    const long off = n * a.tstride;
+    const R *qi = a.q_in + off;
+    const R qc = L.qc, q0 = L.q0;
+    R gxc = R(0), gyc = R(0);
+    if (o2) {   // grad_t2d of the column (k_tr_grad's expressions)
+      R ax = R(0), ay = R(0);
+      for_slots(np, [&](int j) {
+        const R d = L.qn[j] - qc;
+        ax = ax + m.w_gxc[c * NSLOT + j] * d;
+        ay = ay + m.w_gyc[c * NSLOT + j] * d;
+      });
+      if (body) { gxc = ax; gyc = ay; }
+    }
+    R F = R(0);
+    for_slots(np, [&](int j) {
+      R fs = qc, fn = L.qn[j];
+      if (o2) {
+        fs = qc + m.w_dxs[c * NSLOT + j] * gxc + m.w_dys[c * NSLOT + j] * gyc;
+        if ((need >> j) & 1u) {   // wave-uniform: grad_t2d of the upwind neighbour
+          const int u = nb[j];
+          const int npu = m.w_npoly[u];
+          R bx = R(0), by = R(0);
+          for_slots(npu, [&](int i) {
+            const R d = ld(qi, (long)m.w_nbr[u * NSLOT + i] * ldk + l, act) - fn;
+            bx = bx + m.w_gxc[u * NSLOT + i] * d;
+            by = by + m.w_gyc[u * NSLOT + i] * d;
+          });
+          if (!body) { bx = R(0); by = R(0); }
+          fn = fn + m.w_dxn[c * NSLOT + j] * bx + m.w_dyn[c * NSLOT + j] * by;
+        }
       }
       const R f = sel[j] ? fn : fs;
       F = F + dvm[j] * (f - q0);
@@ -584,6 +715,7 @@ struct TrAdj {
   R *mu;             // adjoint of the stage output (overwritten by the implicit pass)
   R *lam0;           // accumulates the adjoint of scp0
   R *nu, *lpart, *lgx, *lgy;
+  P2<R> *lgxy;       // order 2, unlimited stage: the gradient adjoints (lgx, lgy) side by side
   R *lgq[3];         // order 3: adjoints of the quadratic terms
   R *lq;             // adjoint of the stage input
   const R *vmasc, *wmasc, *rho_old, *ffimp;
@@ -799,7 +931,7 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
 // which is rebuilt here from their mu (nu = mu dtr volti / rho, the same expression as
 // in k_tr_adj0) instead of being written and read back.  Per tracer it reads mu of the
 // column and its neighbours and lam0, and writes lam0, lpart, lgx, lgy.
-template <class R, class RP, bool Q3>
+template <class R, class RP, bool Q3, bool PK = false>
 __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
@@ -919,7 +1051,8 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
     if (act) {
       st_nt(a.lam0, off + ic, l0);
       st_nt(a.lpart, off + ic, lqv + P);
-      if (a.order > 1) { st_nt(a.lgx, off + ic, Gx); st_nt(a.lgy, off + ic, Gy); }
+      if constexpr (PK) st_nt(a.lgxy, off + ic, mk2(Gx, Gy));
+      else if (a.order > 1) { st_nt(a.lgx, off + ic, Gx); st_nt(a.lgy, off + ic, Gy); }
       if constexpr (Q3) {
 #pragma unroll
         for (int t = 0; t < 3; ++t) a.lgq[t][off + ic] = body ? Gq[t] : R(0);
@@ -928,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   }
 }
 
-template <class R, class RP>
+template <class R, class RP, bool PK = false>
 __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
@@ -956,7 +1089,16 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     L.gxs = R(0); L.gys = R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j) { L.gxn[j] = R(0); L.gyn[j] = R(0); }
-    if (a.order > 1) {
+    if constexpr (PK) {
+      const P2<R> g = ld2<R>(a.lgxy + off, ic, act);
+      L.gxs = g.x; L.gys = g.y;
+#pragma unroll
+      for (int j = 0; j < 7; ++j)
+        if (j < 6 || np > 6) {
+          const P2<R> gn = ld2<R>(a.lgxy + off, (long)nb[j] * ldk + l, act);
+          L.gxn[j] = gn.x; L.gyn[j] = gn.y;
+        }
+    } else if (a.order > 1) {
       L.gxs = ld(a.lgx + off, ic, act); L.gys = ld(a.lgy + off, ic, act);
 #pragma unroll
       for (int j = 0; j < 7; ++j)
@@ -1319,6 +1461,49 @@ __global__ __launch_bounds__(256) void k_tr_sgsmix_adj(DMesh<R> m, const R *__re
     for_slots(np, [&](int j) { acc = acc + ak[j] * (vtn[j] * ld(lsct + off, (long)nb[j] * ldk + l, act) - vt * lc); });
     if (act) lq[off + ic] = acc;
   }
+}
+
+// ---------------------------------------------------------------- tracer_maxmin
+// check_nans.F90:164-203: per tracer the maximum and minimum of var_p over the prognostic
+// levels lpw..mza of every column.  Floats are ordered through their bit patterns (an int
+// key that sorts as the float does) so each wave's reduction is folded in with one integer
+// atomic max per tracer and bound; max and min are exact, in any order.  NaN values are
+// skipped (fmax / fmin keep the other operand).
+__device__ __forceinline__ int f2key(float f) {
+  const int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
+
+__global__ void k_maxmin_init(int *keys, int n) {
+  const int i = threadIdx.x;
+  if (i < n) keys[i] = f2key(-3.402823466e+38f);   // -huge(1.)
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_tr_maxmin(DMesh<R> m, const R *q, int ntr, long tstride, int *keys) {
+  const int c = wave_item();
+  if (c >= m.nw) return;
+  const int l = lane_id(), k = l + 1;
+  const bool body = k >= 2 && k <= m.mza;
+  const long ic = (long)c * m.ldk + l;
+  for (int n = 0; n < ntr; ++n) {
+    const float v = (float)ld(q + n * tstride, ic, body);
+    float hi = body ? v : -3.402823466e+38f, lo = body ? -v : -3.402823466e+38f;
+    for (int s = 32; s >= 1; s >>= 1) {
+      hi = fmaxf(hi, __shfl_xor(hi, s, WAVE));
+      lo = fmaxf(lo, __shfl_xor(lo, s, WAVE));
+    }
+    if (l == 0) {
+      atomicMax(keys + 2 * n, f2key(hi));
+      atomicMax(keys + 2 * n + 1, f2key(lo));
+    }
+  }
+}
+
+__global__ void k_maxmin_final(const int *keys, int n, float *out) {
+  const int i = threadIdx.x;
+  if (i < n) out[i] = (i & 1) ? -key2f(keys[i]) : key2f(keys[i]);
 }
 
 // ---------------------------------------------------------------- mass sums

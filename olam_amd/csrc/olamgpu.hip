@@ -369,6 +369,7 @@ struct ImplBase {
   virtual void checkpoint(int, void *, int) = 0;
   virtual void set_pdef(int, int) = 0;
   virtual void set_sgsmix(int, int) = 0;
+  virtual void tracer_maxmin(float *) = 0;
 };
 
 // x**n for an integer n >= 0 in the type of x, one rounding per factor (Fortran's
@@ -463,6 +464,12 @@ struct Impl : ImplBase {
     size_t lds = 0;
   } pf, pa;
   bool use_patch = false;   // OLAMGPU_TRACER=patch selects the LDS-staged tracer kernels
+  // Order-2 tracer stages (outside a limited final stage) run as a gradient pass and an
+  // update pass with the gradient pairs stored side by side (default), as the same two
+  // passes with gx and gy in separate arrays (OLAMGPU_TRACER=twopass), or as one pass
+  // that forms every gradient where it is used (OLAMGPU_TRACER=fused; measured 2x slower
+  // at the bench size: it gathers the two-ring stencil of every upwind neighbour).
+  bool two_pass = false, fused_fwd = false;
   bool fuse_adj = true;     // one pass for the column and face parts of the adjoint stage
   // state
   DevBuf thil[2], wmc, wc, vxe, vye, vze, vmc, vc, vmasc, wmasc, alpha, pwfac, pvfac, thilt, vmxet, vmyet, vmzet, vmt;
@@ -490,7 +497,9 @@ struct Impl : ImplBase {
   // tracers
   int ntr;
   long tstride;
-  DevBuf trS, trX, trY, trgx, trgy, trsct;
+  // trG holds the gradients of the stage input: gx then gy (ntr fields each), or at order
+  // 2 outside a limited stage the (gx, gy) pairs side by side; lgG the same for the adjoint
+  DevBuf trS, trX, trY, trG, trsct;
   DevBuf trgq[3];              // order 3: quadratic reconstruction terms
   bool have_sct = false;
   // final-stage limiters (iscal_monot = 1 monotone, 2 positive definite)
@@ -499,9 +508,9 @@ struct Impl : ImplBase {
   DevBuf sgs_dev, rhof, lsct;       // do_sgsmix per tracer; nudged tracer density; adjoint of sct
   DevBuf lim_mode, lim_pre, lim_s1, lim_s2, lim_tsv, lim_tsw;
   bool any_lim = false, any_pre = false, taped_lim = false;
-  DevBuf ffimp, cflmax, nimp, anyimp, masspart, massout;
+  DevBuf ffimp, cflmax, nimp, anyimp, masspart, massout, mmkeys, mmout;
   // adjoint
-  DevBuf adjA, adjB, lam0, nu, lpart, lgx, lgy, lgq[3];
+  DevBuf adjA, adjB, lam0, nu, lpart, lgG, lgq[3];
   int cur_adj = 0;
   bool have_fwd = false;
   // ffimp (and whether any column is implicit) is current for the present vmasc, wmasc
@@ -545,6 +554,8 @@ struct Impl : ImplBase {
     if (const char *e = std::getenv("OLAMGPU_TPW")) tpw = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("OLAMGPU_TRACER")) {
       use_patch = std::strcmp(e, "patch") == 0;
+      two_pass = std::strcmp(e, "twopass") == 0;
+      fused_fwd = std::strcmp(e, "fused") == 0;
       fuse_adj = std::strcmp(e, "split") != 0;   // OLAMGPU_TRACER=split: separate adjoint passes
     }
     build_patch_tables();
@@ -883,7 +894,8 @@ struct Impl : ImplBase {
     ntr = prm.num_scalar;
     tstride = (long)wsz;
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
-    for (DevBuf *b : {&trS, &trX, &trY, &trgx, &trgy}) { b->alloc(tb); zero(*b); }
+    for (DevBuf *b : {&trS, &trX, &trY}) { b->alloc(tb); zero(*b); }
+    trG.alloc(2 * tb); zero(trG);
     if (prm.scal_horiz_adv_order == 3)
       for (DevBuf &b : trgq) { b.alloc(tb); zero(b); }
     pdef.assign(std::max(ntr, 1), 1);
@@ -901,13 +913,15 @@ struct Impl : ImplBase {
     cflmax.alloc(4 * sizeof(unsigned)); nimp.alloc(sizeof(int)); anyimp.alloc(sizeof(int));
     masspart.alloc((size_t)nblk(nw) * 65 * sizeof(double));
     massout.alloc(65 * sizeof(double));
+    mmkeys.alloc(2 * 64 * sizeof(int)); mmout.alloc(2 * 64 * sizeof(float));
     HIPCHK(hipStreamSynchronize(s));
   }
 
   void alloc_adj() {
     if (adjA.p) return;
     size_t tb = std::max<size_t>(1, (size_t)ntr) * wsz * sizeof(R);
-    for (DevBuf *b : {&adjA, &adjB, &lam0, &nu, &lpart, &lgx, &lgy}) { b->alloc(tb); zero(*b); }
+    for (DevBuf *b : {&adjA, &adjB, &lam0, &nu, &lpart}) { b->alloc(tb); zero(*b); }
+    lgG.alloc(2 * tb); zero(lgG);
     if (prm.scal_horiz_adv_order == 3)
       for (DevBuf &b : lgq) { b.alloc(tb); zero(b); }
     HIPCHK(hipStreamSynchronize(s));
@@ -1330,7 +1344,8 @@ struct Impl : ImplBase {
       DevBuf *out = outs[istage - 1];
       TrStage<R, RP> a;
       a.q_in = dp<R>(*in); a.q0 = dp<R>(trS); a.sct = have_sct ? dp<R>(trsct) : nullptr;
-      a.gx = dp<R>(trgx); a.gy = dp<R>(trgy); a.q_out = dp<R>(*out);
+      a.gx = dp<R>(trG); a.gy = dp<R>(trG) + (size_t)std::max(ntr, 1) * tstride; a.q_out = dp<R>(*out);
+      a.gxy = trG.as<P2<R>>();
       for (int t = 0; t < 3; ++t) a.gq[t] = dp<R>(trgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
       a.rho = rho_tr;
@@ -1371,11 +1386,22 @@ struct Impl : ImplBase {
   // reconstruction coefficients of the stage input, then the update of the unlimited
   // tracers (grad_t2d or grad_t2d_quadratic, scalar_v_column + scalar_w_column)
   void tracer_update(const TrStage<R, RP> &a) {
+    // a limited final stage keeps the gradients: the limiter passes read them
+    if (a.order < 3 && fused_fwd && !a.lim) {
+      LAUNCH("k_tr_stage", (k_tr_stage<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+      return;
+    }
+    if (a.order == 2 && !two_pass && !a.lim) {
+      LAUNCH("k_tr_grad", (k_tr_grad<R, true>), dim3(nblk_tr()), dim3(256), dm, a.q_in, nullptr, nullptr,
+             (P2<R> *)a.gxy, ntr, tstride, tpw);
+      LAUNCH("k_tr_update", (k_tr_update<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      return;
+    }
     if (a.order == 2)
-      LAUNCH("k_tr_grad", (k_tr_grad<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, dp<R>(trgx), dp<R>(trgy),
+      LAUNCH("k_tr_grad", (k_tr_grad<R, false>), dim3(nblk_tr()), dim3(256), dm, a.q_in, (R *)a.gx, (R *)a.gy, nullptr,
              ntr, tstride, tpw);
     if (a.order == 3) {
-      const Q5<R> g{{dp<R>(trgx), dp<R>(trgy), dp<R>(trgq[0]), dp<R>(trgq[1]), dp<R>(trgq[2])}};
+      const Q5<R> g{{(R *)a.gx, (R *)a.gy, dp<R>(trgq[0]), dp<R>(trgq[1]), dp<R>(trgq[2])}};
       LAUNCH("k_tr_grad3", (k_tr_grad3<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, g, ntr, tstride, tpw, a.tmode,
              a.lim);
       LAUNCH("k_tr_update", (k_tr_update<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
@@ -1413,7 +1439,8 @@ struct Impl : ImplBase {
       DevBuf &muB = cur_adj_buf();
       DevBuf &lqB = cur_adj ? adjA : adjB;
       a.mu = dp<R>(muB); a.lam0 = dp<R>(lam0); a.nu = dp<R>(nu); a.lpart = dp<R>(lpart);
-      a.lgx = dp<R>(lgx); a.lgy = dp<R>(lgy); a.lq = dp<R>(lqB);
+      a.lgx = dp<R>(lgG); a.lgy = dp<R>(lgG) + (size_t)std::max(ntr, 1) * tstride; a.lq = dp<R>(lqB);
+      a.lgxy = lgG.as<P2<R>>();
       for (int t = 0; t < 3; ++t) a.lgq[t] = dp<R>(lgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
       a.rho = rho_tr;
@@ -1439,8 +1466,9 @@ struct Impl : ImplBase {
         else if (pa.cpw == 2) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 2>), g, b, pa.lds, dm, a, pa.tab);
         else LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 1>), g, b, pa.lds, dm, a, pa.tab);
       } else {
-        adjoint_faces(a);
-        adjoint_coefs(a);
+        const bool pk = a.order == 2 && fuse_adj && !two_pass;
+        adjoint_faces(a, pk);
+        adjoint_coefs(a, pk);
       }
       cur_adj = 1 - cur_adj;
     }
@@ -1452,9 +1480,12 @@ struct Impl : ImplBase {
   }
   // column and face parts of an adjoint stage for the unlimited tracers: fused in one
   // pass (default) or as k_tr_adj0 + k_tr_adj1 (OLAMGPU_TRACER=split)
-  void adjoint_faces(const TrAdj<R, RP> &a) {
+  // pk: the gradient adjoints as (lgx, lgy) pairs (order 2, fused column and face pass)
+  void adjoint_faces(const TrAdj<R, RP> &a, bool pk = false) {
     const bool q3 = a.order == 3;
-    if (fuse_adj) {
+    if (pk) {
+      LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
+    } else if (fuse_adj) {
       if (q3) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
       else LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
     } else {
@@ -1464,8 +1495,9 @@ struct Impl : ImplBase {
     }
   }
   // transpose of the reconstruction coefficients into the stage input's adjoint
-  void adjoint_coefs(const TrAdj<R, RP> &a) {
+  void adjoint_coefs(const TrAdj<R, RP> &a, bool pk = false) {
     if (a.order == 3) LAUNCH("k_tr_adj2q", (k_tr_adj2q<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
+    else if (pk) LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
     else LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
   }
 
@@ -1477,6 +1509,16 @@ struct Impl : ImplBase {
     LAUNCH("k_mass_final", k_mass_final, dim3(1), dim3(256), masspart.as<double>(), (int)nb, 1 + ntr,
                        massout.as<double>());
     HIPCHK(hipMemcpyAsync(out, massout.p, (1 + ntr) * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  // tracer_maxmin (check_nans.F90:164-203): out[2n] = max, out[2n+1] = min of tracer n+1
+  void tracer_maxmin(float *out) override {
+    if (ntr == 0) return;
+    LAUNCH("k_maxmin_init", k_maxmin_init, dim3(1), dim3(128), mmkeys.as<int>(), 2 * ntr);
+    LAUNCH("k_tr_maxmin", (k_tr_maxmin<R>), dim3(nblk(nw)), dim3(256), dm, dp<R>(trS), ntr, tstride,
+           mmkeys.as<int>());
+    LAUNCH("k_maxmin_final", k_maxmin_final, dim3(1), dim3(128), mmkeys.as<int>(), 2 * ntr, mmout.as<float>());
+    HIPCHK(hipMemcpyAsync(out, mmout.p, 2 * ntr * sizeof(float), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   }
   void cfl_report(double *o) override {
@@ -1745,6 +1787,12 @@ int olamgpu_copy_field(olamgpu_ctx *c, int32_t field_id, int32_t index, void *de
   return guarded(c, [&] {
     if (!dev_buf || (direction != 0 && direction != 1)) throw Err(OLAMGPU_EINVAL, "bad copy arguments");
     c->impl->copy_field(field_id, index, dev_buf, direction, elem_bytes);
+  });
+}
+int olamgpu_tracer_maxmin(olamgpu_ctx *c, float *out) {
+  return guarded(c, [&] {
+    if (!out) throw Err(OLAMGPU_EINVAL, "tracer_maxmin needs an output array");
+    c->impl->tracer_maxmin(out);
   });
 }
 int olamgpu_set_scalar_sgsmix(olamgpu_ctx *c, int32_t n, int32_t v) {

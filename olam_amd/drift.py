@@ -1,14 +1,15 @@
-"""FP32-bulk versus FP64 drift and mass conservation over many long steps.
+"""FP32-bulk and reference-precision drift against FP64, and mass conservation, over
+many long steps.
 
     python -m olam_amd.drift [--nxp 320] [--nlev 60] [--ntr 10] [--steps 100] [--dt 30]
 
 Runs the same initial state (the balanced zonal jet with a 1 K theta bump and
 log-normal tracers) through N long steps of tend0, prog_wrtv_rk, timeavg_momsc and
-scalar_transport_rk twice on the GPU: once in the FP32-bulk mode and once in the FP64
-mode (which matches the CPU oracle to 1e-12 on the parity cases), then reports per
-prognostic field and per tracer the relative L2 and Linf differences, and the
-relative change of the dry-air and tracer masses (sum rho volt, sum rho volt q) of
-each run.  Prints one JSON object.
+scalar_transport_rk on the GPU in the FP64 mode (which matches the CPU oracle to 1e-12
+on the parity cases), the FP32-bulk mode and the reference-precision mode (FP64 rho and
+press), then reports per prognostic field and per tracer the relative L2 and Linf
+differences of each reduced mode from FP64, and the relative change of the dry-air and
+tracer masses (sum rho volt, sum rho volt q) of each run.  Prints one JSON object.
 """
 
 from __future__ import annotations
@@ -57,25 +58,33 @@ def rel(a, b):
             float(np.abs(d).max() / nbi) if nbi > 0 else float(np.abs(d).max()))
 
 
-def drift(nxp=320, nlev=60, ntr=10, nsteps=100, dt=30.0):
-    grid = abi.FortranGrid(M.build_mesh(nxp), M.vertical_grid(nlev))
+def drift(nxp=320, nlev=60, ntr=10, nsteps=100, dt=30.0, modes=("f32", "mixed"), grid=None, log=None):
+    if grid is None:
+        grid = abi.FortranGrid(M.build_mesh(nxp), M.vertical_grid(nlev))
     st0 = cases.jet_state(grid)
     tr = cases.lognormal_tracers(grid, ntr)
-    lo, qlo, m0lo, m1lo, slo = run(grid, st0, tr, "f32", nsteps, dt)
-    hi, qhi, m0hi, m1hi, shi = run(grid, st0, tr, "f64", nsteps, dt)
+    runs = {}
+    for prec in ("f64",) + tuple(modes):
+        if log:
+            log(f"[drift] {prec}: {nsteps} steps")
+        runs[prec] = run(grid, st0, tr, prec, nsteps, dt)
+    hi, qhi, m0hi, m1hi, shi = runs["f64"]
     res = {"workload": f"NXP={nxp} ({grid.mwa - 1} columns), L={grid.mza - 1}, "
                        f"{ntr} tracers, {nsteps} steps of {dt} s",
-           "fields": {}, "tracers": {}}
-    for f in FIELDS:
-        l2, li = rel(lo.arrays[f], hi.arrays[f])
-        res["fields"][f] = {"rel_l2": l2, "rel_linf": li}
-    for n in range(ntr):
-        l2, li = rel(qlo[n], qhi[n])
-        res["tracers"][f"q{n + 1}"] = {"rel_l2": l2, "rel_linf": li}
-    res["mass_change_f32"] = ((m1lo - m0lo) / m0lo).tolist()
-    res["mass_change_f64"] = ((m1hi - m0hi) / m0hi).tolist()
-    res["seconds"] = {"f32": slo, "f64": shi}
-    res["max_tracer_rel_l2"] = max(v["rel_l2"] for v in res["tracers"].values())
+           "mass_change": {"f64": ((m1hi - m0hi) / m0hi).tolist()}, "seconds": {"f64": shi}}
+    for prec in modes:
+        lo, qlo, m0lo, m1lo, slo = runs[prec]
+        r = {"fields": {}, "tracers": {}}
+        for f in FIELDS:
+            l2, li = rel(lo.arrays[f], hi.arrays[f])
+            r["fields"][f] = {"rel_l2": l2, "rel_linf": li}
+        for n in range(ntr):
+            l2, li = rel(qlo[n], qhi[n])
+            r["tracers"][f"q{n + 1}"] = {"rel_l2": l2, "rel_linf": li}
+        r["max_tracer_rel_l2"] = max(v["rel_l2"] for v in r["tracers"].values())
+        res[prec] = r
+        res["mass_change"][prec] = ((m1lo - m0lo) / m0lo).tolist()
+        res["seconds"][prec] = slo
     return res
 
 

@@ -97,7 +97,7 @@ module olamgpu_iface
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
        olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_checkpoint_bytes, &
        olamgpu_checkpoint_save, olamgpu_checkpoint_restore, olamgpu_set_scalar_pdef, &
-       olamgpu_set_scalar_sgsmix, &
+       olamgpu_set_scalar_sgsmix, olamgpu_tracer_maxmin, &
        olamgpu_error_message
 
   interface
@@ -151,6 +151,12 @@ module olamgpu_iface
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: n, pdef
      end function olamgpu_set_scalar_pdef
+
+     integer(c_int) function olamgpu_tracer_maxmin(ctx, out) bind(C, name="olamgpu_tracer_maxmin")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: out(*)
+     end function olamgpu_tracer_maxmin
 
      integer(c_int) function olamgpu_set_scalar_sgsmix(ctx, n, do_sgsmix) bind(C, name="olamgpu_set_scalar_sgsmix")
        import :: c_int, c_int32_t, c_ptr

@@ -53,6 +53,7 @@ def lib():
         L.olamgpu_copy_field.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
         L.olamgpu_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
+        L.olamgpu_tracer_maxmin.argtypes = [vp, vp]
         L.olamgpu_set_scalar_sgsmix.argtypes = [vp, C.c_int32, C.c_int32]
         L.olamgpu_checkpoint_bytes.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.olamgpu_checkpoint_save.argtypes = [vp, C.c_int32, vp]
@@ -71,7 +72,7 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
            "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef",
            "olamgpu_comp_alpha_press", "olamgpu_set_turbulence", "olamgpu_set_nudging",
-           "olamgpu_set_scalar_sgsmix"]
+           "olamgpu_set_scalar_sgsmix", "olamgpu_tracer_maxmin"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
@@ -198,6 +199,13 @@ class Context:
         out = np.zeros(1 + self.params.num_scalar)
         self._chk(lib().olamgpu_mass_sums(self.h, _p(out)), "mass_sums")
         return out
+
+    def tracer_maxmin(self):
+        """(num_scalar, 2) float32: max and min of each tracer over the prognostic levels
+        (tracer_maxmin, check_nans.F90:164-203)."""
+        out = np.zeros(2 * max(self.params.num_scalar, 1), np.float32)
+        self._chk(lib().olamgpu_tracer_maxmin(self.h, _p(out)), "tracer_maxmin")
+        return out[: 2 * self.params.num_scalar].reshape(-1, 2)
 
     def cfl_report(self):
         out = np.zeros(4)

@@ -21,6 +21,7 @@
 // vertical order 3 (iorderv = 3), the global domain (mdomain /= 4), no nudging or
 // eddy diffusion (their switches are off in every configuration exercised here).
 
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -101,6 +102,7 @@ struct Base {
   virtual void scalar_transport_rk_adj() = 0;
   virtual void mass_sums(double *) = 0;
   virtual void cfl_report(double *) = 0;
+  virtual void tracer_maxmin(float *) = 0;
   virtual int set_pdef(int, int) = 0;
   virtual int set_sgsmix(int, int) = 0;
   virtual void set_frozen(int) = 0;
@@ -2060,6 +2062,21 @@ struct Model : Base {
     for (size_t n = 0; n < scp.size(); ++n) out[1 + n] = tr[n];
   }
   void cfl_report(double *o) override { for (int i = 0; i < 4; ++i) o[i] = cfl_out[i]; }
+  // tracer_maxmin, check_nans.F90:164-203: smax(1) = max over the prognostic levels,
+  // smax(2) = max of -var_p, from -huge(1.); out[2n] = smax(1), out[2n+1] = -smax(2)
+  void tracer_maxmin(float *out) override {
+    for (size_t n = 0; n < scp.size(); ++n) {
+      float s1 = -FLT_MAX, s2 = -FLT_MAX;
+      for (int iw = 2; iw <= mwa; ++iw)
+        for (int k = 2; k <= mza; ++k) {
+          const float v = (float)scp[n](k, iw);
+          s1 = std::max(s1, v);
+          s2 = std::max(s2, -v);
+        }
+      out[2 * n] = s1;
+      out[2 * n + 1] = -s2;
+    }
+  }
 };
 
 }  // namespace
@@ -2121,6 +2138,7 @@ int oracle_scalar_transport_rk_adj(void *h) {
   return 0;
 }
 int oracle_mass_sums(void *h, double *out) { ((oracle_ctx *)h)->m->mass_sums(out); return 0; }
+int oracle_tracer_maxmin(void *h, float *out) { ((oracle_ctx *)h)->m->tracer_maxmin(out); return 0; }
 int oracle_cfl_report(void *h, double *out) { ((oracle_ctx *)h)->m->cfl_report(out); return 0; }
 int oracle_set_scalar_pdef(void *h, int32_t n, int32_t pdef) { return ((oracle_ctx *)h)->m->set_pdef(n, pdef); }
 int oracle_set_scalar_sgsmix(void *h, int32_t n, int32_t v) { return ((oracle_ctx *)h)->m->set_sgsmix(n, v); }

@@ -47,6 +47,7 @@ def lib():
         L.oracle_scalar_transport_rk.argtypes = [vp, vp]
         L.oracle_mass_sums.argtypes = [vp, vp]
         L.oracle_cfl_report.argtypes = [vp, vp]
+        L.oracle_tracer_maxmin.argtypes = [vp, vp]
         L.oracle_set_threads.argtypes = [C.c_int32]
         L.oracle_tridiffo.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
         L.oracle_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
@@ -183,6 +184,12 @@ class Oracle:
         out = np.zeros(1 + self.params.num_scalar)
         lib().oracle_mass_sums(self.h, _p(out))
         return out
+
+    def tracer_maxmin(self):
+        """(num_scalar, 2) float32: max and min of each tracer over the prognostic levels."""
+        out = np.zeros(2 * self.params.num_scalar, np.float32)
+        lib().oracle_tracer_maxmin(self.h, _p(out))
+        return out.reshape(-1, 2)
 
     def cfl_report(self):
         out = np.zeros(4)

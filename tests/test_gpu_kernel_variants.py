@@ -1,7 +1,10 @@
-"""The tracer stage has three kernel arrangements selected when a context is created
-(OLAMGPU_TRACER): the default (gather kernels, with the column and face parts of the
-adjoint in one pass), "split" (the adjoint's column and face parts as two passes with
-nu written between them) and "patch" (LDS-staged patches of columns).  They evaluate
+"""The tracer stage has five kernel arrangements selected when a context is created
+(OLAMGPU_TRACER): the default (a gradient pass and an update pass with the gradient
+pairs stored side by side, and the adjoint's column and face parts in one pass),
+"twopass" (the same with gx and gy in separate arrays), "fused" (one pass per stage that
+forms every gradient where it is used, at orders 1 and 2), "split" (the adjoint's
+column and face parts as two passes with nu written between them) and "patch"
+(LDS-staged patches of columns).  They evaluate
 the same expressions in the same order, so their outputs agree to the last bit or to
 FMA-contraction rounding; each is checked here against the default, and the default is
 checked against the oracle by the parity tests.  At order 3 "patch" runs the gather
@@ -73,9 +76,10 @@ def test_variants_agree(grid_factory, prec, order, wnoise):
     prm = abi.make_params(prec, num_scalar=ntr, dtsm=600.0, order=order)
     rng = np.random.default_rng(11)
     lam = [rng.standard_normal((grid.mwa, grid.mza)).astype(np.float32) for _ in range(ntr)]
-    res = {v: _run(grid, prm, v, w, st, tr, lam, wnoise) for v in ("default", "split", "patch")}
+    variants = ("twopass", "fused", "split", "patch")
+    res = {v: _run(grid, prm, v, w, st, tr, lam, wnoise) for v in ("default",) + variants}
     q0, a0 = res["default"]
-    for v in ("split", "patch"):
+    for v in variants:
         q, a = res[v]
         for n in range(ntr):
             eq, ea = _rel(q[n], q0[n]), _rel(a[n], a0[n])

@@ -354,3 +354,25 @@ def test_relabelled_mesh_bit_identical(grid_factory):
         outs.append([bk(o.download_scalar(2)), bk(res.arrays["thil"]), bk(res.arrays["vmc"], "v")])
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+def test_tracer_maxmin_matches_numpy(grid_factory):
+    """tracer_maxmin (check_nans.F90:164-203): max and min of each tracer over levels
+    lpw..mza of every real column; the dummy column and level 1 are left out (values
+    planted there must not show), and a NaN is skipped."""
+    grid = grid_factory(6, 10)
+    rng = np.random.default_rng(4)
+    o = Oracle(grid, abi.make_params("f32", num_scalar=3))
+    qs = []
+    for n in range(3):
+        q = (rng.standard_normal((grid.mwa, grid.mza)) * (n + 1) - n).astype(np.float32)
+        q[0, :] = 1e30          # dummy column
+        q[:, 0] = -1e30         # level 1
+        if n == 2:
+            q[5, 4] = np.nan
+        qs.append(q)
+        o.upload_scalar(n + 1, q)
+    mm = o.tracer_maxmin()
+    for n, q in enumerate(qs):
+        body = q[1:, 1:]
+        assert mm[n, 0] == np.nanmax(body) and mm[n, 1] == np.nanmin(body)
