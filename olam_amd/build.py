@@ -4,6 +4,7 @@ Fortran ISO_C_BINDING interface module.
 Everything is built in-tree so the shared objects travel with the repository:
   olam_amd/lib/libolamgpu.so      product: HIP kernels + C-ABI (include/olamgpu.h)
   olam_amd/lib/olamgpu_iface.mod  Fortran interface module (+ .o) when amdflang exists
+  olam_amd/lib/libolamio.so       HDF5 gridfile / history storage (C libhdf5 of the image)
   oracle/build/liboracle.so       CPU restatement used only by tests and the CPU baseline
 """
 
@@ -85,9 +86,18 @@ def build_fortran():
     return exe
 
 
+def build_io(verbose=False):
+    """The HDF5 storage layer of the gridfile / history I/O (olam_amd/gridfile.py)."""
+    from . import gridfile
+    if not os.path.exists(os.path.join(gridfile.HDF5_PREFIX, "include", "hdf5.h")):
+        return None
+    return gridfile.build(verbose)
+
+
 def build_all(verbose=False):
     build_hip(verbose)
     build_copy(verbose)
     build_oracle()
+    build_io(verbose)
     if os.path.exists(os.path.join(FORTRAN, "olamgpu_iface.f90")):
         build_fortran()

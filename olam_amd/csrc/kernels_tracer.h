@@ -715,7 +715,6 @@ struct TrAdj {
   R *mu;             // adjoint of the stage output (overwritten by the implicit pass)
   R *lam0;           // accumulates the adjoint of scp0
   R *nu, *lpart, *lgx, *lgy;
-  P2<R> *lgxy;       // order 2, unlimited stage: the gradient adjoints (lgx, lgy) side by side
   R *lgq[3];         // order 3: adjoints of the quadratic terms
   R *lq;             // adjoint of the stage input
   const R *vmasc, *wmasc, *rho_old, *ffimp;
@@ -931,7 +930,7 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
 // which is rebuilt here from their mu (nu = mu dtr volti / rho, the same expression as
 // in k_tr_adj0) instead of being written and read back.  Per tracer it reads mu of the
 // column and its neighbours and lam0, and writes lam0, lpart, lgx, lgy.
-template <class R, class RP, bool Q3, bool PK = false>
+template <class R, class RP, bool Q3>
 __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
@@ -1051,8 +1050,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
     if (act) {
       st_nt(a.lam0, off + ic, l0);
       st_nt(a.lpart, off + ic, lqv + P);
-      if constexpr (PK) st_nt(a.lgxy, off + ic, mk2(Gx, Gy));
-      else if (a.order > 1) { st_nt(a.lgx, off + ic, Gx); st_nt(a.lgy, off + ic, Gy); }
+      if (a.order > 1) { st_nt(a.lgx, off + ic, Gx); st_nt(a.lgy, off + ic, Gy); }
       if constexpr (Q3) {
 #pragma unroll
         for (int t = 0; t < 3; ++t) a.lgq[t][off + ic] = body ? Gq[t] : R(0);
@@ -1061,7 +1059,7 @@ __global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   }
 }
 
-template <class R, class RP, bool PK = false>
+template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
@@ -1089,16 +1087,7 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
     L.gxs = R(0); L.gys = R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j) { L.gxn[j] = R(0); L.gyn[j] = R(0); }
-    if constexpr (PK) {
-      const P2<R> g = ld2<R>(a.lgxy + off, ic, act);
-      L.gxs = g.x; L.gys = g.y;
-#pragma unroll
-      for (int j = 0; j < 7; ++j)
-        if (j < 6 || np > 6) {
-          const P2<R> gn = ld2<R>(a.lgxy + off, (long)nb[j] * ldk + l, act);
-          L.gxn[j] = gn.x; L.gyn[j] = gn.y;
-        }
-    } else if (a.order > 1) {
+    if (a.order > 1) {
       L.gxs = ld(a.lgx + off, ic, act); L.gys = ld(a.lgy + off, ic, act);
 #pragma unroll
       for (int j = 0; j < 7; ++j)

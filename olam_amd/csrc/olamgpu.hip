@@ -498,7 +498,10 @@ struct Impl : ImplBase {
   int ntr;
   long tstride;
   // trG holds the gradients of the stage input: gx then gy (ntr fields each), or at order
-  // 2 outside a limited stage the (gx, gy) pairs side by side; lgG the same for the adjoint
+  // 2 outside a limited stage the (gx, gy) pairs side by side (measured at NXP=320, L=60,
+  // 10 tracers: grad + update 6.47 ms per stage against 6.66 with separate arrays; the
+  // adjoint's pair layout was slower, 7.64 against 7.20, and is not used); lgG holds the
+  // adjoint's gradient adjoints, lgx then lgy
   DevBuf trS, trX, trY, trG, trsct;
   DevBuf trgq[3];              // order 3: quadratic reconstruction terms
   bool have_sct = false;
@@ -1440,7 +1443,6 @@ struct Impl : ImplBase {
       DevBuf &lqB = cur_adj ? adjA : adjB;
       a.mu = dp<R>(muB); a.lam0 = dp<R>(lam0); a.nu = dp<R>(nu); a.lpart = dp<R>(lpart);
       a.lgx = dp<R>(lgG); a.lgy = dp<R>(lgG) + (size_t)std::max(ntr, 1) * tstride; a.lq = dp<R>(lqB);
-      a.lgxy = lgG.as<P2<R>>();
       for (int t = 0; t < 3; ++t) a.lgq[t] = dp<R>(lgq[t]);
       a.vmasc = dp<R>(vmasc); a.wmasc = dp<R>(wmasc); a.rho_old = dp<R>(rho_old); a.ffimp = dp<R>(ffimp);
       a.rho = rho_tr;
@@ -1466,9 +1468,8 @@ struct Impl : ImplBase {
         else if (pa.cpw == 2) LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 2>), g, b, pa.lds, dm, a, pa.tab);
         else LAUNCH_SH("k_tr_adj_patch", (k_tr_adj_patch<R, RP, 1>), g, b, pa.lds, dm, a, pa.tab);
       } else {
-        const bool pk = a.order == 2 && fuse_adj && !two_pass;
-        adjoint_faces(a, pk);
-        adjoint_coefs(a, pk);
+        adjoint_faces(a);
+        adjoint_coefs(a);
       }
       cur_adj = 1 - cur_adj;
     }
@@ -1480,12 +1481,9 @@ struct Impl : ImplBase {
   }
   // column and face parts of an adjoint stage for the unlimited tracers: fused in one
   // pass (default) or as k_tr_adj0 + k_tr_adj1 (OLAMGPU_TRACER=split)
-  // pk: the gradient adjoints as (lgx, lgy) pairs (order 2, fused column and face pass)
-  void adjoint_faces(const TrAdj<R, RP> &a, bool pk = false) {
+  void adjoint_faces(const TrAdj<R, RP> &a) {
     const bool q3 = a.order == 3;
-    if (pk) {
-      LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
-    } else if (fuse_adj) {
+    if (fuse_adj) {
       if (q3) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
       else LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
     } else {
@@ -1495,9 +1493,8 @@ struct Impl : ImplBase {
     }
   }
   // transpose of the reconstruction coefficients into the stage input's adjoint
-  void adjoint_coefs(const TrAdj<R, RP> &a, bool pk = false) {
+  void adjoint_coefs(const TrAdj<R, RP> &a) {
     if (a.order == 3) LAUNCH("k_tr_adj2q", (k_tr_adj2q<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
-    else if (pk) LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
     else LAUNCH("k_tr_adj2", (k_tr_adj2<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);
   }
 

@@ -527,6 +527,7 @@ def _horizontal_metrics(mesh: Mesh, xyz_unit: np.ndarray):
         dnu=dnu, dnv=dnv, dniu=1.0 / dnu, dniv=1.0 / dnv,
         arw0=arw0, arm0=arm0, glatw=glatw, glonw=glonw,
         farm=farm, farv=farv, gxps_coef=gxps_coef, gyps_coef=gyps_coef,
+        gxps1=gxps1, gyps1=gyps1, gxps2=gxps2, gyps2=gyps2,
         dxps=dxps, dyps=dyps, ecvec_vx=ecvec[..., 0], ecvec_vy=ecvec[..., 1],
         ecvec_vz=ecvec[..., 2], ecvec_ok=ok,
     ))
