@@ -266,6 +266,16 @@ int olamgpu_download_scalar_adj(olamgpu_ctx *ctx, int32_t n, float *lambda);
  * out[1+n] = sum rho*volt*q_n for n = 0..num_scalar-1, all FP64 accumulations. */
 int olamgpu_mass_sums(olamgpu_ctx *ctx, double *out);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The sums of olamgpu_mass_sums, written to out_dev, a DEVICE array of 1 + num_scalar
+ * doubles, with no allocation and no host copy: the partial sums live in a static
+ * device array of 1024 x 65 doubles (520 KiB per GPU, held until the process ends), so
+ * num_scalar may be at most 64.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (from several threads or contexts) race.  Returns after the
+ * context's stream is synchronised, like every entry point. */
+int olamgpu_mass_sums_noalloc(olamgpu_ctx *ctx, double *out_dev);
+
 /* tracer_maxmin (check_nans.F90:164-203): out[2n] and out[2n+1] = the maximum and the
  * minimum of tracer n+1 (n = 0..num_scalar-1) over levels lpw..mza of every column, FP32
  * as var_p; NaN values are skipped.  The reference prints them; here they are returned. */

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -345,6 +346,61 @@ static Patches build_patches(const Topo &t, int P) {
 }
 
 // ------------------------------------------------------------------ implementation
+// ---------------------------------------------------------------- allocation-free mass sums
+// olamgpu_mass_sums_noalloc forms the sums of olamgpu_mass_sums with its per-block partial
+// sums in this file-scope device array instead of a buffer the library allocates: a
+// fixed grid of MASS_NB blocks strides over the columns, each block keeping MASS_NV
+// partials (dry mass and up to 64 tracers).  Each GPU holds its own copy of the array
+// until the process ends; every call on a GPU shares it, so calls that overlap on one GPU
+// race.  A call writes every partial it reads, so nothing needs clearing between calls.
+constexpr int MASS_NB = 1024, MASS_NV = 65;
+static __device__ __align__(256) double scratch[MASS_NB * MASS_NV];     // 520 KiB per GPU
+
+// Device address of `scratch` on each device, taken once per device with that device
+// current (olamgpu_create): host code uses this, never the array's name.
+static std::mutex g_scratch_mu;
+static double *g_scratch[64] = {};
+static void scratch_setup(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= 64) throw Err(OLAMGPU_EINVAL, "device index beyond the scratch table");
+  if (g_scratch[dev]) return;
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[dev] = static_cast<double *>(p);
+}
+static double *scratch_ptr(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (dev >= 0 && dev < 64) ? g_scratch[dev] : nullptr;
+}
+
+// Partial sums of rho volt and rho volt q_n (FP64) per block over the columns
+// blockIdx.x * WPB + wave, stepping by gridDim.x * WPB; each wave keeps its own row of
+// LDS partials, summed in a fixed order, so the result is deterministic.
+template <class R, class RP>
+__global__ __launch_bounds__(256) void k_mass_partial_gs(DMesh<R> m, const RP *rho, const R *q, int ntr,
+                                                         long tstride, double *part) {
+  __shared__ double sh[WPB][MASS_NV];
+  const int w = threadIdx.x >> 6, l = lane_id(), k = l + 1;
+  for (int v = l; v <= ntr; v += WAVE) sh[w][v] = 0.0;
+  __syncthreads();
+  const bool lev = k >= 2 && k <= m.mza;
+  for (int c = blockIdx.x * WPB + w; c < m.nw; c += gridDim.x * WPB) {
+    const long ic = (long)c * m.ldk + l;
+    const double mass = lev ? (double)rho[ic] * m.volt[ic] : 0.0;
+    for (int n = -1; n < ntr; ++n) {
+      double v = n < 0 ? mass : (lev ? mass * (double)q[n * tstride + ic] : 0.0);
+      for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, WAVE);
+      if (l == 0) sh[w][n + 1] += v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x <= ntr) {
+    double s = 0.0;
+    for (int i = 0; i < WPB; ++i) s += sh[i][threadIdx.x];
+    part[(long)blockIdx.x * MASS_NV + threadIdx.x] = s;
+  }
+}
+
 struct ImplBase {
   virtual ~ImplBase() {}
   virtual void upload_state(const olamgpu_state *) = 0;
@@ -370,6 +426,7 @@ struct ImplBase {
   virtual void set_pdef(int, int) = 0;
   virtual void set_sgsmix(int, int) = 0;
   virtual void tracer_maxmin(float *) = 0;
+  virtual void mass_sums_noalloc(double *, double *) = 0;
 };
 
 // x**n for an integer n >= 0 in the type of x, one rounding per factor (Fortran's
@@ -1508,6 +1565,16 @@ struct Impl : ImplBase {
     HIPCHK(hipMemcpyAsync(out, massout.p, (1 + ntr) * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   }
+  // compute_mass_sums into a device array, with the partial sums in the static scratch
+  // (no allocation, no host copy)
+  void mass_sums_noalloc(double *out_dev, double *scr) override {
+    if (1 + ntr > MASS_NV) throw Err(OLAMGPU_EINVAL, "mass_sums_noalloc carries at most 64 tracers");
+    if (!scr) throw Err(OLAMGPU_ESTATE, "scratch address not set up on this device");
+    hipLaunchKernelGGL((k_mass_partial_gs<R, RP>), dim3(MASS_NB), dim3(256), 0, s, dm, dp<RP>(rho), dp<R>(trS), ntr,
+                       tstride, scr);
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, s, (const double *)scr, MASS_NB, 1 + ntr, out_dev);
+    HIPCHK(hipGetLastError());
+  }
   // tracer_maxmin (check_nans.F90:164-203): out[2n] = max, out[2n+1] = min of tracer n+1
   void tracer_maxmin(float *out) override {
     if (ntr == 0) return;
@@ -1698,6 +1765,7 @@ int olamgpu_create(const olamgpu_mesh *mesh, const olamgpu_params *params, int32
     if (dev < 0) HIPCHK(hipGetDevice(&dev));
     c->device = dev;
     HIPCHK(hipSetDevice(dev));
+    scratch_setup(dev);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -1775,6 +1843,12 @@ int olamgpu_download_scalar_adj(olamgpu_ctx *c, int32_t n, float *p) {
 }
 int olamgpu_mass_sums(olamgpu_ctx *c, double *out) {
   return guarded(c, [&] { if (!out) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->mass_sums(out); });
+}
+int olamgpu_mass_sums_noalloc(olamgpu_ctx *c, double *out_dev) {
+  return guarded(c, [&] {
+    if (!out_dev) throw Err(OLAMGPU_EINVAL, "null buffer");
+    c->impl->mass_sums_noalloc(out_dev, scratch_ptr(c->device));
+  });
 }
 int olamgpu_cfl_report(olamgpu_ctx *c, double *out) {
   return guarded(c, [&] { if (!out) throw Err(OLAMGPU_EINVAL, "null buffer"); c->impl->cfl_report(out); });

@@ -97,7 +97,7 @@ module olamgpu_iface
        olamgpu_cfl_report, olamgpu_copy_field, olamgpu_last_call_ms, &
        olamgpu_set_profiling, olamgpu_kernel_profile, olamgpu_checkpoint_bytes, &
        olamgpu_checkpoint_save, olamgpu_checkpoint_restore, olamgpu_set_scalar_pdef, &
-       olamgpu_set_scalar_sgsmix, olamgpu_tracer_maxmin, &
+       olamgpu_set_scalar_sgsmix, olamgpu_tracer_maxmin, olamgpu_mass_sums_noalloc, &
        olamgpu_error_message
 
   interface
@@ -151,6 +151,13 @@ module olamgpu_iface
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: n, pdef
      end function olamgpu_set_scalar_pdef
+
+     ! no counterpart in the reference: for callers that cannot allocate device memory;
+     ! out_dev is a device array of 1 + num_scalar doubles
+     integer(c_int) function olamgpu_mass_sums_noalloc(ctx, out_dev) bind(C, name="olamgpu_mass_sums_noalloc")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, out_dev
+     end function olamgpu_mass_sums_noalloc
 
      integer(c_int) function olamgpu_tracer_maxmin(ctx, out) bind(C, name="olamgpu_tracer_maxmin")
        import :: c_int, c_ptr, c_float

@@ -54,6 +54,7 @@ def lib():
         L.olamgpu_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
         L.olamgpu_set_scalar_pdef.argtypes = [vp, C.c_int32, C.c_int32]
         L.olamgpu_tracer_maxmin.argtypes = [vp, vp]
+        L.olamgpu_mass_sums_noalloc.argtypes = [vp, vp]
         L.olamgpu_set_scalar_sgsmix.argtypes = [vp, C.c_int32, C.c_int32]
         L.olamgpu_checkpoint_bytes.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.olamgpu_checkpoint_save.argtypes = [vp, C.c_int32, vp]
@@ -72,7 +73,7 @@ EXPORTS = ["olamgpu_create", "olamgpu_destroy", "olamgpu_last_error", "olamgpu_u
            "olamgpu_set_profiling", "olamgpu_kernel_profile", "olamgpu_checkpoint_bytes",
            "olamgpu_checkpoint_save", "olamgpu_checkpoint_restore", "olamgpu_set_scalar_pdef",
            "olamgpu_comp_alpha_press", "olamgpu_set_turbulence", "olamgpu_set_nudging",
-           "olamgpu_set_scalar_sgsmix", "olamgpu_tracer_maxmin"]
+           "olamgpu_set_scalar_sgsmix", "olamgpu_tracer_maxmin", "olamgpu_mass_sums_noalloc"]
 
 CKPT_STATE, CKPT_TAPE = 1, 2
 
