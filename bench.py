@@ -23,6 +23,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads: one per core of this process's share, bound close
+# (read by the OpenMP runtime when it starts, so set before anything loads it)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 
@@ -119,9 +123,9 @@ def host_cpu():
     return model, len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(grid, ntr, prec, dt, threads, max_steps=5, time_box=60.0):
-    """Oracle (CPU restatement, OpenMP) on the metric workload: one warm-up step, then
-    timed steps until max_steps or the time box (at least 2); the median step."""
+def cpu_baseline(grid, ntr, prec, dt, threads, max_steps=5, time_box=60.0, warmup=2):
+    """Oracle (CPU restatement, OpenMP) on the metric workload: `warmup` untimed steps,
+    then timed steps until max_steps or the time box (at least 2); the median step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from olam_amd import abi, cases
@@ -142,7 +146,8 @@ def cpu_baseline(grid, ntr, prec, dt, threads, max_steps=5, time_box=60.0):
         o.scalar_transport_rk()
         o.scalar_transport_rk_adj()
 
-    step()                                   # warm-up (allocates scratch)
+    for _ in range(warmup):                  # the first allocates the scratch
+        step()
     times = []
     t_start = time.perf_counter()
     while len(times) < max_steps and (len(times) < 2 or time.perf_counter() - t_start < time_box):
@@ -155,9 +160,11 @@ def cpu_baseline(grid, ntr, prec, dt, threads, max_steps=5, time_box=60.0):
     model, _ = host_cpu()
     return {"value": nw * nlev / sec, "unit": "cell-updates/s", "cores": used, "kind": "port",
             "cpu_model": model, "seconds_per_step": sec, "steps_timed": len(times),
-            "sample": f"oracle (C++ restatement, OpenMP, -O3 -march=x86-64-v3 -ffp-contract=off) on {nw} columns, "
-                      f"L={nlev}, {ntr} tracers, precision {prec}: one warm-up, then the median of {len(times)} "
-                      f"timed steps of dynamics + tracer forward + tracer adjoint (time box {time_box:g} s)"}
+            "sample": f"oracle (C++ restatement, OpenMP {os.environ.get('OMP_PROC_BIND')}/"
+                      f"{os.environ.get('OMP_PLACES')}, -O3 -march={O.ISA} -ffp-contract=off) on {nw} columns, "
+                      f"L={nlev}, {ntr} tracers, precision {prec}: {warmup} warm-up steps, then the median of "
+                      f"{len(times)} timed steps of dynamics + tracer forward + tracer adjoint "
+                      f"(time box {time_box:g} s)"}
 
 
 def main():
@@ -180,8 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=5,
                     help="also time this many steps in the reference-precision mode (0: skip)")
-    ap.add_argument("--host-resident", action="store_true",
-                    help="also time long steps whose state and tracers start and end in host memory")
+    ap.add_argument("--no-host-resident", action="store_true",
+                    help="skip timing long steps whose state and tracers start and end in host memory")
     ap.add_argument("--no-copy", action="store_true", help="skip the achievable-copy-bandwidth probe")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -256,26 +263,70 @@ def main():
             f"{ms / n:8.3f} ms/launch")
 
     host = None
-    if args.host_resident:
-        # the drop-in as INTEGRATION.md's shim drives it when the host owns the state:
-        # upload the state and tracers, one long step, download them again
+    if not args.no_host_resident:
+        # the drop-in as the Fortran shim drives it when the host owns the state.  "shim":
+        # per long step the long-step tendencies (thilt, vmxet, vmyet, vmzet, vmt) and the
+        # tracer tendencies go up, the fields the host's physics reads (thil, rho, press,
+        # vxe, vye, vze, wc) and the tracers come down.  "full": the whole state and the
+        # tracers go up and come down around every step.
         st_out = abi.FortranState(grid)
-        qh = [np.empty((grid.mwa, grid.mza), np.float32) for _ in range(args.ntr)]
-        ts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
+        zero_t = np.zeros((grid.mwa, grid.mza), np.float32)
+        tend = ("thilt", "vmxet", "vmyet", "vmzet", "vmt")
+        phys = ("thil", "rho", "press", "vxe", "vye", "vze", "wc")
+        st_tend = abi.FortranState(grid)
+
+        def shim_step():
+            ctx.tend0()
+            ctx.upload_state(st_tend, only=tend)
+            ctx.prog_wrtv_rk()
+            ctx.timeavg_momsc()
+            for n in range(args.ntr):
+                ctx.upload_scalar(n + 1, None, zero_t)
+            ctx.scalar_transport_rk()
+            ctx.scalar_transport_rk_adj()
+            ctx.download_state(st_out, only=phys)
+            for n in range(args.ntr):
+                ctx.download_scalar(n + 1)
+
+        def full_step():
             ctx.upload_state(jet)
             for n in range(args.ntr):
                 ctx.upload_scalar(n + 1, tr[n])
             step()
             ctx.download_state(st_out)
             for n in range(args.ntr):
-                qh[n][...] = ctx.download_scalar(n + 1)
-            ts.append(time.perf_counter() - t0)
-        hsec = sorted(ts)[1]
-        host = {"ms_per_step": hsec * 1e3, "transfer_ms": (hsec - sec) * 1e3,
-                "what": "upload of the full state and tracers, one long step, download of both (median of 3)"}
-        log(f"[bench] host-resident step {hsec * 1e3:.1f} ms ({(hsec - sec) * 1e3:.1f} ms transfers)")
+                ctx.download_scalar(n + 1)
+
+        def timed(fn):
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return sorted(ts)[1]
+
+        w4, v4, w8 = grid.mwa * grid.mza * 4, grid.mva * grid.mza * 4, grid.mwa * grid.mza * 8
+        shim_up = 4 * w4 + v4 + args.ntr * w4
+        shim_down = 5 * w4 + 2 * w8 + args.ntr * w4           # rho and press are real(r8)
+        nv_f = sum(1 for _, k, _ in abi.STATE_FIELDS if k == "v")
+        nw8 = sum(1 for _, k, d in abi.STATE_FIELDS if k == "w" and d == np.float64)
+        nw4 = len(abi.STATE_FIELDS) - nv_f - nw8
+        full_b = nw4 * w4 + nw8 * w8 + nv_f * v4 + args.ntr * w4
+        ssec = timed(shim_step)
+        fsec = timed(full_step)
+        host = {"shim": {"ms_per_step": ssec * 1e3, "transfer_ms": (ssec - sec) * 1e3,
+                         "bytes_up": shim_up, "bytes_down": shim_down,
+                         "GBps": (shim_up + shim_down) / max(ssec - sec, 1e-9) / 1e9,
+                         "value": cells / ssec},
+                "full": {"ms_per_step": fsec * 1e3, "transfer_ms": (fsec - sec) * 1e3,
+                         "bytes_up": full_b, "bytes_down": full_b,
+                         "GBps": 2 * full_b / max(fsec - sec, 1e-9) / 1e9,
+                         "value": cells / fsec},
+                "what": "host-resident long steps (median of 3): shim = the Fortran shim's per-step "
+                        "transfers, full = the whole state and tracers up and down every step; "
+                        "bytes are the host arrays' (Fortran layout)"}
+        log(f"[bench] host-resident shim step {ssec * 1e3:.1f} ms ({(ssec - sec) * 1e3:.1f} ms transfers, "
+            f"{host['shim']['GBps']:.1f} GB/s); full {fsec * 1e3:.1f} ms ({host['full']['GBps']:.1f} GB/s)")
     ctx.close()
 
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
@@ -321,7 +372,7 @@ def main():
             f"({cpu['seconds_per_step']:.2f} s/step)")
         if args.cpu_1thread_nxp:
             g1 = abi.FortranGrid(mesh.build_mesh(args.cpu_1thread_nxp), mesh.vertical_grid(args.nlev))
-            cpu1 = cpu_baseline(g1, args.ntr, args.cpu_prec, args.dt, 1, 3, 20.0)
+            cpu1 = cpu_baseline(g1, args.ntr, args.cpu_prec, args.dt, 1, 5, 20.0)
             log(f"[bench] CPU 1 thread {cpu1['value']:.3e} cell-updates/s")
     out = {
         "metric": "cell-updates/s per RK3 step (fwd+adjoint) at NXP=320, L=60; % HBM roofline",

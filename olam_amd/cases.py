@@ -26,6 +26,8 @@ point id), so values do not depend on the order in which points are numbered.
 from __future__ import annotations
 
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -52,6 +54,21 @@ def counter_normal(seed, stream, ids, nlev):
     gen = np.random.Generator(np.random.Philox(key=key))
     z = gen.standard_normal((int(ids.max()) + 1, nlev))
     return z[ids]
+
+
+def _workers():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _chunked(n, fn, size=32768):
+    """fn(slice) over [0, n) in chunks on a thread pool (numpy releases the GIL)."""
+    sls = [slice(a, min(n, a + size)) for a in range(0, n, size)]
+    with ThreadPoolExecutor(_workers()) as ex:
+        list(ex.map(fn, sls))
 
 
 def _latlon_w(mesh):
@@ -122,13 +139,15 @@ def cosine_bell(grid: FortranGrid, lon0=1.5 * math.pi, lat0=0.0, radius=M.ERAD /
 
 def lognormal_tracers(grid: FortranGrid, ntr: int, mu=-20.0, sigma=2.0, seed=7):
     mesh = grid.mesh
-    out = []
-    for n in range(ntr):
+
+    def one(n):
         z = counter_normal(seed, 100 + n, mesh.w_canon, grid.mza)
         q = np.exp(mu + sigma * z).astype(np.float32)
         q[:, 0] = q[:, 1]
-        out.append(grid.to_f(q, "w"))
-    return out
+        return grid.to_f(q, "w")
+
+    with ThreadPoolExecutor(min(_workers(), max(1, ntr))) as ex:
+        return list(ex.map(one, range(ntr)))
 
 
 def seeded_field(grid: FortranGrid, seed, stream, kind="w", scale=1.0):
@@ -164,31 +183,38 @@ def jet_state(grid: FortranGrid, seed: int = 3, u_jet: float = 35.0, bump: float
     blat, blon = math.radians(rng.uniform(-40, 40)), math.radians(rng.uniform(0, 360))
     rdist = M.ERAD * np.arccos(np.clip(np.sin(blat) * np.sin(lat) + np.cos(blat) * np.cos(lat) * np.cos(lon - blon), -1, 1))
     horiz = np.exp(-(rdist / 1.0e6) ** 2)
+    vert = [math.exp(-((z - 5000.0) / 2000.0) ** 2) for z in zt]
     theta = np.zeros((mza + 1, nw))
-    for k in range(2, mza + 1):
-        theta[k] = theta0[k] + bump * horiz * math.exp(-((zt[k] - 5000.0) / 2000.0) ** 2)
-    # discrete hydrostatic rebalance, per column, upward from level 2
+    pr = np.zeros((mza + 1, nw))
+    rh = np.zeros((mza + 1, nw))
     alpha = M.ALPHA_DRY
+    dzm = vg["dzm"]
+    zb, ztp = vg["zwgt_bot8"], vg["zwgt_top8"]
 
     def rho_of(pp, th):
         return (pp / alpha) ** (1.0 / M.CPOCV) / th
 
-    pr = np.zeros((mza + 1, nw))
-    rh = np.zeros((mza + 1, nw))
-    pr[2] = p[2]
-    rh[2] = rho_of(pr[2], theta[2])
-    dzm = vg["dzm"]
-    zb, ztp = vg["zwgt_bot8"], vg["zwgt_top8"]
-    for k in range(2, mza):
-        pk, rk = pr[k], rh[k]
-        pn = pk * np.exp(-dzm[k] * M.GRAV / (M.RDRY * 250.0))
-        for _ in range(8):
-            rn = rho_of(pn, theta[k + 1])
-            f = pk - pn - dzm[k] * M.GRAV * (zb[k] * rk + ztp[k] * rn)
-            dfdp = -1.0 - dzm[k] * M.GRAV * ztp[k] * rn / (M.CPOCV * pn)
-            pn = pn - f / dfdp
-        pr[k + 1] = pn
-        rh[k + 1] = rho_of(pn, theta[k + 1])
+    def balance(sl):
+        """theta with the bump, then the discrete hydrostatic rebalance per column,
+        upward from level 2, for the columns in slice sl (elementwise: the chunking
+        changes no value)."""
+        th, p_, r_ = theta[:, sl], pr[:, sl], rh[:, sl]
+        for k in range(2, mza + 1):
+            th[k] = theta0[k] + bump * horiz[sl] * vert[k]
+        p_[2] = p[2]
+        r_[2] = rho_of(p_[2], th[2])
+        for k in range(2, mza):
+            pk, rk = p_[k], r_[k]
+            pn = pk * np.exp(-dzm[k] * M.GRAV / (M.RDRY * 250.0))
+            for _ in range(8):
+                rn = rho_of(pn, th[k + 1])
+                f = pk - pn - dzm[k] * M.GRAV * (zb[k] * rk + ztp[k] * rn)
+                dfdp = -1.0 - dzm[k] * M.GRAV * ztp[k] * rn / (M.CPOCV * pn)
+                pn = pn - f / dfdp
+            p_[k + 1] = pn
+            r_[k + 1] = rho_of(pn, th[k + 1])
+
+    _chunked(nw, balance)
     # press must be the model's diagnostic alpha (rho theta)^(cp/cv)
     pr[2:] = alpha * (rh[2:] * theta[2:]) ** M.CPOCV
     # zonal jet on V faces
@@ -201,22 +227,33 @@ def jet_state(grid: FortranGrid, seed: int = 3, u_jet: float = 35.0, bump: float
     iw1, iw2 = mesh.v_iw[:, 0], mesh.v_iw[:, 1]
     vc = np.zeros((mza + 1, nv))
     vmc = np.zeros((mza + 1, nv))
-    for k in range(2, mza + 1):
-        pf = 0.5 * (pr[k, iw1] + pr[k, iw2])
-        F = np.exp(-0.5 * (np.log(pf / 25000.0) / 0.5) ** 2)
-        u = u_jet * np.sin(2.0 * latv) ** 2 * F
-        vc[k] = (u * en).astype(np.float32)
-        vmc[k] = vc[k] * 0.5 * (rh[k, iw1] + rh[k, iw2])
+    s2 = np.sin(2.0 * latv) ** 2
+
+    def jet(sl):
+        for k in range(2, mza + 1):
+            a1, a2 = iw1[sl], iw2[sl]
+            pf = 0.5 * (pr[k, a1] + pr[k, a2])
+            F = np.exp(-0.5 * (np.log(pf / 25000.0) / 0.5) ** 2)
+            u = u_jet * s2[sl] * F
+            vc[k, sl] = (u * en[sl]).astype(np.float32)
+            vmc[k, sl] = vc[k, sl] * 0.5 * (rh[k, a1] + rh[k, a2])
+
+    _chunked(nv, jet)
     # cell velocities from vc (vel_t3d_hex with wc = 0); padded slots have zero weight
     vcT = vc.T                                                    # (nv, mza+1)
     ivs = np.where(np.arange(7)[None, :] < mesh.npoly[:, None], mesh.w_iv, 0)
-    vxe = np.zeros((nw, mza + 1)); vye = np.zeros_like(vxe); vze = np.zeros_like(vxe)
-    for j in range(7):
-        col = vcT[ivs[:, j]]
-        vxe += g["ecvec_vx"][:, j, None] * col
-        vye += g["ecvec_vy"][:, j, None] * col
-        vze += g["ecvec_vz"][:, j, None] * col
-    vxe, vye, vze = vxe.T.copy(), vye.T.copy(), vze.T.copy()
+    vxe = np.zeros((mza + 1, nw)); vye = np.zeros_like(vxe); vze = np.zeros_like(vxe)
+
+    def cell(sl):
+        ax = np.zeros((sl.stop - sl.start, mza + 1)); ay = np.zeros_like(ax); az = np.zeros_like(ax)
+        for j in range(7):
+            col = vcT[ivs[sl, j]]
+            ax += g["ecvec_vx"][sl, j, None] * col
+            ay += g["ecvec_vy"][sl, j, None] * col
+            az += g["ecvec_vz"][sl, j, None] * col
+        vxe[:, sl], vye[:, sl], vze[:, sl] = ax.T, ay.T, az.T
+
+    _chunked(nw, cell)
     for a in (vxe, vye, vze, theta):
         a[1] = a[2]
     st = FortranState(grid)

@@ -17,6 +17,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/olamgpu.h"
@@ -54,6 +55,96 @@ struct DevBuf {
     if (b) HIPCHK(hipMalloc(&p, b));
   }
   template <class T> T *as() const { return (T *)p; }
+};
+
+// ------------------------------------------------------------------ host transfers
+// A host array in the reference's Fortran layout (mza, n + 1), dummy column first, moves
+// as raw bytes between host memory and a device staging buffer through two pinned
+// bounce buffers (the host-side copy of one chunk overlaps the DMA of the other, and
+// the host-side copy is split over threads); kernels then convert it to or from the
+// device layout, renumbered columns of ldk levels (levels mza..ldk-1 zero).
+template <class D, class H>
+__global__ __launch_bounds__(256) void k_unpack(D *dst, const H *src, const int *perm, int n, int mza, int ldk) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * ldk) return;
+  const int c = (int)(i / ldk), l = (int)(i % ldk);
+  dst[i] = l < mza ? (D)src[(size_t)(perm[c] + 1) * mza + l] : D(0);
+}
+template <class D, class H>
+__global__ __launch_bounds__(256) void k_pack(H *dst, const D *src, const int *inv, int n, int mza, int ldk) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)(n + 1) * mza) return;
+  const int j = (int)(i / mza), l = (int)(i % mza);
+  dst[i] = j == 0 ? H(0) : (H)src[(size_t)inv[j - 1] * ldk + l];
+}
+
+struct HostXfer {
+  static constexpr size_t CHUNK = size_t(64) << 20;
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  DevBuf stage;
+  int nthr = 1;
+  ~HostXfer() {
+    for (int k = 0; k < 2; ++k) {
+      if (pin[k]) (void)hipHostFree(pin[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+  }
+  void ready(size_t bytes, hipStream_t s) {
+    if (!pin[0]) {
+      for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipHostMalloc(&pin[k], CHUNK, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        HIPCHK(hipEventRecord(ev[k], s));
+      }
+      nthr = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    }
+    if (stage.bytes < bytes) stage.alloc(bytes);
+  }
+  void copy(void *dst, const void *src, size_t nb) const {
+    const size_t part = (nb + nthr - 1) / nthr;
+    if (nthr == 1 || nb < (size_t(4) << 20)) { std::memcpy(dst, src, nb); return; }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nthr; ++k) {
+      const size_t o = (size_t)k * part;
+      if (o >= nb) break;
+      th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, std::min(part, nb - o)); });
+    }
+    for (auto &x : th) x.join();
+  }
+  // host -> stage
+  void up(const void *h, size_t bytes, hipStream_t s) {
+    ready(bytes, s);
+    for (size_t o = 0, i = 0; o < bytes; o += CHUNK, ++i) {
+      const int k = (int)(i & 1);
+      const size_t nb = std::min(CHUNK, bytes - o);
+      HIPCHK(hipEventSynchronize(ev[k]));          // the DMA that last read pin[k] is done
+      copy(pin[k], (const char *)h + o, nb);
+      HIPCHK(hipMemcpyAsync((char *)stage.p + o, pin[k], nb, hipMemcpyHostToDevice, s));
+      HIPCHK(hipEventRecord(ev[k], s));
+    }
+  }
+  // stage -> host (the stream's earlier work, the packing kernel, comes first)
+  void down(void *h, size_t bytes, hipStream_t s) {
+    ready(bytes, s);
+    size_t prev_o = 0, prev_nb = 0;
+    int prev_k = -1;
+    for (size_t o = 0, i = 0; o < bytes; o += CHUNK, ++i) {
+      const int k = (int)(i & 1);
+      const size_t nb = std::min(CHUNK, bytes - o);
+      HIPCHK(hipMemcpyAsync(pin[k], (const char *)stage.p + o, nb, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipEventRecord(ev[k], s));
+      if (prev_k >= 0) {
+        HIPCHK(hipEventSynchronize(ev[prev_k]));
+        copy((char *)h + prev_o, pin[prev_k], prev_nb);
+      }
+      prev_k = k; prev_o = o; prev_nb = nb;
+    }
+    if (prev_k >= 0) {
+      HIPCHK(hipEventSynchronize(ev[prev_k]));
+      copy((char *)h + prev_o, pin[prev_k], prev_nb);
+    }
+  }
 };
 
 // ------------------------------------------------------------------ kernel timing
@@ -988,40 +1079,41 @@ struct Impl : ImplBase {
   }
 
   // -------------------------------------------------------------- transfers
-  // Fortran (mza, n) host array with dummy column <-> device (n, ldk) renumbered
-  template <class D, class H> void h2d(DevBuf &b, const H *h, const std::vector<int> &perm) {
-    size_t n = perm.size();
-    std::vector<D> tmp(n * ldk, D(0));
-    for (size_t c = 0; c < n; ++c) {
-      const H *src = h + (size_t)(perm[c] + 1) * mza;
-      D *dst = tmp.data() + c * ldk;
-      for (int l = 0; l < mza; ++l) dst[l] = (D)src[l];
-    }
-    HIPCHK(hipMemcpyAsync(b.p, tmp.data(), tmp.size() * sizeof(D), hipMemcpyHostToDevice, s));
+  // Fortran (mza, n + 1) host array with dummy column <-> device (n, ldk) renumbered,
+  // repacked on the device
+  HostXfer xfer;
+  DevBuf dperm_w, dperm_v, dinv_w, dinv_v;
+  const int *dev_perm(const std::vector<int> &perm, bool inverse) {
+    const bool w = &perm == &t.perm_w;
+    if (!w && &perm != &t.perm_v) throw Err(OLAMGPU_EINVAL, "transfer with an unknown numbering");
+    DevBuf &p = w ? dperm_w : dperm_v, &q = w ? dinv_w : dinv_v;
+    if (!p.p) { upv(p, perm); upv(q, w ? t.inv_w : t.inv_v); }
+    return (inverse ? q : p).as<int>();
+  }
+  template <class D, class H> void h2d_at(void *dst, const H *h, const std::vector<int> &perm) {
+    const int n = (int)perm.size();
+    xfer.up(h, (size_t)(n + 1) * mza * sizeof(H), s);
+    const size_t cnt = (size_t)n * ldk;
+    hipLaunchKernelGGL((k_unpack<D, H>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       (D *)dst, xfer.stage.as<const H>(), dev_perm(perm, false), n, mza, ldk);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
   }
-  template <class D, class H> void d2h(const DevBuf &b, H *h, const std::vector<int> &perm, size_t off = 0) {
-    size_t n = perm.size();
-    std::vector<D> tmp(n * ldk);
-    HIPCHK(hipMemcpyAsync(tmp.data(), (const char *)b.p + off * sizeof(D), tmp.size() * sizeof(D),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (int l = 0; l < mza; ++l) h[l] = H(0);
-    for (size_t c = 0; c < n; ++c) {
-      const D *src = tmp.data() + c * ldk;
-      H *dst = h + (size_t)(perm[c] + 1) * mza;
-      for (int l = 0; l < mza; ++l) dst[l] = (H)src[l];
-    }
+  template <class D, class H> void h2d(DevBuf &b, const H *h, const std::vector<int> &perm) {
+    h2d_at<D>(b.p, h, perm);
   }
   template <class D, class H> void h2d_off(DevBuf &b, size_t off, const H *h, const std::vector<int> &perm) {
-    size_t n = perm.size();
-    std::vector<D> tmp(n * ldk, D(0));
-    for (size_t c = 0; c < n; ++c) {
-      const H *src = h + (size_t)(perm[c] + 1) * mza;
-      for (int l = 0; l < mza; ++l) tmp[c * ldk + l] = (D)src[l];
-    }
-    HIPCHK(hipMemcpyAsync((char *)b.p + off * sizeof(D), tmp.data(), tmp.size() * sizeof(D), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    h2d_at<D>((char *)b.p + off * sizeof(D), h, perm);
+  }
+  template <class D, class H> void d2h(const DevBuf &b, H *h, const std::vector<int> &perm, size_t off = 0) {
+    const int n = (int)perm.size();
+    const size_t cnt = (size_t)(n + 1) * mza;
+    xfer.ready(cnt * sizeof(H), s);
+    hipLaunchKernelGGL((k_pack<D, H>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       xfer.stage.as<H>(), (const D *)((const char *)b.p + off * sizeof(D)),
+                       dev_perm(perm, true), n, mza, ldk);
+    HIPCHK(hipGetLastError());
+    xfer.down(h, cnt * sizeof(H), s);
   }
 
   DevBuf &cur_thil_buf() { return thil[cur_thil]; }

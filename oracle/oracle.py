@@ -15,11 +15,29 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "build", "liboracle.so")
+LIB_V3 = os.path.join(HERE, "build", "liboracle.so")
+LIB_V4 = os.path.join(HERE, "build", "liboracle_avx512.so")
+
+
+def _has_avx512():
+    need = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512cd"}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return need <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+# the x86-64-v4 build where the CPU has AVX-512 (same bits, wider vectors);
+# OLAM_ORACLE_ISA=v3 forces the AVX2 build
+ISA = "x86-64-v4" if os.environ.get("OLAM_ORACLE_ISA", "") != "v3" and _has_avx512() else "x86-64-v3"
+LIB = LIB_V4 if ISA == "x86-64-v4" else LIB_V3
 
 
 def build(force=False):
-    if force or not os.path.exists(LIB):
+    if force or not (os.path.exists(LIB_V3) and os.path.exists(LIB_V4)):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
 
