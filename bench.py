@@ -46,12 +46,14 @@ def alg_bytes(ntr, prec):
 
 def stage_alg_bytes(name, ntr, prec):
     """Algorithmic bytes per cell-level of the RK stage that one launch of the named
-    kernel carries out, from the per-stage byte model of the design notes: a tracer stage
-    (all tracers, forward or adjoint) is (120 + 48 n) / 3, the dynamics' W-column phase
-    132 (152 with FP64 rho/press), its V phase 92 (100) and its velocity phase 28.  A
-    stage that runs as two kernels credits each with the whole stage's bytes.  This is
-    what the headline roofline uses, whatever the kernel moves besides."""
-    tr = (120 + 48 * ntr) / 3.0
+    kernel carries out, from the per-stage byte model: a tracer stage (all tracers,
+    forward or adjoint) reads rhos, volti, wmasc and vmasc on 3 faces (24 B) and per
+    tracer reads scp, scp0, sct and writes scp (16 B), so 24 + 16 n (the long step's
+    48 B of setup is not a stage's); the dynamics' W-column phase 132 (152 with FP64
+    rho/press), its V phase 92 (100) and its velocity phase 28.  A stage that runs as
+    two kernels credits each with the whole stage's bytes.  This is what the headline
+    roofline uses, whatever the kernel moves besides."""
+    tr = 24.0 + 16.0 * ntr
     mixed = prec == "mixed"
     per = {
         "k_tr_grad": tr, "k_tr_update": tr, "k_tr_stage": tr,
@@ -341,7 +343,7 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "traffic": traffic, "traffic_source": tsrc,
                 "frac_of_copy": (ach / (copy_bw * 1e9)) if copy_bw else None,
-                "alg_bytes_per_launch": sb * cells, "alg_bytes_basis": "stage model (per-stage bytes)",
+                "alg_bytes_per_launch": sb * cells, "alg_bytes_basis": "per-stage bytes (tracer stage 24 + 16 n; dynamics phases 132, 92, 28)",
                 "avg_launch_ms": per_launch_s * 1e3,
                 "frac_kernel_arrays": (kb * cells / per_launch_s / PEAK_HBM) if kb else None}
     bd, bf, ba = alg_bytes(args.ntr, args.prec)

@@ -11,6 +11,16 @@
 #pragma once
 #include "common.h"
 
+// Register budgets of the hottest tracer kernels, as the minimum number of 256-thread
+// blocks per CU (= waves per SIMD) the compiler must allow: 4 gives 128 VGPRs, 5 gives
+// 96, 6 gives 80.  Measured choices; a build may override them (-D...) to re-tune.
+#ifndef OLAMGPU_BLOCKS_UPDATE
+#define OLAMGPU_BLOCKS_UPDATE 4
+#endif
+#ifndef OLAMGPU_BLOCKS_ADJ01
+#define OLAMGPU_BLOCKS_ADJ01 1
+#endif
+
 template <class R, class RP>
 struct TrStage {
   const R *q_in, *q0, *sct, *gx, *gy;
@@ -123,7 +133,7 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 // One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
 // (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
 template <class R, class RP, bool Q3, bool PK = false>
-__global__ __launch_bounds__(256, Q3 ? 2 : 4) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
+__global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
   if (c >= m.nw) return;
@@ -931,7 +941,7 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
 // in k_tr_adj0) instead of being written and read back.  Per tracer it reads mu of the
 // column and its neighbours and lam0, and writes lam0, lpart, lgx, lgy.
 template <class R, class RP, bool Q3>
-__global__ __launch_bounds__(256) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
+__global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
   const int c = ti.c;
   if (c >= m.nw) return;
