@@ -216,6 +216,15 @@ __device__ __forceinline__ P2<R> mk2(R x, R y) {
   v.x = x; v.y = y;
   return v;
 }
+// Unpredicated column load: a wave-uniform column base and the lane index, so the load
+// takes a scalar base and a 32-bit lane offset (no per-lane 64-bit address arithmetic,
+// no exec-mask branch).  Every lane reads an element of its own column (the column
+// stride is at least one wave); lanes above the column's levels read its padding, so
+// use it only where those lanes' results are discarded.
+template <class T>
+__device__ __forceinline__ T ldl(const T *col, unsigned lane) { return col[lane]; }
+template <class T>
+__device__ __forceinline__ T ldl_nt(const T *col, unsigned lane) { return __builtin_nontemporal_load(col + lane); }
 template <class T>
 __device__ __forceinline__ T ldv(const T *p, long i, bool ok, T dflt) {
   const T v = p[i];

@@ -606,16 +606,22 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     bool sel;
     if constexpr (NUD) sel = sel_j[j];
     else sel = (dv < R(0)) ? (vmca < R(0)) : !(vmca < R(0));
-    const bool nact = act && ((need >> j) & 1u);
+    const bool take = (need >> j) & 1u;     // wave-uniform
+    const bool nact = act && take;
+    const long nbb = (long)nbr * ldk;
     R f[4];
     for (int q = 0; q < 4; ++q) {
       const int ord = q == 0 ? b.thil_order : b.wind_order;
       const R *src = q == 0 ? b.thil_in : (q == 1 ? b.vxe : (q == 2 ? b.vye : b.vze));
-      const R qn = ld(src, in, nact);
+      // unpredicated neighbour loads: a lane above mza only feeds its own face sums,
+      // which the vertical terms at mza select away
+      const R qn = take ? ldl(src + nbb, (unsigned)l) : R(0);
       R fs = qc[q], fn = qn;
       if (ord > 1) {
         fs = qc[q] + dxs * gxs[q] + dys * gys[q];
-        fn = qn + dxn * ld(b.g[2 * q], in, nact) + dyn * ld(b.g[2 * q + 1], in, nact);
+        const R gxn = take ? ldl(b.g[2 * q] + nbb, (unsigned)l) : R(0);
+        const R gyn = take ? ldl(b.g[2 * q + 1] + nbb, (unsigned)l) : R(0);
+        fn = qn + dxn * gxn + dyn * gyn;
       }
       if (Q3 && ord == 3) {   // the unbounded fit at the bounded offsets xx0_v (:1057-1071)
         const long e = (long)c * NSLOT + j;

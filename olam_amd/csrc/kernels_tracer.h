@@ -57,12 +57,14 @@ __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict
     cy[j] = m.w_gyc[c * NSLOT + j];
   }
   const long ldk = m.ldk;
+  const unsigned ul = (unsigned)l;
   for (int n = ti.n0; n < ti.n1; ++n) {
+    // unpredicated column loads: lanes above mza only feed their own (unstored) results
     const R *qn = q + n * tstride;
-    const R qc = ld(qn, c * ldk + l, act);
+    const R qc = ldl(qn + (long)c * ldk, ul);
     R ax = R(0), ay = R(0);
     for_slots(np, [&](int j) {
-      const R d = ld(qn, nb[j] * ldk + l, act) - qc;
+      const R d = ldl(qn + (long)nb[j] * ldk, ul) - qc;
       ax = ax + cx[j] * d;
       ay = ay + cy[j] * d;
     });
@@ -145,11 +147,12 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
   const int np = m.w_npoly[c];
   constexpr int NQ = Q3 ? 3 : 1;
   int nb[7];
-  R dvm[7], dxs[7], dys[7], dxn[7], dyn[7], oqs[7][NQ], oqn[7][NQ];
+  R dvm[7], dxs[7], dys[7], dxn[7], dyn[7], oqs[7][NQ], oqn[7][NQ], cgx[7], cgy[7];
   bool sel[7];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
     nb[j] = m.w_nbr[c * NSLOT + j];
+    if constexpr (PK) { cgx[j] = m.w_gxc[c * NSLOT + j]; cgy[j] = m.w_gyc[c * NSLOT + j]; }
     const int iv = m.w_face[c * NSLOT + j];
     const R dv = m.w_dirv[c * NSLOT + j];
     dxs[j] = m.w_dxs[c * NSLOT + j]; dys[j] = m.w_dys[c * NSLOT + j];
@@ -179,13 +182,20 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
     const R *qi = a.q_in + off;
-    L.qc = ld(qi, ic, act);
-    L.q0 = ld_nt(a.q0 + off, ic, act);
-    L.st = a.sct ? ld_nt(a.sct + off, ic, act) : R(0);
+    if constexpr (PK) {   // unpredicated: the vertical terms at mza select away lane mza
+      const long cb = off + (long)c * ldk;
+      L.qc = ldl(a.q_in + cb, (unsigned)l);
+      L.q0 = ldl_nt(a.q0 + cb, (unsigned)l);
+      L.st = a.sct ? ldl_nt(a.sct + cb, (unsigned)l) : R(0);
+    } else {
+      L.qc = ld(qi, ic, act);
+      L.q0 = ld_nt(a.q0 + off, ic, act);
+      L.st = a.sct ? ld_nt(a.sct + off, ic, act) : R(0);
+    }
     L.gxc = R(0); L.gyc = R(0);
     if constexpr (PK) {
-      const P2<R> g = ld2<R>(a.gxy + off, ic, act);
-      L.gxc = g.x; L.gyc = g.y;
+      // the column's own gradient is rebuilt from the neighbour values below (every
+      // neighbour is read) instead of being streamed from the gradient array
     } else if (a.order > 1) {
       L.gxc = ld(a.gx + off, ic, act); L.gyc = ld(a.gy + off, ic, act);
     }
@@ -200,11 +210,14 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
 #pragma unroll
         for (int t = 0; t < 3; ++t) L.gqn[j][t] = R(0);
       }
+      if constexpr (PK) {   // values of lanes above mza only reach discarded lanes
+        if (j < 6 || np > 6) L.qn[j] = ldl(qi + (long)nb[j] * ldk, (unsigned)l);
+      }
       if ((j < 6 || np > 6) && ((need >> j) & 1u)) {
         const long in = (long)nb[j] * ldk + l;
-        L.qn[j] = ld(qi, in, act);
+        if constexpr (!PK) L.qn[j] = ld(qi, in, act);
         if constexpr (PK) {
-          const P2<R> g = ld2<R>(a.gxy + off, in, act);
+          const P2<R> g = ldl(a.gxy + off + (long)nb[j] * ldk, (unsigned)l);
           L.gxn[j] = g.x; L.gyn[j] = g.y;
         } else if (a.order > 1) {
           L.gxn[j] = ld(a.gx + off, in, act);
@@ -225,11 +238,22 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
     if (a.limited(n)) continue;
     const long off = n * a.tstride;
     const R qc = L.qc, q0 = L.q0;
+    R gxc = L.gxc, gyc = L.gyc;
+    if constexpr (PK) {   // grad_t2d of the column, the arithmetic and order of k_tr_grad
+      R ax = R(0), ay = R(0);
+      for_slots(np, [&](int j) {
+        const R d = L.qn[j] - qc;
+        ax = ax + cgx[j] * d;
+        ay = ay + cgy[j] * d;
+      });
+      gxc = body ? ax : R(0);
+      gyc = body ? ay : R(0);
+    }
     R F = R(0);
     for_slots(np, [&](int j) {
       R fs = qc, fn = L.qn[j];
       if (a.order > 1) {
-        fs = qc + dxs[j] * L.gxc + dys[j] * L.gyc;
+        fs = qc + dxs[j] * gxc + dys[j] * gyc;
         fn = L.qn[j] + dxn[j] * L.gxn[j] + dyn[j] * L.gyn[j];
       }
       if constexpr (Q3) {   // bounded offsets xx0_v (:515-524)
@@ -1006,11 +1030,13 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
   struct Ld { R mu, l0, mun[7]; };
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
-    L.mu = ld(a.mu + off, ic, act);
-    L.l0 = rd0 ? ld_nt(a.lam0 + off, ic, act) : R(0);
+    // unpredicated column loads: lanes above mza only feed their own (unstored) results,
+    // and nu is zero there
+    L.mu = ldl(a.mu + off + (long)c * ldk, (unsigned)l);
+    L.l0 = rd0 ? ldl_nt(a.lam0 + off + (long)c * ldk, (unsigned)l) : R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j)
-      L.mun[j] = ((j < 6 || np > 6) && ((need >> j) & 1u)) ? ld(a.mu + off, (long)nb[j] * ldk + l, act) : R(0);
+      L.mun[j] = ((j < 6 || np > 6) && ((need >> j) & 1u)) ? ldl(a.mu + off + (long)nb[j] * ldk, (unsigned)l) : R(0);
   };
   Ld nx;
   if (ti.n0 < ti.n1) load(ti.n0, nx);
@@ -1092,19 +1118,22 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
   struct Ld { R lp, l0, gxs, gys, gxn[7], gyn[7]; };
   auto load = [&](int n, Ld &L) {
     const long off = n * a.tstride;
-    L.lp = ld_nt(a.lpart + off, ic, act);
-    L.l0 = a.last ? ld_nt(a.lam0 + off, ic, act) : R(0);
+    // unpredicated column loads: every result is per level and stored for act lanes only
+    const unsigned ul = (unsigned)l;
+    const long cb = off + (long)c * ldk;
+    L.lp = ldl_nt(a.lpart + cb, ul);
+    L.l0 = a.last ? ldl_nt(a.lam0 + cb, ul) : R(0);
     L.gxs = R(0); L.gys = R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j) { L.gxn[j] = R(0); L.gyn[j] = R(0); }
     if (a.order > 1) {
-      L.gxs = ld(a.lgx + off, ic, act); L.gys = ld(a.lgy + off, ic, act);
+      L.gxs = ldl(a.lgx + cb, ul); L.gys = ldl(a.lgy + cb, ul);
 #pragma unroll
       for (int j = 0; j < 7; ++j)
         if (j < 6 || np > 6) {
-          const long in = (long)nb[j] * ldk + l;
-          L.gxn[j] = ld(a.lgx + off, in, act);
-          L.gyn[j] = ld(a.lgy + off, in, act);
+          const long nbb = off + (long)nb[j] * ldk;
+          L.gxn[j] = ldl(a.lgx + nbb, ul);
+          L.gyn[j] = ldl(a.lgy + nbb, ul);
         }
     }
   };
