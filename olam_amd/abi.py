@@ -256,10 +256,14 @@ class FortranState:
 
     def desc(self, only=None):
         d = StateDesc()
-        for name, _, dt in STATE_FIELDS:
+        for name, kind, dt in STATE_FIELDS:
             if only is not None and name not in only:
                 continue
             a = self.arrays[name]
+            n = self.grid.mva if kind == "v" else self.grid.mwa
+            if a.dtype != dt or a.shape != (n, self.grid.mza) or not a.flags["C_CONTIGUOUS"]:
+                raise ValueError(f"state member {name} must be a contiguous {np.dtype(dt).name} "
+                                 f"array of shape {(n, self.grid.mza)}")
             setattr(d, name, _ptr(a, C.c_double if dt == np.float64 else C.c_float))
         return d
 

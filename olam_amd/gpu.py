@@ -122,9 +122,19 @@ class Context:
         d = st.desc(only)
         self._chk(lib().olamgpu_download_state(self.h, C.byref(d)), "download_state")
 
+    def _wfield(self, a, dt, name):
+        """A host W field in the Fortran layout, (mwa, mza) contiguous, or ValueError
+        (the library reads exactly that many elements)."""
+        a = np.ascontiguousarray(a, dtype=dt)
+        if a.shape != (self.grid.mwa, self.grid.mza):
+            raise ValueError(f"{name} must be (mwa, mza) = {(self.grid.mwa, self.grid.mza)}, got {a.shape}")
+        return a
+
     def upload_scalar(self, n, q, t=None):
-        q = np.ascontiguousarray(q, dtype=np.float32)
-        t = None if t is None else np.ascontiguousarray(t, dtype=np.float32)
+        """Tracer n (var_p) and, if given, its long-step tendency (var_t); either may be
+        None (a NULL pointer: the device copy of that array is kept)."""
+        q = None if q is None else self._wfield(q, np.float32, "q")
+        t = None if t is None else self._wfield(t, np.float32, "t")
         self._chk(lib().olamgpu_upload_scalar(self.h, n, _p(q), _p(t)), "upload_scalar")
 
     def download_scalar(self, n):
@@ -139,7 +149,7 @@ class Context:
         self._chk(lib().olamgpu_set_scalar_sgsmix(self.h, int(n), int(bool(on))), "set_scalar_sgsmix")
 
     def upload_scalar_adj(self, n, lam):
-        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        lam = self._wfield(lam, np.float32, "lambda")
         self._chk(lib().olamgpu_upload_scalar_adj(self.h, n, _p(lam)), "upload_scalar_adj")
 
     def download_scalar_adj(self, n):
@@ -163,8 +173,12 @@ class Context:
     def set_nudging(self, wmanud=None, vmanud=None, rhot_nud=None):
         """mem_nudge fields: wmanud (mwa, mza), vmanud (mva, mza) FP32, rhot_nud (mwa, mza)
         FP64; None leaves a field as it was."""
+        g = self.grid
         a = [None if x is None else np.ascontiguousarray(x, dtype=dt)
              for x, dt in ((wmanud, np.float32), (vmanud, np.float32), (rhot_nud, np.float64))]
+        for x, n, name in zip(a, (g.mwa, g.mva, g.mwa), ("wmanud", "vmanud", "rhot_nud")):
+            if x is not None and x.shape != (n, g.mza):
+                raise ValueError(f"{name} must be ({n}, {g.mza}), got {x.shape}")
         self._chk(lib().olamgpu_set_nudging(self.h, *(_p(x) for x in a)), "set_nudging")
 
     def set_turbulence(self, vkh, vkm, khtop, kmtop, akm_sfc):
@@ -189,7 +203,7 @@ class Context:
         self._chk(lib().olamgpu_comp_alpha_press(self.h, *(_p(x) for x in a)), "comp_alpha_press")
 
     def scalar_transport_rk(self, rho_old=None):
-        r = None if rho_old is None else np.ascontiguousarray(rho_old, dtype=np.float32)
+        r = None if rho_old is None else self._wfield(rho_old, np.float32, "rho_old")
         self._chk(lib().olamgpu_scalar_transport_rk(self.h, _p(r)), "scalar_transport_rk")
 
     def scalar_transport_rk_adj(self):
