@@ -83,7 +83,33 @@ def build_fortran():
         subprocess.run([fc, "-O2", "-I", LIBDIR, "-o", exe, driver, os.path.join(LIBDIR, "olamgpu_iface.o"),
                         "-L", LIBDIR, "-lolamgpu", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN"],
                        check=True)
+    build_shim_driver(fc)
     return exe
+
+
+SHIM_DRIVER = os.path.join(LIBDIR, "olam_shim_driver")
+
+
+def build_shim_driver(fc):
+    """The drop-in shim (olam_amd/fortran/olam_shim.f90: modules wrtv_rk and
+    scalar_transport) compiled against the test stand-ins of OLAM's host modules
+    (tests/fortran/host_modules.f90) with the test host tests/fortran/shim_driver.f90.
+    Modules and objects go to olam_amd/lib/shim/, apart from the product's module."""
+    tdir = os.path.join(ROOT, "tests", "fortran")
+    srcs = [os.path.join(tdir, "host_modules.f90"), os.path.join(FORTRAN, "olam_shim.f90"),
+            os.path.join(tdir, "shim_driver.f90")]
+    if not all(os.path.exists(x) for x in srcs):
+        return None
+    out = os.path.join(LIBDIR, "shim")
+    os.makedirs(out, exist_ok=True)
+    objs = []
+    for src in srcs:
+        obj = os.path.join(out, os.path.basename(src)[:-4] + ".o")
+        subprocess.run([fc, "-c", "-O2", "-J", out, "-I", LIBDIR, "-o", obj, src], check=True)
+        objs.append(obj)
+    subprocess.run([fc, "-O2", "-o", SHIM_DRIVER, *objs, os.path.join(LIBDIR, "olamgpu_iface.o"),
+                    "-L", LIBDIR, "-lolamgpu", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN"], check=True)
+    return SHIM_DRIVER
 
 
 def build_io(verbose=False):

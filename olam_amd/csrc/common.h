@@ -2,8 +2,8 @@
 //
 // Layout in HBM.  Every 3-D field is stored column by column exactly as the
 // reference's Fortran arrays (vertical index contiguous), with the dummy point
-// removed, points renumbered for locality, and the column stride padded to LDK
-// (a multiple of 16 elements, >= mza).  Fortran level k sits at offset k-1 of its
+// removed, points renumbered for locality, and the column stride padded to LDK = 64
+// elements (one wave; mza <= 64), levels above mza zero.  Fortran level k sits at offset k-1 of its
 // column.  One 64-lane wavefront owns one column: lane l holds level k = l + 1, so
 // every column load/store is a single coalesced 4*mza-byte access and neighbour
 // columns are gathered whole.  Per-point neighbour records are padded to 8 slots so

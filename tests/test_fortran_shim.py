@@ -1,0 +1,84 @@
+"""The drop-in shim olam_amd/fortran/olam_shim.f90 (modules wrtv_rk and
+scalar_transport with OLAM's public procedures init_wrtv_rk, prog_wrtv_rk and
+scalar_transport_rk), compiled against stand-ins of OLAM's host modules
+(tests/fortran/host_modules.f90: mem_grid, mem_ijtabs with the itab_w/itab_v/itab_m
+derived types, mem_adv, mem_basic, mem_tend, misc_coms, consts_coms, oname_coms,
+var_tables) and driven by tests/fortran/shim_driver.f90 as olam_run / timestep call it.
+
+CPU: the shim and the driver build (build_fortran) and the shim names, in its host
+modules' `use` lists, the variables OLAM declares there.
+GPU: two long steps with two acoustic steps each through the shim give the fields it
+brings back every long step, the whole state after olamgpu_shim_download_all and the
+tracers bit-identical to the same steps made from Python through ctypes.
+"""
+
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from olam_amd import abi, build, cases, mesh as M
+from helpers import write_fortran_case
+
+SHIM = os.path.join(build.FORTRAN, "olam_shim.f90")
+
+
+def test_shim_uses_reference_module_names():
+    text = open(SHIM).read()
+    mods = set(re.findall(r"^\s*module\s+(\w+)\s*$", text, re.M))
+    assert {"wrtv_rk", "scalar_transport", "olamgpu_host"} <= mods
+    pub = re.findall(r"public\s*::\s*([\w, ]+)", text)
+    names = {x.strip() for p in pub for x in p.split(",")}
+    assert {"init_wrtv_rk", "prog_wrtv_rk", "scalar_transport_rk"} <= names
+    assert re.search(r"subroutine\s+scalar_transport_rk\s*\(\s*rho_old\s*\)", text)
+    used = set(re.findall(r"use\s+(\w+)\s*,\s*only", text))
+    assert {"mem_grid", "mem_ijtabs", "mem_adv", "mem_basic", "mem_tend", "misc_coms",
+            "consts_coms", "oname_coms", "var_tables"} <= used
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mixed", "f32"])
+def test_shim_long_steps_match_ctypes(tmp_path, prec):
+    from olam_amd.gpu import Context
+    exe = build.SHIM_DRIVER
+    if not os.path.exists(exe):
+        pytest.skip("shim driver not built (no amdflang at build time)")
+    grid = abi.FortranGrid(M.build_mesh(10), M.vertical_grid(20))
+    st = cases.jet_state(grid)
+    tr = cases.lognormal_tracers(grid, 3)
+    prm = abi.make_params(prec, num_scalar=3, dtsm=120.0, nacoust=2, dtlm=240.0, adjoint_tape=0)
+    nsteps = 2
+    write_fortran_case(str(tmp_path), grid, prm, st, tr, nsteps)
+    print(f"shim driver: nxp=10 L=20 ntr=3 {prec} nsteps=2 nacoust=2", flush=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    ctx = Context(grid, prm, device=0)
+    ctx.upload_state(st)
+    ctx.init_wrtv_rk()
+    for n in range(3):
+        ctx.upload_scalar(n + 1, tr[n])
+    for _ in range(nsteps):
+        ctx.tend0()
+        for _ in range(prm.nacoust):
+            ctx.prog_wrtv_rk()
+        ctx.timeavg_momsc()
+        ctx.scalar_transport_rk()
+    out = abi.FortranState(grid)
+    ctx.download_state(out)
+    qs = [ctx.download_scalar(n + 1) for n in range(3)]
+    ctx.close()
+
+    def shim(name, like):
+        return np.fromfile(str(tmp_path / f"{name}.bin"), dtype=like.dtype).reshape(like.shape)
+
+    for name in ("thil", "rho", "press", "vxe", "vye", "vze", "wc"):
+        a = out.arrays[name]
+        assert np.array_equal(shim(f"shim_phys_{name}", a)[1:], a[1:]), name
+    for name in ("thil", "wmc", "wc", "vxe", "vye", "vze", "rho", "press", "vmc", "vc", "wmasc", "vmasc"):
+        a = out.arrays[name]
+        assert np.array_equal(shim(f"shim_{name}", a)[1:], a[1:]), name
+    for n in range(3):
+        assert np.array_equal(shim(f"shim_scalar{n + 1}", qs[n])[1:], qs[n][1:]), n
