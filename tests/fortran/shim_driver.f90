@@ -2,8 +2,10 @@
 ! modules (tests/fortran/host_modules.f90) from a case directory written by
 ! tests/helpers.py:write_fortran_case (params.txt, fields.txt, one raw .bin per array),
 ! then calls the shim as OLAM's olam_run / timestep do: init_wrtv_rk once, and per long
-! step nacoust x prog_wrtv_rk and scalar_transport_rk(rho_old).  Long-step tendencies
-! stay zero (no physics).
+! step nacoust x prog_wrtv_rk and scalar_transport_rk(rho_old).  The long-step
+! tendencies thilt, vmxet, vmyet, vmzet, vmt of the case stand for the host physics'
+! (the same every long step); tracer tendencies come from the case's 'scalart' arrays,
+! zero where it has none.
 !
 ! Writes <dir>/shim_phys_<name>.bin for the fields the shim brings back each long step,
 ! <dir>/shim_<name>.bin for every state member after olamgpu_shim_download_all, and
@@ -33,6 +35,8 @@ program shim_driver
   integer, allocatable :: i2(:,:)
   real, allocatable :: r2(:,:)
   real, allocatable :: rho_old(:,:)
+  real, allocatable :: thilt0(:,:), vmxet0(:,:), vmyet0(:,:), vmzet0(:,:), vmt0(:,:)
+  integer :: nt
 
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) stop 'usage: olam_shim_driver <dir>'
@@ -57,6 +61,7 @@ program shim_driver
   allocate(itab_w(mwa), itab_v(mva), itab_m(mma), scalar_tab(num_scalar))
 
   n = 0
+  nt = 0
   open(newunit=u, file=trim(dir)//'/fields.txt', status='old', action='read')
   do
      read(u, *, iostat=ios) group, name, nbytes
@@ -71,14 +76,19 @@ program shim_driver
         allocate(scalar_tab(n)%var_p(mza, mwa), scalar_tab(n)%var_t(mza, mwa))
         call rd_r2(trim(name), scalar_tab(n)%var_p)
         scalar_tab(n)%var_t = 0.
+     case ('scalart')
+        nt = nt + 1
+        call rd_r2(trim(name), scalar_tab(nt)%var_t)
      end select
   end do
   close(u)
   allocate(rho_old(mza, mwa))
+  thilt0 = thilt; vmxet0 = vmxet; vmyet0 = vmyet; vmzet0 = vmzet; vmt0 = vmt
 
   call init_wrtv_rk()
   do istep = 1, nsteps
-     thilt = 0.; vmxet = 0.; vmyet = 0.; vmzet = 0.; vmt = 0.      ! the host's tend0
+     ! the host's tend0 and physics
+     thilt = thilt0; vmxet = vmxet0; vmyet = vmyet0; vmzet = vmzet0; vmt = vmt0
      rho_old = real(rho)
      do iac = 1, nacoust
         call prog_wrtv_rk()

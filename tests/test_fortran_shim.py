@@ -9,7 +9,9 @@ CPU: the shim and the driver build (build_fortran) and the shim names, in its ho
 modules' `use` lists, the variables OLAM declares there.
 GPU: two long steps with two acoustic steps each through the shim give the fields it
 brings back every long step, the whole state after olamgpu_shim_download_all and the
-tracers bit-identical to the same steps made from Python through ctypes.
+tracers bit-identical to the same steps made from Python through ctypes, without and
+with long-step tendencies of the host physics (seeded thilt, vmxet, vmyet, vmzet, vmt
+and tracer tendencies, which the shim uploads every long step).
 """
 
 import os
@@ -39,8 +41,8 @@ def test_shim_uses_reference_module_names():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["mixed", "f32"])
-def test_shim_long_steps_match_ctypes(tmp_path, prec):
+@pytest.mark.parametrize("prec,physics", [("mixed", False), ("f32", False), ("mixed", True)])
+def test_shim_long_steps_match_ctypes(tmp_path, prec, physics):
     from olam_amd.gpu import Context
     exe = build.SHIM_DRIVER
     if not os.path.exists(exe):
@@ -50,8 +52,25 @@ def test_shim_long_steps_match_ctypes(tmp_path, prec):
     tr = cases.lognormal_tracers(grid, 3)
     prm = abi.make_params(prec, num_scalar=3, dtsm=120.0, nacoust=2, dtlm=240.0, adjoint_tape=0)
     nsteps = 2
+    tends = None
+    if physics:   # small seeded tendencies of the host physics, the same every long step
+        rng = np.random.default_rng(21)
+        for name, scale in (("thilt", 1e-3), ("vmxet", 1e-3), ("vmyet", 1e-3), ("vmzet", 1e-4), ("vmt", 1e-3)):
+            a = st.arrays[name]
+            a[...] = (scale * rng.standard_normal(a.shape)).astype(np.float32)
+            a[0] = 0
+        tends = []
+        for n in range(3):
+            t = (1e-9 * rng.standard_normal((grid.mwa, grid.mza))).astype(np.float32)
+            t[0] = 0
+            tends.append(t)
     write_fortran_case(str(tmp_path), grid, prm, st, tr, nsteps)
-    print(f"shim driver: nxp=10 L=20 ntr=3 {prec} nsteps=2 nacoust=2", flush=True)
+    if tends:
+        with open(tmp_path / "fields.txt", "a") as f:
+            for n, t in enumerate(tends):
+                t.tofile(str(tmp_path / f"scalart{n + 1}.bin"))
+                f.write(f"scalart scalart{n + 1} {t.nbytes}\n")
+    print(f"shim driver: nxp=10 L=20 ntr=3 {prec} nsteps=2 nacoust=2 physics={physics}", flush=True)
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
@@ -60,11 +79,15 @@ def test_shim_long_steps_match_ctypes(tmp_path, prec):
     ctx.init_wrtv_rk()
     for n in range(3):
         ctx.upload_scalar(n + 1, tr[n])
+    tend = ("thilt", "vmxet", "vmyet", "vmzet", "vmt")
     for _ in range(nsteps):
         ctx.tend0()
+        ctx.upload_state(st, only=tend)
         for _ in range(prm.nacoust):
             ctx.prog_wrtv_rk()
         ctx.timeavg_momsc()
+        for n in range(3):
+            ctx.upload_scalar(n + 1, None, tends[n] if tends else np.zeros((grid.mwa, grid.mza), np.float32))
         ctx.scalar_transport_rk()
     out = abi.FortranState(grid)
     ctx.download_state(out)
