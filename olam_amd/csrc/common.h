@@ -6,7 +6,9 @@
 // elements (one wave; mza <= 64), levels above mza zero.  Fortran level k sits at offset k-1 of its
 // column.  One 64-lane wavefront owns one column: lane l holds level k = l + 1, so
 // every column load/store is a single coalesced 4*mza-byte access and neighbour
-// columns are gathered whole.  Per-point neighbour records are padded to 8 slots so
+// columns are gathered whole.  Tall columns (64 < mza <= 128) have LDK = 128 and two
+// consecutive waves of a block per column, the first holding levels 1..64 and the second
+// 65..128; the vertical moves between them go through LDS (vshdn, vshup, vlane0, vpcr).  Per-point neighbour records are padded to 8 slots so
 // a wave reads them with scalar loads.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -54,6 +56,7 @@ struct DMesh {
   // on, b20 (the W factor of the implicit solve) unless W friction is on
   const R *rayf_cof, *rayf_cofmix, *th01d, *dn01d, *b20;
   int krayf_bot, krayfmix_bot;
+  int tall;      // 1: two waves per column (64 < mza <= 128)
 };
 
 // Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous range of
@@ -71,14 +74,29 @@ __device__ __forceinline__ int wave_item() {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Work item (column, V face or M point) of a wave and the level index l (0-based, so
+// Fortran level k = l + 1) of its lane.  With tall columns the two waves of an item are
+// consecutive in the block: half h holds levels 64 h + 1 .. 64 h + 64.
+struct WPos { int item, l, h; };
+__device__ __forceinline__ WPos wave_pos(int tall) {
+  const int w = wave_item();
+  WPos p;
+  p.h = tall ? (w & 1) : 0;
+  p.item = tall ? (w >> 1) : w;
+  p.l = p.h * WAVE + lane_id();
+  return p;
+}
+
 // Work item of a tracer kernel: column c and the tracer range [n0, n1) handled by this
 // wave.  The tpw tracers of one group share a wave; the groups of one column are
 // consecutive items, so they run in the same block and share the column's gathers.
-struct TrItem { int c, n0, n1; };
-__device__ __forceinline__ TrItem tr_item(int nw, int ntr, int tpw) {
+struct TrItem { int c, n0, n1, l; };
+__device__ __forceinline__ TrItem tr_item(int nw, int ntr, int tpw, int tall) {
   const int ng = (ntr + tpw - 1) / tpw;
-  const int it = wave_item();
+  const WPos p = wave_pos(tall);
+  const int it = p.item;
   TrItem t;
+  t.l = p.l;
   t.c = it / ng;
   t.n0 = (it - t.c * ng) * tpw;
   t.n1 = min(ntr, t.n0 + tpw);
@@ -132,27 +150,97 @@ __device__ __forceinline__ T shup(T v, int d) {
   return __shfl_up(v, d, WAVE);
 }
 
+// Vertical moves that also work for tall columns.  The two waves of a tall column swap
+// the words the other half needs through LDS (one 64-bit word per lane, two barriers),
+// so every wave of the block must make the same calls: only in code that all waves of
+// the block run alike, never inside a branch on the item's own data.
+__device__ __forceinline__ unsigned long long *xbuf() {
+  __shared__ unsigned long long x[WPB * WAVE];
+  return x;
+}
+template <class T>
+__device__ __forceinline__ unsigned long long to_u64(T v) {
+  unsigned long long u = 0;
+  __builtin_memcpy(&u, &v, sizeof(T));
+  return u;
+}
+template <class T>
+__device__ __forceinline__ T from_u64(unsigned long long u) {
+  T v;
+  __builtin_memcpy(&v, &u, sizeof(T));
+  return v;
+}
+// level l + d (a level without one keeps its own value, as shdn)
+template <class T>
+__device__ __forceinline__ T vshdn(int tall, T v, int d) {
+  T r = shdn(v, d);
+  if (tall) {
+    unsigned long long *x = xbuf();
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    x[w * WAVE + lane] = to_u64(v);
+    __syncthreads();
+    if (!(w & 1) && lane + d >= WAVE) r = from_u64<T>(x[(w + 1) * WAVE + lane + d - WAVE]);
+    __syncthreads();
+  }
+  return r;
+}
+// level l - d
+template <class T>
+__device__ __forceinline__ T vshup(int tall, T v, int d) {
+  T r = shup(v, d);
+  if (tall) {
+    unsigned long long *x = xbuf();
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    x[w * WAVE + lane] = to_u64(v);
+    __syncthreads();
+    if ((w & 1) && lane < d) r = from_u64<T>(x[(w - 1) * WAVE + WAVE - d + lane]);
+    __syncthreads();
+  }
+  return r;
+}
+// the value at level 1 in every lane of the column
+template <class T>
+__device__ __forceinline__ T vlane0(int tall, T v) {
+  T r = lane0(v);
+  if (tall) {
+    unsigned long long *x = xbuf();
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    if (!(w & 1) && lane == 0) x[w * WAVE] = to_u64(v);
+    __syncthreads();
+    if (w & 1) r = from_u64<T>(x[(w - 1) * WAVE]);
+    __syncthreads();
+  }
+  return r;
+}
+
 // Parallel cyclic reduction of a tridiagonal system held one row per lane:
 // a x(l-1) + b x(l) + c x(l+1) = d.  Rows outside the system must be identity rows
 // (a = c = d = 0, b = 1).  Returns x for this lane.  FP64 throughout.
-__device__ __forceinline__ double pcr_solve(double a, double b, double c, double d) {
-  const int l = lane_id();
+__device__ __forceinline__ void pcr_step(int tall, int l, int nrow, int s, double &a, double &b, double &c,
+                                         double &d) {
+  double rb = 1.0 / b;
+  double am = vshup(tall, a, s), rbm = vshup(tall, rb, s), cm = vshup(tall, c, s), dm = vshup(tall, d, s);
+  double ap = vshdn(tall, a, s), rbp = vshdn(tall, rb, s), cp = vshdn(tall, c, s), dp = vshdn(tall, d, s);
+  if (l < s) { am = 0.0; rbm = 1.0; cm = 0.0; dm = 0.0; }
+  if (l + s >= nrow) { ap = 0.0; rbp = 1.0; cp = 0.0; dp = 0.0; }
+  double alpha = -a * rbm;
+  double gamma = -c * rbp;
+  double na = alpha * am;
+  double nc = gamma * cp;
+  double nb = b + alpha * cm + gamma * ap;
+  double nd = d + alpha * dm + gamma * dp;
+  a = na; b = nb; c = nc; d = nd;
+}
+// l: the row's level index; tall columns have 128 rows over the two waves
+__device__ __forceinline__ double vpcr(int tall, int l, double a, double b, double c, double d) {
+  const int nrow = tall ? 2 * WAVE : WAVE;
 #pragma unroll
-  for (int s = 1; s < WAVE; s <<= 1) {
-    double rb = 1.0 / b;
-    double am = shup(a, s), rbm = shup(rb, s), cm = shup(c, s), dm = shup(d, s);
-    double ap = shdn(a, s), rbp = shdn(rb, s), cp = shdn(c, s), dp = shdn(d, s);
-    if (l < s) { am = 0.0; rbm = 1.0; cm = 0.0; dm = 0.0; }
-    if (l + s >= WAVE) { ap = 0.0; rbp = 1.0; cp = 0.0; dp = 0.0; }
-    double alpha = -a * rbm;
-    double gamma = -c * rbp;
-    double na = alpha * am;
-    double nc = gamma * cp;
-    double nb = b + alpha * cm + gamma * ap;
-    double nd = d + alpha * dm + gamma * dp;
-    a = na; b = nb; c = nc; d = nd;
-  }
+  for (int s = 1; s < WAVE; s <<= 1) pcr_step(tall, l, nrow, s, a, b, c, d);
+  if (tall) pcr_step(tall, l, nrow, WAVE, a, b, c, d);
   return d / b;
+}
+__device__ __forceinline__ double pcr_solve(double a, double b, double c, double d) {
+  return vpcr(0, lane_id(), a, b, c, d);
 }
 
 // pcr_solve for three right-hand sides of the same matrix (the eliminations of each d are

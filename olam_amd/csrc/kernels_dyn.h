@@ -79,9 +79,10 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
                                                      RP *rho0, R *wmc0, R *thilt_s, R *vmxet_s, R *vmyet_s,
                                                      R *vmzet_s, const R *vxe, const R *vye, const R *vze) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   const bool act = k >= 2 && k <= mza;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   // own-column values; lanes outside 2..mza read the padding of this column
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
       const R dtom = dts * vti / (R)r;
       const R dtomass = forw_imp * dtom;
       const R varp = th + dtom * tt;
-      const R varp1 = shdn(varp, 1), dtomass1 = shdn(dtomass, 1);
+      const R varp1 = vshdn(m.tall, varp, 1), dtomass1 = vshdn(m.tall, dtomass, 1);
       const bool row = k >= 2 && k <= khtop;
       double a = 0.0, b = 1.0, cc = 0.0, d = 0.0;
       if (row) {
@@ -126,9 +127,9 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
         cc = (k < khtop) ? (double)v7 : 0.0;
         d = (double)rhs;
       }
-      const double x = pcr_solve(a, b, cc, d);
+      const double x = vpcr(m.tall, l, a, b, cc, d);
       const R flux = row ? (R)x : R(0);
-      const R flux_m1 = shup(flux, 1);
+      const R flux_m1 = vshup(m.tall, flux, 1);
       if (k >= 2 && k <= kmax) tt = tt + (R)((double)flux_m1 - (double)flux);
     }
     // solve_eddy_diff_vxe, pbl_driver.f90:609-742: levels ka..kmax = max(kmtop + 1, ka),
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
       const int kmtop = tb.kmtop[c];
       const int kmax = max(kmtop + 1, 2);
       const R akodz = (k >= 2 && k <= kmtop) ? arw * tb.vkm[ic] * dzim : R(0);
-      const R akodz_m1 = shup(akodz, 1);
+      const R akodz_m1 = vshup(m.tall, akodz, 1);
       const bool row = k >= 2 && k <= kmax;
       const R dtom = (R)((RP)(dts * vti) / r);
       const R dtomass = forw_imp * dtom;
@@ -160,9 +161,9 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
       pcr_solve3(a, b, cc, d0, d1, d2, x[0], x[1], x[2]);
       for (int q = 0; q < 3; ++q) {
         const R so = row ? (R)x[q] : R(0);
-        const R so1 = shdn(so, 1);
+        const R so1 = vshdn(m.tall, so, 1);
         const R v2 = (k >= 2 && k <= kmax - 1) ? akodz * (so - so1) : R(0);
-        const R v2m = shup(v2, 1);
+        const R v2m = vshup(m.tall, v2, 1);
         if (k == 2) vm[q] = vm[q] + (v2m - v2 - vctr3 * so);
         else if (k >= 3 && k <= kmax) vm[q] = vm[q] + v2m - v2;
       }
@@ -173,12 +174,12 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
   // vflux(k-1) - vflux(k) on ka..mza.  Every lane takes part in the shifts.
   if (m.rayf_cofmix) {
     const bool fl = k >= m.krayfmix_bot && k <= mza - 1;
-    const RP r1 = shdn(r, 1);
+    const RP r1 = vshdn(m.tall, r, 1);
     const R fact = m.arw0[c] * m.rayf_cofmix[l] * (R)(r1 + r);
     for (int q = 0; q < 3; ++q) {
-      const R a1 = shdn(vv[q], 1);
+      const R a1 = vshdn(m.tall, vv[q], 1);
       const R fk = fl ? fact * (vv[q] - a1) : R(0);
-      const R fkm1 = shup(fk, 1);
+      const R fkm1 = vshup(m.tall, fk, 1);
       if (act && k >= m.krayfmix_bot) vm[q] = vm[q] + fkm1 - fk;
     }
   }
@@ -210,9 +211,10 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_horiz_k(DMesh<R> m, DTurb<R> tb, const RP *rho, R dtl, R akmin) {
   NOCONTRACT
-  const int iv = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int iv = wp_.item;
   if (iv >= m.nv) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const long ldk = m.ldk, ie = (long)iv * ldk + l;
   const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
   const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void k_horiz_k(DMesh<R> m, DTurb<R> tb, const 
   const R fact2 = R(0.95) / (dtl * (R)m.w_npoly[iw2]);
   // every lane loads and shifts (k-1 levels of the W-level diffusivities)
   const R m1 = tb.vkm[i1], m2 = tb.vkm[i2], h1 = tb.vkh[i1], h2 = tb.vkh[i2];
-  const R m1m = shup(m1, 1), m2m = shup(m2, 1), h1m = shup(h1, 1), h2m = shup(h2, 1);
+  const R m1m = vshup(m.tall, m1, 1), m2m = vshup(m.tall, m2, 1), h1m = vshup(m.tall, h1, 1), h2m = vshup(m.tall, h2, 1);
   const RP r1 = rho[i1], r2 = rho[i2];
   const bool amin = akmin > R(1.e-7);
   int khv, kmv;
@@ -267,11 +269,12 @@ template <class R>
 __global__ __launch_bounds__(256) void k_alpha_w(DMesh<R> m, const R *rr_v, const R *rr_w, const R *theta,
                                                  const R *thil, R *alpha, R *pwfac, R pc1) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const long ic = (long)c * m.ldk + l;
-  const R w0 = rr_w[ic], w1 = shdn(w0, 1);    // rr_w(k), rr_w(k+1); every lane shifts
+  const R w0 = rr_w[ic], w1 = vshdn(m.tall, w0, 1);    // rr_w(k), rr_w(k+1); every lane shifts
   if (k < 2 || k > m.mza) return;
   const R rdry = R(287), rvap = R(461), cpocv = (R)(1004.0 / 717.0);
   alpha[ic] = pc1 * pow_r((rdry + rvap * rr_v[ic]) * theta[ic] / thil[ic], cpocv);
@@ -286,9 +289,10 @@ __global__ __launch_bounds__(256) void k_alpha_w(DMesh<R> m, const R *rr_v, cons
 template <class R>
 __global__ __launch_bounds__(256) void k_alpha_v(DMesh<R> m, const R *rr_w, R *pvfac) {
   NOCONTRACT
-  const int iv = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int iv = wp_.item;
   if (iv >= m.nv) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k < 2 || k > m.mza) return;
   const long ldk = m.ldk;
   const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
@@ -303,9 +307,10 @@ __global__ __launch_bounds__(256) void k_alpha_v(DMesh<R> m, const R *rr_w, R *p
 template <class R>
 __global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, const R *div_in, R *div_out) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k < 2 || k > m.mza) return;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
@@ -321,9 +326,10 @@ __global__ __launch_bounds__(256) void k_div(DMesh<R> m, const R *vmc, const R *
 template <class R>
 __global__ __launch_bounds__(256) void k_lap_w(DMesh<R> m, const R *src, R *dst) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k > m.mza) return;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
@@ -339,9 +345,10 @@ __global__ __launch_bounds__(256) void k_lap_w(DMesh<R> m, const R *src, R *dst)
 template <class R>
 __global__ __launch_bounds__(256) void k_vort(DMesh<R> m, const R *vc, R *vort) {
   NOCONTRACT
-  const int im = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int im = wp_.item;
   if (im >= m.nm) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k > m.mza) return;
   const long ldk = m.ldk;
   R v = R(0);
@@ -354,9 +361,10 @@ __global__ __launch_bounds__(256) void k_vort(DMesh<R> m, const R *vc, R *vort) 
 template <class R>
 __global__ __launch_bounds__(256) void k_lap_m(DMesh<R> m, const R *src, R *dst) {
   NOCONTRACT
-  const int im = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int im = wp_.item;
   if (im >= m.nm) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k > m.mza) return;
   const long ldk = m.ldk, ic = (long)im * ldk + l;
   R d = R(0);
@@ -377,9 +385,10 @@ __global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, c
                                                      const R *fdivh, const R *vort, const R *fvorth,
                                                      const RP *rho, const R *vc, const R *vc03d) {
   NOCONTRACT
-  const int iv = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int iv = wp_.item;
   if (iv >= m.nv) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k > m.mza) return;
   const long ldk = m.ldk, ie = (long)iv * ldk + l;
   vmc0[ie] = vmc[ie];
@@ -406,9 +415,10 @@ __global__ __launch_bounds__(256) void k_dyn_grad(DMesh<R> m, const R *thil, con
                                                   const R *vze, R *g0, R *g1, R *g2, R *g3, R *g4, R *g5, R *g6,
                                                   R *g7, int do_thil, int do_wind) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
@@ -439,9 +449,10 @@ __global__ __launch_bounds__(256) void k_dyn_grad3(DMesh<R> m, const R *thil, co
                                                    const R *vze, DynGrad3<R> out, int do_thil, int do_wind,
                                                    int bounded_mask) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
@@ -469,15 +480,15 @@ __global__ __launch_bounds__(256) void k_dyn_grad3(DMesh<R> m, const R *thil, co
 // vertical upwinded face value at W level k of a column held one level per lane
 // (fluxes_wrtv scalar_w_column :1085-1126 with grad_z_quadratic, bounded)
 template <class R>
-__device__ __forceinline__ R vert_upw(R q, R wsign, int k, int mza, R a11, R a21, R a12, R a22) {
+__device__ __forceinline__ R vert_upw(int tall, R q, R wsign, int k, int mza, R a11, R a21, R a12, R a22) {
   NOCONTRACT
   const bool inner = k >= 2 && k <= mza - 1;
-  const R qp1 = shdn(q, 1);
+  const R qp1 = vshdn(tall, q, 1);
   const R ds = inner ? qp1 - q : R(0);
-  const R dsm1 = shup(ds, 1);
+  const R dsm1 = vshup(tall, ds, 1);
   const R scpb = q + ds * a11 + dsm1 * a21;
   const R scpt = q + ds * a12 + dsm1 * a22;
-  const R scpb_p1 = shdn(scpb, 1);
+  const R scpb_p1 = vshdn(tall, scpb, 1);
   R u = (wsign < R(0)) ? scpb_p1 : scpt;
   if (k == 1) u = qp1;
   if (k == mza) u = q;
@@ -494,15 +505,15 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
   const bool inner = k >= 2 && k <= mza - 1;
   R sb, st;
   lim_vrec(m, q, k, mza, 1, sb, st);
-  const R sc = ld(scpt, ic, act), sc_p1 = shdn(sc, 1);
+  const R sc = ld(scpt, ic, act), sc_p1 = vshdn(m.tall, sc, 1);
   const R low = (wmcc < R(0)) ? sc_p1 : sc;
-  const R sb_p1 = shdn(sb, 1);
+  const R sb_p1 = vshdn(m.tall, sb, 1);
   const R hiw = inner ? ((wmcc < R(0)) ? sb_p1 : st) - low : R(0);
   const R sfw = inner ? arw * wmcc * hiw : R(0);
   const R s1c = ld(s1, ic, act), s2c = ld(s2, ic, act);
-  const R s1p = shdn(s1c, 1), s2p = shdn(s2c, 1);
+  const R s1p = vshdn(m.tall, s1c, 1), s2p = vshdn(m.tall, s2c, 1);
   const R scale = (sfw < R(0)) ? fmin(s1c, s2p) : fmin(s1p, s2c);
-  const R qp1 = shdn(q, 1);
+  const R qp1 = vshdn(m.tall, q, 1);
   R u = low + hiw * scale;
   if (k == 1) u = qp1;
   if (k == mza) u = q;
@@ -516,13 +527,14 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
 // The FP32 default instance is held to 64 VGPRs (8 waves per SIMD; a few spills, 3 %
 // faster at NXP=320, L=60); the other instances would spill far more under that bound.
 // Q3: some field has horizontal order 3 (the quadratic terms are only kept then).
-template <class R, class RP, bool MONO, bool NUD, bool Q3>
+template <class R, class RP, bool MONO, bool NUD, bool Q3, bool TALL = false>
 __global__ __launch_bounds__(256, (sizeof(R) == 4 && sizeof(RP) == 4 && !MONO && !NUD && !Q3) ? 8 : 1)
 void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(TALL);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
@@ -552,18 +564,18 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   // nudging mass flux wmanud carrying the vertically upwinded values (:605-621,
   // nudflag == 1 with nud_preserve_total_mass), before the face sums
   if (NUD && b.nud_w) {
-    const R wn = ldv(b.wmanud, ic, act, R(0)), wn_m1 = shup(wn, 1);
+    const R wn = ldv(b.wmanud, ic, act, R(0)), wn_m1 = vshup(TALL, wn, 1);
     const R wmc_n = ld(b.wmc, ic, act);
     R up[4];
     for (int q = 0; q < 4; ++q) {
-      up[q] = vert_upw(qc[q], wmc_n, k, mza, m.aw11[l], m.aw21[l], m.aw12[l], m.aw22[l]);
+      up[q] = vert_upw(TALL, qc[q], wmc_n, k, mza, m.aw11[l], m.aw21[l], m.aw12[l], m.aw22[l]);
       if constexpr (MONO)
         if (b.mono[q]) up[q] = vert_upw_monot(m, qc[q], b.scpt[q], b.ms1[q], b.ms2[q], ic, act, wmc_n, ld(m.arw, ic, act), k, mza);
     }
     const RP vt = inner ? (RP)(wn * up[0]) : RP(0);
     const R vx = inner ? wn * up[1] : R(0), vy = inner ? wn * up[2] : R(0), vz = inner ? wn * up[3] : R(0);
-    const RP vt_m1 = shup(vt, 1);
-    const R vx_m1 = shup(vx, 1), vy_m1 = shup(vy, 1), vz_m1 = shup(vz, 1);
+    const RP vt_m1 = vshup(TALL, vt, 1);
+    const R vx_m1 = vshup(TALL, vx, 1), vy_m1 = vshup(TALL, vy, 1), vz_m1 = vshup(TALL, vz, 1);
     if (body) {
       hflx = (RP)(wn_m1 - wn);
       tt = (R)((RP)tt + vt_m1 - vt);
@@ -677,7 +689,7 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
   R upws[4];
   for (int q = 0; q < 4; ++q) {
-    upws[q] = vert_upw(qc[q], wmcc, k, mza, a11, a21, a12, a22);
+    upws[q] = vert_upw(TALL, qc[q], wmcc, k, mza, a11, a21, a12, a22);
     if constexpr (MONO)
       if (b.mono[q]) upws[q] = vert_upw_monot(m, qc[q], b.scpt[q], b.ms1[q], b.ms2[q], ic, act, wmcc, ld(m.arw, ic, act), k, mza);
   }
@@ -700,8 +712,8 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const RP press_ex = b5 * (rhothil + pc2 * delex_rt);
   const RP rho_ex = rho_c + fr * delex_rho;
   // explicit W momentum (:727-745), W levels ka..mza-1
-  const RP pex_p1 = shdn(press_ex, 1), rex_p1 = shdn(rho_ex, 1);
-  const R tx_p1 = shdn(tx, 1), ty_p1 = shdn(ty, 1), tz_p1 = shdn(tz, 1);
+  const RP pex_p1 = vshdn(TALL, press_ex, 1), rex_p1 = vshdn(TALL, rho_ex, 1);
+  const R tx_p1 = vshdn(TALL, tx, 1), ty_p1 = vshdn(TALL, ty, 1), tz_p1 = vshdn(TALL, tz, 1);
   const R pwf = ld_nt(b.pwfac, ic, act), vwi = ld(m.volwi, ic, act);
   R delex_wm = R(0);
   if (inner)
@@ -715,10 +727,10 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R c8 = dts * (R)pc2;
   const R c9 = -dts * dts * (R)fr;
   const R wcc = ld(b.wc, ic, act);
-  const R wc_m1 = shup(wcc, 1), wc_p1 = shdn(wcc, 1);
-  const R b15_p1 = shdn(b15, 1), vti_p1 = shdn(vti, 1);
-  const R arw = ld(m.arw, ic, act), arw_m1 = shup(arw, 1), arw_p1 = shdn(arw, 1);
-  const R tupw_m1 = shup(tupw, 1), tupw_p1 = shdn(tupw, 1);
+  const R wc_m1 = vshup(TALL, wcc, 1), wc_p1 = vshdn(TALL, wcc, 1);
+  const R b15_p1 = vshdn(TALL, b15, 1), vti_p1 = vshdn(TALL, vti, 1);
+  const R arw = ld(m.arw, ic, act), arw_m1 = vshup(TALL, arw, 1), arw_p1 = vshdn(TALL, arw, 1);
+  const R tupw_m1 = vshup(TALL, tupw, 1), tupw_p1 = vshdn(TALL, tupw, 1);
   double ma = 0.0, mb = 1.0, mc = 0.0, md = 0.0;
   if (inner) {
     const R b7 = dts * vwi;
@@ -735,7 +747,7 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     md = (double)delex_wm;
   }
   // tridiffo (tridiag_lib.f90:10-36) replaced by FP64 PCR within the wavefront
-  const double wmf = pcr_solve(ma, mb, mc, md);
+  const double wmf = vpcr(TALL, l, ma, mb, mc, md);
   const R wmff = (R)wmf;
   R wma = inner ? wmff * arw : R(0);
   if (NUD && b.nud_vm && inner) wma = wma + b.wmanud[ic];                                 // :782-788
@@ -743,9 +755,9 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   const R vfx = wma * xupw, vfy = wma * yupw, vfz = wma * zupw;
   if (b.istage == b.nrk && inner) b.wmasc[ic] = b.wmasc[ic] + wma;
   // T-level updates (:818-844)
-  const R wma_m1 = shup(wma, 1);
-  const RP vft_m1 = shup(vflux_thil, 1);
-  const R vfx_m1 = shup(vfx, 1), vfy_m1 = shup(vfy, 1), vfz_m1 = shup(vfz, 1);
+  const R wma_m1 = vshup(TALL, wma, 1);
+  const RP vft_m1 = vshup(TALL, vflux_thil, 1);
+  const R vfx_m1 = vshup(TALL, vfx, 1), vfy_m1 = vshup(TALL, vfy, 1), vfz_m1 = vshup(TALL, vfz, 1);
   RP rho_new = rho_c;
   R thil_new = thc;
   if (body) {
@@ -759,14 +771,14 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     st_nt(b.vmzet_rk, ic, tz + vfz_m1 - vfz);
   }
   // W-level updates (:869-882)
-  const RP rho_new_p1 = shdn(rho_new, 1);
+  const RP rho_new_p1 = vshdn(TALL, rho_new, 1);
   R wc_new = wcc;
   if (inner) {
     st_nt(b.wmc, ic, wmff);
     wc_new = wmff / (R)((RP)m.zwgt_bot8[l] * rho_new + (RP)m.zwgt_top8[l] * rho_new_p1);
   }
-  const R wc_new_p1 = shdn(wc_new, 1);
-  const R thil_new_p1 = shdn(thil_new, 1);
+  const R wc_new_p1 = vshdn(TALL, wc_new, 1);
+  const R thil_new_p1 = vshdn(TALL, thil_new, 1);
   if (k == 1) { wc_new = wc_new_p1; thil_new = thil_new_p1; }
   if (act && k <= mza - 1) st_nt(b.wc, ic, wc_new);
   if (act) st_nt(b.thil_out, ic, thil_new);
@@ -785,9 +797,10 @@ struct DynMono {
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_monot_pre(DMesh<R> m, DynMono<R, RP> mo, const RP *rho0, R dts) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k < 2 || k > m.mza) return;
   const long ic = (long)c * m.ldk + l;
   const R vti = m.volti[ic], r0 = (R)rho0[ic];
@@ -803,9 +816,10 @@ __global__ __launch_bounds__(256) void k_dyn_monot_pre(DMesh<R> m, DynMono<R, RP
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, RP> b, DynMono<R, RP> mo) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
@@ -822,7 +836,7 @@ __global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, 
   }
   const R wmcc = ld(b.wmc, ic, act), arw = ld(m.arw, ic, act);
   const R wmca = inner ? wmcc * arw : R(0);
-  const R wmca_m1 = shup(wmca, 1);
+  const R wmca_m1 = vshup(m.tall, wmca, 1);
   const R vti = ld(m.volti, ic, act);
   const R rhos = act ? (R)b.rho[ic] : R(1);
   for (int q = 0; q < 4; ++q) {
@@ -838,13 +852,13 @@ __global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, 
     // vertical low-order and high-order values and fluxes (scalar_wflux_monot)
     R sb, st;
     lim_vrec(m, qc, k, mza, 1, sb, st);
-    const R sc_p1 = shdn(sc, 1), sc_m1 = shup(sc, 1), sb_p1 = shdn(sb, 1), qp1 = shdn(qc, 1);
+    const R sc_p1 = vshdn(m.tall, sc, 1), sc_m1 = vshup(m.tall, sc, 1), sb_p1 = vshdn(m.tall, sb, 1), qp1 = vshdn(m.tall, qc, 1);
     R upwl = (wmcc < R(0)) ? sc_p1 : sc;
     if (k == 1) upwl = qp1;
     if (k == mza) upwl = qc;
     const R hiw = inner ? ((wmcc < R(0)) ? sb_p1 : st) - upwl : R(0);
     const R sfw = inner ? arw * wmcc * hiw : R(0);
-    const R sfw_m1 = shup(sfw, 1), upwl_m1 = shup(upwl, 1);
+    const R sfw_m1 = vshup(m.tall, sfw, 1), upwl_m1 = vshup(m.tall, upwl, 1);
     // upwind-biased extrema (vertical)
     R smax = sc, smin = sc;
     if (k >= 3 && wmca_m1 > R(0)) { smax = fmax(smax, sc_m1); smin = fmin(smin, sc_m1); }
@@ -903,9 +917,10 @@ __global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int acc_vmasc) {
   NOCONTRACT
-  const int iv = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int iv = wp_.item;
   if (iv >= m.nv) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   if (k < 2 || k > m.mza) return;
   const long ldk = m.ldk, ie = (long)iv * ldk + l;
   const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
@@ -926,13 +941,14 @@ __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const R wcc = ld(b.wc, ic, act);
-  const R wst = shup(wcc, 1) + wcc;
+  const R wst = vshup(m.tall, wcc, 1) + wcc;
   R x = wst * (m.wnx[c] * R(0.5)), y = wst * (m.wny[c] * R(0.5)), z = wst * (m.wnz[c] * R(0.5));
   const int np = m.w_npoly[c];
   for_slots(np, [&](int j) {
@@ -941,7 +957,7 @@ __global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
     y = y + m.w_ecy[c * NSLOT + j] * v;
     z = z + m.w_ecz[c * NSLOT + j] * v;
   });
-  const R x2 = shdn(x, 1), y2 = shdn(y, 1), z2 = shdn(z, 1);
+  const R x2 = vshdn(m.tall, x, 1), y2 = vshdn(m.tall, y, 1), z2 = vshdn(m.tall, z, 1);
   if (k == 1) { x = x2; y = y2; z = z2; }
   if (act) { st_nt(b.vxe, ic, x); st_nt(b.vye, ic, y); st_nt(b.vze, ic, z); }
   (void)body;

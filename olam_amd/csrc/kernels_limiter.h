@@ -49,7 +49,7 @@ __device__ __forceinline__ void lim_col(const DMesh<R> &m, const TrStage<R, RP> 
     L.vm[j] = (j < 6 || L.np > 6) ? ld(a.vmasc, (long)L.iv[j] * ldk + l, act) : R(0);
   }
   L.wm = ld(a.wmasc, ic, act);
-  L.wm_m1 = shup(L.wm, 1);
+  L.wm_m1 = vshup(m.tall, L.wm, 1);
   L.fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
   L.fexpl = R(1) - L.fimpl;
   L.vt = ld(m.volti, ic, act);
@@ -102,10 +102,10 @@ __device__ __forceinline__ void lim_face(const DMesh<R> &m, const TrStage<R, RP>
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_lim_scales(DMesh<R> m, TrStage<R, RP> a, TrLim<R> T) {
   NOCONTRACT
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
@@ -123,20 +123,20 @@ __global__ __launch_bounds__(256) void k_tr_lim_scales(DMesh<R> m, TrStage<R, RP
     if (a.order == 3)
       for (int t = 0; t < 3; ++t) gqc[t] = ld(a.gq[t] + off, ic, act);
     // vertical: low-order upwind values from scp0, high-order from the stage input
-    const R q0p1 = shdn(q0c, 1), qp1 = shdn(qc, 1);
+    const R q0p1 = vshdn(m.tall, q0c, 1), qp1 = vshdn(m.tall, qc, 1);
     R upwl = (L.wm < R(0)) ? q0p1 : q0c;
     if (k == 1) upwl = (mode == 1) ? q0p1 : qp1;
     if (k == mza) upwl = (mode == 1) ? q0c : qc;
     R sb, st;
     lim_vrec(m, qc, k, mza, mode, sb, st);
-    const R sb_p1 = shdn(sb, 1);
+    const R sb_p1 = vshdn(m.tall, sb, 1);
     const R hiw = inner ? ((L.wm < R(0)) ? sb_p1 : st) - upwl : R(0);
     const R sfw = inner ? L.fexpl * L.wm * hiw : R(0);
-    const R sfw_m1 = shup(sfw, 1);
-    const R upwl_m1 = shup(upwl, 1);
+    const R sfw_m1 = vshup(m.tall, sfw, 1);
+    const R upwl_m1 = vshup(m.tall, upwl, 1);
     R smax = q0c, smin = q0c;
     if (mode == 1) {
-      const R q0m1 = shup(q0c, 1);
+      const R q0m1 = vshup(m.tall, q0c, 1);
       if (k >= 3 && L.wm_m1 > R(0)) { smax = fmax(smax, q0m1); smin = fmin(smin, q0m1); }
       if (inner && L.wm < R(0)) { smax = fmax(smax, q0p1); smin = fmin(smin, q0p1); }
     }
@@ -196,10 +196,10 @@ __global__ __launch_bounds__(256) void k_tr_lim_scales(DMesh<R> m, TrStage<R, RP
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_lim_update(DMesh<R> m, TrStage<R, RP> a, TrLim<R> T) {
   NOCONTRACT
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
@@ -235,23 +235,23 @@ __global__ __launch_bounds__(256) void k_tr_lim_update(DMesh<R> m, TrStage<R, RP
       if (T.tsv && iw1side && body) T.tsv[n * T.vstride + (long)L.iv[j] * ldk + l] = scale;
     });
     // vertical: renormalised fluxes (:1003-1008, :1396-1401)
-    const R q0p1 = shdn(q0c, 1), qp1 = shdn(qc, 1);
+    const R q0p1 = vshdn(m.tall, q0c, 1), qp1 = vshdn(m.tall, qc, 1);
     R upwl = (L.wm < R(0)) ? q0p1 : q0c;
     if (k == 1) upwl = (mode == 1) ? q0p1 : qp1;
     if (k == mza) upwl = (mode == 1) ? q0c : qc;
     R sb, st;
     lim_vrec(m, qc, k, mza, mode, sb, st);
-    const R sb_p1 = shdn(sb, 1);
+    const R sb_p1 = vshdn(m.tall, sb, 1);
     const R hiw = inner ? ((L.wm < R(0)) ? sb_p1 : st) - upwl : R(0);
     const R sfw = inner ? L.fexpl * L.wm * hiw : R(0);
-    const R s1p = shdn(s1c, 1), s2p = shdn(s2c, 1);
+    const R s1p = vshdn(m.tall, s1c, 1), s2p = vshdn(m.tall, s2c, 1);
     R vs = R(0);
     if (mode == 1) vs = (sfw < R(0)) ? fmin(s1c, s2p) : fmin(s1p, s2c);
     else vs = (sfw < R(0)) ? s1p : s1c;
     R upw = upwl;
     if (inner) upw = upwl + hiw * vs;
     if (T.tsw && inner) T.tsw[off + ic] = vs;
-    const R upw_m1 = shup(upw, 1);
+    const R upw_m1 = vshup(m.tall, upw, 1);
     if (L.fimpl > R(.001)) fd = fd + L.fexpl * (L.wm_m1 * (upw_m1 - q0c) - L.wm * (upw - q0c));
     else fd = fd + L.wm_m1 * (upw_m1 - q0c) - L.wm * (upw - q0c);
     // scalar_tend_monot / scalar_tend_pd: level 1 keeps the stage input
@@ -270,10 +270,10 @@ __global__ __launch_bounds__(256) void k_tr_lim_update(DMesh<R> m, TrStage<R, RP
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a, TrLim<R> T) {
   NOCONTRACT
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     const int iv = m.w_face[c * NSLOT + j];
     gsum = gsum + m.w_dirv[c * NSLOT + j] * ld(a.vmasc, (long)iv * ldk + l, act);
   });
-  const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
+  const R wm = ld(a.wmasc, ic, act), wm_m1 = vshup(m.tall, wm, 1);
   const R ff = a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R fexp = (ff > R(0.001)) ? R(1) - ff : R(1);
   const R vt = ld(m.volti, ic, act);
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     if (mode == 0) continue;
     const long off = n * a.tstride;
     R mu = ld(a.mu + off, ic, act);
-    const R mu1 = lane0(mu);
+    const R mu1 = vlane0(m.tall, mu);
     if (k == 1) mu = R(0);
     // on the first adjoint stage lam0 holds only what the implicit pass wrote
     R l0 = (!a.first || ff > R(0.001)) ? ld(a.lam0 + off, ic, act) : R(0);
@@ -302,16 +302,16 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     if (body) l0 = l0 + mu - nu * (gsum + fexp * (wm_m1 - wm));
     // every lane takes part in a shuffle: a lane that skipped it would hand 0 to its
     // neighbour (here lane mza - 1 would lose nu(mza))
-    const R nu_dn = shdn(nu, 1);
+    const R nu_dn = vshdn(m.tall, nu, 1);
     const R nu_p1 = (k + 1 <= mza) ? nu_dn : R(0);
     const R psi = act ? fexp * wm * (nu_p1 - nu) : R(0);          // adjoint of upw(k)
-    const R psi_m1 = shup(psi, 1);
+    const R psi_m1 = vshup(m.tall, psi, 1);
     const R s = inner ? ld(T.tsw + off, ic, act) : R(0);
     const R low = inner ? (R(1) - s) * psi : R(0);
     const R lowB = neg ? low : R(0), lowA = neg ? R(0) : low;
     const R h = inner ? s * psi : R(0);
     const R hneg = neg ? h : R(0), hpos = neg ? R(0) : h;
-    const R lowB_m1 = shup(lowB, 1);
+    const R lowB_m1 = vshup(m.tall, lowB, 1);
     if (body) {
       if (mode == 1) {
         if (k == 2) l0 = l0 + psi_m1;          // upw(1) = scp0(2)
@@ -324,17 +324,17 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
     const R A21 = mode == 1 ? m.awu21[l] : m.aw21[l];
     const R A12 = mode == 1 ? m.awu12[l] : m.aw12[l];
     const R A22 = mode == 1 ? m.awu22[l] : m.aw22[l];
-    const R A21_p1 = shdn(A21, 1);
-    const R hneg_m1 = shup(hneg, 1);
-    R lds = hneg_m1 * A11 + hneg * A21_p1 + hpos * A12 + shdn(hpos * A22, 1);
+    const R A21_p1 = vshdn(m.tall, A21, 1);
+    const R hneg_m1 = vshup(m.tall, hneg, 1);
+    R lds = hneg_m1 * A11 + hneg * A21_p1 + hpos * A12 + vshdn(m.tall, hpos * A22, 1);
     if (k == 1 && mode != 1) lds = R(0);
     if (mode == 1) {                                // ds(1) = ds(2), ds(mza) = ds(mza-1)
-      const R l_m1 = shup(lds, 1), l_p1 = shdn(lds, 1);
+      const R l_m1 = vshup(m.tall, lds, 1), l_p1 = vshdn(m.tall, lds, 1);
       if (k == 2) lds = lds + l_m1;
       if (k == mza - 1) lds = lds + l_p1;
     }
     if (!inner) lds = R(0);
-    const R lds_m1 = shup(lds, 1);
+    const R lds_m1 = vshup(m.tall, lds, 1);
     R lqv = hpos + hneg_m1 + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
     if (mode == 2) {
       if (k == 2) lqv = lqv + psi_m1;          // upw(1) = scp(2)
@@ -353,10 +353,10 @@ __global__ __launch_bounds__(256) void k_tr_lim_adj0(DMesh<R> m, TrAdj<R, RP> a,
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_lim_adj1(DMesh<R> m, TrAdj<R, RP> a, TrLim<R> T) {
   NOCONTRACT
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = (long)c * ldk + l;
   const int np = m.w_npoly[c];
@@ -411,9 +411,10 @@ template <class R>
 __global__ __launch_bounds__(256) void k_tr_lim_pre(DMesh<R> m, R *q0, const R *sct, const R *rho_old,
                                                     const int *pre, int ntr, long tstride, R dtr) {
   NOCONTRACT
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const long ic = (long)c * m.ldk + l;
   const R ro = ldv(rho_old, ic, body, R(1));

@@ -42,10 +42,10 @@ template <class R, bool PK>
 __global__ __launch_bounds__(256) void k_tr_grad(DMesh<R> m, const R *__restrict__ q, R *__restrict__ gx,
                                                  R *__restrict__ gy, P2<R> *__restrict__ gxy, int ntr, long tstride,
                                                  int tpw) {
-  const TrItem ti = tr_item(m.nw, ntr, tpw);
+  const TrItem ti = tr_item(m.nw, ntr, tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = ti.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const int np = m.w_npoly[c];
   int nb[7];
@@ -92,10 +92,10 @@ struct Q5 { R *p[5]; };
 template <class R>
 __global__ __launch_bounds__(256) void k_tr_grad3(DMesh<R> m, const R *__restrict__ q, Q5<R> g, int ntr,
                                                   long tstride, int tpw, const int *tmode, int lim) {
-  const TrItem ti = tr_item(m.nw, ntr, tpw);
+  const TrItem ti = tr_item(m.nw, ntr, tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = ti.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const int np = m.w_npoly[c];
   int nb[7];
@@ -134,12 +134,12 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 // ---------------------------------------------------------------- update_scalar
 // One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
 // (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
-template <class R, class RP, bool Q3, bool PK = false>
+template <class R, class RP, bool Q3, bool PK = false, bool TALL = false>
 __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, TALL);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;      // W levels with a flux
   const long ldk = m.ldk;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
     sel[j] = (dv < R(0)) ? (vm < R(0)) : !(vm < R(0));
   }
   const R wm = ld(a.wmasc, ic, act);
-  const R wm_m1 = shup(wm, 1);
+  const R wm_m1 = vshup(TALL, wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
@@ -264,19 +264,19 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
       F = F + dvm[j] * (f - q0);
     });
     // vertical: grad_z_quadratic (bounded, zero-gradient ends) and upwinding
-    const R qp1 = shdn(qc, 1);
+    const R qp1 = vshdn(TALL, qc, 1);
     const R ds = inner ? qp1 - qc : R(0);
-    const R dsm1 = shup(ds, 1);
+    const R dsm1 = vshup(TALL, ds, 1);
     const R scpb = qc + ds * a11 + dsm1 * a21;
     const R scpt = qc + ds * a12 + dsm1 * a22;
-    const R scpb_p1 = shdn(scpb, 1);
+    const R scpb_p1 = vshdn(TALL, scpb, 1);
     R upw = (wm < R(0)) ? scpb_p1 : scpt;
     if (k == 1) upw = qp1;
     if (k == mza) upw = qc;
-    const R upw_m1 = shup(upw, 1);
+    const R upw_m1 = vshup(TALL, upw, 1);
     F = F + fexpl * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
     R qn_new = q0 + a.dtr * (L.st + vt * F) * rinv;
-    const R lev1 = shdn(qn_new, 1);             // level 1 copies level 2
+    const R lev1 = vshdn(TALL, qn_new, 1);             // level 1 copies level 2
     if (k == 1) qn_new = lev1;
     if (act) st_nt(a.q_out, off + ic, qn_new);
   }
@@ -294,10 +294,10 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
 // the output.
 template <class R, class RP>
 __global__ __launch_bounds__(256, 4) void k_tr_stage(DMesh<R> m, TrStage<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk;
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256, 4) void k_tr_stage(DMesh<R> m, TrStage<R, RP> 
     sel[j] = (dv < R(0)) ? (vm < R(0)) : !(vm < R(0));
   }
   const R wm = ld(a.wmasc, ic, act);
-  const R wm_m1 = shup(wm, 1);
+  const R wm_m1 = vshup(m.tall, wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
@@ -381,19 +381,19 @@ __global__ __launch_bounds__(256, 4) void k_tr_stage(DMesh<R> m, TrStage<R, RP> 
       F = F + dvm[j] * (f - q0);
     });
     // vertical: grad_z_quadratic (bounded, zero-gradient ends) and upwinding
-    const R qp1 = shdn(qc, 1);
+    const R qp1 = vshdn(m.tall, qc, 1);
     const R ds = inner ? qp1 - qc : R(0);
-    const R dsm1 = shup(ds, 1);
+    const R dsm1 = vshup(m.tall, ds, 1);
     const R scpb = qc + ds * a11 + dsm1 * a21;
     const R scpt = qc + ds * a12 + dsm1 * a22;
-    const R scpb_p1 = shdn(scpb, 1);
+    const R scpb_p1 = vshdn(m.tall, scpb, 1);
     R upw = (wm < R(0)) ? scpb_p1 : scpt;
     if (k == 1) upw = qp1;
     if (k == mza) upw = qc;
-    const R upw_m1 = shup(upw, 1);
+    const R upw_m1 = vshup(m.tall, upw, 1);
     F = F + fexpl * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
     R qn_new = q0 + a.dtr * (L.st + vt * F) * rinv;
-    const R lev1 = shdn(qn_new, 1);             // level 1 copies level 2
+    const R lev1 = vshdn(m.tall, qn_new, 1);             // level 1 copies level 2
     if (k == 1) qn_new = lev1;
     if (act) st_nt(a.q_out, off + ic, qn_new);
   }
@@ -590,20 +590,20 @@ __global__ __launch_bounds__(512) void k_tr_patch(DMesh<R> m, TrStage<R, RP> a, 
         F = F + dvm[t][j] * (f - q0);
       });
       const R wm = wmr[t];
-      const R wm_m1 = shup(wm, 1);
-      const R qp1 = shdn(qc, 1);
+      const R wm_m1 = vshup(m.tall, wm, 1);
+      const R qp1 = vshdn(m.tall, qc, 1);
       const R ds = inner ? qp1 - qc : R(0);
-      const R dsm1 = shup(ds, 1);
+      const R dsm1 = vshup(m.tall, ds, 1);
       const R scpb = qc + ds * a11 + dsm1 * a21;
       const R scpt = qc + ds * a12 + dsm1 * a22;
-      const R scpb_p1 = shdn(scpb, 1);
+      const R scpb_p1 = vshdn(m.tall, scpb, 1);
       R upw = (wm < R(0)) ? scpb_p1 : scpt;
       if (k == 1) upw = qp1;
       if (k == mza) upw = qc;
-      const R upw_m1 = shup(upw, 1);
+      const R upw_m1 = vshup(m.tall, upw, 1);
       F = F + fexr[t] * (wm_m1 * (upw_m1 - q0) - wm * (upw - q0));
       R qn_new = q0 + a.dtr * (stc[t] + vtr[t] * F) * rinvr[t];
-      const R lev1 = shdn(qn_new, 1);
+      const R lev1 = vshdn(m.tall, qn_new, 1);
       if (k == 1) qn_new = lev1;
       if (act) a.q_out[off + icr[t]] = qn_new;
     }
@@ -616,23 +616,24 @@ __global__ __launch_bounds__(512) void k_tr_patch(DMesh<R> m, TrStage<R, RP> a, 
 // PCR.  Level 1 keeps the explicit value, as in the reference.
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_implicit(DMesh<R> m, TrStage<R, RP> a) {
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
   const R fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
   if (!(fimpl > R(0.001))) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = c * ldk + l;
   const R wm = ld(a.wmasc, ic, act);
-  const R wm_m1 = shup(wm, 1);
+  const R wm_m1 = vshup(m.tall, wm, 1);
   const R vt = ld(m.volti, ic, act);
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R ro = ld(a.rho_old, ic, act);
   const R rs = stage_rho(ro, rf, a.istage, a.nrk);
   const R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
   const R wdn = inner ? (wm < R(0) ? wm : R(0)) : R(0);
-  const R wup_m1 = shup(wup, 1), wdn_m1 = shup(wdn, 1);
+  const R wup_m1 = vshup(m.tall, wup, 1), wdn_m1 = vshup(m.tall, wdn, 1);
   const R c2 = fimpl * a.dtr * vt / (body ? rs : R(1));
   double ib1 = 0.0, ib2 = 1.0, ib3 = 0.0;
   if (body) {
@@ -644,11 +645,11 @@ __global__ __launch_bounds__(256) void k_tr_implicit(DMesh<R> m, TrStage<R, RP> 
     const long off = n * a.tstride;
     const R q0 = ld(a.q0 + off, ic, act);
     const R qe = ld(a.q_out + off, ic, act);
-    const R q0p1 = shdn(q0, 1);
+    const R q0p1 = vshdn(m.tall, q0, 1);
     const R dq0 = inner ? q0p1 - q0 : R(0);
-    const R dq0_m1 = shup(dq0, 1);
+    const R dq0_m1 = vshup(m.tall, dq0, 1);
     const R delex = qe - q0 - c2 * ((wm < R(0) ? wm : R(0)) * dq0 + (wm_m1 > R(0) ? wm_m1 : R(0)) * dq0_m1);
-    const double del = pcr_solve(ib1, ib2, ib3, body ? (double)delex : 0.0);
+    const double del = vpcr(m.tall, l, ib1, ib2, ib3, body ? (double)delex : 0.0);
     if (body) a.q_out[off + ic] = q0 + (R)del;
   }
 }
@@ -663,9 +664,10 @@ __global__ __launch_bounds__(256) void k_tr_cfl(DMesh<R> m, const R *vmasc, cons
                                                 const R *wc, R dt, int nrk, R *ffimp, unsigned *maxbits,
                                                 int *nimp, int *any_imp, int report) {
   __shared__ float sh[WPB][4];
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   const int w = threadIdx.x >> 6;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   float v0 = 0.f, v1 = 0.f, wmx = 0.f;
   bool any = false;
   if (c < m.nw) {
@@ -679,7 +681,7 @@ __global__ __launch_bounds__(256) void k_tr_cfl(DMesh<R> m, const R *vmasc, cons
       const R x = dv * ld(vmasc, (long)iv * ldk + l, act);
       cflh = cflh - (x < R(0) ? x : R(0));
     });
-    const R wm = ld(wmasc, ic, act), wm_m1 = shup(wm, 1);
+    const R wm = ld(wmasc, ic, act), wm_m1 = vshup(m.tall, wm, 1);
     R fimp[3] = {R(0), R(0), R(0)};
     R cfl = R(0);
     if (body) {
@@ -698,6 +700,11 @@ __global__ __launch_bounds__(256) void k_tr_cfl(DMesh<R> m, const R *vmasc, cons
     for (int n = 0; n < nrk; ++n) {
       R v = fimp[n];
       for (int s = 32; s >= 1; s >>= 1) { R o = __shfl_xor(v, s, WAVE); v = v > o ? v : o; }
+      if (m.tall) {                    // the maximum over both halves of the column
+        const R o = vshdn(m.tall, v, WAVE), p = vshup(m.tall, v, WAVE);
+        v = v > o ? v : o;
+        v = v > p ? v : p;
+      }
       v = v > R(0) ? v : R(0);         // fimp starts at 0 (:1558)
       R f = v < R(1) ? v : R(1);
       if (f < R(0.02)) f = R(0);
@@ -717,7 +724,8 @@ __global__ __launch_bounds__(256) void k_tr_cfl(DMesh<R> m, const R *vmasc, cons
   }
   if (any && l == 0) *any_imp = 1;
   if (!report) return;
-  if (l == 0) { sh[w][0] = v0; sh[w][1] = v1; sh[w][2] = wmx; sh[w][3] = any ? 1.f : 0.f; }
+  // each column counted once (by the wave holding level 1)
+  if (lane_id() == 0) { sh[w][0] = v0; sh[w][1] = v1; sh[w][2] = wmx; sh[w][3] = (any && l == 0) ? 1.f : 0.f; }
   __syncthreads();
   if (threadIdx.x == 0) {
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
@@ -764,45 +772,46 @@ struct TrAdj {
 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP> a) {
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
   const R fimpl = a.ffimp[(long)(a.istage - 1) * m.nw + c];
   if (!(fimpl > R(0.001))) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = wp_.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = c * ldk + l;
-  const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
+  const R wm = ld(a.wmasc, ic, act), wm_m1 = vshup(m.tall, wm, 1);
   const R vt = ld(m.volti, ic, act);
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R ro = ld(a.rho_old, ic, act);
   const R rs = stage_rho(ro, rf, a.istage, a.nrk);
   const R wup = inner ? (wm > R(0) ? wm : R(0)) : R(0);
   const R wdn = inner ? (wm < R(0) ? wm : R(0)) : R(0);
-  const R wup_m1 = shup(wup, 1), wdn_m1 = shup(wdn, 1);
+  const R wup_m1 = vshup(m.tall, wup, 1), wdn_m1 = vshup(m.tall, wdn, 1);
   const R c2 = fimpl * a.dtr * vt / (body ? rs : R(1));
   const R b1 = body ? -c2 * wup_m1 : R(0);
   const R b2 = body ? R(1) + c2 * (wup - wdn_m1) : R(1);
   const R b3 = body ? c2 * wdn : R(0);
   // transposed system: sub-diagonal b3(k-1), diagonal b2(k), super-diagonal b1(k+1)
-  const R b3_m1 = shup(b3, 1), b1_p1 = shdn(b1, 1);
+  const R b3_m1 = vshup(m.tall, b3, 1), b1_p1 = vshdn(m.tall, b1, 1);
   double tb1 = 0.0, tb2 = 1.0, tb3 = 0.0;
   if (body) { tb1 = (double)b3_m1; tb2 = (double)b2; tb3 = (double)(k < mza ? b1_p1 : R(0)); }
   for (int n = 0; n < a.ntr; ++n) {
     const long off = n * a.tstride;
     const bool limn = a.limited(n);
     R mu = ld(a.mu + off, ic, act);
-    const R mu1 = lane0(mu);
+    const R mu1 = vlane0(m.tall, mu);
     if (k == 1) mu = R(0);
     R l0 = a.first ? R(0) : ld(a.lam0 + off, ic, act);
-    const double le = pcr_solve(tb1, tb2, tb3, body ? (double)mu : 0.0);
+    const double le = vpcr(m.tall, l, tb1, tb2, tb3, body ? (double)mu : 0.0);
     R lef = body ? (R)le : R(0);
     l0 = l0 + mu - lef;
     // -c2 (min(wm(k),0) dq0(k) + max(wm(k-1),0) dq0(k-1)),  dq0(k) = q0(k+1) - q0(k)
     const R av = -c2 * lef;
     const R t1 = inner ? av * (wm < R(0) ? wm : R(0)) : R(0);
     const R t2 = (body && k - 1 >= 2) ? av * (wm_m1 > R(0) ? wm_m1 : R(0)) : R(0);
-    const R t1_m1 = shup(t1, 1), t2_p1 = shdn(t2, 1);
+    const R t1_m1 = vshup(m.tall, t1, 1), t2_p1 = vshdn(m.tall, t2, 1);
     if (body) l0 = l0 - t1 + (k - 1 >= 2 ? t1_m1 : R(0)) + t2 - (k + 1 <= mza ? t2_p1 : R(0));
     // level 1 kept the explicit level-2 value, so its adjoint joins that of the explicit
     // result at level 2 (on a limited stage level 1 is the stage input: its adjoint
@@ -818,10 +827,10 @@ __global__ __launch_bounds__(256) void k_tr_adj_implicit(DMesh<R> m, TrAdj<R, RP
 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = c * ldk + l;
@@ -832,16 +841,16 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
     const R dv = m.w_dirv[c * NSLOT + j];
     csum = csum + dv * ld(a.vmasc, (long)iv * ldk + l, act);
   });
-  const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
+  const R wm = ld(a.wmasc, ic, act), wm_m1 = vshup(m.tall, wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   const R vt = ld(m.volti, ic, act);
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R rinv = R(1) / (body ? rf : R(1));
   const R cc = csum + fexpl * (wm_m1 - wm);
   const R a11 = m.aw11[l], a12 = m.aw12[l];
-  const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
+  const R a21_p1 = vshdn(m.tall, m.aw21[l], 1), a22_p1 = vshdn(m.tall, m.aw22[l], 1);
   const bool neg = wm < R(0);
-  const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
+  const bool neg_m1 = vshup(m.tall, (int)neg, 1) != 0, neg_p1 = vshdn(m.tall, (int)neg, 1) != 0;
   // the next tracer's loads are issued before the current one is computed (all of a
   // tracer's loads are in the prefetch, so no later load waits on them)
   R nmu = R(0), nl0 = R(0);
@@ -856,15 +865,15 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
     }
     if (a.limited(n)) continue;
     const long off = n * a.tstride;
-    const R mu1 = lane0(mu);
+    const R mu1 = vlane0(m.tall, mu);
     if (k == 1) mu = R(0);
     if (k == 2) mu = mu + mu1;                 // level 1 copied level 2 (explicit)
     const R nu = body ? mu * a.dtr * vt * rinv : R(0);
     if (body) l0 = l0 + mu - nu * cc;
     // vertical transpose: upw(j) feeds F(j+1) with +fexpl wm(j), F(j) with -fexpl wm(j)
-    const R nu_p1 = shdn(nu, 1);
+    const R nu_p1 = vshdn(m.tall, nu, 1);
     const R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
-    const R lupw_m1 = shup(lupw, 1), lupw_p1 = shdn(lupw, 1);
+    const R lupw_m1 = vshup(m.tall, lupw, 1), lupw_p1 = vshdn(m.tall, lupw, 1);
     R lqv = R(0), lds = R(0);
     if (body) {
       if (inner && !neg) lqv = lqv + lupw;
@@ -877,7 +886,7 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
       if (k + 1 <= mza - 1 && !neg_p1) lds = lds + lupw_p1 * a22_p1;
     }
     if (!inner) lds = R(0);
-    const R lds_m1 = shup(lds, 1);
+    const R lds_m1 = vshup(m.tall, lds, 1);
     if (body) lqv = lqv + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
     if (act) {
       a.lam0[off + ic] = l0;
@@ -889,10 +898,10 @@ __global__ __launch_bounds__(256) void k_tr_adj0(DMesh<R> m, TrAdj<R, RP> a) {
 
 template <class R, class RP, bool Q3>
 __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
@@ -964,12 +973,12 @@ __global__ __launch_bounds__(256) void k_tr_adj1(DMesh<R> m, TrAdj<R, RP> a) {
 // which is rebuilt here from their mu (nu = mu dtr volti / rho, the same expression as
 // in k_tr_adj0) instead of being written and read back.  Per tracer it reads mu of the
 // column and its neighbours and lam0, and writes lam0, lpart, lgx, lgy.
-template <class R, class RP, bool Q3>
+template <class R, class RP, bool Q3, bool TALL = false>
 __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01(DMesh<R> m, TrAdj<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, TALL);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const bool inner = k >= 2 && k <= mza - 1;
   const long ldk = m.ldk, ic = c * ldk + l;
@@ -1007,7 +1016,7 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
     const R rfn = take ? (R)ldv(a.rho, in, act, RP(1)) : R(1);
     rinvn[j] = R(1) / (body ? rfn : R(1));
   }
-  const R wm = ld(a.wmasc, ic, act), wm_m1 = shup(wm, 1);
+  const R wm = ld(a.wmasc, ic, act), wm_m1 = vshup(TALL, wm, 1);
   const R fexpl = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
   // on the first adjoint stage lam0 holds only what the implicit pass of this column wrote
   const bool rd0 = !a.first || a.ffimp[(long)(a.istage - 1) * m.nw + c] > R(0.001);
@@ -1016,12 +1025,12 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
   const R rinv = R(1) / (body ? rf : R(1));
   const R cc = csum + fexpl * (wm_m1 - wm);
   const R a11 = m.aw11[l], a12 = m.aw12[l];
-  const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
+  const R a21_p1 = vshdn(TALL, m.aw21[l], 1), a22_p1 = vshdn(TALL, m.aw22[l], 1);
   const bool neg = wm < R(0);
-  const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
+  const bool neg_m1 = vshup(TALL, (int)neg, 1) != 0, neg_p1 = vshdn(TALL, (int)neg, 1) != 0;
   // mu of the stage output with level 1 folded into level 2 (the explicit copy)
   auto fold = [&](R v) {
-    const R v1 = lane0(v);
+    const R v1 = vlane0(TALL, v);
     if (k == 1) v = R(0);
     if (k == 2) v = v + v1;
     return v;
@@ -1050,9 +1059,9 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
     const R nu = body ? mu * a.dtr * vt * rinv : R(0);
     if (body) l0 = l0 + mu - nu * cc;
     // vertical transpose (as k_tr_adj0)
-    const R nu_p1 = shdn(nu, 1);
+    const R nu_p1 = vshdn(TALL, nu, 1);
     const R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
-    const R lupw_m1 = shup(lupw, 1), lupw_p1 = shdn(lupw, 1);
+    const R lupw_m1 = vshup(TALL, lupw, 1), lupw_p1 = vshdn(TALL, lupw, 1);
     R lqv = R(0), lds = R(0);
     if (body) {
       if (inner && !neg) lqv = lqv + lupw;
@@ -1063,7 +1072,7 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
       if (k + 1 <= mza - 1 && !neg_p1) lds = lds + lupw_p1 * a22_p1;
     }
     if (!inner) lds = R(0);
-    const R lds_m1 = shup(lds, 1);
+    const R lds_m1 = vshup(TALL, lds, 1);
     if (body) lqv = lqv + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
     // face adjoints to the upwind side (as k_tr_adj1)
     R P = R(0), Gx = R(0), Gy = R(0), Gq[NQ];
@@ -1097,10 +1106,10 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
 
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = ti.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
@@ -1162,10 +1171,10 @@ __global__ __launch_bounds__(256) void k_tr_adj2(DMesh<R> m, TrAdj<R, RP> a) {
 // otherwise).  Padded slots carry no coefficients (their back slot is not a real one).
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj2q(DMesh<R> m, TrAdj<R, RP> a) {
-  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw);
+  const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, m.tall);
   const int c = ti.c;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = ti.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
@@ -1229,7 +1238,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
   R *sgy = sgx + (size_t)pt.rmax * WAVE;
   stage_pcells(m, pt, p, nr, sc);
   const R a11 = m.aw11[l], a12 = m.aw12[l];
-  const R a21_p1 = shdn(m.aw21[l], 1), a22_p1 = shdn(m.aw22[l], 1);
+  const R a21_p1 = vshdn(m.tall, m.aw21[l], 1), a22_p1 = vshdn(m.tall, m.aw22[l], 1);
   // tracer-independent data of this wave's owned columns
   R vmj[CPW][7], cc[CPW], wmr[CPW], fexr[CPW];
   unsigned upm[CPW], iw1m[CPW];
@@ -1259,7 +1268,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
       }
       wmr[t] = ld(a.wmasc, ic, act);
       fexr[t] = R(1) - a.ffimp[(long)(a.istage - 1) * m.nw + c];
-      cc[t] = csum + fexr[t] * (shup(wmr[t], 1) - wmr[t]);
+      cc[t] = csum + fexr[t] * (vshup(m.tall, wmr[t], 1) - wmr[t]);
     }
   }
   for (int n = 0; n < a.ntr; ++n) {
@@ -1269,7 +1278,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
     for (int i = w; i < n_tot; i += nwv) {
       const long ix = (long)cells[i] * ldk + l;
       R mu = ld(a.mu + off, ix, act);
-      const R mu1 = lane0(mu);
+      const R mu1 = vlane0(m.tall, mu);
       if (k == 1) mu = R(0);
       if (k == 2) mu = mu + mu1;
       const R vt = ld(m.volti, ix, act);
@@ -1289,7 +1298,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
       const long ic = (long)c * ldk + l;
       const PCell<R> &me = sc[i];
       R mu = ld(a.mu + off, ic, act);
-      const R mu1 = lane0(mu);
+      const R mu1 = vlane0(m.tall, mu);
       if (k == 1) mu = R(0);
       if (k == 2) mu = mu + mu1;
       // on the first adjoint stage lam0 holds only what the implicit pass wrote
@@ -1299,10 +1308,10 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
       if (body) l0 = l0 + mu - nu * cc[t];
       const R wm = wmr[t], fexpl = fexr[t];
       const bool neg = wm < R(0);
-      const bool neg_m1 = shup((int)neg, 1) != 0, neg_p1 = shdn((int)neg, 1) != 0;
-      const R nu_p1 = shdn(nu, 1);
+      const bool neg_m1 = vshup(m.tall, (int)neg, 1) != 0, neg_p1 = vshdn(m.tall, (int)neg, 1) != 0;
+      const R nu_p1 = vshdn(m.tall, nu, 1);
       const R lupw = inner ? fexpl * wm * ((k + 1 <= mza ? nu_p1 : R(0)) - nu) : R(0);
-      const R lupw_m1 = shup(lupw, 1), lupw_p1 = shdn(lupw, 1);
+      const R lupw_m1 = vshup(m.tall, lupw, 1), lupw_p1 = vshdn(m.tall, lupw, 1);
       R lqv = R(0), lds = R(0);
       if (body) {
         if (inner && !neg) lqv = lqv + lupw;
@@ -1313,7 +1322,7 @@ __global__ __launch_bounds__(512) void k_tr_adj_patch(DMesh<R> m, TrAdj<R, RP> a
         if (k + 1 <= mza - 1 && !neg_p1) lds = lds + lupw_p1 * a22_p1;
       }
       if (!inner) lds = R(0);
-      const R lds_m1 = shup(lds, 1);
+      const R lds_m1 = vshup(m.tall, lds, 1);
       if (body) lqv = lqv + (k - 1 >= 2 ? lds_m1 : R(0)) - lds;
       R P = R(0), Gx = R(0), Gy = R(0);
       for_slots(me.np, [&](int j) {
@@ -1401,9 +1410,10 @@ template <class R>
 __global__ __launch_bounds__(256) void k_tr_sgsmix(DMesh<R> m, const R *__restrict__ q, R *__restrict__ sct,
                                                    const R *akhodx, const int *khtopv, const int *sgs, int ntr,
                                                    long tstride) {
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool act = k <= m.mza;
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
@@ -1435,9 +1445,10 @@ __global__ __launch_bounds__(256) void k_tr_sgsmix(DMesh<R> m, const R *__restri
 // implicit pass has already moved it there).  The first adjoint stage starts from zero.
 template <class R, class RP>
 __global__ __launch_bounds__(256) void k_tr_adj_sct(DMesh<R> m, TrAdj<R, RP> a, R *lsct, const int *sgs) {
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool act = k <= m.mza, body = act && k >= 2;
   const long ldk = m.ldk, ic = c * ldk + l;
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
@@ -1446,7 +1457,7 @@ __global__ __launch_bounds__(256) void k_tr_adj_sct(DMesh<R> m, TrAdj<R, RP> a, 
     if (!sgs[n]) continue;
     const long off = n * a.tstride;
     R mu = ld(a.mu + off, ic, act);
-    const R mu1 = lane0(mu);
+    const R mu1 = vlane0(m.tall, mu);
     if (k == 1) mu = R(0);
     if (k == 2) mu = mu + mu1;
     const R v = body ? mu * a.dtr * rinv : R(0);
@@ -1461,9 +1472,10 @@ template <class R>
 __global__ __launch_bounds__(256) void k_tr_sgsmix_adj(DMesh<R> m, const R *__restrict__ lsct, R *__restrict__ lq,
                                                        const R *akhodx, const int *khtopv, const int *sgs, int ntr,
                                                        long tstride) {
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool act = k <= m.mza;
   const long ldk = m.ldk, ic = c * ldk + l;
   const int np = m.w_npoly[c];
@@ -1510,9 +1522,10 @@ __global__ void k_maxmin_init(int *keys, int n) {
 
 template <class R>
 __global__ __launch_bounds__(256) void k_tr_maxmin(DMesh<R> m, const R *q, int ntr, long tstride, int *keys) {
-  const int c = wave_item();
+  const WPos wp_ = wave_pos(m.tall);
+  const int c = wp_.item;
   if (c >= m.nw) return;
-  const int l = lane_id(), k = l + 1;
+  const int l = wp_.l, k = l + 1;
   const bool body = k >= 2 && k <= m.mza;
   const long ic = (long)c * m.ldk + l;
   for (int n = 0; n < ntr; ++n) {
@@ -1522,7 +1535,7 @@ __global__ __launch_bounds__(256) void k_tr_maxmin(DMesh<R> m, const R *q, int n
       hi = fmaxf(hi, __shfl_xor(hi, s, WAVE));
       lo = fmaxf(lo, __shfl_xor(lo, s, WAVE));
     }
-    if (l == 0) {
+    if (lane_id() == 0) {
       atomicMax(keys + 2 * n, f2key(hi));
       atomicMax(keys + 2 * n + 1, f2key(lo));
     }
@@ -1542,15 +1555,16 @@ __global__ __launch_bounds__(256) void k_mass_partial(DMesh<R> m, const RP *rho,
                                                       double *part) {
   __shared__ double sh[WPB][65];
   const int b = blockIdx.x;
-  const int w = threadIdx.x >> 6, l = lane_id(), k = l + 1;
-  const int c = b * WPB + w;
+  const int w = threadIdx.x >> 6, gw = b * WPB + w;
+  const int c = m.tall ? gw >> 1 : gw;                       // a tall column: two waves
+  const int l = (m.tall ? (gw & 1) * WAVE : 0) + lane_id(), k = l + 1;
   const bool ok = c < m.nw && k >= 2 && k <= m.mza;
   const long ic = (long)c * m.ldk + l;
   const double mass = ok ? (double)rho[ic] * m.volt[ic] : 0.0;
   for (int n = -1; n < ntr && n < 64; ++n) {
     double v = n < 0 ? mass : (ok ? mass * (double)q[n * tstride + ic] : 0.0);
     for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, WAVE);
-    if (l == 0) sh[w][n + 1] = v;
+    if (lane_id() == 0) sh[w][n + 1] = v;
   }
   __syncthreads();
   if ((int)threadIdx.x <= ntr && threadIdx.x < 65) {

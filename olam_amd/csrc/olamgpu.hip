@@ -254,7 +254,8 @@ struct Topo {
 
 // Column stride: one wave.  Lane l of a column's wave always addresses element l of
 // that column, which the column loads rely on (see ld in common.h).
-static int ld_k(int mza) { return mza <= WAVE ? WAVE : ((mza + 15) / 16) * 16; }
+// column stride: one wave of levels, or two for tall columns (64 < mza <= 128)
+static int ld_k(int mza) { return mza <= WAVE ? WAVE : 2 * WAVE; }
 
 // Fortran accessors on the input descriptor (1-based, dummy index 1)
 struct In {
@@ -273,7 +274,11 @@ static Topo build_topo(const olamgpu_mesh *m) {
   t.nw = m->mwa - 1; t.nv = m->mva - 1; t.nm = m->mma - 1; t.mza = m->mza; t.ldk = ld_k(m->mza);
   In in{m};
   if (t.nw < 4 || t.nv < 6 || t.nm < 4) throw Err(OLAMGPU_EINVAL, "mesh too small");
-  if (m->mza < 4 || m->mza > 64) throw Err(OLAMGPU_EINVAL, "mza must be in [4, 64]");
+  if (m->mza < 4 || m->mza > 2 * WAVE) throw Err(OLAMGPU_EINVAL, "mza must be in [4, 128]");
+  // a tall column takes two waves of a block; whole blocks of whole items keep every
+  // wave of a block on the same sequence of LDS exchanges
+  if (m->mza > WAVE && ((t.nw | t.nv | t.nm) & 1))
+    throw Err(OLAMGPU_EINVAL, "more than 64 levels need even numbers of W, V and M points");
   for (int i = 1; i < m->mwa; ++i)
     if (m->lpw[i] != 2) throw Err(OLAMGPU_EINVAL, "only flat terrain (lpw = 2) is supported");
   for (int i = 1; i < m->mva; ++i)
@@ -471,17 +476,20 @@ template <class R, class RP>
 __global__ __launch_bounds__(256) void k_mass_partial_gs(DMesh<R> m, const RP *rho, const R *q, int ntr,
                                                          long tstride, double *part) {
   __shared__ double sh[WPB][MASS_NV];
-  const int w = threadIdx.x >> 6, l = lane_id(), k = l + 1;
-  for (int v = l; v <= ntr; v += WAVE) sh[w][v] = 0.0;
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  for (int v = lane; v <= ntr; v += WAVE) sh[w][v] = 0.0;
   __syncthreads();
-  const bool lev = k >= 2 && k <= m.mza;
-  for (int c = blockIdx.x * WPB + w; c < m.nw; c += gridDim.x * WPB) {
+  const int nitem = m.tall ? 2 * m.nw : m.nw;                // a tall column: two waves
+  for (int g = blockIdx.x * WPB + w; g < nitem; g += gridDim.x * WPB) {
+    const int c = m.tall ? g >> 1 : g;
+    const int l = (m.tall ? (g & 1) * WAVE : 0) + lane, k = l + 1;
+    const bool lev = k >= 2 && k <= m.mza;
     const long ic = (long)c * m.ldk + l;
     const double mass = lev ? (double)rho[ic] * m.volt[ic] : 0.0;
     for (int n = -1; n < ntr; ++n) {
       double v = n < 0 ? mass : (lev ? mass * (double)q[n * tstride + ic] : 0.0);
       for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, WAVE);
-      if (l == 0) sh[w][n + 1] += v;
+      if (lane == 0) sh[w][n + 1] += v;
     }
   }
   __syncthreads();
@@ -678,7 +686,9 @@ struct Impl : ImplBase {
   }
   template <class T> void upv(DevBuf &b, const std::vector<T> &v) { up(b, v.data(), v.size() * sizeof(T)); }
 
-  unsigned nblk(int n) const { return (unsigned)((n + WPB - 1) / WPB); }
+  // waves per point: 2 for tall columns (each item's halves in one block)
+  int halves = 1;
+  unsigned nblk(int n) const { return (unsigned)(((long)n * halves + WPB - 1) / WPB); }
   // tracers per wave in the tracer kernels: all of them up to 16 (measured at NXP=320,
   // L=60, 10 tracers: 1 -> 151, 2 -> 121, 5 -> 107, 10 -> 104 ms per step, because the
   // per-column gathers of winds and metrics are shared); OLAMGPU_TPW overrides
@@ -699,6 +709,18 @@ struct Impl : ImplBase {
     }
     if (prm.iscal_monot < 0 || prm.iscal_monot > 2) throw Err(OLAMGPU_EINVAL, "iscal_monot must be 0, 1 or 2");
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
+    if (mza > WAVE) {
+      // tall columns: the default configuration of the path (orders 1-2, no limiters,
+      // no monotone fluxes, no Rayleigh friction, no nudging; turbulence and the moist
+      // factors are refused by their own calls)
+      halves = 2;
+      const bool rayf = prm.rayf_fact > 1e-7f || prm.rayfw_fact > 1e-7f || prm.rayfdiv_fact > 1e-7f ||
+                        prm.rayfmix_fact > 1e-7f;
+      if (prm.thil_horiz_adv_order > 2 || prm.wind_horiz_adv_order > 2 || prm.scal_horiz_adv_order > 2 ||
+          prm.iscal_monot != 0 || prm.ithil_monot > 0 || prm.iwind_monot > 0 || rayf || prm.nudflag > 0)
+        throw Err(OLAMGPU_EINVAL, "more than 64 levels support orders 1-2 without limiters, monotone fluxes, "
+                                  "Rayleigh friction or nudging");
+    }
     build_mesh(m);
     alloc_state();
     if (prm.nudflag > 0) alloc_nudging();
@@ -708,6 +730,13 @@ struct Impl : ImplBase {
       two_pass = std::strcmp(e, "twopass") == 0;
       fused_fwd = std::strcmp(e, "fused") == 0;
       fuse_adj = std::strcmp(e, "split") != 0;   // OLAMGPU_TRACER=split: separate adjoint passes
+    }
+    if (halves == 2) {
+      // one tracer group (every item of a block runs the same tracer loop) and the
+      // default kernel arrangement
+      tpw = std::max(ntr, 1);
+      use_patch = two_pass = fused_fwd = false;
+      fuse_adj = true;
     }
     build_patch_tables();
   }
@@ -836,11 +865,11 @@ struct Impl : ImplBase {
         xxyym[i * 4 + j] = m->xx_yy_m[(size_t)(o + 1) * 3 + j];
       }
     }
-    // per-level arrays (WAVE entries: every lane may read its level)
+    // per-level arrays (ldk entries: every lane may read its level)
     std::vector<R> lev[11];
-    for (auto &v : lev) v.assign(WAVE, R(0));
+    for (auto &v : lev) v.assign(ldk, R(0));
     std::vector<double> lev8[4];
-    for (auto &v : lev8) v.assign(WAVE, 0.0);
+    for (auto &v : lev8) v.assign(ldk, 0.0);
     for (int k = 1; k <= mza; ++k) {
       int l = k - 1;
       lev[0][l] = m->dzim[l];
@@ -914,6 +943,7 @@ struct Impl : ImplBase {
     verify_dev(b_nbr, t.w_nbr); verify_dev(b_face, t.w_face); verify_dev(b_viw, t.v_iw); verify_dev(b_miv, t.m_iv);
 
     dm.nw = nw; dm.nv = nv; dm.nm = nm; dm.mza = mza; dm.ldk = ldk;
+    dm.tall = mza > WAVE ? 1 : 0;
     dm.w_npoly = dp<int>(b_npoly); dm.w_nbr = dp<int>(b_nbr); dm.w_face = dp<int>(b_face); dm.w_back = dp<int>(b_back);
     dm.w_dirv = dp<R>(b_dirv); dm.w_gxc = dp<R>(b_gxc); dm.w_gyc = dp<R>(b_gyc);
     dm.w_dxs = dp<R>(b_dxs); dm.w_dys = dp<R>(b_dys); dm.w_dxn = dp<R>(b_dxn); dm.w_dyn = dp<R>(b_dyn);
@@ -1190,6 +1220,7 @@ struct Impl : ImplBase {
   // the PBL scheme's mem_turb fields, then comp_horiz_k (pbl_driver.f90:136-238)
   void set_turbulence(const float *vkh, const float *vkm, const int32_t *khtop, const int32_t *kmtop,
                       const float *akm_sfc) override {
+    if (halves == 2) throw Err(OLAMGPU_EINVAL, "the PBL turbulence is not supported with more than 64 levels");
     if (!vkh || !vkm || !khtop || !kmtop || !akm_sfc)
       throw Err(OLAMGPU_EINVAL, "set_turbulence needs vkh, vkm, khtop, kmtop and akm_sfc");
     if (!tb_dzitb.p) throw Err(OLAMGPU_EINVAL, "set_turbulence needs dzit_bot in the mesh descriptor");
@@ -1224,6 +1255,7 @@ struct Impl : ImplBase {
   // comp_alpha_press (timestep.f90:500-559) from host moisture fields
   DevBuf moist[3];
   void comp_alpha_press(const float *rr_v, const float *rr_w, const float *theta) override {
+    if (halves == 2) throw Err(OLAMGPU_EINVAL, "the moist factors are not supported with more than 64 levels");
     if (!rr_v || !rr_w || !theta) throw Err(OLAMGPU_EINVAL, "comp_alpha_press needs rr_v, rr_w and theta");
     const float *h[3] = {rr_v, rr_w, theta};
     for (int i = 0; i < 3; ++i) {
@@ -1246,7 +1278,11 @@ struct Impl : ImplBase {
       else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, true, false>), dim3(gw), dim3(256), dm, b);
     } else {
       if (q3) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, true>), dim3(gw), dim3(256), dm, b);
-      else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, false>), dim3(gw), dim3(256), dm, b);
+      else if constexpr (!MONO) {
+        // tall columns: the instance whose vertical moves cross the two waves of a column
+        if (dm.tall) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, false, false, true>), dim3(gw), dim3(256), dm, b);
+        else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, false, false>), dim3(gw), dim3(256), dm, b);
+      } else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, false>), dim3(gw), dim3(256), dm, b);
     }
   }
 
@@ -1546,7 +1582,8 @@ struct Impl : ImplBase {
     if (a.order == 2 && !two_pass && !a.lim) {
       LAUNCH("k_tr_grad", (k_tr_grad<R, true>), dim3(nblk_tr()), dim3(256), dm, a.q_in, nullptr, nullptr,
              (P2<R> *)a.gxy, ntr, tstride, tpw);
-      LAUNCH("k_tr_update", (k_tr_update<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (dm.tall) LAUNCH("k_tr_update", (k_tr_update<R, RP, false, true, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else LAUNCH("k_tr_update", (k_tr_update<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
       return;
     }
     if (a.order == 2)
@@ -1558,7 +1595,8 @@ struct Impl : ImplBase {
              a.lim);
       LAUNCH("k_tr_update", (k_tr_update<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
     } else {
-      LAUNCH("k_tr_update", (k_tr_update<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (dm.tall) LAUNCH("k_tr_update", (k_tr_update<R, RP, false, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else LAUNCH("k_tr_update", (k_tr_update<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
     }
   }
 
@@ -1634,6 +1672,7 @@ struct Impl : ImplBase {
     const bool q3 = a.order == 3;
     if (fuse_adj) {
       if (q3) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else if (dm.tall) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
       else LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
     } else {
       LAUNCH("k_tr_adj0", (k_tr_adj0<R, RP>), dim3(nblk_tr()), dim3(256), dm, a);

@@ -29,6 +29,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build() (the HIP path has no fallback)")
+        # torch ships its own HIP runtime, which its libraries request as libamdhip64.so
+        # while this library requests the soname libamdhip64.so.7.  Loaded after this
+        # library, torch would bring a second HIP and HSA runtime into the process (and
+        # then finds no GPU); loaded first, its runtime carries the soname
+        # libamdhip64.so.7 and this library binds to it, so the process has one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB)
         vp = C.c_void_p
         L.olamgpu_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]
