@@ -231,12 +231,46 @@ __device__ __forceinline__ void pcr_step(int tall, int l, int nrow, int s, doubl
   double nd = d + alpha * dm + gamma * dp;
   a = na; b = nb; c = nc; d = nd;
 }
+// Tall columns: the 128 rows of a column lie in two waves.  Each step publishes every
+// row's (a, 1/b, c, d) in an LDS row array of the column and reads rows l - s and l + s
+// from it, with the arithmetic of pcr_step.  Two arrays used in turn need one barrier
+// per step: a wave writes the array of step k + 2 only after the barrier of step k + 1,
+// which every wave reaches after its reads of step k; the barrier at the end does the
+// same for the next call.  Every wave of the block must make the same calls.
+struct PcrRow {
+  double a, rb, c, d;
+};
+__device__ __forceinline__ double vpcr_tall(int l, double a, double b, double c, double d) {
+  __shared__ PcrRow rows[2][WPB / 2][2 * WAVE];
+  const int col = threadIdx.x >> 7;   // the block's two waves of one column
+  const int nrow = 2 * WAVE;
+  int k = 0;
+#pragma unroll
+  for (int s = 1; s < nrow; s <<= 1, ++k) {
+    PcrRow *r = rows[k & 1][col];
+    const double rb = 1.0 / b;
+    r[l] = PcrRow{a, rb, c, d};
+    __syncthreads();
+    double am = 0.0, rbm = 1.0, cm = 0.0, dm = 0.0;
+    double ap = 0.0, rbp = 1.0, cp = 0.0, dp = 0.0;
+    if (l >= s) { const PcrRow q = r[l - s]; am = q.a; rbm = q.rb; cm = q.c; dm = q.d; }
+    if (l + s < nrow) { const PcrRow q = r[l + s]; ap = q.a; rbp = q.rb; cp = q.c; dp = q.d; }
+    double alpha = -a * rbm;
+    double gamma = -c * rbp;
+    double na = alpha * am;
+    double nc = gamma * cp;
+    double nb = b + alpha * cm + gamma * ap;
+    double nd = d + alpha * dm + gamma * dp;
+    a = na; b = nb; c = nc; d = nd;
+  }
+  __syncthreads();
+  return d / b;
+}
 // l: the row's level index; tall columns have 128 rows over the two waves
 __device__ __forceinline__ double vpcr(int tall, int l, double a, double b, double c, double d) {
-  const int nrow = tall ? 2 * WAVE : WAVE;
+  if (tall) return vpcr_tall(l, a, b, c, d);
 #pragma unroll
-  for (int s = 1; s < WAVE; s <<= 1) pcr_step(tall, l, nrow, s, a, b, c, d);
-  if (tall) pcr_step(tall, l, nrow, WAVE, a, b, c, d);
+  for (int s = 1; s < WAVE; s <<= 1) pcr_step(0, l, WAVE, s, a, b, c, d);
   return d / b;
 }
 __device__ __forceinline__ double pcr_solve(double a, double b, double c, double d) {
