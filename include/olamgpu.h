@@ -28,9 +28,9 @@
  *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere) and 4 <= mza <= 128 are
  *    supported; horizontal advection orders 1, 2 and 3 are built (order 3 needs the
  *    mem_adv least-squares arrays in the mesh descriptor).  Columns of more than 64
- *    levels (two wavefronts per column) take orders 1 and 2 without the monotone
- *    limiters (iscal_monot = ithil_monot = iwind_monot = 0), Rayleigh friction,
- *    nudging, turbulence or moist pressure factors.  create() rejects anything else
+ *    levels (two wavefronts per column) take orders 1 and 2 and the tracer limiters,
+ *    without the monotone dynamics fluxes (ithil_monot = iwind_monot = 0), Rayleigh
+ *    friction, nudging, turbulence or moist pressure factors.  create() rejects anything else
  *    with OLAMGPU_EINVAL, and olamgpu_set_turbulence / olamgpu_comp_alpha_press
  *    return OLAMGPU_EINVAL for such columns.
  */

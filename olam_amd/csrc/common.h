@@ -379,15 +379,15 @@ template <class R>
 __device__ __forceinline__ void lim_vrec(const DMesh<R> &m, R qc, int k, int mza, int mode, R &sb, R &st) {
   NOCONTRACT
   const int l = k - 1;
-  const R qp1 = shdn(qc, 1);
+  const R qp1 = vshdn(m.tall, qc, 1);
   const bool inner = k >= 2 && k <= mza - 1;
   R ds = inner ? qp1 - qc : R(0);
-  const R ds_p1 = shdn(ds, 1), ds_m1 = shup(ds, 1);
+  const R ds_p1 = vshdn(m.tall, ds, 1), ds_m1 = vshup(m.tall, ds, 1);
   if (mode == 1) {
     if (k == 1) ds = ds_p1;
     if (k == mza) ds = ds_m1;
   }
-  const R dsm1 = shup(ds, 1);
+  const R dsm1 = vshup(m.tall, ds, 1);
   if (mode == 1) {
     sb = qc + ds * m.awu11[l] + dsm1 * m.awu21[l];
     st = qc + ds * m.awu12[l] + dsm1 * m.awu22[l];

@@ -710,15 +710,15 @@ struct Impl : ImplBase {
     if (prm.iscal_monot < 0 || prm.iscal_monot > 2) throw Err(OLAMGPU_EINVAL, "iscal_monot must be 0, 1 or 2");
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
     if (mza > WAVE) {
-      // tall columns: the default configuration of the path (orders 1-2, no limiters,
-      // no monotone fluxes, no Rayleigh friction, no nudging; turbulence and the moist
-      // factors are refused by their own calls)
+      // tall columns: orders 1-2 and the tracer limiters; no monotone dynamics fluxes,
+      // Rayleigh friction or nudging (turbulence and the moist factors are refused by
+      // their own calls)
       halves = 2;
       const bool rayf = prm.rayf_fact > 1e-7f || prm.rayfw_fact > 1e-7f || prm.rayfdiv_fact > 1e-7f ||
                         prm.rayfmix_fact > 1e-7f;
       if (prm.thil_horiz_adv_order > 2 || prm.wind_horiz_adv_order > 2 || prm.scal_horiz_adv_order > 2 ||
-          prm.iscal_monot != 0 || prm.ithil_monot > 0 || prm.iwind_monot > 0 || rayf || prm.nudflag > 0)
-        throw Err(OLAMGPU_EINVAL, "more than 64 levels support orders 1-2 without limiters, monotone fluxes, "
+          prm.ithil_monot > 0 || prm.iwind_monot > 0 || rayf || prm.nudflag > 0)
+        throw Err(OLAMGPU_EINVAL, "more than 64 levels support orders 1-2 without monotone dynamics fluxes, "
                                   "Rayleigh friction or nudging");
     }
     build_mesh(m);

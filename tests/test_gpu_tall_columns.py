@@ -3,8 +3,8 @@ vertical moves and column solves go through LDS) against the oracle, at L=70 (mz
 the level count of the largest single-GPU configuration) and L=127 (mza = 128, both
 waves full): dynamics steps, a tracer step with its adjoint and dot-product test, the
 implicit vertical tracer branch, the CFL report, mass sums (also the allocation-free
-ones), tracer maxima and minima, the dry and tracer mass of long steps, and the options
-refused for tall columns.  Tolerances as for L=63 (test_gpu_edge_levels.py).
+ones), tracer maxima and minima, the dry and tracer mass of long steps, the final-stage
+tracer limiters with their adjoint, and the options refused for tall columns.  Tolerances as for L=63 (test_gpu_edge_levels.py).
 """
 
 import ctypes as C
@@ -180,10 +180,55 @@ def test_tall_long_steps_mass(grid_factory, nlev):
     o.close()
 
 
+@pytest.mark.parametrize("mode,prec,wnoise", [(1, "f64", 0.01), (2, "f64", 0.01), (1, "f32", 0.01),
+                                              (2, "f32", 0.01), (1, "f64", 40.0), (2, "f64", 40.0)])
+def test_tall_limited_step_and_adjoint(grid_factory, mode, prec, wnoise):
+    """The final-stage limiters (1 monotone, 2 positive definite) at L=70: the forward
+    step and the adjoint with the scales frozen, against the oracle, with the cases and
+    bounds of test_gpu_limiters.py (wnoise = 40 forces implicit columns)."""
+    from test_gpu_limiters import _case, _run
+    grid = grid_factory(10, 70)
+    print(f"tall columns: limiter mode={mode} {prec} wnoise={wnoise} NXP=10 L=70", flush=True)
+    w, st, tr, sct, prm = _case(grid, mode, 2, prec, wnoise=wnoise)
+    rng = np.random.default_rng(5)
+    lam = [rng.standard_normal((grid.mwa, grid.mza)).astype(np.float32) for _ in range(3)]
+    res = {}
+    for name, be in (("gpu", _gpu(grid, prm)), ("oracle", Oracle(grid, prm))):
+        q = _run(be, grid, w, st, tr, sct, nsteps=1)
+        rep = be.cfl_report()
+        for n in range(3):
+            be.upload_scalar_adj(n + 1, lam[n])
+        be.scalar_transport_rk_adj()
+        res[name] = (q, [be.download_scalar_adj(n + 1) for n in range(3)], rep)
+        be.close()
+    if wnoise > 1.0:
+        assert res["oracle"][2][3] > 0 and res["gpu"][2][3] == res["oracle"][2][3]
+    for n in range(3):
+        e = rel_linf(res["gpu"][0][n], res["oracle"][0][n])
+        ea = rel_linf(res["gpu"][1][n], res["oracle"][1][n])
+        print(f"  tracer {n + 1}: forward rel Linf {e:.3e}, adjoint {ea:.3e}")
+        # with implicit columns the column solve (parallel cyclic reduction against the
+        # oracle's Thomas sweep) differs at 1e-16, and the monotone limiter's max/min
+        # comparisons can turn on that: measured 6.0e-8 (one FP32 ulp of the tracer's
+        # maximum 1) in FP64 at L=70, mode 1
+        tol = (2e-7 if wnoise > 1.0 else 1e-11) if prec == "f64" else 2e-5
+        assert e < tol
+        assert ea < (1e-10 if prec == "f64" else 5e-5)
+    q1, q1o = res["gpu"][0][0][1:, 1:], res["oracle"][0][0][1:, 1:]
+    print(f"  step tracer min {q1.min():.3e} max {q1.max():.6f} (oracle {q1o.min():.3e} {q1o.max():.6f})")
+    if mode == 1:
+        assert q1.min() > -1e-5 and q1.max() < 1 + 1e-5
+    else:
+        # positive definite up to rounding: in FP32 a few ulps of the tracer's maximum 1
+        # (measured -6.0e-7 at L=70 in FP32), as the oracle gives
+        assert q1.min() > (-1e-7 if prec == "f64" else -2e-6)
+        assert abs(q1.min() - q1o.min()) < (1e-12 if prec == "f64" else 1e-6)
+
+
 def test_tall_refused_options(grid_factory):
     from olam_amd.gpu import OlamGpuError
     grid = grid_factory(10, 70)
-    for kw in ({"order": 3}, {"iscal_monot": 1}, {"ithil_monot": 1}, {"nudflag": 1},
+    for kw in ({"order": 3}, {"ithil_monot": 1}, {"iwind_monot": 1}, {"nudflag": 1},
                {"rayf": {"rayfw": (20000.0, 0.01, 1.0)}}):
         with pytest.raises(OlamGpuError):
             _gpu(grid, abi.make_params("f32", num_scalar=1, dtsm=300.0, **kw))
