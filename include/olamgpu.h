@@ -26,13 +26,10 @@
  *  - Calls are ordered on the context's HIP stream and return after the stream is
  *    synchronised.  One host thread per context.
  *  - Only flat terrain (lpw = lpv = lpm = 2 everywhere) and 4 <= mza <= 128 are
- *    supported; horizontal advection orders 1, 2 and 3 are built (order 3 needs the
- *    mem_adv least-squares arrays in the mesh descriptor).  Columns of more than 64
- *    levels (two wavefronts per column) take orders 1 and 2 and the tracer limiters,
- *    without the monotone dynamics fluxes (ithil_monot = iwind_monot = 0), Rayleigh
- *    friction, nudging, turbulence or moist pressure factors.  create() rejects anything else
- *    with OLAMGPU_EINVAL, and olamgpu_set_turbulence / olamgpu_comp_alpha_press
- *    return OLAMGPU_EINVAL for such columns.
+ *    supported (columns of more than 64 levels run as two wavefronts per column, with
+ *    every option); horizontal advection orders 1, 2 and 3 are built (order 3 needs the
+ *    mem_adv least-squares arrays in the mesh descriptor).  create() rejects anything
+ *    else with OLAMGPU_EINVAL.
  */
 #ifndef OLAMGPU_H
 #define OLAMGPU_H

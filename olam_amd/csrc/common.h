@@ -303,6 +303,19 @@ __device__ __forceinline__ void pcr_solve3(double a, double b, double c, double 
   }
   x0 = d0 / b; x1 = d1 / b; x2 = d2 / b;
 }
+// pcr_solve3 for a column that may be tall (l: the row's level index).  A tall column
+// solves each right-hand side in turn: the eliminations of each d are those of the
+// joint solve, and the column's LDS row array holds one right-hand side.
+__device__ __forceinline__ void vpcr3(int tall, int l, double a, double b, double c, double d0, double d1,
+                                      double d2, double &x0, double &x1, double &x2) {
+  if (tall) {
+    x0 = vpcr_tall(l, a, b, c, d0);
+    x1 = vpcr_tall(l, a, b, c, d1);
+    x2 = vpcr_tall(l, a, b, c, d2);
+    return;
+  }
+  pcr_solve3(a, b, c, d0, d1, d2, x0, x1, x2);
+}
 
 // Column loads.  ld predicates the load per lane (lanes outside the column's levels
 // read nothing); its result may stay in flight past later loads, which the pipelined

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_dyn_setup_w(DMesh<R> m, DTurb<R> tb, R 
         d0 = (double)vv[0]; d1 = (double)vv[1]; d2 = (double)vv[2];
       }
       double x[3];
-      pcr_solve3(a, b, cc, d0, d1, d2, x[0], x[1], x[2]);
+      vpcr3(m.tall, l, a, b, cc, d0, d1, d2, x[0], x[1], x[2]);
       for (int q = 0; q < 3; ++q) {
         const R so = row ? (R)x[q] : R(0);
         const R so1 = vshdn(m.tall, so, 1);

@@ -709,18 +709,7 @@ struct Impl : ImplBase {
     }
     if (prm.iscal_monot < 0 || prm.iscal_monot > 2) throw Err(OLAMGPU_EINVAL, "iscal_monot must be 0, 1 or 2");
     if (prm.num_scalar < 0 || prm.num_scalar > 64) throw Err(OLAMGPU_EINVAL, "num_scalar must be in [0, 64]");
-    if (mza > WAVE) {
-      // tall columns: orders 1-2 and the tracer limiters; no monotone dynamics fluxes,
-      // Rayleigh friction or nudging (turbulence and the moist factors are refused by
-      // their own calls)
-      halves = 2;
-      const bool rayf = prm.rayf_fact > 1e-7f || prm.rayfw_fact > 1e-7f || prm.rayfdiv_fact > 1e-7f ||
-                        prm.rayfmix_fact > 1e-7f;
-      if (prm.thil_horiz_adv_order > 2 || prm.wind_horiz_adv_order > 2 || prm.scal_horiz_adv_order > 2 ||
-          prm.ithil_monot > 0 || prm.iwind_monot > 0 || rayf || prm.nudflag > 0)
-        throw Err(OLAMGPU_EINVAL, "more than 64 levels support orders 1-2 without monotone dynamics fluxes, "
-                                  "Rayleigh friction or nudging");
-    }
+    if (mza > WAVE) halves = 2;   // tall columns: two waves per column
     build_mesh(m);
     alloc_state();
     if (prm.nudflag > 0) alloc_nudging();
@@ -1220,7 +1209,6 @@ struct Impl : ImplBase {
   // the PBL scheme's mem_turb fields, then comp_horiz_k (pbl_driver.f90:136-238)
   void set_turbulence(const float *vkh, const float *vkm, const int32_t *khtop, const int32_t *kmtop,
                       const float *akm_sfc) override {
-    if (halves == 2) throw Err(OLAMGPU_EINVAL, "the PBL turbulence is not supported with more than 64 levels");
     if (!vkh || !vkm || !khtop || !kmtop || !akm_sfc)
       throw Err(OLAMGPU_EINVAL, "set_turbulence needs vkh, vkm, khtop, kmtop and akm_sfc");
     if (!tb_dzitb.p) throw Err(OLAMGPU_EINVAL, "set_turbulence needs dzit_bot in the mesh descriptor");
@@ -1255,7 +1243,6 @@ struct Impl : ImplBase {
   // comp_alpha_press (timestep.f90:500-559) from host moisture fields
   DevBuf moist[3];
   void comp_alpha_press(const float *rr_v, const float *rr_w, const float *theta) override {
-    if (halves == 2) throw Err(OLAMGPU_EINVAL, "the moist factors are not supported with more than 64 levels");
     if (!rr_v || !rr_w || !theta) throw Err(OLAMGPU_EINVAL, "comp_alpha_press needs rr_v, rr_w and theta");
     const float *h[3] = {rr_v, rr_w, theta};
     for (int i = 0; i < 3; ++i) {
@@ -1271,18 +1258,19 @@ struct Impl : ImplBase {
   }
 
   // k_dyn_w instance for the stage: monotone fluxes, nudging, order-3 fields
+  // (tall columns: the instances whose vertical moves cross the two waves of a column)
+  template <bool MONO, bool NUD, bool Q3> void launch_dyn_w_inst(const DynBuf<R, RP> &b, unsigned gw) {
+    if (dm.tall) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, NUD, Q3, true>), dim3(gw), dim3(256), dm, b);
+    else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, NUD, Q3>), dim3(gw), dim3(256), dm, b);
+  }
   template <bool MONO> void launch_dyn_w(const DynBuf<R, RP> &b, bool nud, unsigned gw) {
     const bool q3 = b.thil_order == 3 || b.wind_order == 3;
     if (nud) {
-      if (q3) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, true, true>), dim3(gw), dim3(256), dm, b);
-      else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, true, false>), dim3(gw), dim3(256), dm, b);
+      if (q3) launch_dyn_w_inst<MONO, true, true>(b, gw);
+      else launch_dyn_w_inst<MONO, true, false>(b, gw);
     } else {
-      if (q3) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, true>), dim3(gw), dim3(256), dm, b);
-      else if constexpr (!MONO) {
-        // tall columns: the instance whose vertical moves cross the two waves of a column
-        if (dm.tall) LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, false, false, true>), dim3(gw), dim3(256), dm, b);
-        else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, false, false, false>), dim3(gw), dim3(256), dm, b);
-      } else LAUNCH("k_dyn_w", (k_dyn_w<R, RP, MONO, false, false>), dim3(gw), dim3(256), dm, b);
+      if (q3) launch_dyn_w_inst<MONO, false, true>(b, gw);
+      else launch_dyn_w_inst<MONO, false, false>(b, gw);
     }
   }
 
@@ -1593,7 +1581,8 @@ struct Impl : ImplBase {
       const Q5<R> g{{(R *)a.gx, (R *)a.gy, dp<R>(trgq[0]), dp<R>(trgq[1]), dp<R>(trgq[2])}};
       LAUNCH("k_tr_grad3", (k_tr_grad3<R>), dim3(nblk_tr()), dim3(256), dm, a.q_in, g, ntr, tstride, tpw, a.tmode,
              a.lim);
-      LAUNCH("k_tr_update", (k_tr_update<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (dm.tall) LAUNCH("k_tr_update", (k_tr_update<R, RP, true, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else LAUNCH("k_tr_update", (k_tr_update<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
     } else {
       if (dm.tall) LAUNCH("k_tr_update", (k_tr_update<R, RP, false, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
       else LAUNCH("k_tr_update", (k_tr_update<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
@@ -1671,7 +1660,8 @@ struct Impl : ImplBase {
   void adjoint_faces(const TrAdj<R, RP> &a) {
     const bool q3 = a.order == 3;
     if (fuse_adj) {
-      if (q3) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      if (q3 && dm.tall) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true, true>), dim3(nblk_tr()), dim3(256), dm, a);
+      else if (q3) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, true>), dim3(nblk_tr()), dim3(256), dm, a);
       else if (dm.tall) LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false, true>), dim3(nblk_tr()), dim3(256), dm, a);
       else LAUNCH("k_tr_adj01", (k_tr_adj01<R, RP, false>), dim3(nblk_tr()), dim3(256), dm, a);
     } else {

@@ -1,6 +1,6 @@
 """GPU parity of comp_alpha_press (timestep.f90:500-559) against the oracle through the
 C-ABI: the moist factors alpha_press, pwfac, pvfac themselves, then acoustic steps
-that read them.  Tolerances as in test_gpu_parity.py.
+that read them.  Tolerances as in test_gpu_parity.py; also at L=70 (tall columns).
 """
 
 import numpy as np
@@ -14,9 +14,9 @@ from oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_moist_factors_and_step_parity(prec):
-    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(20))
+@pytest.mark.parametrize("prec,nlev", [("f32", 20), ("f64", 20), ("f32", 70), ("f64", 70)])
+def test_moist_factors_and_step_parity(prec, nlev):
+    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(nlev))
     st = cases.jet_state(grid)
     rng = np.random.default_rng(12)
     shape = (grid.mwa, grid.mza)
@@ -24,7 +24,7 @@ def test_moist_factors_and_step_parity(prec):
     rr_w = (rr_v + rng.uniform(0.0, 0.005, shape)).astype(np.float32)
     theta = (st.arrays["thil"] * (1.0 + 0.01 * rng.uniform(0.0, 1.0, shape))).astype(np.float32)
     prm = abi.make_params(prec, num_scalar=1, dtsm=60.0)
-    print(f"moist factors {prec}", flush=True)
+    print(f"moist factors {prec} L={nlev}", flush=True)
     fac, dyn = {}, {}
     for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
         be.upload_state(st)
@@ -51,4 +51,9 @@ def test_moist_factors_and_step_parity(prec):
         tol = tol_small if f in ("wmc", "wc", "wmasc") else tol_big
         if prec == "f64" and f == "wc":
             tol = 1e-7
+        if prec == "f64" and nlev > 63 and f in ("wmc", "wmasc", "vze", "vc"):
+            # tall columns: the 128-row PCR against the oracle's Thomas sweep differs in
+            # wmc at 1e-9 (test_gpu_tall_columns.py), which reaches vze and, through the
+            # pressure gradient of the next step, vc (measured 5.4e-11 at L=70)
+            tol = 1e-9 if f in ("vze", "vc") else 1e-7
         assert e < tol, (f, e)

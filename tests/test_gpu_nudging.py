@@ -2,7 +2,7 @@
 703-706,782-788) against the oracle through the C-ABI, in the three modes (nudging
 mass fluxes with and without the vertical nudging fluxes of nudflag == 1, and the
 density tendency rhot_nud), including the long-step mass fluxes vmasc/wmasc.
-Tolerances as in test_gpu_turbulence.py.
+Tolerances as in test_gpu_turbulence.py; also at L=70 (tall columns).
 """
 
 import numpy as np
@@ -17,16 +17,17 @@ from test_nudging import nudging_fields
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec,nudflag,preserve", [("f64", 1, 1), ("f32", 1, 1), ("f64", 2, 1), ("f64", 1, 0),
-                                                   ("f32", 1, 0)])
-def test_nudging_step_parity(prec, nudflag, preserve):
-    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(20))
+@pytest.mark.parametrize("prec,nudflag,preserve,nlev", [("f64", 1, 1, 20), ("f32", 1, 1, 20), ("f64", 2, 1, 20),
+                                                        ("f64", 1, 0, 20), ("f32", 1, 0, 20), ("f64", 1, 1, 70),
+                                                        ("f32", 2, 0, 70)])
+def test_nudging_step_parity(prec, nudflag, preserve, nlev):
+    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(nlev))
     st = cases.jet_state(grid)
     rng = np.random.default_rng(31)
     st.arrays["thil"][1:] += rng.standard_normal(st.arrays["thil"][1:].shape).astype(np.float32)
     fields = nudging_fields(grid, 7)
     prm = abi.make_params(prec, num_scalar=1, dtsm=60.0, nudflag=nudflag, nud_preserve_total_mass=preserve)
-    print(f"nudging {prec} nudflag {nudflag} preserve {preserve}", flush=True)
+    print(f"nudging {prec} nudflag {nudflag} preserve {preserve} L={nlev}", flush=True)
     out = {}
     for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
         be.upload_state(st)
@@ -45,4 +46,6 @@ def test_nudging_step_parity(prec, nudflag, preserve):
         tol = tol_small if f in ("wmc", "wc", "wmasc") else tol_big
         if prec == "f64" and f == "wc":
             tol = 1e-7
+        if prec == "f64" and nlev > 63 and f == "vze":
+            tol = 1e-9     # tall columns: wc's PCR differences reach vze (test_gpu_tall_columns.py)
         assert e < tol, (f, e)

@@ -8,7 +8,8 @@ turbulence (idiffk = 2: eddy and horizontal diffusion from the PBL scheme's fiel
 which also diffuse the tracers horizontally, scalar_transport_rk.F90:244-274).
 Three long steps of tend0, 3 x prog_wrtv_rk, timeavg_momsc and scalar_transport_rk with
 three tracers, then the tracer adjoint of the last step, against the oracle through the
-C-ABI.  FP32 tolerances as in test_gpu_parity.py; FP64 as explained below.
+C-ABI, at 30 levels and at 70 (tall columns: two waves per column).  FP32 tolerances as
+in test_gpu_parity.py; FP64 as explained below.
 """
 
 import numpy as np
@@ -24,9 +25,9 @@ from test_turbulence import turbulence
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_olamin_configuration_parity(prec):
-    grid = abi.FortranGrid(M.build_mesh(16), M.vertical_grid(30))
+@pytest.mark.parametrize("prec,nlev", [("f64", 30), ("f32", 30), ("f64", 70), ("f32", 70)])
+def test_olamin_configuration_parity(prec, nlev):
+    grid = abi.FortranGrid(M.build_mesh(16), M.vertical_grid(nlev))
     st = cases.jet_state(grid)
     rng = np.random.default_rng(41)
     shape = (grid.mwa, grid.mza)
@@ -39,7 +40,7 @@ def test_olamin_configuration_parity(prec):
                           nacoust=3, divh_damp_level=3, divh_damp_fact=0.12, vort_damp_level=3,
                           vort_damp_fact=0.075, initial=2, rayf=OLAMIN_RAYF)
     lam = [rng.standard_normal(shape).astype(np.float32) for _ in range(3)]
-    print(f"OLAMIN configuration {prec}: NXP=16, L=30, 3 long steps", flush=True)
+    print(f"OLAMIN configuration {prec}: NXP=16, L={nlev}, 3 long steps", flush=True)
     out, trc, adj = {}, {}, {}
     for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
         be.upload_state(st)

@@ -24,8 +24,8 @@ OLAMIN = {"rayf": (20000.0, 0.0, 1.0), "rayfw": (20000.0, 0.5, 1.0),
           "rayfdiv": (20000.0, 0.15, 1.0), "rayfmix": (20000.0, 0.25, 1.0)}
 
 
-def _grid():
-    g = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(20))
+def _grid(nlev=20):
+    g = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(nlev))
     st = cases.jet_state(g)
     g.set_sounding(*cases.reference_sounding(g, st))
     rng = np.random.default_rng(6)
@@ -34,15 +34,16 @@ def _grid():
     return g, st
 
 
-@pytest.mark.parametrize("prec,divl,rf", [("f64", 3, "all"), ("f32", 3, "all"), ("f64", 2, "all"),
-                                          ("f64", 1, "all"), ("f64", 0, "all"), ("f64", 3, "olamin"),
-                                          ("f32", 3, "olamin")])
-def test_rayleigh_dynamics_parity(prec, divl, rf):
-    grid, st = _grid()
+@pytest.mark.parametrize("prec,divl,rf,nlev", [("f64", 3, "all", 20), ("f32", 3, "all", 20), ("f64", 2, "all", 20),
+                                               ("f64", 1, "all", 20), ("f64", 0, "all", 20),
+                                               ("f64", 3, "olamin", 20), ("f32", 3, "olamin", 20),
+                                               ("f64", 3, "all", 70), ("f32", 3, "olamin", 70)])
+def test_rayleigh_dynamics_parity(prec, divl, rf, nlev):
+    grid, st = _grid(nlev)
     prm = abi.make_params(prec, num_scalar=1, dtsm=60.0, divh_damp_level=divl,
                           divh_damp_fact=0.12 if rf == "olamin" else 0.1,
                           rayf=ALL if rf == "all" else OLAMIN)
-    print(f"rayleigh {rf} {prec} divh level {divl}", flush=True)
+    print(f"rayleigh {rf} {prec} divh level {divl} L={nlev}", flush=True)
     out = {}
     for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
         be.upload_state(st)
@@ -63,6 +64,8 @@ def test_rayleigh_dynamics_parity(prec, divl, rf):
         tol = tol_small if f in ("wmc", "wc", "wmasc") else tol_big
         if prec == "f64" and f == "wc":
             tol = 1e-7
+        if prec == "f64" and nlev > 63 and f in ("wmc", "wmasc", "vze"):
+            tol = 1e-7     # tall columns: the 128-row PCR (test_gpu_tall_columns.py)
         assert e < tol, (f, e)
 
 

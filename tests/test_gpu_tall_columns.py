@@ -4,7 +4,7 @@ the level count of the largest single-GPU configuration) and L=127 (mza = 128, b
 waves full): dynamics steps, a tracer step with its adjoint and dot-product test, the
 implicit vertical tracer branch, the CFL report, mass sums (also the allocation-free
 ones), tracer maxima and minima, the dry and tracer mass of long steps, the final-stage
-tracer limiters with their adjoint, and the options refused for tall columns.  Tolerances as for L=63 (test_gpu_edge_levels.py).
+tracer limiters with their adjoint, and the refusal of more than 128 levels.  Tolerances as for L=63 (test_gpu_edge_levels.py).
 """
 
 import ctypes as C
@@ -225,10 +225,12 @@ def test_tall_limited_step_and_adjoint(grid_factory, mode, prec, wnoise):
         assert abs(q1.min() - q1o.min()) < (1e-12 if prec == "f64" else 1e-6)
 
 
-def test_tall_refused_options(grid_factory):
+def test_more_than_128_levels_refused(grid_factory):
+    """Columns hold at most two waves: mza = 129 (L=128) is refused at create.  Every
+    option is compared with the oracle at L=70 in its own test file (limiters above;
+    order 3, monotone fluxes, nudging, Rayleigh friction, turbulence, moist factors,
+    tracer diffusion and the OLAMIN configuration elsewhere)."""
     from olam_amd.gpu import OlamGpuError
-    grid = grid_factory(10, 70)
-    for kw in ({"order": 3}, {"ithil_monot": 1}, {"iwind_monot": 1}, {"nudflag": 1},
-               {"rayf": {"rayfw": (20000.0, 0.01, 1.0)}}):
-        with pytest.raises(OlamGpuError):
-            _gpu(grid, abi.make_params("f32", num_scalar=1, dtsm=300.0, **kw))
+    grid = grid_factory(10, 128)
+    with pytest.raises(OlamGpuError):
+        _gpu(grid, abi.make_params("f32", num_scalar=1, dtsm=300.0))

@@ -2,7 +2,8 @@
 the nudging switch on: the horizontal diffusion of do_sgsmix tracers into their
 tendencies (scalar_transport_rk.F90:244-274) with its transpose in the adjoint, and
 the nudged final-stage density rho - dtl rhot_nud (:120-133).  Tolerances as in
-test_gpu_parity.py's tracer cases (F32 1e-5, F64 1e-12 relative Linf).
+test_gpu_parity.py's tracer cases (F32 1e-5, F64 1e-12 relative Linf); also at L=70
+(tall columns).
 """
 
 import numpy as np
@@ -19,16 +20,16 @@ pytestmark = pytest.mark.gpu
 DT = 600.0
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_tracer_diffusion_and_adjoint_parity(prec):
-    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(16))
+@pytest.mark.parametrize("prec,nlev", [("f32", 16), ("f64", 16), ("f32", 70), ("f64", 70)])
+def test_tracer_diffusion_and_adjoint_parity(prec, nlev):
+    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(nlev))
     w, st, tr = tracer_case(grid, dt=DT, ntr=3)
     st.arrays["rho"][:] = w["rho"]
     turb = strong_turbulence(grid, st, 3)
     rng = np.random.default_rng(12)
     lam = [rng.standard_normal((grid.mwa, grid.mza)).astype(np.float32) for _ in range(3)]
     prm = abi.make_params(prec, num_scalar=3, dtsm=DT)
-    print(f"tracer diffusion {prec}", flush=True)
+    print(f"tracer diffusion {prec} L={nlev}", flush=True)
     out = {}
     for name, be in (("gpu", Context(grid, prm, device=0)), ("oracle", Oracle(grid, prm))):
         be.upload_state(st)
@@ -50,9 +51,9 @@ def test_tracer_diffusion_and_adjoint_parity(prec):
         assert e < tol, (i, e)
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_nudged_density_parity(prec):
-    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(16))
+@pytest.mark.parametrize("prec,nlev", [("f32", 16), ("f64", 16), ("f64", 70)])
+def test_nudged_density_parity(prec, nlev):
+    grid = abi.FortranGrid(M.build_mesh(12), M.vertical_grid(nlev))
     w, st, tr = tracer_case(grid, dt=DT, ntr=2)
     rng = np.random.default_rng(9)
     rt = np.zeros((grid.mwa, grid.mza))
