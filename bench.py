@@ -8,6 +8,11 @@ Workload: global icosahedral-hexagonal mesh NXP=320 (1,024,002 columns), 60 leve
 10 tracers with log-normal mixing ratios, dt = 30 s, synthetic jet state (no
 datasets).  Metric: cell-updates/s = columns x levels / seconds per step.
 
+Beside the metric the line carries the roofline of the dominant kernel, the step's
+fraction of the HBM roofline, host-resident (PCIe-inclusive) rates, the reference-
+precision mode, one GPU's share of the 70-level, 20-tracer configuration (NXP=320:
+a quarter of NXP=640's columns, two waves per column) and the CPU baseline.
+
 Prints one JSON line.  --gpus N > 1 exits with a message: this build targets one GPU.
 """
 
@@ -187,6 +192,9 @@ def main():
     ap.add_argument("--cpu-1thread-nxp", type=int, default=40,
                     help="mesh of the additional 1-thread CPU figure (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c5-steps", type=int, default=5,
+                    help="timed steps of one GPU's share of the 70-level, 20-tracer configuration (0: skip)")
+    ap.add_argument("--c5-nxp", type=int, default=320)
     ap.add_argument("--mixed-steps", type=int, default=5,
                     help="also time this many steps in the reference-precision mode (0: skip)")
     ap.add_argument("--no-host-resident", action="store_true",
@@ -363,6 +371,39 @@ def main():
                  "dtype": "fp32 + fp64 rho/press (the reference's precision map)"}
         log(f"[bench] reference-precision mode: {msec * 1e3:.2f} ms/step")
 
+    c5 = None
+    if args.c5_steps > 0:
+        # one GPU's share of NXP=640 x 70 levels x 20 tracers (a quarter of its columns),
+        # at its real column shape: 70 levels run as two waves per column
+        t = time.perf_counter()
+        g5 = abi.FortranGrid(mesh.build_mesh(args.c5_nxp), mesh.vertical_grid(70))
+        jet5 = cases.jet_state(g5)
+        tr5 = cases.lognormal_tracers(g5, 20)
+        c = Context(g5, abi.make_params("f32", num_scalar=20, dtsm=args.dt), device=0)
+        c.upload_state(jet5)
+        for n in range(20):
+            c.upload_scalar(n + 1, tr5[n])
+            c.upload_scalar_adj(n + 1, tr5[n])
+        del tr5
+        log(f"[bench] 70-level, 20-tracer share NXP={args.c5_nxp}: set-up {time.perf_counter() - t:.1f}s")
+        s5 = stepper(c)
+        for _ in range(2):
+            s5()
+        c.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(args.c5_steps):
+            s5()
+        sec5 = (time.perf_counter() - t0) / args.c5_steps
+        p5 = c.kernel_profile()
+        c.close()
+        cells5 = (g5.mwa - 1) * 70
+        c5 = {"workload": f"prog_wrtv_rk + scalar_transport_rk + adjoint, NXP={args.c5_nxp} ({g5.mwa - 1} columns, "
+                          f"one GPU's share of NXP=640), L=70 (two waves per column), 20 tracers, FP32",
+              "ms_per_step": sec5 * 1e3, "value": cells5 / sec5, "unit": "cell-updates/s", "steps": args.c5_steps,
+              "kernel_ms_per_step": {k: round(v[1] / args.c5_steps, 3)
+                                     for k, v in sorted(p5.items(), key=lambda kv: -kv[1][1])[:6]}}
+        log(f"[bench] 70-level, 20-tracer share: {sec5 * 1e3:.2f} ms/step, {cells5 / sec5:.3e} cell-updates/s")
+
     cpu = cpu1 = None
     if not args.no_cpu:
         threads = args.cpu_threads or host_cpu()[1]
@@ -394,6 +435,7 @@ def main():
                           "frac_of_copy": ((bd + bf + ba) * cells / sec / (copy_bw * 1e9)) if copy_bw else None},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kern_ms.items()},
         "reference_precision": mixed,
+        "l70_20tracer_share": c5,
         "host_resident": host,
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,

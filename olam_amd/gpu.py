@@ -5,6 +5,15 @@ method names follow the Fortran procedures they replace (``prog_wrtv_rk``,
 ``scalar_transport_rk``, ``tend0``, ``timeavg_momsc``, ``mass_sums``).  There is no
 CPU fallback: if the HIP library is missing or no GPU is visible, construction
 raises.
+
+A process that also uses torch must import torch before the library is loaded (the
+first Context or lib() call): torch ships its own HIP runtime, which its libraries
+request as libamdhip64.so while this library requests the soname libamdhip64.so.7.
+Loaded first, torch's runtime carries that soname and this library binds to it; loaded
+after, torch starts a second HIP runtime in the process, which finds no GPU.
+olam_amd.adjoint imports torch first.  The library does not import torch itself: with
+OMP_PROC_BIND set (bench.py sets it for the CPU baseline), importing torch binds the
+calling thread to one core, and the library's threaded host copies would share it.
 """
 
 from __future__ import annotations
@@ -29,15 +38,6 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build() (the HIP path has no fallback)")
-        # torch ships its own HIP runtime, which its libraries request as libamdhip64.so
-        # while this library requests the soname libamdhip64.so.7.  Loaded after this
-        # library, torch would bring a second HIP and HSA runtime into the process (and
-        # then finds no GPU); loaded first, its runtime carries the soname
-        # libamdhip64.so.7 and this library binds to it, so the process has one runtime.
-        try:
-            import torch  # noqa: F401
-        except ImportError:
-            pass
         L = C.CDLL(LIB)
         vp = C.c_void_p
         L.olamgpu_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]

@@ -2,6 +2,9 @@ import os
 import sys
 
 import pytest
+# torch before libolamgpu in every test process: torch's HIP runtime then serves both
+# (loaded after the library, torch would start a second runtime that finds no GPU)
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "oracle")):
