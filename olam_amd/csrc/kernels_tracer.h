@@ -173,7 +173,8 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
   const R rf = (R)ldv(a.rho, ic, act, RP(1));
   const R rinv = R(1) / (body ? rf : R(1));
   const R a11 = m.aw11[l], a21 = m.aw21[l], a12 = m.aw12[l], a22 = m.aw22[l];
-  // a neighbour is read only if some level takes that face's value from it
+  // outside the packed path a neighbour is read only if some level takes that face's
+  // value from it
   unsigned need = 0;
 #pragma unroll
   for (int j = 0; j < 7; ++j) need |= (__ballot(sel[j] && act) != 0 ? 1u : 0u) << j;
@@ -213,7 +214,10 @@ __global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) voi
       if constexpr (PK) {   // values of lanes above mza only reach discarded lanes
         if (j < 6 || np > 6) L.qn[j] = ldl(qi + (long)nb[j] * ldk, (unsigned)l);
       }
-      if ((j < 6 || np > 6) && ((need >> j) & 1u)) {
+      // the packed path gathers every neighbour's gradient pair: skipping the faces no
+      // level takes from the neighbour costs more in branches and held masks than the
+      // cache reads it saves (4 % of the kernel)
+      if ((j < 6 || np > 6) && (PK || ((need >> j) & 1u))) {
         const long in = (long)nb[j] * ldk + l;
         if constexpr (!PK) L.qn[j] = ld(qi, in, act);
         if constexpr (PK) {
@@ -1045,7 +1049,7 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
     L.l0 = rd0 ? ldl_nt(a.lam0 + off + (long)c * ldk, (unsigned)l) : R(0);
 #pragma unroll
     for (int j = 0; j < 7; ++j)
-      L.mun[j] = ((j < 6 || np > 6) && ((need >> j) & 1u)) ? ldl(a.mu + off + (long)nb[j] * ldk, (unsigned)l) : R(0);
+      L.mun[j] = (j < 6 || np > 6) ? ldl(a.mu + off + (long)nb[j] * ldk, (unsigned)l) : R(0);
   };
   Ld nx;
   if (ti.n0 < ti.n1) load(ti.n0, nx);
