@@ -618,7 +618,10 @@ void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
     bool sel;
     if constexpr (NUD) sel = sel_j[j];
     else sel = (dv < R(0)) ? (vmca < R(0)) : !(vmca < R(0));
-    const bool take = (need >> j) & 1u;     // wave-uniform
+    // wave-uniform: every neighbour is gathered in the instances without order-3 or
+    // monotone terms (the branches and held masks of the gating cost more than the
+    // cache reads it saves); the others read only faces some level takes
+    const bool take = (!Q3 && !MONO) || ((need >> j) & 1u);
     const bool nact = act && take;
     const long nbb = (long)nbr * ldk;
     R f[4];
