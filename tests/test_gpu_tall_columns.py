@@ -1,10 +1,12 @@
 """Tall columns (64 < mza <= 128: two waves per column, levels 1..64 and 65..128, whose
-vertical moves and column solves go through LDS) against the oracle, at L=70 (mza = 71,
+vertical moves and column solves go through LDS) against the oracle, at L=64 (mza = 65,
+the smallest tall column: the second wave holds the top level alone), L=70 (mza = 71,
 the level count of the largest single-GPU configuration) and L=127 (mza = 128, both
 waves full): dynamics steps, a tracer step with its adjoint and dot-product test, the
 implicit vertical tracer branch, the CFL report, mass sums (also the allocation-free
 ones), tracer maxima and minima, the dry and tracer mass of long steps, the final-stage
-tracer limiters with their adjoint, and the refusal of more than 128 levels.  Tolerances as for L=63 (test_gpu_edge_levels.py).
+tracer limiters with their adjoint, and the refusal of more than 128 levels.  Tolerances
+as for L=63 (test_gpu_edge_levels.py).
 """
 
 import ctypes as C
@@ -18,7 +20,7 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-LEVELS = [70, 127]
+LEVELS = [64, 70, 127]   # mza = 65: the second wave holds only the top level
 
 
 def _gpu(grid, prm):
