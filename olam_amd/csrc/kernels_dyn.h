@@ -376,6 +376,135 @@ __global__ __launch_bounds__(256) void k_lap_m(DMesh<R> m, const R *src, R *dst)
   dst[ic] = d;
 }
 
+// k_div, k_lap_w, k_vort and k_lap_m with N points per wave (64-level columns): the
+// gathers of all N points are issued before any is reduced, with each kernel's own
+// arithmetic.  Points past the last one read the wave's first point and store nothing.
+template <class R, int N>
+__global__ __launch_bounds__(256) void k_divn(DMesh<R> m, const R *vmc, const R *div_in, R *div_out) {
+  NOCONTRACT
+  const int w = wave_item();
+  const int l = lane_id(), k = l + 1;
+  if (N * w >= m.nw || k < 2 || k > m.mza) return;
+  const long ldk = m.ldk;
+  R d0[N], av[N][7], vv[N][7];
+  int cs[N], nps[N];
+  bool ok[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nw;
+    const int c = ok[r] ? N * w + r : N * w;
+    cs[r] = c;
+    nps[r] = m.w_npoly[c];
+    d0[r] = div_in[(long)c * ldk + l];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const long iv = (long)m.w_face[c * NSLOT + j] * ldk + l;
+      const bool use = j < 6 || nps[r] > 6;
+      av[r][j] = use ? m.arv[iv] : R(0);
+      vv[r][j] = use ? vmc[iv] : R(0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (!ok[r]) continue;
+    const int c = cs[r];
+    const long ic = (long)c * ldk + l;
+    R d = d0[r];
+    for_slots(nps[r], [&](int j) { d = d - av[r][j] * m.w_dirv[c * NSLOT + j] * vv[r][j]; });
+    div_out[ic] = d * m.volti[ic];
+  }
+}
+
+template <class R, int N>
+__global__ __launch_bounds__(256) void k_lap_wn(DMesh<R> m, const R *src, R *dst) {
+  NOCONTRACT
+  const int w = wave_item();
+  const int l = lane_id(), k = l + 1;
+  if (N * w >= m.nw || k > m.mza) return;
+  const long ldk = m.ldk;
+  R s0[N], sn[N][7];
+  int cs[N], nps[N];
+  bool ok[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nw;
+    const int c = ok[r] ? N * w + r : N * w;
+    cs[r] = c;
+    nps[r] = m.w_npoly[c];
+    s0[r] = src[(long)c * ldk + l];
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+      sn[r][j] = (j < 6 || nps[r] > 6) ? src[(long)m.w_nbr[c * NSLOT + j] * ldk + l] : R(0);
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (!ok[r]) continue;
+    const int c = cs[r];
+    R d = R(0);
+    if (k >= 2) for_slots(nps[r], [&](int j) { d = d + (sn[r][j] - s0[r]) * m.w_xxyy[c * NSLOT + j]; });
+    dst[(long)c * ldk + l] = d;
+  }
+}
+
+template <class R, int N>
+__global__ __launch_bounds__(256) void k_vortn(DMesh<R> m, const R *vc, R *vort) {
+  NOCONTRACT
+  const int w = wave_item();
+  const int l = lane_id(), k = l + 1;
+  if (N * w >= m.nm || k > m.mza) return;
+  const long ldk = m.ldk;
+  R vj[N][3];
+  int ims[N];
+  bool ok[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nm;
+    const int im = ok[r] ? N * w + r : N * w;
+    ims[r] = im;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) vj[r][j] = vc[(long)m.m_iv[im * 4 + j] * ldk + l];
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (!ok[r]) continue;
+    const int im = ims[r];
+    R v = R(0);
+    if (k >= 2)
+      for (int j = 0; j < 3; ++j) v = v + vj[r][j] * m.m_vfact[im * 4 + j];
+    vort[(long)im * ldk + l] = v;
+  }
+}
+
+template <class R, int N>
+__global__ __launch_bounds__(256) void k_lap_mn(DMesh<R> m, const R *src, R *dst) {
+  NOCONTRACT
+  const int w = wave_item();
+  const int l = lane_id(), k = l + 1;
+  if (N * w >= m.nm || k > m.mza) return;
+  const long ldk = m.ldk;
+  R s0[N], sn[N][3];
+  int ims[N];
+  bool ok[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nm;
+    const int im = ok[r] ? N * w + r : N * w;
+    ims[r] = im;
+    s0[r] = src[(long)im * ldk + l];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sn[r][j] = src[(long)m.m_im[im * 4 + j] * ldk + l];
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (!ok[r]) continue;
+    const int im = ims[r];
+    R d = R(0);
+    if (k >= 2)
+      for (int j = 0; j < 3; ++j) d = d + (sn[r][j] - s0[r]) * m.m_xxyym[im * 4 + j];
+    dst[(long)im * ldk + l] = d;
+  }
+}
+
 // V setup (prog_wrtv_rk.f90:256-262), then the Rayleigh friction of vmc toward vc03d
 // (:268-284, global domain) and the damping tendencies (vort_div_damp.f90:169-185,
 // 216-228 and 424-441), added to vmt_short in the reference's order.
@@ -939,6 +1068,56 @@ __global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int 
   if (acc_vmasc) b.vmasc[ie] = b.vmasc[ie] + (b.nud_vm ? v * m.arv[ie] + b.vmanud[ie] : v * m.arv[ie]);
 }
 
+// prog_v_begs with N faces per wave (64-level columns): the loads of all N faces are
+// issued before any is computed, so a wave keeps N times the loads in flight; the
+// arithmetic is k_dyn_v's.  Faces past the last one read the wave's first face's data
+// and store nothing.
+template <class R, class RP, int N>
+__global__ __launch_bounds__(256) void k_dyn_vn(DMesh<R> m, DynBuf<R, RP> b, int acc_vmasc) {
+  NOCONTRACT
+  const int w = wave_item();
+  const int l = lane_id(), k = l + 1;
+  if (N * w >= m.nv || k < 2 || k > m.mza) return;
+  const long ldk = m.ldk;
+  const R dts = (R)b.dt8;
+  struct Face {
+    long ie, i1, i2;
+    RP p1, p2, r1, r2;
+    R pv, v0, vt, x1, x2, y1, y2, z1, z2, vvi, nx, ny, nz;
+  };
+  Face f[N];
+  bool ok[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nv;
+    const int iv = ok[r] ? N * w + r : N * w;
+    Face &F = f[r];
+    F.ie = (long)iv * ldk + l;
+    const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
+    F.i1 = (long)iw1 * ldk + l; F.i2 = (long)iw2 * ldk + l;
+    F.p1 = b.press[F.i1]; F.p2 = b.press[F.i2];
+    F.pv = __builtin_nontemporal_load(b.pvfac + F.ie);
+    F.v0 = __builtin_nontemporal_load(b.vmc0 + F.ie);
+    F.vt = __builtin_nontemporal_load(b.vmt_s + F.ie);
+    F.x1 = b.vmxet_rk[F.i1]; F.x2 = b.vmxet_rk[F.i2];
+    F.y1 = b.vmyet_rk[F.i1]; F.y2 = b.vmyet_rk[F.i2];
+    F.z1 = b.vmzet_rk[F.i1]; F.z2 = b.vmzet_rk[F.i2];
+    F.vvi = m.volvi[F.ie];
+    F.r1 = b.rho[F.i1]; F.r2 = b.rho[F.i2];
+    F.nx = m.vnx[iv]; F.ny = m.vny[iv]; F.nz = m.vnz[iv];
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (!ok[r]) continue;
+    const Face &F = f[r];
+    const R pgf = (R)(F.p1 - F.p2) * F.pv;
+    const R v = F.v0 + dts * (F.vt + pgf + (F.nx * (F.x1 + F.x2) + F.ny * (F.y1 + F.y2) + F.nz * (F.z1 + F.z2)) * F.vvi);
+    st_nt(b.vmc, F.ie, v);
+    st_nt(b.vc, F.ie, R(2) * v / (R)(F.r1 + F.r2));
+    if (acc_vmasc) b.vmasc[F.ie] = b.vmasc[F.ie] + (b.nud_vm ? v * m.arv[F.ie] + b.vmanud[F.ie] : v * m.arv[F.ie]);
+  }
+}
+
 // ---------------------------------------------------------------- vel_t3d_hex
 // vel_t3d.f90:39-108 (no closed faces on flat terrain)
 template <class R, class RP>
@@ -964,4 +1143,47 @@ __global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
   if (k == 1) { x = x2; y = y2; z = z2; }
   if (act) { st_nt(b.vxe, ic, x); st_nt(b.vye, ic, y); st_nt(b.vze, ic, z); }
   (void)body;
+}
+
+// vel_t3d_hex with N columns per wave (64-level columns): the loads of all N columns
+// are issued before any is reduced; the arithmetic is k_dyn_vel's.  Columns past the
+// last one read the wave's first column's data and store nothing.
+template <class R, class RP, int N>
+__global__ __launch_bounds__(256) void k_dyn_veln(DMesh<R> m, DynBuf<R, RP> b) {
+  NOCONTRACT
+  const int w = wave_item();
+  if (N * w >= m.nw) return;
+  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const bool act = k <= mza;
+  const long ldk = m.ldk;
+  R wcc[N], vj[N][7];
+  int cs[N], nps[N];
+  bool ok[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nw;
+    const int c = ok[r] ? N * w + r : N * w;
+    cs[r] = c;
+    nps[r] = m.w_npoly[c];
+    wcc[r] = ld(b.wc, (long)c * ldk + l, act);
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+      vj[r][j] = (j < 6 || nps[r] > 6) ? ld(b.vc, (long)m.w_face[c * NSLOT + j] * ldk + l, act) : R(0);
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    const int c = cs[r];
+    const long ic = (long)c * ldk + l;
+    const R wst = shup(wcc[r], 1) + wcc[r];
+    R x = wst * (m.wnx[c] * R(0.5)), y = wst * (m.wny[c] * R(0.5)), z = wst * (m.wnz[c] * R(0.5));
+    for_slots(nps[r], [&](int j) {
+      const R v = vj[r][j];
+      x = x + m.w_ecx[c * NSLOT + j] * v;
+      y = y + m.w_ecy[c * NSLOT + j] * v;
+      z = z + m.w_ecz[c * NSLOT + j] * v;
+    });
+    const R x2 = shdn(x, 1), y2 = shdn(y, 1), z2 = shdn(z, 1);
+    if (k == 1) { x = x2; y = y2; z = z2; }
+    if (act && ok[r]) { st_nt(b.vxe, ic, x); st_nt(b.vye, ic, y); st_nt(b.vze, ic, z); }
+  }
 }

@@ -22,6 +22,15 @@
 
 #include "../../include/olamgpu.h"
 #include "common.h"
+#ifndef OLAMGPU_V_ITEMS
+#define OLAMGPU_V_ITEMS 2
+#endif
+#ifndef OLAMGPU_VEL_ITEMS
+#define OLAMGPU_VEL_ITEMS 8
+#endif
+#ifndef OLAMGPU_DAMP_ITEMS
+#define OLAMGPU_DAMP_ITEMS 4
+#endif
 #include "kernels_dyn.h"
 #include "kernels_tracer.h"
 #include "kernels_limiter.h"
@@ -689,6 +698,8 @@ struct Impl : ImplBase {
   // waves per point: 2 for tall columns (each item's halves in one block)
   int halves = 1;
   unsigned nblk(int n) const { return (unsigned)(((long)n * halves + WPB - 1) / WPB); }
+  // blocks for q items per wave (64-level columns)
+  unsigned nblk_per(int n, int q) const { return (unsigned)((((long)n + q - 1) / q + WPB - 1) / WPB); }
   // tracers per wave in the tracer kernels: all of them up to 16 (measured at NXP=320,
   // L=60, 10 tracers: 1 -> 151, 2 -> 121, 5 -> 107, 10 -> 104 ms per step, because the
   // per-column gathers of winds and metrics are shared); OLAMGPU_TPW overrides
@@ -1307,7 +1318,10 @@ struct Impl : ImplBase {
     // fdivl still sees it after the two Laplacians have overwritten div2d[0].
     if (do_divl || do_divh) {
       const int dst = div2d[2].p ? 2 : 0;
-      LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]), dp<R>(div2d[dst]));
+      if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+        LAUNCH("k_div", (k_divn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nw, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
+               dp<R>(vmc), dp<R>(div2d[0]), dp<R>(div2d[dst]));
+      else LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]), dp<R>(div2d[dst]));
       div_l = dst;
       int ih = dst;
       if (do_divh) {
@@ -1316,19 +1330,28 @@ struct Impl : ImplBase {
         const int order[3] = {dst, 1, 0};
         for (int nn = 1; nn <= prm.divh_damp_level - 1; ++nn) {
           const int il = order[nn - 1], io = order[nn];
-          LAUNCH("k_lap_w", (k_lap_w<R>), dim3(gw), dim3(256), dm, dp<R>(div2d[il]), dp<R>(div2d[io]));
+          if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+            LAUNCH("k_lap_w", (k_lap_wn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nw, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
+                   dp<R>(div2d[il]), dp<R>(div2d[io]));
+          else LAUNCH("k_lap_w", (k_lap_w<R>), dim3(gw), dim3(256), dm, dp<R>(div2d[il]), dp<R>(div2d[io]));
           ih = io;
         }
       }
       div_ih = ih;
     }
     if (do_vort) {
-      LAUNCH("k_vort", (k_vort<R>), dim3(gm), dim3(256), dm, dp<R>(vc), dp<R>(vort2d[0]));
+      if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+        LAUNCH("k_vort", (k_vortn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nm, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
+               dp<R>(vc), dp<R>(vort2d[0]));
+      else LAUNCH("k_vort", (k_vort<R>), dim3(gm), dim3(256), dm, dp<R>(vc), dp<R>(vort2d[0]));
       int ih = 0;
       for (int nn = 1; nn <= prm.vort_damp_level - 1; ++nn) {
         int il = (nn - 1) % 2;
         ih = nn % 2;
-        LAUNCH("k_lap_m", (k_lap_m<R>), dim3(gm), dim3(256), dm, dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
+        if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+          LAUNCH("k_lap_m", (k_lap_mn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nm, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
+                 dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
+        else LAUNCH("k_lap_m", (k_lap_m<R>), dim3(gm), dim3(256), dm, dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
       }
       vort_ih = ih;
     }
@@ -1402,8 +1425,15 @@ struct Impl : ImplBase {
         launch_dyn_w<false>(b, nud, gw);
       }
       cur_thil = 1 - cur_thil;
-      LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
-      LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);
+      // 64-level columns: several faces or columns per wave, their loads issued together
+      if (!dm.tall) {
+        constexpr int NV = OLAMGPU_V_ITEMS, NVEL = OLAMGPU_VEL_ITEMS;
+        LAUNCH("k_dyn_v", (k_dyn_vn<R, RP, NV>), dim3(nblk_per(nv, NV)), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
+        LAUNCH("k_dyn_vel", (k_dyn_veln<R, RP, NVEL>), dim3(nblk_per(nw, NVEL)), dim3(256), dm, b);
+      } else {
+        LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
+        LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);
+      }
     }
     HIPCHK(hipGetLastError());
   }
