@@ -537,6 +537,65 @@ __global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, c
   vmt_s[ie] = t;
 }
 
+// k_dyn_setup_v with N faces per wave (64-level columns): the loads of all N faces are
+// issued before any tendency is formed, with k_dyn_setup_v's arithmetic.  Faces past the
+// last one read the wave's first face and store nothing.
+template <class R, class RP, int N>
+__global__ __launch_bounds__(256) void k_dyn_setup_vn(DMesh<R> m, const R *vmc, const R *vmt, R *vmc0, R *vmt_s,
+                                                      const R *divl, const R *fdivl, int kdivl, const R *divh,
+                                                      const R *fdivh, const R *vort, const R *fvorth,
+                                                      const RP *rho, const R *vc, const R *vc03d) {
+  NOCONTRACT
+  const int w = wave_item();
+  const int l = lane_id(), k = l + 1;
+  if (N * w >= m.nv || k > m.mza) return;
+  const long ldk = m.ldk;
+  struct Face {
+    long ie;
+    R mc, mt, c03, cc, dl1, dl2, fl, dh1, dh2, fh, vo1, vo2, fv;
+    RP r1, r2;
+  };
+  Face f[N];
+  bool ok[N];
+  const bool rayf = m.rayf_cof && k >= m.krayf_bot;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    ok[r] = N * w + r < m.nv;
+    const int iv = ok[r] ? N * w + r : N * w;
+    Face &F = f[r];
+    F.ie = (long)iv * ldk + l;
+    F.mc = vmc[F.ie];
+    F.mt = vmt[F.ie];
+    if (k >= 2) {
+      const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
+      const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
+      if (rayf || vort) { F.r1 = rho[i1]; F.r2 = rho[i2]; }
+      if (rayf) { F.c03 = vc03d[F.ie]; F.cc = vc[F.ie]; }
+      if (divl && k >= kdivl) { F.fl = fdivl[F.ie]; F.dl1 = divl[i1]; F.dl2 = divl[i2]; }
+      if (divh) { F.fh = fdivh[F.ie]; F.dh1 = divh[i1]; F.dh2 = divh[i2]; }
+      if (vort) {
+        const int im1 = m.v_im[iv * 2], im2 = m.v_im[iv * 2 + 1];
+        F.fv = fvorth[iv];
+        F.vo1 = vort[(long)im1 * ldk + l]; F.vo2 = vort[(long)im2 * ldk + l];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (!ok[r]) continue;
+    const Face &F = f[r];
+    vmc0[F.ie] = F.mc;
+    R t = F.mt;
+    if (k >= 2) {
+      if (rayf) t = (R)((RP)t + (RP)(m.rayf_cof[l] * R(0.5)) * (F.r1 + F.r2) * (RP)(F.c03 - F.cc));
+      if (divl && k >= kdivl) t = t + F.fl * (F.dl2 - F.dl1);
+      if (divh) t = t + F.fh * (F.dh2 - F.dh1);
+      if (vort) t = t + F.fv * (F.vo2 - F.vo1) * (R)(F.r1 + F.r2);
+    }
+    vmt_s[F.ie] = t;
+  }
+}
+
 // ---------------------------------------------------------------- gradients
 // grad_t2d of thil, vxe, vye, vze (fluxes_wrtv, prog_wrtv_rk.f90:976-987)
 template <class R>

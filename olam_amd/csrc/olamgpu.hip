@@ -28,6 +28,9 @@
 #ifndef OLAMGPU_VEL_ITEMS
 #define OLAMGPU_VEL_ITEMS 8
 #endif
+#ifndef OLAMGPU_SETUPV_ITEMS
+#define OLAMGPU_SETUPV_ITEMS 2
+#endif
 #ifndef OLAMGPU_DAMP_ITEMS
 #define OLAMGPU_DAMP_ITEMS 4
 #endif
@@ -1355,11 +1358,19 @@ struct Impl : ImplBase {
       }
       vort_ih = ih;
     }
-    LAUNCH("k_dyn_setup_v", (k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
-                       dp<R>(vmt_s), do_divl ? dp<R>(div2d[div_l]) : (R *)nullptr, dp<R>(fdivl), kdivl,
-                       do_divh ? dp<R>(div2d[div_ih]) : (R *)nullptr, dp<R>(fdivh),
-                       do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc),
-                       dp<R>(vc03d));
+    {
+      R *dl = do_divl ? dp<R>(div2d[div_l]) : (R *)nullptr, *dh = do_divh ? dp<R>(div2d[div_ih]) : (R *)nullptr;
+      R *vo = do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr;
+      // 64-level columns: several faces per wave, their loads issued together
+      if (!dm.tall)
+        LAUNCH("k_dyn_setup_v", (k_dyn_setup_vn<R, RP, OLAMGPU_SETUPV_ITEMS>), dim3(nblk_per(nv, OLAMGPU_SETUPV_ITEMS)),
+               dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0), dp<R>(vmt_s), dl, dp<R>(fdivl), kdivl, dh,
+               dp<R>(fdivh), vo, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc), dp<R>(vc03d));
+      else
+        LAUNCH("k_dyn_setup_v", (k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
+               dp<R>(vmt_s), dl, dp<R>(fdivl), kdivl, dh, dp<R>(fdivh), vo, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc),
+               dp<R>(vc03d));
+    }
     const int nrk = prm.nrk_wrtv;
     for (int istage = 1; istage <= nrk; ++istage) {
       double frac = (nrk <= 2) ? (istage == 1 ? 0.5 : 1.0)
