@@ -1009,7 +1009,9 @@ __global__ __launch_bounds__(256, Q3 ? 1 : OLAMGPU_BLOCKS_ADJ01) void k_tr_adj01
     iw1[j] = dv < R(0);
     up[j] = iw1[j] ? !(vm < R(0)) : (vm < R(0));
   }
-  // a neighbour is read only if this column is upwind of that face at some level
+  // a neighbour's volume and density are read only if this column is upwind of that face
+  // at some level (its mu, per tracer, is read for every face: gating those loads costs
+  // more than the reads)
 #pragma unroll
   for (int j = 0; j < 7; ++j) need |= (__ballot(up[j] && act) != 0 ? 1u : 0u) << j;
 #pragma unroll
