@@ -8,6 +8,9 @@
 // tridiagonal W solve by parallel cyclic reduction inside the wavefront),
 // k_dyn_v (prog_v_begs) and k_dyn_vel (vel_t3d_hex).
 #pragma once
+#ifndef OLAMGPU_BLOCKS_DYNW
+#define OLAMGPU_BLOCKS_DYNW 8
+#endif
 #include "common.h"
 
 // All dynamics kernels evaluate their expressions exactly as written (no FMA
@@ -716,7 +719,7 @@ __device__ __forceinline__ R vert_upw_monot(const DMesh<R> &m, R q, const R *scp
 // faster at NXP=320, L=60); the other instances would spill far more under that bound.
 // Q3: some field has horizontal order 3 (the quadratic terms are only kept then).
 template <class R, class RP, bool MONO, bool NUD, bool Q3, bool TALL = false>
-__global__ __launch_bounds__(256, (sizeof(R) == 4 && sizeof(RP) == 4 && !MONO && !NUD && !Q3) ? 8 : 1)
+__global__ __launch_bounds__(256, (sizeof(R) == 4 && sizeof(RP) == 4 && !MONO && !NUD && !Q3) ? OLAMGPU_BLOCKS_DYNW : 1)
 void k_dyn_w(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
   const WPos wp_ = wave_pos(TALL);
