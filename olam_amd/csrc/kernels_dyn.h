@@ -6,7 +6,7 @@
 // OpenMP loops: k_dyn_grad (grad_t2d of thil, vxe, vye, vze), k_dyn_w (fluxes_wrtv's
 // face and vertical upwinding fused with prog_wrt_begs, including the FP64
 // tridiagonal W solve by parallel cyclic reduction inside the wavefront),
-// k_dyn_v (prog_v_begs) and k_dyn_vel (vel_t3d_hex).
+// k_dyn_vn (prog_v_begs) and k_dyn_vel / k_dyn_veln (vel_t3d_hex).
 #pragma once
 #ifndef OLAMGPU_BLOCKS_DYNW
 #define OLAMGPU_BLOCKS_DYNW 8
@@ -379,14 +379,16 @@ __global__ __launch_bounds__(256) void k_lap_m(DMesh<R> m, const R *src, R *dst)
   dst[ic] = d;
 }
 
-// k_div, k_lap_w, k_vort and k_lap_m with N points per wave (64-level columns): the
+// k_div, k_lap_w, k_vort and k_lap_m with N points per wave (per wave pair for tall
+// columns, no move crosses the two waves): the
 // gathers of all N points are issued before any is reduced, with each kernel's own
 // arithmetic.  Points past the last one read the wave's first point and store nothing.
 template <class R, int N>
 __global__ __launch_bounds__(256) void k_divn(DMesh<R> m, const R *vmc, const R *div_in, R *div_out) {
   NOCONTRACT
-  const int w = wave_item();
-  const int l = lane_id(), k = l + 1;
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
+  const int l = p.l, k = l + 1;
   if (N * w >= m.nw || k < 2 || k > m.mza) return;
   const long ldk = m.ldk;
   R d0[N], av[N][7], vv[N][7];
@@ -421,8 +423,9 @@ __global__ __launch_bounds__(256) void k_divn(DMesh<R> m, const R *vmc, const R 
 template <class R, int N>
 __global__ __launch_bounds__(256) void k_lap_wn(DMesh<R> m, const R *src, R *dst) {
   NOCONTRACT
-  const int w = wave_item();
-  const int l = lane_id(), k = l + 1;
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
+  const int l = p.l, k = l + 1;
   if (N * w >= m.nw || k > m.mza) return;
   const long ldk = m.ldk;
   R s0[N], sn[N][7];
@@ -452,8 +455,9 @@ __global__ __launch_bounds__(256) void k_lap_wn(DMesh<R> m, const R *src, R *dst
 template <class R, int N>
 __global__ __launch_bounds__(256) void k_vortn(DMesh<R> m, const R *vc, R *vort) {
   NOCONTRACT
-  const int w = wave_item();
-  const int l = lane_id(), k = l + 1;
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
+  const int l = p.l, k = l + 1;
   if (N * w >= m.nm || k > m.mza) return;
   const long ldk = m.ldk;
   R vj[N][3];
@@ -481,8 +485,9 @@ __global__ __launch_bounds__(256) void k_vortn(DMesh<R> m, const R *vc, R *vort)
 template <class R, int N>
 __global__ __launch_bounds__(256) void k_lap_mn(DMesh<R> m, const R *src, R *dst) {
   NOCONTRACT
-  const int w = wave_item();
-  const int l = lane_id(), k = l + 1;
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
+  const int l = p.l, k = l + 1;
   if (N * w >= m.nm || k > m.mza) return;
   const long ldk = m.ldk;
   R s0[N], sn[N][3];
@@ -508,49 +513,22 @@ __global__ __launch_bounds__(256) void k_lap_mn(DMesh<R> m, const R *src, R *dst
   }
 }
 
+
 // V setup (prog_wrtv_rk.f90:256-262), then the Rayleigh friction of vmc toward vc03d
 // (:268-284, global domain) and the damping tendencies (vort_div_damp.f90:169-185,
 // 216-228 and 424-441), added to vmt_short in the reference's order.
-template <class R, class RP>
-__global__ __launch_bounds__(256) void k_dyn_setup_v(DMesh<R> m, const R *vmc, const R *vmt, R *vmc0, R *vmt_s,
-                                                     const R *divl, const R *fdivl, int kdivl, const R *divh,
-                                                     const R *fdivh, const R *vort, const R *fvorth,
-                                                     const RP *rho, const R *vc, const R *vc03d) {
-  NOCONTRACT
-  const WPos wp_ = wave_pos(m.tall);
-  const int iv = wp_.item;
-  if (iv >= m.nv) return;
-  const int l = wp_.l, k = l + 1;
-  if (k > m.mza) return;
-  const long ldk = m.ldk, ie = (long)iv * ldk + l;
-  vmc0[ie] = vmc[ie];
-  R t = vmt[ie];
-  if (k >= 2) {
-    const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
-    const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
-    if (m.rayf_cof && k >= m.krayf_bot)
-      t = (R)((RP)t + (RP)(m.rayf_cof[l] * R(0.5)) * (rho[i1] + rho[i2]) * (RP)(vc03d[ie] - vc[ie]));
-    if (divl && k >= kdivl) t = t + fdivl[ie] * (divl[i2] - divl[i1]);
-    if (divh) t = t + fdivh[ie] * (divh[i2] - divh[i1]);
-    if (vort) {
-      const int im1 = m.v_im[iv * 2], im2 = m.v_im[iv * 2 + 1];
-      t = t + fvorth[iv] * (vort[(long)im2 * ldk + l] - vort[(long)im1 * ldk + l]) * (R)(rho[i1] + rho[i2]);
-    }
-  }
-  vmt_s[ie] = t;
-}
-
-// k_dyn_setup_v with N faces per wave (64-level columns): the loads of all N faces are
-// issued before any tendency is formed, with k_dyn_setup_v's arithmetic.  Faces past the
-// last one read the wave's first face and store nothing.
+// N faces per wave (a wave pair for tall columns): the loads of all N faces are issued
+// before any tendency is formed.  Faces past the last one read the wave's first face
+// and store nothing.
 template <class R, class RP, int N>
 __global__ __launch_bounds__(256) void k_dyn_setup_vn(DMesh<R> m, const R *vmc, const R *vmt, R *vmc0, R *vmt_s,
                                                       const R *divl, const R *fdivl, int kdivl, const R *divh,
                                                       const R *fdivh, const R *vort, const R *fvorth,
                                                       const RP *rho, const R *vc, const R *vc03d) {
   NOCONTRACT
-  const int w = wave_item();
-  const int l = lane_id(), k = l + 1;
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
+  const int l = p.l, k = l + 1;
   if (N * w >= m.nv || k > m.mza) return;
   const long ldk = m.ldk;
   struct Face {
@@ -1108,37 +1086,15 @@ __global__ __launch_bounds__(256) void k_dyn_monot_scales(DMesh<R> m, DynBuf<R, 
 // ---------------------------------------------------------------- prog_v_begs
 // prog_wrtv_rk.f90:888-946.  At stage nrk-1 the new vmc*arv is what the last stage
 // uses as vmca, so it is added to vmasc here (:356-365).
-template <class R, class RP>
-__global__ __launch_bounds__(256) void k_dyn_v(DMesh<R> m, DynBuf<R, RP> b, int acc_vmasc) {
-  NOCONTRACT
-  const WPos wp_ = wave_pos(m.tall);
-  const int iv = wp_.item;
-  if (iv >= m.nv) return;
-  const int l = wp_.l, k = l + 1;
-  if (k < 2 || k > m.mza) return;
-  const long ldk = m.ldk, ie = (long)iv * ldk + l;
-  const int iw1 = m.v_iw[iv * 2], iw2 = m.v_iw[iv * 2 + 1];
-  const long i1 = (long)iw1 * ldk + l, i2 = (long)iw2 * ldk + l;
-  const R dts = (R)b.dt8;
-  const R pgf = (R)(b.press[i1] - b.press[i2]) * __builtin_nontemporal_load(b.pvfac + ie);
-  const R v = __builtin_nontemporal_load(b.vmc0 + ie) + dts * (__builtin_nontemporal_load(b.vmt_s + ie) + pgf +
-                                  (m.vnx[iv] * (b.vmxet_rk[i1] + b.vmxet_rk[i2]) +
-                                   m.vny[iv] * (b.vmyet_rk[i1] + b.vmyet_rk[i2]) +
-                                   m.vnz[iv] * (b.vmzet_rk[i1] + b.vmzet_rk[i2])) * m.volvi[ie]);
-  st_nt(b.vmc, ie, v);
-  st_nt(b.vc, ie, R(2) * v / (R)(b.rho[i1] + b.rho[i2]));
-  if (acc_vmasc) b.vmasc[ie] = b.vmasc[ie] + (b.nud_vm ? v * m.arv[ie] + b.vmanud[ie] : v * m.arv[ie]);
-}
-
-// prog_v_begs with N faces per wave (64-level columns): the loads of all N faces are
-// issued before any is computed, so a wave keeps N times the loads in flight; the
-// arithmetic is k_dyn_v's.  Faces past the last one read the wave's first face's data
-// and store nothing.
+// N faces per wave (a wave pair for tall columns): the loads of all N faces are issued
+// before any is computed, so a wave keeps N times the loads in flight.  Faces past the
+// last one read the wave's first face and store nothing.
 template <class R, class RP, int N>
 __global__ __launch_bounds__(256) void k_dyn_vn(DMesh<R> m, DynBuf<R, RP> b, int acc_vmasc) {
   NOCONTRACT
-  const int w = wave_item();
-  const int l = lane_id(), k = l + 1;
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
+  const int l = p.l, k = l + 1;
   if (N * w >= m.nv || k < 2 || k > m.mza) return;
   const long ldk = m.ldk;
   const R dts = (R)b.dt8;

@@ -701,8 +701,8 @@ struct Impl : ImplBase {
   // waves per point: 2 for tall columns (each item's halves in one block)
   int halves = 1;
   unsigned nblk(int n) const { return (unsigned)(((long)n * halves + WPB - 1) / WPB); }
-  // blocks for q items per wave (64-level columns)
-  unsigned nblk_per(int n, int q) const { return (unsigned)((((long)n + q - 1) / q + WPB - 1) / WPB); }
+  // blocks for q items per wave (per wave pair for tall columns)
+  unsigned nblk_per(int n, int q) const { return (unsigned)((((long)n + q - 1) / q * halves + WPB - 1) / WPB); }
   // tracers per wave in the tracer kernels: all of them up to 16 (measured at NXP=320,
   // L=60, 10 tracers: 1 -> 151, 2 -> 121, 5 -> 107, 10 -> 104 ms per step, because the
   // per-column gathers of winds and metrics are shared); OLAMGPU_TPW overrides
@@ -1321,7 +1321,7 @@ struct Impl : ImplBase {
     // fdivl still sees it after the two Laplacians have overwritten div2d[0].
     if (do_divl || do_divh) {
       const int dst = div2d[2].p ? 2 : 0;
-      if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+      if (OLAMGPU_DAMP_ITEMS > 1)
         LAUNCH("k_div", (k_divn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nw, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
                dp<R>(vmc), dp<R>(div2d[0]), dp<R>(div2d[dst]));
       else LAUNCH("k_div", (k_div<R>), dim3(gw), dim3(256), dm, dp<R>(vmc), dp<R>(div2d[0]), dp<R>(div2d[dst]));
@@ -1333,7 +1333,7 @@ struct Impl : ImplBase {
         const int order[3] = {dst, 1, 0};
         for (int nn = 1; nn <= prm.divh_damp_level - 1; ++nn) {
           const int il = order[nn - 1], io = order[nn];
-          if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+          if (OLAMGPU_DAMP_ITEMS > 1)
             LAUNCH("k_lap_w", (k_lap_wn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nw, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
                    dp<R>(div2d[il]), dp<R>(div2d[io]));
           else LAUNCH("k_lap_w", (k_lap_w<R>), dim3(gw), dim3(256), dm, dp<R>(div2d[il]), dp<R>(div2d[io]));
@@ -1343,7 +1343,7 @@ struct Impl : ImplBase {
       div_ih = ih;
     }
     if (do_vort) {
-      if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+      if (OLAMGPU_DAMP_ITEMS > 1)
         LAUNCH("k_vort", (k_vortn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nm, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
                dp<R>(vc), dp<R>(vort2d[0]));
       else LAUNCH("k_vort", (k_vort<R>), dim3(gm), dim3(256), dm, dp<R>(vc), dp<R>(vort2d[0]));
@@ -1351,7 +1351,7 @@ struct Impl : ImplBase {
       for (int nn = 1; nn <= prm.vort_damp_level - 1; ++nn) {
         int il = (nn - 1) % 2;
         ih = nn % 2;
-        if (OLAMGPU_DAMP_ITEMS > 1 && !dm.tall)
+        if (OLAMGPU_DAMP_ITEMS > 1)
           LAUNCH("k_lap_m", (k_lap_mn<R, OLAMGPU_DAMP_ITEMS>), dim3(nblk_per(nm, OLAMGPU_DAMP_ITEMS)), dim3(256), dm,
                  dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
         else LAUNCH("k_lap_m", (k_lap_m<R>), dim3(gm), dim3(256), dm, dp<R>(vort2d[il]), dp<R>(vort2d[ih]));
@@ -1361,15 +1361,10 @@ struct Impl : ImplBase {
     {
       R *dl = do_divl ? dp<R>(div2d[div_l]) : (R *)nullptr, *dh = do_divh ? dp<R>(div2d[div_ih]) : (R *)nullptr;
       R *vo = do_vort ? dp<R>(vort2d[vort_ih]) : (R *)nullptr;
-      // 64-level columns: several faces per wave, their loads issued together
-      if (!dm.tall)
-        LAUNCH("k_dyn_setup_v", (k_dyn_setup_vn<R, RP, OLAMGPU_SETUPV_ITEMS>), dim3(nblk_per(nv, OLAMGPU_SETUPV_ITEMS)),
-               dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0), dp<R>(vmt_s), dl, dp<R>(fdivl), kdivl, dh,
-               dp<R>(fdivh), vo, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc), dp<R>(vc03d));
-      else
-        LAUNCH("k_dyn_setup_v", (k_dyn_setup_v<R, RP>), dim3(gv), dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0),
-               dp<R>(vmt_s), dl, dp<R>(fdivl), kdivl, dh, dp<R>(fdivh), vo, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc),
-               dp<R>(vc03d));
+      // several faces per wave, their loads issued together
+      LAUNCH("k_dyn_setup_v", (k_dyn_setup_vn<R, RP, OLAMGPU_SETUPV_ITEMS>), dim3(nblk_per(nv, OLAMGPU_SETUPV_ITEMS)),
+             dim3(256), dm, dp<R>(vmc), dp<R>(vmt), dp<R>(vmc0), dp<R>(vmt_s), dl, dp<R>(fdivl), kdivl, dh,
+             dp<R>(fdivh), vo, dp<R>(fvorth), dp<RP>(rho), dp<R>(vc), dp<R>(vc03d));
     }
     const int nrk = prm.nrk_wrtv;
     for (int istage = 1; istage <= nrk; ++istage) {
@@ -1436,15 +1431,12 @@ struct Impl : ImplBase {
         launch_dyn_w<false>(b, nud, gw);
       }
       cur_thil = 1 - cur_thil;
-      // 64-level columns: several faces or columns per wave, their loads issued together
-      if (!dm.tall) {
-        constexpr int NV = OLAMGPU_V_ITEMS, NVEL = OLAMGPU_VEL_ITEMS;
-        LAUNCH("k_dyn_v", (k_dyn_vn<R, RP, NV>), dim3(nblk_per(nv, NV)), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
-        LAUNCH("k_dyn_vel", (k_dyn_veln<R, RP, NVEL>), dim3(nblk_per(nw, NVEL)), dim3(256), dm, b);
-      } else {
-        LAUNCH("k_dyn_v", (k_dyn_v<R, RP>), dim3(gv), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
-        LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);
-      }
+      // several faces or columns per wave, their loads issued together (vel_t3d_hex's
+      // vertical moves cross the two waves of a tall column: one column per wave there)
+      constexpr int NV = OLAMGPU_V_ITEMS, NVEL = OLAMGPU_VEL_ITEMS;
+      LAUNCH("k_dyn_v", (k_dyn_vn<R, RP, NV>), dim3(nblk_per(nv, NV)), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
+      if (!dm.tall) LAUNCH("k_dyn_vel", (k_dyn_veln<R, RP, NVEL>), dim3(nblk_per(nw, NVEL)), dim3(256), dm, b);
+      else LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);
     }
     HIPCHK(hipGetLastError());
   }
