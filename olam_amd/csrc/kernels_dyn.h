@@ -6,7 +6,7 @@
 // OpenMP loops: k_dyn_grad (grad_t2d of thil, vxe, vye, vze), k_dyn_w (fluxes_wrtv's
 // face and vertical upwinding fused with prog_wrt_begs, including the FP64
 // tridiagonal W solve by parallel cyclic reduction inside the wavefront),
-// k_dyn_vn (prog_v_begs) and k_dyn_vel / k_dyn_veln (vel_t3d_hex).
+// k_dyn_vn (prog_v_begs) and k_dyn_veln (vel_t3d_hex).
 #pragma once
 #ifndef OLAMGPU_BLOCKS_DYNW
 #define OLAMGPU_BLOCKS_DYNW 8
@@ -1137,41 +1137,17 @@ __global__ __launch_bounds__(256) void k_dyn_vn(DMesh<R> m, DynBuf<R, RP> b, int
 }
 
 // ---------------------------------------------------------------- vel_t3d_hex
-// vel_t3d.f90:39-108 (no closed faces on flat terrain)
-template <class R, class RP>
-__global__ __launch_bounds__(256) void k_dyn_vel(DMesh<R> m, DynBuf<R, RP> b) {
-  NOCONTRACT
-  const WPos wp_ = wave_pos(m.tall);
-  const int c = wp_.item;
-  if (c >= m.nw) return;
-  const int l = wp_.l, k = l + 1, mza = m.mza;
-  const bool act = k <= mza, body = act && k >= 2;
-  const long ldk = m.ldk, ic = (long)c * ldk + l;
-  const R wcc = ld(b.wc, ic, act);
-  const R wst = vshup(m.tall, wcc, 1) + wcc;
-  R x = wst * (m.wnx[c] * R(0.5)), y = wst * (m.wny[c] * R(0.5)), z = wst * (m.wnz[c] * R(0.5));
-  const int np = m.w_npoly[c];
-  for_slots(np, [&](int j) {
-    const R v = ld(b.vc, (long)m.w_face[c * NSLOT + j] * ldk + l, act);
-    x = x + m.w_ecx[c * NSLOT + j] * v;
-    y = y + m.w_ecy[c * NSLOT + j] * v;
-    z = z + m.w_ecz[c * NSLOT + j] * v;
-  });
-  const R x2 = vshdn(m.tall, x, 1), y2 = vshdn(m.tall, y, 1), z2 = vshdn(m.tall, z, 1);
-  if (k == 1) { x = x2; y = y2; z = z2; }
-  if (act) { st_nt(b.vxe, ic, x); st_nt(b.vye, ic, y); st_nt(b.vze, ic, z); }
-  (void)body;
-}
-
-// vel_t3d_hex with N columns per wave (64-level columns): the loads of all N columns
-// are issued before any is reduced; the arithmetic is k_dyn_vel's.  Columns past the
-// last one read the wave's first column's data and store nothing.
+// vel_t3d.f90:39-108 (no closed faces on flat terrain).
+// N columns per wave (a wave pair for tall columns): the loads of all N columns are
+// issued before any is reduced.  Columns past the last one read the wave's first column
+// and store nothing.
 template <class R, class RP, int N>
 __global__ __launch_bounds__(256) void k_dyn_veln(DMesh<R> m, DynBuf<R, RP> b) {
   NOCONTRACT
-  const int w = wave_item();
+  const WPos p = wave_pos(m.tall);   // tall columns: the half from the wave index
+  const int w = p.item;
   if (N * w >= m.nw) return;
-  const int l = lane_id(), k = l + 1, mza = m.mza;
+  const int l = p.l, k = l + 1, mza = m.mza;
   const bool act = k <= mza;
   const long ldk = m.ldk;
   R wcc[N], vj[N][7];
@@ -1192,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_dyn_veln(DMesh<R> m, DynBuf<R, RP> b) {
   for (int r = 0; r < N; ++r) {
     const int c = cs[r];
     const long ic = (long)c * ldk + l;
-    const R wst = shup(wcc[r], 1) + wcc[r];
+    const R wst = vshup(m.tall, wcc[r], 1) + wcc[r];   // every item shifts, in both waves
     R x = wst * (m.wnx[c] * R(0.5)), y = wst * (m.wny[c] * R(0.5)), z = wst * (m.wnz[c] * R(0.5));
     for_slots(nps[r], [&](int j) {
       const R v = vj[r][j];
@@ -1200,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_dyn_veln(DMesh<R> m, DynBuf<R, RP> b) {
       y = y + m.w_ecy[c * NSLOT + j] * v;
       z = z + m.w_ecz[c * NSLOT + j] * v;
     });
-    const R x2 = shdn(x, 1), y2 = shdn(y, 1), z2 = shdn(z, 1);
+    const R x2 = vshdn(m.tall, x, 1), y2 = vshdn(m.tall, y, 1), z2 = vshdn(m.tall, z, 1);
     if (k == 1) { x = x2; y = y2; z = z2; }
     if (act && ok[r]) { st_nt(b.vxe, ic, x); st_nt(b.vye, ic, y); st_nt(b.vze, ic, z); }
   }

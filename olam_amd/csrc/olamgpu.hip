@@ -1431,12 +1431,10 @@ struct Impl : ImplBase {
         launch_dyn_w<false>(b, nud, gw);
       }
       cur_thil = 1 - cur_thil;
-      // several faces or columns per wave, their loads issued together (vel_t3d_hex's
-      // vertical moves cross the two waves of a tall column: one column per wave there)
+      // several faces or columns per wave, their loads issued together
       constexpr int NV = OLAMGPU_V_ITEMS, NVEL = OLAMGPU_VEL_ITEMS;
       LAUNCH("k_dyn_v", (k_dyn_vn<R, RP, NV>), dim3(nblk_per(nv, NV)), dim3(256), dm, b, istage == nrk - 1 ? 1 : 0);
-      if (!dm.tall) LAUNCH("k_dyn_vel", (k_dyn_veln<R, RP, NVEL>), dim3(nblk_per(nw, NVEL)), dim3(256), dm, b);
-      else LAUNCH("k_dyn_vel", (k_dyn_vel<R, RP>), dim3(gw), dim3(256), dm, b);
+      LAUNCH("k_dyn_vel", (k_dyn_veln<R, RP, NVEL>), dim3(nblk_per(nw, NVEL)), dim3(256), dm, b);
     }
     HIPCHK(hipGetLastError());
   }
