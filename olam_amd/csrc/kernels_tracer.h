@@ -14,6 +14,9 @@
 // Register budgets of the hottest tracer kernels, as the minimum number of 256-thread
 // blocks per CU (= waves per SIMD) the compiler must allow: 4 gives 128 VGPRs, 5 gives
 // 96, 6 gives 80.  Measured choices; a build may override them (-D...) to re-tune.
+#ifndef OLAMGPU_BLOCKS_UPDATE64
+#define OLAMGPU_BLOCKS_UPDATE64 3
+#endif
 #ifndef OLAMGPU_BLOCKS_UPDATE
 #define OLAMGPU_BLOCKS_UPDATE 4
 #endif
@@ -135,7 +138,11 @@ __device__ __forceinline__ R stage_rho(R ro, R rf, int istage, int nrk) {
 // One explicit RK stage of all tracers: scalar_v_column + scalar_w_column
 // (:474-611), with the explicit weight fexpl = 1 - ff_implic of :578.
 template <class R, class RP, bool Q3, bool PK = false, bool TALL = false>
-__global__ __launch_bounds__(256, Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)) void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
+// FP64 instances get the register room of fewer waves (held to the FP32 bounds they
+// spilled 38-88 registers per lane; the packed one ran 2.2x faster without)
+__global__ __launch_bounds__(256, sizeof(R) == 8 ? (Q3 ? 1 : (PK ? OLAMGPU_BLOCKS_UPDATE64 : 2))
+                                                : (Q3 ? 2 : (PK ? OLAMGPU_BLOCKS_UPDATE : 4)))
+void k_tr_update(DMesh<R> m, TrStage<R, RP> a) {
   const TrItem ti = tr_item(m.nw, a.ntr, a.tpw, TALL);
   const int c = ti.c;
   if (c >= m.nw) return;
